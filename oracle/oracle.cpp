@@ -1,0 +1,1110 @@
+// ORACLE (test infrastructure only).  CPU restatement of the reference
+// pipeline (DavidXu-JJ/PoissonRecon_GPU, main.cu), stage by stage, used as the
+// parity checker for the HIP product and as the CPU baseline in bench.py.
+// Nothing in the product links or calls this file.
+//
+// Each stage cites the reference code it restates.  Where the reference has
+// undefined behaviour the intended semantics are used:
+//   - every sibling slot's parent is its true parent (main.cu:442-448 writes
+//     the parent of one hashed child only);
+//   - pidx of a leaf is the first sorted index of its point run (the
+//     insertMin race of Hash.cuh:53-71 aside);
+//   - children[k] = block base + k for interior nodes and -1 for leaves and
+//     empty nodes (main.cu:426,466-471 leave some of these uninitialised);
+//   - coarse-depth divergence (depth <= 4) uses the same double accumulation
+//     as the finer depths instead of a float thrust::reduce of unspecified
+//     order (main.cu:3449).
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "oracle_funcspace.h"
+
+namespace {
+
+// Marching-cubes triangle table (data; MarchingCubes.cuh:394-651).
+#include "../poissonrecon_gpu_amd/csrc/mc_triangle_table.inc"
+const int kTriTable[256][16] = {PRG_MC_TRIANGLE_TABLE_DATA};
+
+// edgeVertex (MarchingCubes.cuh:693-706): the two square-numbered corners of
+// each edge kind.
+const int kEdgeVertex[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 3}, {1, 2},
+                                {4, 7}, {5, 6}, {0, 4}, {1, 5}, {3, 7}, {2, 6}};
+// childrenVertexKind (MarchingCubes.cuh:720-723), used literally (Q9).
+const int kChildrenVertexKind[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+
+inline int tri_count(int cube) {
+    int n = 0;
+    while (n < 16 && kTriTable[cube][n] >= 0) n++;
+    return n / 3;
+}
+
+// LUTparent / LUTchild (main.cu:80-99) derived from their definition: for a
+// child in slot s = x<<2|y<<1|z and neighbour offset j = 9(dx+1)+3(dy+1)+(dz+1),
+// the neighbour lies in the parent's neighbour floor((c+d)/2) at child slot
+// (c+d) mod 2 on every axis.
+struct Luts {
+    int parent[8][27], child[8][27];
+    Luts() {
+        for (int s = 0; s < 8; s++)
+            for (int j = 0; j < 27; j++) {
+                int c[3] = {(s >> 2) & 1, (s >> 1) & 1, s & 1};
+                int d[3] = {j / 9 - 1, (j / 3) % 3 - 1, j % 3 - 1};
+                int pj = 0, cs = 0;
+                for (int a = 0; a < 3; a++) {
+                    int v = c[a] + d[a];
+                    int po = (v < 0) ? -1 : (v > 1 ? 1 : 0);
+                    int co = v - 2 * po;
+                    pj = pj * 3 + (po + 1);
+                    cs = (cs << 1) | co;
+                }
+                parent[s][j] = pj;
+                child[s][j] = cs;
+            }
+    }
+};
+const Luts kLuts;
+
+}  // namespace
+
+struct orc_state {
+    int D = 0;
+    int64_t N = 0;
+    std::vector<float> pts, nrm;  // normalised; sorted after orc_build_octree
+    float center[3] = {0, 0, 0};
+    float scale = 1.0f;
+    bool have_points = false, have_tree = false, have_div = false, have_x = false;
+
+    std::vector<int64_t> counts, bases;
+    int64_t M = 0;
+    std::vector<uint32_t> key;
+    std::vector<int32_t> pidx, pnum, parent, didx, dnum, children, neighs, point_leaf;
+    std::vector<int32_t> depth_of;
+
+    std::unique_ptr<orc::FunctionSpace> fs;
+    std::vector<orc::CPoly> base_functions;  // res entries
+    orc::CPoly base_maxdepth;
+    std::vector<std::vector<float>> div_table;  // per depth, 3*2^(D-d) entries
+    std::vector<std::vector<float>> stencil;    // per depth, 27 entries
+
+    std::vector<float> vfield, div, x, pvals, leaf_corner_vals;
+    float iso = 0.0f;
+    bool iso_set = false;
+
+    std::vector<float> mesh_v;
+    std::vector<int32_t> mesh_t;
+
+    void ensure_fs() {
+        if (fs) return;
+        fs.reset(new orc::FunctionSpace(D));
+        base_functions.resize(fs->res);
+        for (int64_t i = 0; i < fs->res; i++) base_functions[i] = orc::cpoly_from(fs->base_function(i));
+        base_maxdepth = orc::cpoly_from(fs->base_maxdepth());
+        // Divergence weights dDot[node + res*leaf] (main.cu:1030-1054) depend only
+        // on the node depth d and the leaf's offset u = o_leaf - (o_node-1)*2^(D-d)
+        // (the entries are exact functions of relative position).
+        div_table.assign(D + 1, {});
+        for (int d = 0; d <= D; d++) {
+            int64_t s = 1LL << (D - d);
+            div_table[d].resize(3 * s);
+            for (int64_t u = 0; u < 3 * s; u++)
+                div_table[d][u] = float(fs->entry(1, d, 0, D, u - s));
+        }
+        stencil.assign(D + 1, std::vector<float>(27, 0.0f));
+        for (int d = 0; d <= D; d++) {
+            double dt[3], d2[3];
+            for (int o = -1; o <= 1; o++) {
+                // dot_F_F[idx_node*res + idx_neigh] (main.cu:1195-1199)
+                dt[o + 1] = fs->entry(0, d, o, d, 0);
+                d2[o + 1] = fs->entry(2, d, o, d, 0);
+            }
+            for (int j = 0; j < 27; j++) {
+                int ox = j / 9, oy = (j / 3) % 3, oz = j % 3;
+                // GetLaplacianEntry (main.cu:1143-1158)
+                double v = d2[ox] * dt[oy] * dt[oz] + d2[oy] * dt[ox] * dt[oz] + d2[oz] * dt[ox] * dt[oy];
+                // kept when |v| > EPSILON (main.cu:1204)
+                stencil[d][j] = (std::fabs(v) > double(float(1e-6))) ? float(v) : 0.0f;
+            }
+        }
+    }
+    int64_t MD() const { return counts.empty() ? 0 : counts[D]; }
+};
+
+namespace {
+
+// encodePoint (main.cu:132-164): successive halvings with strict '>'.
+uint32_t encode_point(const float* p, int D) {
+    uint32_t key = 0;
+    float c[3] = {0.5f, 0.5f, 0.5f};
+    float w = 0.25f;
+    for (int i = D - 1; i >= 0; --i) {
+        for (int a = 0; a < 3; a++) {
+            if (p[a] > c[a]) {
+                key |= 1u << (3 * i + 2 - a);
+                c[a] = c[a] + w;
+            } else {
+                c[a] = c[a] - w;
+            }
+        }
+        w = w / 2;
+    }
+    return key;
+}
+
+// getFunctionIdxOfNode (main.cu:861-885): per-axis offsets at depth d.
+inline void cell_offsets(uint32_t key, int d, int D, int64_t o[3]) {
+    o[0] = o[1] = o[2] = 0;
+    for (int l = 1; l <= d; l++) {
+        int sh = 3 * (D - l);
+        o[0] = (o[0] << 1) | ((key >> (sh + 2)) & 1);
+        o[1] = (o[1] << 1) | ((key >> (sh + 1)) & 1);
+        o[2] = (o[2] << 1) | ((key >> sh) & 1);
+    }
+}
+
+// getNodeCenterAllDepth (main.cu:1403-1420)
+inline void node_center(uint32_t key, int depth, int D, float c[3]) {
+    c[0] = c[1] = c[2] = 0.5f;
+    float w = 0.25f;
+    for (int i = D - 1; i >= D - depth; --i) {
+        for (int a = 0; a < 3; a++) {
+            if ((key >> (3 * i + 2 - a)) & 1)
+                c[a] = c[a] + w;
+            else
+                c[a] = c[a] - w;
+        }
+        w = w / 2;
+    }
+}
+
+// SquareDistance in float (Geometry.inl:37-40)
+inline float sqdist(const float* a, const float* b) {
+    float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+    float s = dx * dx;
+    s = s + dy * dy;
+    s = s + dz * dz;
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+orc_state* orc_create(int32_t max_depth) {
+    if (max_depth < 2 || max_depth > 10) return nullptr;
+    orc_state* s = new orc_state();
+    s->D = max_depth;
+    return s;
+}
+
+void orc_destroy(orc_state* s) { delete s; }
+
+int32_t orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+void orc_set_num_threads(int32_t n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+// ---- A1: bounding box and normalisation (main.cu:530-571) -------------------
+int32_t orc_set_points(orc_state* s, const float* xyz, const float* nxyz, int64_t n) {
+    if (n < 1 || n >= (1LL << 31)) return 1;
+    float mn[3], mx[3];
+    for (int64_t i = 0; i < n; i++)
+        for (int a = 0; a < 3; a++) {
+            float v = xyz[3 * i + a];
+            if (!i || v < mn[a]) mn[a] = v;
+            if (!i || v > mx[a]) mx[a] = v;
+        }
+    float scale = 1.0f;
+    float center[3];
+    for (int a = 0; a < 3; a++) {
+        if (!a || scale < mx[a] - mn[a]) scale = float(mx[a] - mn[a]);
+        center[a] = float(mx[a] + mn[a]) / 2;
+    }
+    scale = scale * 1.25f;
+    for (int a = 0; a < 3; a++) center[a] = center[a] - scale / 2;
+    s->N = n;
+    s->pts.resize(3 * n);
+    s->nrm.resize(3 * n);
+    const float nscale = float(2 << s->D);
+    for (int64_t i = 0; i < n; i++) {
+        for (int a = 0; a < 3; a++) s->pts[3 * i + a] = (xyz[3 * i + a] - center[a]) / scale;
+        const float* q = nxyz + 3 * i;
+        float sq = q[0] * q[0];
+        sq = sq + q[1] * q[1];
+        sq = sq + q[2] * q[2];
+        float len = float(std::sqrt(double(sq)));
+        if (len > float(1e-6)) len = 1.0f / len;
+        len = len * nscale;
+        for (int a = 0; a < 3; a++) s->nrm[3 * i + a] = q[a] * len;
+    }
+    std::memcpy(s->center, center, sizeof(center));
+    s->scale = scale;
+    s->have_points = true;
+    s->have_tree = s->have_div = s->have_x = false;
+    return 0;
+}
+
+void orc_get_transform(orc_state* s, float* center3, float* scale) {
+    std::memcpy(center3, s->center, 3 * sizeof(float));
+    *scale = s->scale;
+}
+
+// ---- A2-A8: octree (pipelineBuildNodeArray, main.cu:511-841) ----------------
+int32_t orc_build_octree(orc_state* s) {
+    if (!s->have_points) return 6;
+    const int D = s->D;
+    const int64_t N = s->N;
+    // A2 generateCode: code = key << 32 | original index (main.cu:166-173)
+    std::vector<uint64_t> code(N);
+#pragma omp parallel for
+    for (int64_t i = 0; i < N; i++) code[i] = (uint64_t(encode_point(&s->pts[3 * i], D)) << 32) | uint64_t(i);
+    // A3 sort_by_key (main.cu:596-603); codes are unique so the order is total
+    std::sort(code.begin(), code.end());
+    {
+        std::vector<float> p(3 * N), q(3 * N);
+        for (int64_t r = 0; r < N; r++) {
+            int64_t i = int64_t(code[r] & 0xffffffffu);
+            for (int a = 0; a < 3; a++) {
+                p[3 * r + a] = s->pts[3 * i + a];
+                q[3 * r + a] = s->nrm[3 * i + a];
+            }
+        }
+        s->pts.swap(p);
+        s->nrm.swap(q);
+    }
+    // A4 unique leaves: run starts (generateMark + copy_if, main.cu:195-224,615-641)
+    struct Node {
+        uint32_t key = 0;
+        int32_t pidx = 0, pnum = 0, didx = 0, dnum = 0;
+        int32_t child_block = -1;  // block index in the next finer level
+    };
+    std::vector<Node> uniq;
+    for (int64_t r = 0; r < N; r++) {
+        uint32_t k = uint32_t(code[r] >> 32);
+        if (r == 0 || k != uint32_t(code[r - 1] >> 32)) {
+            Node nd;
+            nd.key = k;
+            nd.pidx = int32_t(r);
+            nd.pnum = 0;
+            uniq.push_back(nd);
+        }
+        uniq.back().pnum++;
+    }
+    // Per-depth sibling-complete arrays, built finest first.
+    std::vector<std::vector<Node>> level(D + 1);
+    std::vector<std::vector<int32_t>> block_parent_pos(D + 1);  // per level: block -> parent slot
+    s->point_leaf.assign(N, -1);
+    // A5 leaf level (generateNodeNums + inclusive_scan + generateNodeArrayD, main.cu:226-249,643-684)
+    {
+        std::vector<int64_t> addr(uniq.size());
+        int64_t a = 0;
+        for (size_t u = 0; u < uniq.size(); u++) {
+            if (u > 0 && (uniq[u].key >> 3) != (uniq[u - 1].key >> 3)) a += 8;
+            addr[u] = a;
+        }
+        int64_t cnt = a + 8;
+        level[D].assign(cnt, Node());
+        for (size_t u = 0; u < uniq.size(); u++) {
+            int64_t idx = addr[u] + (uniq[u].key & 7);
+            level[D][idx].key = uniq[u].key;
+            level[D][idx].pidx = uniq[u].pidx;
+            level[D][idx].pnum = uniq[u].pnum;
+            for (int32_t r = uniq[u].pidx; r < uniq[u].pidx + uniq[u].pnum; r++) s->point_leaf[r] = int32_t(idx);
+        }
+        // initNodeArrayD_DIdxDnum (main.cu:264-273)
+        for (int64_t i = 0; i < cnt; i++) {
+            level[D][i].dnum = 1;
+            level[D][i].didx = int32_t(i);
+        }
+    }
+    // A6 coarser levels, D -> 1 (main.cu:703-770)
+    for (int d = D; d >= 1; --d) {
+        std::vector<Node>& fine = level[d];
+        int64_t nblocks = int64_t(fine.size()) / 8;
+        std::vector<Node> up(nblocks);
+        const int sh = 3 * (D - d);
+        for (int64_t b = 0; b < nblocks; b++) {
+            Node nd;
+            nd.pidx = 0x7fffffff;
+            nd.didx = 0x7fffffff;
+            int v = 0;
+            while (v < 8 && fine[8 * b + v].pnum == 0) v++;
+            nd.key = fine[8 * b + v].key & ~(7u << sh);
+            for (int j = 0; j < 8; j++) {
+                const Node& c = fine[8 * b + j];
+                if (c.pnum != 0) {
+                    nd.pnum += c.pnum;
+                    nd.pidx = std::min(nd.pidx, c.pidx);
+                }
+                if (c.pnum != 0 || c.dnum != 0) {
+                    nd.dnum += c.dnum;
+                    nd.didx = std::min(nd.didx, c.didx);
+                }
+            }
+            nd.child_block = int32_t(b);
+            up[b] = nd;
+        }
+        block_parent_pos[d].resize(nblocks);
+        if (d > 1) {
+            // generateNodeNumsD_1 + inclusive_scan + generateNodeArrayD_1 (main.cu:336-359,383-402)
+            const int gsh = 3 * (D - d + 2);
+            const int ssh = 3 * (D - d + 1);
+            int64_t a = 0;
+            std::vector<int64_t> addr(nblocks);
+            for (int64_t b = 0; b < nblocks; b++) {
+                if (b > 0 && (up[b].key >> gsh) != (up[b - 1].key >> gsh)) a += 8;
+                addr[b] = a;
+            }
+            level[d - 1].assign(a + 8, Node());
+            for (int64_t b = 0; b < nblocks; b++) {
+                int64_t pos = addr[b] + ((up[b].key >> ssh) & 7);
+                level[d - 1][pos] = up[b];
+                block_parent_pos[d][b] = int32_t(pos);
+            }
+        } else {
+            level[0] = up;  // the root (main.cu:757-762)
+            block_parent_pos[1][0] = 0;
+        }
+    }
+    // A7 concatenate and fix up (main.cu:772-799, updateParentChildren :404-431,
+    // updateEmptyNodeInfo :433-490), intended semantics.
+    s->counts.assign(D + 1, 0);
+    s->bases.assign(D + 1, 0);
+    for (int d = 0; d <= D; d++) s->counts[d] = int64_t(level[d].size());
+    for (int d = 1; d <= D; d++) s->bases[d] = s->bases[d - 1] + s->counts[d - 1];
+    const int64_t M = s->bases[D] + s->counts[D];
+    s->M = M;
+    s->key.assign(M, 0);
+    s->pidx.assign(M, 0);
+    s->pnum.assign(M, 0);
+    s->parent.assign(M, -1);
+    s->didx.assign(M, 0);
+    s->dnum.assign(M, 0);
+    s->children.assign(8 * M, -1);
+    s->neighs.assign(27 * M, -1);
+    s->depth_of.assign(M, 0);
+    for (int d = 0; d <= D; d++) {
+        const int64_t base = s->bases[d];
+        const std::vector<Node>& L = level[d];
+        for (int64_t i = 0; i < int64_t(L.size()); i++) {
+            int64_t g = base + i;
+            s->key[g] = L[i].key;
+            s->pidx[g] = L[i].pidx;
+            s->pnum[g] = L[i].pnum;
+            s->didx[g] = L[i].didx;
+            s->dnum[g] = L[i].dnum;
+            s->depth_of[g] = d;
+            if (d < D && L[i].pnum != 0)
+                for (int k = 0; k < 8; k++) s->children[8 * g + k] = int32_t(s->bases[d + 1] + 8 * L[i].child_block + k);
+        }
+        if (d == 0) continue;
+        const int sh = 3 * (D - d);
+        for (int64_t b = 0; b < int64_t(L.size()) / 8; b++) {
+            int v = 0;
+            while (v < 8 && L[8 * b + v].pnum == 0) v++;
+            int32_t nowP = L[8 * b + v].pidx, nowD = L[8 * b + v].didx;
+            uint32_t baseKey = L[8 * b + v].key - (L[8 * b + v].key & (7u << sh));
+            int32_t par = int32_t(s->bases[d - 1] + block_parent_pos[d][b]);
+            for (int j = 0; j < 8; j++) {
+                int64_t g = base + 8 * b + j;
+                s->key[g] = baseKey + (uint32_t(j) << sh);
+                s->pidx[g] = nowP;
+                nowP += L[8 * b + j].pnum;
+                if (d != D) {
+                    s->didx[g] = nowD;
+                    nowD += L[8 * b + j].dnum;
+                }
+                s->parent[g] = par;
+            }
+        }
+    }
+    s->parent[0] = -1;
+    // A8 neighbours (computeNeighbor, main.cu:492-509; root row :803-808)
+    s->neighs[13] = 0;
+    for (int d = 1; d <= D; d++) {
+        const int sh = 3 * (D - d);
+        const int64_t lo = s->bases[d], hi = lo + s->counts[d];
+#pragma omp parallel for
+        for (int64_t i = lo; i < hi; i++) {
+            int sonKey = (s->key[i] >> sh) & 7;
+            int32_t p = s->parent[i];
+            for (int j = 0; j < 27; j++) {
+                int32_t np = s->neighs[27 * int64_t(p) + kLuts.parent[sonKey][j]];
+                int32_t v = -1;
+                if (np != -1) v = s->children[8 * int64_t(np) + kLuts.child[sonKey][j]];
+                s->neighs[27 * i + j] = v;
+            }
+        }
+    }
+    s->have_tree = true;
+    s->have_div = s->have_x = false;
+    s->x.assign(M, 0.0f);
+    s->div.assign(M, 0.0f);
+    return 0;
+}
+
+void orc_level_info(orc_state* s, int64_t* counts, int64_t* bases) {
+    for (int d = 0; d <= s->D; d++) {
+        counts[d] = s->counts.empty() ? 0 : s->counts[d];
+        bases[d] = s->bases.empty() ? 0 : s->bases[d];
+    }
+}
+
+// ---- A11-A13: vector field and divergence -----------------------------------
+int32_t orc_compute_divergence(orc_state* s) {
+    if (!s->have_tree) return 6;
+    s->ensure_fs();
+    const int D = s->D;
+    const int64_t MD = s->MD(), baseD = s->bases[D];
+    s->vfield.assign(3 * MD, 0.0f);
+    const float width = float(1.0 / double(1 << D));
+    // computeVectorField + F_center_width_Point (main.cu:913-964)
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t t = 0; t < MD; t++) {
+        const int64_t i = baseD + t;
+        int64_t o[3];
+        cell_offsets(s->key[i], D, D, o);
+        float oc[3];
+        for (int a = 0; a < 3; a++) oc[a] = float((0.5 + double(o[a])) * double(width));
+        float val[3] = {0.0f, 0.0f, 0.0f};
+        for (int j = 0; j < 27; j++) {
+            int32_t nb = s->neighs[27 * i + j];
+            if (nb == -1) continue;
+            for (int32_t k = 0; k < s->pnum[nb]; k++) {
+                const int64_t pi = int64_t(s->pidx[nb]) + k;
+                const float* p = &s->pts[3 * pi];
+                float fx = orc::cpoly_value(orc::cpoly_shift(s->base_maxdepth, p[0]), oc[0]);
+                float fy = orc::cpoly_value(orc::cpoly_shift(s->base_maxdepth, p[1]), oc[1]);
+                float fz = orc::cpoly_value(orc::cpoly_shift(s->base_maxdepth, p[2]), oc[2]);
+                float w = fx * fy;
+                w = w * fz;
+                for (int a = 0; a < 3; a++) val[a] = val[a] + w * s->nrm[3 * pi + a];
+            }
+        }
+        for (int a = 0; a < 3; a++) s->vfield[3 * t + a] = 0.0f + val[a];
+    }
+    // computeEncodedFinerNodesDivergence (main.cu:1007-1060) for every node;
+    // the depth <= 4 nodes (main.cu:1062-1141,3417-3459) use the same sum.
+    s->div.assign(s->M, 0.0f);
+    for (int d = 0; d <= D; d++) {
+        const int64_t lo = s->bases[d], hi = lo + s->counts[d];
+        const int64_t sc = 1LL << (D - d);
+        const std::vector<float>& T = s->div_table[d];
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t i = lo; i < hi; i++) {
+            int64_t oi[3];
+            cell_offsets(s->key[i], d, D, oi);
+            double val = 0.0;
+            for (int j = 0; j < 27; j++) {
+                int32_t nb = s->neighs[27 * i + j];
+                if (nb == -1) continue;
+                for (int32_t k = 0; k < s->dnum[nb]; k++) {
+                    const int64_t leaf = int64_t(s->didx[nb]) + k;
+                    int64_t ol[3];
+                    cell_offsets(s->key[baseD + leaf], D, D, ol);
+                    float uo[3];
+                    for (int a = 0; a < 3; a++) uo[a] = T[ol[a] - (oi[a] - 1) * sc];
+                    const float* vo = &s->vfield[3 * leaf];
+                    // DotProduct (main.cu:966-972) in float
+                    float r = 0.0f;
+                    r = r + vo[0] * uo[0];
+                    r = r + vo[1] * uo[1];
+                    r = r + vo[2] * uo[2];
+                    val += double(r);
+                }
+            }
+            s->div[i] = float(val);
+        }
+    }
+    s->have_div = true;
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Float SpMV y = A x for depth d: entries in neighbour order j = 0..26 as the
+// CSR rows of LaplacianIteration hold them (main.cu:1182-1210); each product
+// val*x is rounded, then accumulated in float (gpuSpMV, CG_CUDA.cuh:186-203).
+void spmv_depth(const orc_state* s, int d, const float* xin, float* y) {
+    const int64_t lo = s->bases[d], n = s->counts[d];
+    const float* L = s->stencil[d].data();
+#pragma omp parallel for
+    for (int64_t t = 0; t < n; t++) {
+        const int64_t i = lo + t;
+        float out = 0.0f;
+        for (int j = 0; j < 27; j++) {
+            int32_t nb = s->neighs[27 * i + j];
+            if (nb == -1 || L[j] == 0.0f) continue;
+            float prod = L[j] * xin[nb - lo];
+            out = out + prod;
+        }
+        y[t] = out;
+    }
+}
+
+double dot_dd(const float* a, const float* b, int64_t n) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; i++) s += double(a[i]) * double(b[i]);
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- A14-A15 per-depth solves -------------------------------------------------
+int32_t orc_solve(orc_state* s, int32_t mode, float tol, int32_t max_iters, int32_t refresh,
+                  int32_t fixed_iters, int32_t* iters_out, double* relres_out) {
+    if (!s->have_div) return 6;
+    s->ensure_fs();
+    const int D = s->D;
+    s->x.assign(s->M, 0.0f);
+    for (int d = 0; d <= D; d++) {
+        const int64_t lo = s->bases[d], n = s->counts[d];
+        const float* b = &s->div[lo];
+        float* x = &s->x[lo];
+        std::vector<float> r(n), p(n), q(n);
+        int it = 0;
+        double rel = 0.0;
+        if (mode == 0) {
+            // The build's CG: FP32 vectors and SpMV, FP64 dots and scalars,
+            // relative stop ||r|| <= tol ||b||, r recomputed every `refresh`.
+            spmv_depth(s, d, x, q.data());
+            for (int64_t i = 0; i < n; i++) r[i] = b[i] - q[i];
+            double bb = dot_dd(b, b, n);
+            double rr = dot_dd(r.data(), r.data(), n);
+            p = r;
+            const double tol2 = double(tol) * double(tol) * bb;
+            while (bb > 0.0 && it < max_iters && (fixed_iters > 0 ? it < fixed_iters : rr > tol2)) {
+                spmv_depth(s, d, p.data(), q.data());
+                double pq = dot_dd(p.data(), q.data(), n);
+                float af = float(rr / pq);
+                for (int64_t i = 0; i < n; i++) {
+                    float t1 = af * p[i];
+                    x[i] = x[i] + t1;
+                    float t2 = af * q[i];
+                    r[i] = r[i] - t2;
+                }
+                it++;
+                if (refresh > 0 && it % refresh == 0) {
+                    spmv_depth(s, d, x, q.data());
+                    for (int64_t i = 0; i < n; i++) r[i] = b[i] - q[i];
+                }
+                double rr2 = dot_dd(r.data(), r.data(), n);
+                float bf = float(rr2 / rr);
+                rr = rr2;
+                for (int64_t i = 0; i < n; i++) {
+                    float t = bf * p[i];
+                    p[i] = r[i] + t;
+                }
+            }
+            rel = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+        } else {
+            // The reference's CG (gpuConjugateGradient, CG_CUDA.cuh:256-324):
+            // x0 = 0, r = b, float scalars, dot = sum double(float(a*b)),
+            // stop when r.r <= tol^2 (absolute).
+            for (int64_t i = 0; i < n; i++) r[i] = b[i];
+            auto dotf = [&](const float* a, const float* c) {
+                double acc = 0.0;
+                for (int64_t i = 0; i < n; i++) acc += double(a[i] * c[i]);
+                return acc;
+            };
+            float r1 = float(dotf(r.data(), r.data())), r0 = 0.0f;
+            int k = 1;
+            const float tt = tol * tol;
+            while ((fixed_iters > 0 ? k <= fixed_iters : r1 > tt) && k <= max_iters) {
+                if (k > 1) {
+                    float bta = r1 / r0;
+                    for (int64_t i = 0; i < n; i++) {
+                        float a1 = 1.0f * r[i];
+                        float a2 = bta * p[i];
+                        p[i] = a1 + a2;
+                    }
+                } else {
+                    p = r;
+                }
+                spmv_depth(s, d, p.data(), q.data());
+                double dv = dotf(p.data(), q.data());
+                float a = float(double(r1) / dv);
+                for (int64_t i = 0; i < n; i++) {
+                    float t = a * p[i];
+                    x[i] = t + x[i];
+                }
+                float na = -a;
+                for (int64_t i = 0; i < n; i++) {
+                    float t = na * q[i];
+                    r[i] = t + r[i];
+                }
+                r0 = r1;
+                r1 = float(dotf(r.data(), r.data()));
+                k++;
+            }
+            it = k - 1;
+            double bb = dot_dd(b, b, n);
+            rel = bb > 0.0 ? std::sqrt(double(r1) / bb) : 0.0;
+        }
+        if (iters_out) iters_out[d] = it;
+        if (relres_out) relres_out[d] = rel;
+    }
+    s->have_x = true;
+    return 0;
+}
+
+// FP64 CG on the same float systems (error statements for the FP32 solves).
+int32_t orc_solve_fp64(orc_state* s, double tol, int32_t max_iters, double* x64) {
+    if (!s->have_div) return 6;
+    s->ensure_fs();
+    for (int d = 0; d <= s->D; d++) {
+        const int64_t lo = s->bases[d], n = s->counts[d];
+        const float* L = s->stencil[d].data();
+        auto Ax = [&](const std::vector<double>& v, std::vector<double>& y) {
+#pragma omp parallel for
+            for (int64_t t = 0; t < n; t++) {
+                const int64_t i = lo + t;
+                double out = 0.0;
+                for (int j = 0; j < 27; j++) {
+                    int32_t nb = s->neighs[27 * i + j];
+                    if (nb == -1 || L[j] == 0.0f) continue;
+                    out += double(L[j]) * v[nb - lo];
+                }
+                y[t] = out;
+            }
+        };
+        std::vector<double> x(n, 0.0), r(n), p(n), q(n);
+        double bb = 0.0;
+        for (int64_t i = 0; i < n; i++) {
+            r[i] = double(s->div[lo + i]);
+            bb += r[i] * r[i];
+        }
+        p = r;
+        double rr = bb;
+        int it = 0;
+        while (bb > 0.0 && rr > tol * tol * bb && it < max_iters) {
+            Ax(p, q);
+            double pq = 0.0;
+            for (int64_t i = 0; i < n; i++) pq += p[i] * q[i];
+            double a = rr / pq;
+            for (int64_t i = 0; i < n; i++) {
+                x[i] += a * p[i];
+                r[i] -= a * q[i];
+            }
+            double rr2 = 0.0;
+            for (int64_t i = 0; i < n; i++) rr2 += r[i] * r[i];
+            double bt = rr2 / rr;
+            rr = rr2;
+            for (int64_t i = 0; i < n; i++) p[i] = r[i] + bt * p[i];
+            it++;
+        }
+        for (int64_t i = 0; i < n; i++) x64[lo + i] = x[i];
+    }
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// 27-neighbour contribution of node `nd` to f at position pos
+// (calculatePointsImplicitFunctionValue main.cu:1353-1374 and
+// computeVertexImplicitFunctionValue main.cu:2277-2322): float, j = 0..26.
+inline float node_contrib(const orc_state* s, int64_t nd, const float* pos, float val) {
+    const int D = s->D;
+    for (int j = 0; j < 27; j++) {
+        int32_t nb = s->neighs[27 * nd + j];
+        if (nb == -1) continue;
+        int d = s->depth_of[nb];
+        int64_t o[3];
+        cell_offsets(s->key[nb], d, D, o);
+        const int64_t fb = (1LL << d) - 1;
+        float vx = orc::cpoly_value(s->base_functions[fb + o[0]], pos[0]);
+        float vy = orc::cpoly_value(s->base_functions[fb + o[1]], pos[1]);
+        float vz = orc::cpoly_value(s->base_functions[fb + o[2]], pos[2]);
+        float t = s->x[nb] * vx;
+        t = t * vy;
+        t = t * vz;
+        val = val + t;
+    }
+    return val;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- A16 isovalue ---------------------------------------------------------------
+float orc_isovalue(orc_state* s) {
+    if (!s->have_x) return 0.0f;
+    s->ensure_fs();
+    const int64_t N = s->N, baseD = s->bases[s->D];
+    s->pvals.assign(N, 0.0f);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < N; i++) {
+        int64_t nd = baseD + s->point_leaf[i];
+        float val = 0.0f;
+        while (nd != -1) {
+            val = node_contrib(s, nd, &s->pts[3 * i], val);
+            nd = s->parent[nd];
+        }
+        s->pvals[i] = val;
+    }
+    double sum = 0.0;
+    for (int64_t i = 0; i < N; i++) sum += double(s->pvals[i]);
+    s->iso = float(sum) / float(N);
+    s->iso_set = true;
+    return s->iso;
+}
+
+void orc_set_isovalue(orc_state* s, float iso) {
+    s->iso = iso;
+    s->iso_set = true;
+}
+
+// ---- A17-A18 leaf-level marching cubes (main.cu:3502-3795) ------------------------
+int32_t orc_extract_leaf_mesh(orc_state* s, int64_t* n_vertices, int64_t* n_triangles) {
+    if (!s->have_x || !s->iso_set) return 6;
+    s->ensure_fs();
+    const int D = s->D;
+    const int64_t M = s->M, baseD = s->bases[D], MD = s->MD();
+    // precomputeDepthAndCenter (main.cu:1423-1436)
+    std::vector<float> C(3 * M);
+#pragma omp parallel for
+    for (int64_t i = 0; i < M; i++) node_center(s->key[i], s->depth_of[i], D, &C[3 * i]);
+    // initVertexOwner (main.cu:1438-1498) for every node; validVertex drops
+    // owner 0 (main.cu:1634-1638)
+    std::vector<uint8_t> own(8 * M, 0);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t i = 1; i < M; i++) {
+        const int d = s->depth_of[i];
+        const float hw = 1.0f / float(1 << (d + 1));
+        const float W = 1.0f / float(1 << d);
+        const float W2 = W * W;
+        for (int j = 0; j < 8; j++) {
+            float vp[3];
+            vp[0] = C[3 * i + 0] + float(2 * (j & 1) - 1) * hw;
+            vp[1] = C[3 * i + 1] + float(2 * ((j & 2) >> 1) - 1) * hw;
+            vp[2] = C[3 * i + 2] + float(2 * ((j & 4) >> 2) - 1) * hw;
+            uint32_t best = 0x7fffffffu;
+            int64_t owner = -1;
+            for (int k = 0; k < 27; k++) {
+                int32_t nb = s->neighs[27 * i + k];
+                if (nb == -1 || !(sqdist(vp, &C[3 * int64_t(nb)]) < W2)) continue;
+                if (best > s->key[nb]) {
+                    best = s->key[nb];
+                    owner = nb;
+                }
+            }
+            own[8 * i + j] = (owner == i);
+        }
+    }
+    std::vector<int64_t> vnode, vkind;  // VertexArray (copy_if order)
+    for (int64_t v = 0; v < 8 * M; v++)
+        if (own[v]) {
+            vnode.push_back(v / 8);
+            vkind.push_back(v % 8);
+        }
+    const int64_t V = int64_t(vnode.size());
+    std::vector<float> vpos(3 * V);
+    std::vector<int32_t> vertices(8 * M, 0);
+    // maintainVertexNodePointerNonAtomic (main.cu:1640-1681)
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t v = 0; v < V; v++) {
+        const int64_t o = vnode[v];
+        const int j = int(vkind[v]);
+        const int d = s->depth_of[o];
+        const float hw = 1.0f / float(1 << (d + 1));
+        const float W = 1.0f / float(1 << d);
+        const float W2 = W * W;
+        float* vp = &vpos[3 * v];
+        vp[0] = C[3 * o + 0] + float(2 * (j & 1) - 1) * hw;
+        vp[1] = C[3 * o + 1] + float(2 * ((j & 2) >> 1) - 1) * hw;
+        vp[2] = C[3 * o + 2] + float(2 * ((j & 4) >> 2) - 1) * hw;
+        for (int k = 0; k < 27; k++) {
+            int32_t nb = s->neighs[27 * o + k];
+            if (nb == -1) continue;
+            const float* nc = &C[3 * int64_t(nb)];
+            if (!(sqdist(vp, nc) < W2)) continue;
+            int idx = 0;
+            if (nc[0] - vp[0] < 0) idx |= 1;
+            if (nc[2] - vp[2] < 0) idx |= 4;
+            if (nc[1] - vp[1] < 0) idx += (idx & 1) ? 1 : 3;
+            vertices[8 * int64_t(nb) + idx] = int32_t(v + 1);
+        }
+    }
+    // initEdgeArray + maintainEdgeNodePointer on the leaf level (main.cu:1795-1865,2021-2082)
+    auto edge_center = [&](int64_t i, int j, float hw, float ec[3]) {
+        int orient = j >> 2;
+        int off[2] = {j & 1, (j & 2) >> 1};
+        int dim = 0;
+        for (int a = 0; a < 3; a++) {
+            int m = 0;
+            if (a != orient) m = 2 * off[dim++] - 1;
+            ec[a] = C[3 * i + a] + float(m) * hw;
+        }
+    };
+    std::vector<uint8_t> eown(12 * MD, 0);
+    {
+        const float hw = 1.0f / float(1 << (D + 1));
+        const float W = 1.0f / float(1 << D);
+        const float W2 = W * W;
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t t = 0; t < MD; t++) {
+            const int64_t i = baseD + t;
+            for (int j = 0; j < 12; j++) {
+                float ec[3];
+                edge_center(i, j, hw, ec);
+                uint32_t best = 0x7fffffffu;
+                int64_t owner = -1;
+                for (int k = 0; k < 27; k++) {
+                    int32_t nb = s->neighs[27 * i + k];
+                    if (nb == -1 || !(sqdist(ec, &C[3 * int64_t(nb)]) < W2)) continue;
+                    if (best > s->key[nb]) {
+                        best = s->key[nb];
+                        owner = nb;
+                    }
+                }
+                eown[12 * t + j] = (owner == i);
+            }
+        }
+    }
+    std::vector<int64_t> enode;
+    std::vector<int> ekind;
+    for (int64_t e = 0; e < 12 * MD; e++)
+        if (eown[e]) {
+            enode.push_back(baseD + e / 12);
+            ekind.push_back(int(e % 12));
+        }
+    const int64_t E = int64_t(enode.size());
+    std::vector<int32_t> edges(12 * MD, 0);
+    {
+        const float hw = 1.0f / float(1 << (D + 1));
+        const float W = 1.0f / float(1 << D);
+        const float W2 = W * W;
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t e = 0; e < E; e++) {
+            const int64_t o = enode[e];
+            const int j = ekind[e];
+            const int orient = j >> 2;
+            float ec[3];
+            edge_center(o, j, hw, ec);
+            for (int k = 0; k < 27; k++) {
+                int32_t nb = s->neighs[27 * o + k];
+                if (nb == -1) continue;
+                const float* nc = &C[3 * int64_t(nb)];
+                if (!(sqdist(ec, nc) < W2)) continue;
+                int idx = orient << 2, dim = 0;
+                for (int a = 0; a < 3; a++)
+                    if (a != orient) {
+                        if (nc[a] - ec[a] < 0) idx |= (1 << dim);
+                        ++dim;
+                    }
+                edges[12 * (int64_t(nb) - baseD) + idx] = int32_t(e + 1);
+            }
+        }
+    }
+    // computeVertexImplicitFunctionValue (main.cu:2259-2326) for every vertex
+    std::vector<float> vval(V);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t v = 0; v < V; v++) {
+        const float* pos = &vpos[3 * v];
+        int depth = s->depth_of[vnode[v]];
+        const int exceed = kChildrenVertexKind[vkind[v]];
+        float val = 0.0f;
+        int64_t nd = vnode[v];
+        if (nd > 0) {
+            while (nd != -1) {
+                val = node_contrib(s, nd, pos, val);
+                nd = s->parent[nd];
+            }
+            nd = vnode[v];
+            while (depth < D) {
+                ++depth;
+                nd = s->children[8 * nd + exceed];
+                if (nd == -1) break;
+                val = node_contrib(s, nd, pos, val);
+            }
+        }
+        vval[v] = val - s->iso;
+    }
+    s->leaf_corner_vals.assign(8 * MD, 0.0f);
+    for (int64_t t = 0; t < MD; t++)
+        for (int j = 0; j < 8; j++) s->leaf_corner_vals[8 * t + j] = vval[vertices[8 * (baseD + t) + j] - 1];
+    // generateVexNums + exclusive_scan (main.cu:2457-2479,3670-3690)
+    std::vector<int32_t> vexAddr(E);
+    int64_t nv = 0;
+    for (int64_t e = 0; e < E; e++) {
+        const int64_t o = enode[e];
+        const int kind = ekind[e];
+        float f1 = vval[vertices[8 * o + kEdgeVertex[kind][0]] - 1];
+        float f2 = vval[vertices[8 * o + kEdgeVertex[kind][1]] - 1];
+        vexAddr[e] = int32_t(nv);
+        if (f1 * f2 <= 0) nv++;
+    }
+    // generateIntersectionPoint + interpolatePoint (main.cu:2584-2627)
+    s->mesh_v.assign(3 * nv, 0.0f);
+    for (int64_t e = 0; e < E; e++) {
+        const int64_t o = enode[e];
+        const int kind = ekind[e];
+        const int v1 = vertices[8 * o + kEdgeVertex[kind][0]] - 1;
+        const int v2 = vertices[8 * o + kEdgeVertex[kind][1]] - 1;
+        const float f1 = vval[v1], f2 = vval[v2];
+        if (!(f1 * f2 <= 0)) continue;
+        const int dim = kind >> 2;
+        float out[3];
+        for (int a = 0; a < 3; a++)
+            if (a != dim) out[a] = vpos[3 * v1 + a];
+        float pivot = f1 / (f1 - f2);
+        float another = 1 - pivot;
+        float t1 = vpos[3 * v2 + dim] * pivot;
+        float t2 = vpos[3 * v1 + dim] * another;
+        out[dim] = t1 + t2;
+        for (int a = 0; a < 3; a++) s->mesh_v[3 * int64_t(vexAddr[e]) + a] = out[a];
+    }
+    // generateTriNums + generateTrianglePos (main.cu:2540-2560,2699-2757)
+    s->mesh_t.clear();
+    for (int64_t t = 0; t < MD; t++) {
+        int cube = 0;
+        for (int j = 0; j < 8; j++)
+            if (s->leaf_corner_vals[8 * t + j] < 0) cube |= 1 << j;
+        const int nt = tri_count(cube);
+        for (int k = 0; k < 3 * nt; k++) {
+            const int ek = kTriTable[cube][k];
+            s->mesh_t.push_back(vexAddr[edges[12 * t + ek] - 1]);
+        }
+    }
+    *n_vertices = nv;
+    *n_triangles = int64_t(s->mesh_t.size()) / 3;
+    return 0;
+}
+
+int32_t orc_copy_mesh(orc_state* s, float* vertices, int32_t* triangles) {
+    std::memcpy(vertices, s->mesh_v.data(), s->mesh_v.size() * sizeof(float));
+    std::memcpy(triangles, s->mesh_t.data(), s->mesh_t.size() * sizeof(int32_t));
+    return 0;
+}
+
+// ---- buffers (ids of prg_buffer, include/prg.h) ------------------------------------
+int64_t orc_buffer_size(orc_state* s, int32_t which) {
+    const int64_t M = s->M, N = s->N, MD = s->MD();
+    switch (which) {
+        case 0:
+        case 1: return 12 * N;
+        case 2: return 4 * N;
+        case 3:
+        case 4:
+        case 5:
+        case 6:
+        case 9:
+        case 10: return 4 * M;
+        case 7: return 32 * M;
+        case 8: return 108 * M;
+        case 11: return 12 * MD;
+        case 12:
+        case 13: return 4 * M;
+        case 14: return 4 * N;
+        case 15: return 32 * MD;
+        default: return -1;
+    }
+}
+
+int32_t orc_read_buffer(orc_state* s, int32_t which, void* dst, int64_t bytes) {
+    const void* src = nullptr;
+    switch (which) {
+        case 0: src = s->pts.data(); break;
+        case 1: src = s->nrm.data(); break;
+        case 2: src = s->point_leaf.data(); break;
+        case 3: src = s->key.data(); break;
+        case 4: src = s->pidx.data(); break;
+        case 5: src = s->pnum.data(); break;
+        case 6: src = s->parent.data(); break;
+        case 7: src = s->children.data(); break;
+        case 8: src = s->neighs.data(); break;
+        case 9: src = s->didx.data(); break;
+        case 10: src = s->dnum.data(); break;
+        case 11: src = s->vfield.data(); break;
+        case 12: src = s->div.data(); break;
+        case 13: src = s->x.data(); break;
+        case 14: src = s->pvals.data(); break;
+        case 15: src = s->leaf_corner_vals.data(); break;
+        default: return 1;
+    }
+    if (bytes != orc_buffer_size(s, which)) return 1;
+    std::memcpy(dst, src, size_t(bytes));
+    return 0;
+}
+
+int32_t orc_write_buffer(orc_state* s, int32_t which, const void* src, int64_t bytes) {
+    if (bytes != orc_buffer_size(s, which)) return 1;
+    if (which == 12) {
+        std::memcpy(s->div.data(), src, size_t(bytes));
+        s->have_div = true;
+        return 0;
+    }
+    if (which == 13) {
+        std::memcpy(s->x.data(), src, size_t(bytes));
+        s->have_x = true;
+        return 0;
+    }
+    return 1;
+}
+
+// ---- function-space probes ------------------------------------------------------------
+double orc_fs_entry(int32_t max_depth, int32_t which, int32_t da, int64_t oa, int32_t db, int64_t ob) {
+    static std::map<int, std::unique_ptr<orc::FunctionSpace>> cache;
+    auto& p = cache[max_depth];
+    if (!p) p.reset(new orc::FunctionSpace(max_depth));
+    return p->entry(which, da, oa, db, ob);
+}
+
+void orc_fs_base_function(int32_t max_depth, int64_t idx, float* out20) {
+    orc::FunctionSpace fs(max_depth);
+    orc::CPoly c = orc::cpoly_from(fs.base_function(idx));
+    for (int i = 0; i < 4; i++) {
+        out20[5 * i] = c.start[i];
+        for (int j = 0; j < 4; j++) out20[5 * i + 1 + j] = c.c[i][j];
+    }
+}
+
+void orc_fs_base_maxdepth(int32_t max_depth, float* out16) {
+    orc::FunctionSpace fs(max_depth);
+    orc::CPoly c = orc::cpoly_from(fs.base_maxdepth());
+    for (int i = 0; i < 4; i++) {
+        out16[4 * i] = c.start[i];
+        for (int j = 0; j < 3; j++) out16[4 * i + 1 + j] = c.c[i][j];
+    }
+}
+
+float orc_fs_value(int32_t max_depth, int64_t idx, float x) {
+    orc::FunctionSpace fs(max_depth);
+    return orc::cpoly_value(orc::cpoly_from(fs.base_function(idx)), x);
+}
+
+void orc_fs_stencil(int32_t max_depth, int32_t depth, float* out27) {
+    orc_state tmp;
+    tmp.D = max_depth;
+    tmp.ensure_fs();
+    for (int j = 0; j < 27; j++) out27[j] = tmp.stencil[depth][j];
+}
+
+}  // extern "C"

@@ -1,0 +1,425 @@
+// C ABI of the library (include/prg.h): argument checks, state machine,
+// error capture and host <-> device buffer access.  Every entry point
+// catches prg::Error and returns its status; nothing aborts the process.
+#include <cstring>
+#include <new>
+
+#include "ctx.h"
+
+using prg::Error;
+
+namespace {
+
+prg_status fail(prg_ctx* c, prg_status st, const std::string& msg) {
+    if (c) c->err = msg;
+    return st;
+}
+
+template <class F>
+prg_status guarded(prg_ctx* c, F&& f) {
+    if (!c) return PRG_E_ARG;
+    try {
+        return f();
+    } catch (const Error& e) {
+        c->err = e.msg;
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        c->err = "host allocation failed";
+        return PRG_E_NOMEM;
+    } catch (const std::exception& e) {
+        c->err = e.what();
+        return PRG_E_HIP;
+    }
+}
+
+struct BufInfo {
+    int64_t bytes;
+};
+
+int64_t buffer_bytes(const prg_ctx* c, int which) {
+    const int64_t N = c->N, M = c->M, MD = c->MD();
+    switch (which) {
+        case PRG_BUF_POINTS:
+        case PRG_BUF_NORMALS: return c->have_points ? 12 * N : -1;
+        case PRG_BUF_POINT_LEAF: return c->have_tree ? 4 * N : -1;
+        case PRG_BUF_KEY:
+        case PRG_BUF_PIDX:
+        case PRG_BUF_PNUM:
+        case PRG_BUF_PARENT:
+        case PRG_BUF_DIDX:
+        case PRG_BUF_DNUM: return c->have_tree ? 4 * M : -1;
+        case PRG_BUF_CHILDREN: return c->have_tree ? 32 * M : -1;
+        case PRG_BUF_NEIGHS: return c->have_tree ? 108 * M : -1;
+        case PRG_BUF_VECTOR_FIELD: return c->have_div ? 12 * MD : -1;
+        case PRG_BUF_DIVERGENCE:
+        case PRG_BUF_SOLUTION: return c->have_tree ? 4 * M : -1;
+        case PRG_BUF_POINT_VALUES: return c->have_iso ? 4 * N : -1;
+        case PRG_BUF_LEAF_CORNER_VALUES: return (c->mesh_nv > 0 || c->mesh_nt > 0 || c->corner_vals.get()) ? 32 * MD : -1;
+        default: return -1;
+    }
+}
+
+void d2h(prg_ctx* c, void* dst, const void* src, size_t bytes) {
+    PRG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+}
+
+// Interleaves three device arrays of n floats into host AoS.
+void read_xyz(prg_ctx* c, const float* a, const float* b, const float* d, int64_t n, float* host) {
+    std::vector<float> t(static_cast<size_t>(3 * n));
+    d2h(c, t.data(), a, size_t(n) * 4);
+    d2h(c, t.data() + n, b, size_t(n) * 4);
+    d2h(c, t.data() + 2 * n, d, size_t(n) * 4);
+    PRG_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < n; i++) {
+        host[3 * i] = t[i];
+        host[3 * i + 1] = t[n + i];
+        host[3 * i + 2] = t[2 * n + i];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+prg_status prg_config_default(prg_config* cfg) {
+    if (!cfg) return PRG_E_ARG;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->max_depth = 8;
+    cfg->device = 0;
+    cfg->cg_tol = 1e-6f;
+    cfg->cg_max_iters = 2000;
+    cfg->cg_refresh = 20;
+    cfg->cg_fixed_iters = 0;
+    cfg->warm_start = PRG_WARM_NONE;
+    cfg->timing = 0;
+    return PRG_OK;
+}
+
+prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
+    if (!out) return PRG_E_ARG;
+    *out = nullptr;
+    prg_config c0;
+    prg_config_default(&c0);
+    const prg_config& cf = cfg ? *cfg : c0;
+    if (cf.max_depth < 2 || cf.max_depth > prg::kMaxDepth) return PRG_E_ARG;
+    if (!(cf.cg_tol >= 0.0f) || cf.cg_max_iters < 1) return PRG_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PRG_E_NODEVICE;
+    if (cf.device < 0 || cf.device >= ndev) return PRG_E_ARG;
+    prg_ctx* c = new (std::nothrow) prg_ctx();
+    if (!c) return PRG_E_NOMEM;
+    c->cfg = cf;
+    c->D = cf.max_depth;
+    prg_status st = guarded(c, [&]() {
+        PRG_HIP(hipSetDevice(cf.device));
+        PRG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        PRG_HIP(hipEventCreate(&c->ev_a));
+        PRG_HIP(hipEventCreate(&c->ev_b));
+        c->kt.enabled = cf.timing != 0;
+        c->kt.stream = c->stream;
+        return PRG_OK;
+    });
+    if (st != PRG_OK) {
+        prg_destroy(c);
+        return st;
+    }
+    *out = c;
+    return PRG_OK;
+}
+
+void prg_destroy(prg_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->cfg.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ev_a) (void)hipEventDestroy(c->ev_a);
+    if (c->ev_b) (void)hipEventDestroy(c->ev_b);
+    hipStream_t s = c->stream;
+    delete c;  // releases device buffers
+    if (s) (void)hipStreamDestroy(s);
+}
+
+const char* prg_last_error(const prg_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+prg_status prg_set_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n, int32_t on_device) {
+    return guarded(c, [&]() {
+        if (!xyz || !nxyz) return fail(c, PRG_E_ARG, "null point pointer");
+        if (n < 1 || n >= (int64_t(1) << 31)) return fail(c, PRG_E_ARG, "point count must be in [1, 2^31)");
+        PRG_HIP(hipSetDevice(c->cfg.device));
+        c->have_points = c->have_tree = c->have_div = c->have_x = c->have_iso = c->iso_ready = false;
+        c->mesh_nv = c->mesh_nt = 0;
+        c->stage_ms.fill(0.0);
+        prg::StageTimer st(c, PRG_STAGE_NORMALIZE);
+        prg::normalize_points(c, xyz, nxyz, n, on_device != 0);
+        c->have_points = true;
+        return PRG_OK;
+    });
+}
+
+prg_status prg_build_octree(prg_ctx* c) {
+    return guarded(c, [&]() {
+        if (!c->have_points) return fail(c, PRG_E_STATE, "prg_set_points first");
+        PRG_HIP(hipSetDevice(c->cfg.device));
+        {
+            prg::StageTimer st(c, PRG_STAGE_OCTREE);
+            prg::build_octree(c);
+        }
+        {
+            prg::StageTimer st(c, PRG_STAGE_TABLES);
+            prg::ensure_tables(c);
+        }
+        c->have_tree = true;
+        c->have_div = c->have_x = c->have_iso = false;
+        PRG_HIP(hipMemsetAsync(c->x.reserve(c->M), 0, size_t(c->M) * 4, c->stream));
+        PRG_HIP(hipMemsetAsync(c->div.reserve(c->M), 0, size_t(c->M) * 4, c->stream));
+        PRG_HIP(hipStreamSynchronize(c->stream));
+        return PRG_OK;
+    });
+}
+
+prg_status prg_level_info(prg_ctx* c, int64_t* counts, int64_t* bases) {
+    return guarded(c, [&]() {
+        if (!c->have_tree) return fail(c, PRG_E_STATE, "prg_build_octree first");
+        if (!counts || !bases) return fail(c, PRG_E_ARG, "null output");
+        for (int d = 0; d <= c->D; d++) {
+            counts[d] = c->counts[d];
+            bases[d] = c->bases[d];
+        }
+        return PRG_OK;
+    });
+}
+
+prg_status prg_level_view(prg_ctx* c, int32_t depth, prg_level_desc* v) {
+    return guarded(c, [&]() {
+        if (!c->have_tree) return fail(c, PRG_E_STATE, "prg_build_octree first");
+        if (!v || depth < 0 || depth > c->D) return fail(c, PRG_E_ARG, "bad depth");
+        const int64_t b = c->bases[depth];
+        v->depth = depth;
+        v->count = c->counts[depth];
+        v->base = b;
+        v->key = c->key.get() + b;
+        v->pidx = c->pidx.get() + b;
+        v->pnum = c->pnum.get() + b;
+        v->parent = c->parent.get() + b;
+        v->child_base = c->child_base.get() + b;
+        v->didx = c->didx.get() + b;
+        v->dnum = c->dnum.get() + b;
+        v->neighs = c->neigh.get() + b;
+        v->neigh_stride = c->M;
+        return PRG_OK;
+    });
+}
+
+prg_status prg_compute_divergence(prg_ctx* c) {
+    return guarded(c, [&]() {
+        if (!c->have_tree) return fail(c, PRG_E_STATE, "prg_build_octree first");
+        PRG_HIP(hipSetDevice(c->cfg.device));
+        prg::compute_divergence(c);
+        c->have_div = true;
+        c->have_x = c->have_iso = false;
+        return PRG_OK;
+    });
+}
+
+prg_status prg_solve(prg_ctx* c, prg_solve_stats* per_depth) {
+    return guarded(c, [&]() {
+        if (!c->have_div) return fail(c, PRG_E_STATE, "prg_compute_divergence first");
+        PRG_HIP(hipSetDevice(c->cfg.device));
+        prg_status st = prg::solve_all(c, per_depth);
+        c->have_x = true;
+        c->have_iso = c->iso_ready = false;
+        if (st == PRG_E_NOT_CONVERGED) c->err = "CG reached its iteration cap";
+        return st;
+    });
+}
+
+prg_status prg_isovalue(prg_ctx* c, float* out) {
+    return guarded(c, [&]() {
+        if (!c->have_x) return fail(c, PRG_E_STATE, "prg_solve first");
+        PRG_HIP(hipSetDevice(c->cfg.device));
+        prg::compute_isovalue(c);
+        c->have_iso = true;
+        c->iso_ready = true;
+        if (out) *out = c->iso;
+        return PRG_OK;
+    });
+}
+
+prg_status prg_set_isovalue(prg_ctx* c, float iso) {
+    return guarded(c, [&]() {
+        c->iso = iso;
+        c->iso_ready = true;
+        return PRG_OK;
+    });
+}
+
+prg_status prg_extract_mesh(prg_ctx* c, int32_t subdivide, prg_mesh* out) {
+    return guarded(c, [&]() {
+        if (!c->have_x) return fail(c, PRG_E_STATE, "prg_solve first");
+        if (!c->iso_ready) return fail(c, PRG_E_STATE, "prg_isovalue or prg_set_isovalue first");
+        if (!out) return fail(c, PRG_E_ARG, "null mesh");
+        if (subdivide) return fail(c, PRG_E_ARG, "coarse-leaf subdivision is not built yet");
+        PRG_HIP(hipSetDevice(c->cfg.device));
+        prg::extract_leaf_mesh(c);
+        std::memset(out, 0, sizeof(*out));
+        prg::StageTimer st(c, PRG_STAGE_MESH_COPY);
+        out->n_vertices = c->mesh_nv;
+        out->n_triangles = c->mesh_nt;
+        out->n_leaf_vertices = c->mesh_nv;
+        out->n_leaf_triangles = c->mesh_nt;
+        out->vertices = static_cast<float*>(std::malloc(size_t(3 * c->mesh_nv + 1) * 4));
+        out->triangles = static_cast<int32_t*>(std::malloc(size_t(3 * c->mesh_nt + 1) * 4));
+        if (!out->vertices || !out->triangles) {
+            prg_mesh_release(out);
+            return fail(c, PRG_E_NOMEM, "mesh host allocation");
+        }
+        if (c->mesh_nv) d2h(c, out->vertices, c->mesh_v.get(), size_t(3 * c->mesh_nv) * 4);
+        if (c->mesh_nt) d2h(c, out->triangles, c->mesh_t.get(), size_t(3 * c->mesh_nt) * 4);
+        PRG_HIP(hipStreamSynchronize(c->stream));
+        for (int a = 0; a < 3; a++) out->center[a] = c->center[a];
+        out->scale = c->scale;
+        return PRG_OK;
+    });
+}
+
+void prg_mesh_release(prg_mesh* m) {
+    if (!m) return;
+    std::free(m->vertices);
+    std::free(m->triangles);
+    m->vertices = nullptr;
+    m->triangles = nullptr;
+    m->n_vertices = m->n_triangles = 0;
+}
+
+prg_status prg_reconstruct(prg_ctx* c, int32_t subdivide, prg_mesh* out, prg_solve_stats* per_depth) {
+    prg_status st = prg_build_octree(c);
+    if (st != PRG_OK) return st;
+    if ((st = prg_compute_divergence(c)) != PRG_OK) return st;
+    st = prg_solve(c, per_depth);
+    if (st != PRG_OK && st != PRG_E_NOT_CONVERGED) return st;
+    prg_status st2 = prg_isovalue(c, nullptr);
+    if (st2 != PRG_OK) return st2;
+    st2 = prg_extract_mesh(c, subdivide, out);
+    if (st2 != PRG_OK) return st2;
+    return st;
+}
+
+prg_status prg_stage_times(prg_ctx* c, prg_timings* out) {
+    return guarded(c, [&]() {
+        if (!out) return fail(c, PRG_E_ARG, "null output");
+        for (int i = 0; i < PRG_STAGE_COUNT; i++) out->ms[i] = c->stage_ms[i];
+        return PRG_OK;
+    });
+}
+
+prg_status prg_buffer_size(prg_ctx* c, int32_t which, int64_t* bytes) {
+    return guarded(c, [&]() {
+        if (!bytes) return fail(c, PRG_E_ARG, "null output");
+        int64_t b = buffer_bytes(c, which);
+        if (b < 0) return fail(c, PRG_E_STATE, "buffer not available");
+        *bytes = b;
+        return PRG_OK;
+    });
+}
+
+prg_status prg_read_buffer(prg_ctx* c, int32_t which, void* host, int64_t bytes) {
+    return guarded(c, [&]() {
+        const int64_t b = buffer_bytes(c, which);
+        if (b < 0) return fail(c, PRG_E_STATE, "buffer not available");
+        if (!host || bytes != b) return fail(c, PRG_E_ARG, "size mismatch");
+        PRG_HIP(hipSetDevice(c->cfg.device));
+        const int64_t M = c->M;
+        switch (which) {
+            case PRG_BUF_POINTS: read_xyz(c, c->px.get(), c->py.get(), c->pz.get(), c->N, (float*)host); break;
+            case PRG_BUF_NORMALS: read_xyz(c, c->nx.get(), c->ny.get(), c->nz.get(), c->N, (float*)host); break;
+            case PRG_BUF_POINT_LEAF: d2h(c, host, c->point_leaf.get(), size_t(b)); break;
+            case PRG_BUF_KEY: d2h(c, host, c->key.get(), size_t(b)); break;
+            case PRG_BUF_PIDX: d2h(c, host, c->pidx.get(), size_t(b)); break;
+            case PRG_BUF_PNUM: d2h(c, host, c->pnum.get(), size_t(b)); break;
+            case PRG_BUF_PARENT: d2h(c, host, c->parent.get(), size_t(b)); break;
+            case PRG_BUF_DIDX: d2h(c, host, c->didx.get(), size_t(b)); break;
+            case PRG_BUF_DNUM: d2h(c, host, c->dnum.get(), size_t(b)); break;
+            case PRG_BUF_CHILDREN: {
+                std::vector<int32_t> cb(static_cast<size_t>(M));
+                d2h(c, cb.data(), c->child_base.get(), size_t(M) * 4);
+                PRG_HIP(hipStreamSynchronize(c->stream));
+                int32_t* o = (int32_t*)host;
+                for (int64_t i = 0; i < M; i++)
+                    for (int k = 0; k < 8; k++) o[8 * i + k] = cb[i] < 0 ? -1 : cb[i] + k;
+                break;
+            }
+            case PRG_BUF_NEIGHS: {
+                std::vector<int32_t> t(static_cast<size_t>(27 * M));
+                d2h(c, t.data(), c->neigh.get(), size_t(27 * M) * 4);
+                PRG_HIP(hipStreamSynchronize(c->stream));
+                int32_t* o = (int32_t*)host;
+                for (int64_t i = 0; i < M; i++)
+                    for (int j = 0; j < 27; j++) o[27 * i + j] = t[size_t(j) * M + i];
+                break;
+            }
+            case PRG_BUF_VECTOR_FIELD: read_xyz(c, c->vfx.get(), c->vfy.get(), c->vfz.get(), c->MD(), (float*)host); break;
+            case PRG_BUF_DIVERGENCE: d2h(c, host, c->div.get(), size_t(b)); break;
+            case PRG_BUF_SOLUTION: d2h(c, host, c->x.get(), size_t(b)); break;
+            case PRG_BUF_POINT_VALUES: d2h(c, host, c->pvals.get(), size_t(b)); break;
+            case PRG_BUF_LEAF_CORNER_VALUES: d2h(c, host, c->corner_vals.get(), size_t(b)); break;
+            default: return fail(c, PRG_E_ARG, "unknown buffer");
+        }
+        PRG_HIP(hipStreamSynchronize(c->stream));
+        return PRG_OK;
+    });
+}
+
+prg_status prg_write_buffer(prg_ctx* c, int32_t which, const void* host, int64_t bytes) {
+    return guarded(c, [&]() {
+        if (which != PRG_BUF_DIVERGENCE && which != PRG_BUF_SOLUTION)
+            return fail(c, PRG_E_ARG, "buffer is not writable");
+        const int64_t b = buffer_bytes(c, which);
+        if (b < 0) return fail(c, PRG_E_STATE, "prg_build_octree first");
+        if (!host || bytes != b) return fail(c, PRG_E_ARG, "size mismatch");
+        PRG_HIP(hipSetDevice(c->cfg.device));
+        float* dst = (which == PRG_BUF_DIVERGENCE) ? c->div.reserve(c->M) : c->x.reserve(c->M);
+        PRG_HIP(hipMemcpyAsync(dst, host, size_t(b), hipMemcpyHostToDevice, c->stream));
+        PRG_HIP(hipStreamSynchronize(c->stream));
+        if (which == PRG_BUF_DIVERGENCE) {
+            prg::ensure_tables(c);
+            c->have_div = true;
+        } else {
+            c->have_x = true;
+        }
+        return PRG_OK;
+    });
+}
+
+prg_status prg_kernel_stats(prg_ctx* c, const char* name, int64_t* launches, double* total_ms,
+                            double* bytes_per_launch) {
+    return guarded(c, [&]() {
+        if (!name) return fail(c, PRG_E_ARG, "null name");
+        c->kt.flush();
+        auto it = c->kt.recs.find(name);
+        if (it == c->kt.recs.end()) return fail(c, PRG_E_ARG, std::string("no launches of ") + name);
+        if (launches) *launches = it->second.launches;
+        if (total_ms) *total_ms = it->second.total_ms;
+        if (bytes_per_launch)
+            *bytes_per_launch = it->second.launches ? it->second.bytes / double(it->second.launches) : 0.0;
+        return PRG_OK;
+    });
+}
+
+prg_status prg_kernel_stats_reset(prg_ctx* c) {
+    return guarded(c, [&]() {
+        c->kt.flush();
+        c->kt.recs.clear();
+        return PRG_OK;
+    });
+}
+
+prg_status prg_get_transform(prg_ctx* c, float* center3, float* scale) {
+    return guarded(c, [&]() {
+        if (!c->have_points) return fail(c, PRG_E_STATE, "prg_set_points first");
+        if (center3)
+            for (int a = 0; a < 3; a++) center3[a] = c->center[a];
+        if (scale) *scale = c->scale;
+        return PRG_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// Internal context of one reconstruction (one HIP device, one stream).
+// Device layout (all structure-of-arrays, node arrays indexed by the global
+// node index; depth d occupies [base[d], base[d] + count[d]) as in the
+// reference's concatenated NodeArray, main.cu:772-794):
+//   points     px,py,pz,nx,ny,nz      float[N]   normalised, then sorted
+//   nodes      key                    uint32[M]
+//              pidx,pnum,parent,child_base,didx,dnum   int32[M]
+//              neigh                  int32[27][M]  (plane j holds neighbour j of every node)
+//   fields     vfield_{x,y,z}         float[M_D]
+//              div, x                 float[M]
+#pragma once
+
+#include <array>
+#include <memory>
+
+#include "common.h"
+
+struct prg_ctx {
+    prg_config cfg{};
+    int D = 8;
+    hipStream_t stream = nullptr;
+    std::string err;
+    prg::KernelTimer kt;
+    prg::Scratch scratch;
+
+    // points
+    int64_t N = 0;
+    float center[3] = {0, 0, 0};
+    float scale = 1.0f;
+    prg::DBuf<float> px, py, pz, nx, ny, nz;       // normalised input, sorted in place by the octree build
+    prg::DBuf<float> tx, ty, tz, tnx, tny, tnz;    // sort scratch
+    prg::DBuf<int32_t> point_leaf;                 // local leaf-level slot of each sorted point
+
+    // octree
+    bool have_points = false, have_tree = false, have_div = false, have_x = false, have_iso = false;
+    std::vector<int64_t> counts, bases;
+    int64_t M = 0;
+    prg::DBuf<uint32_t> key;
+    prg::DBuf<int32_t> pidx, pnum, parent, child_base, didx, dnum, neigh;
+
+    // function space
+    std::unique_ptr<prg::FsTables> fs;
+    prg::DBuf<float> d_base_fn, d_stencil, d_div_table;
+
+    // fields
+    prg::DBuf<float> vfx, vfy, vfz, div, x, pvals;
+    prg::DBuf<double> dscalars;  // reductions and CG scalars
+    float iso = 0.0f;
+    bool iso_ready = false;  // iso computed or set by the caller
+
+    // leaf-level marching cubes results (device)
+    prg::DBuf<float> corner_vals;  // 8 * M_D, square corner numbering
+    prg::DBuf<float> mesh_v;
+    prg::DBuf<int32_t> mesh_t;
+    int64_t mesh_nv = 0, mesh_nt = 0;
+
+    // stage timing
+    std::array<double, PRG_STAGE_COUNT> stage_ms{};
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+
+    int64_t MD() const { return counts.empty() ? 0 : counts[D]; }
+    int64_t base_of(int d) const { return bases[d]; }
+};
+
+namespace prg {
+
+// Stage entry points implemented in the .hip files.
+void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n, bool on_device);
+void build_octree(prg_ctx* c);
+void ensure_tables(prg_ctx* c);
+void compute_divergence(prg_ctx* c);
+prg_status solve_all(prg_ctx* c, prg_solve_stats* stats);
+void compute_isovalue(prg_ctx* c);
+void extract_leaf_mesh(prg_ctx* c);
+
+// Measures one stage with hipEvents on the context stream.
+struct StageTimer {
+    prg_ctx* c;
+    int stage;
+    StageTimer(prg_ctx* ctx, int st) : c(ctx), stage(st) { PRG_HIP(hipEventRecord(c->ev_a, c->stream)); }
+    ~StageTimer() noexcept(false) {
+        PRG_HIP(hipEventRecord(c->ev_b, c->stream));
+        PRG_HIP(hipEventSynchronize(c->ev_b));
+        float ms = 0.0f;
+        PRG_HIP(hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
+        c->stage_ms[stage] += ms;
+    }
+};
+
+}  // namespace prg

@@ -1,0 +1,402 @@
+// Isovalue and leaf-level marching cubes (reference:
+// calculatePointsImplicitFunctionValue main.cu:1334-1381, the MC topology
+// kernels :1438-2082, computeVertexImplicitFunctionValue :2259-2326,
+// generateVexNums/TriNums/IntersectionPoint/TrianglePos :2457-2757, driver
+// :3502-3795).
+//
+// The implicit function at a position is the float sum, over the node's
+// ancestor chain (leaf first) and their 27 neighbours (j = 0..26), of
+// x_j * B_x(p_x) * B_y(p_y) * B_z(p_z) with the reference's per-index float
+// basis polynomials.
+//
+// Topology uses the reference's ownership rule without its pre-arrays of
+// 56-byte vertex records: a corner (edge) of the leaf level belongs to the
+// node with the smallest key among the nodes sharing it; corner values are
+// evaluated once, by the owner; mesh vertices are the owned edges with a sign
+// change, numbered by an exclusive scan over (owner node, edge kind) - the
+// order of the reference's EdgeArray - so triangle indices come out
+// identical.
+#include <cmath>
+
+#include "ctx.h"
+
+namespace prg {
+namespace {
+
+constexpr int kBlock = 256;
+
+#include "mc_triangle_table.inc"
+__constant__ int8_t c_tri[256][16] = {PRG_MC_TRIANGLE_TABLE_DATA};
+// edgeVertex (MarchingCubes.cuh:693-706): square-numbered corners of each edge kind
+__constant__ int8_t c_edge_vertex[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 3}, {1, 2},
+                                            {4, 7}, {5, 6}, {0, 4}, {1, 5}, {3, 7}, {2, 6}};
+
+__device__ __forceinline__ uint32_t compact3(uint32_t x) {
+    x &= 0x09249249u;
+    x = (x ^ (x >> 2)) & 0x030c30c3u;
+    x = (x ^ (x >> 4)) & 0x0300f00fu;
+    x = (x ^ (x >> 8)) & 0xff0000ffu;
+    x = (x ^ (x >> 16)) & 0x000003ffu;
+    return x;
+}
+
+// value() of baseFunctions[idx] (ConfirmedPPolynomial<3,4>, ConfirmedPPolynomial.cuh:78-91)
+__device__ __forceinline__ float basis_value(const float* __restrict__ bf, int64_t idx, float x) {
+    const float* f = bf + idx * 20;
+    float res = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (!(x > f[5 * i])) break;
+        float temp = 1.0f, v = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            v = v + temp * f[5 * i + 1 + j];
+            temp = temp * x;
+        }
+        res = res + v;
+    }
+    return res;
+}
+
+// f(pos) restricted to the ancestor chain of node `nd` at depth `d`.
+__device__ float implicit_value(int64_t nd, int d, float px, float py, float pz, const uint32_t* __restrict__ key,
+                                const int32_t* __restrict__ neigh, const int32_t* __restrict__ parent,
+                                const float* __restrict__ x, const float* __restrict__ bf, int64_t M, int D) {
+    float val = 0.0f;
+    while (nd != -1) {
+        const uint32_t k = key[nd];
+        const int64_t fb = (int64_t(1) << d) - 1;
+        const int64_t ox = fb + (compact3(k >> 2) >> (D - d)), oy = fb + (compact3(k >> 1) >> (D - d)),
+                      oz = fb + (compact3(k) >> (D - d));
+        for (int j = 0; j < 27; j++) {
+            const int32_t nb = neigh[int64_t(j) * M + nd];
+            if (nb == -1) continue;
+            const float vx = basis_value(bf, ox + (j / 9 - 1), px);
+            const float vy = basis_value(bf, oy + ((j / 3) % 3 - 1), py);
+            const float vz = basis_value(bf, oz + (j % 3 - 1), pz);
+            float t = x[nb] * vx;
+            t = t * vy;
+            t = t * vz;
+            val = val + t;
+        }
+        nd = parent[nd];
+        d--;
+    }
+    return val;
+}
+
+__global__ void __launch_bounds__(kBlock) k_point_values(const float* px, const float* py, const float* pz,
+                                                         const int32_t* point_leaf, int64_t n, int64_t baseD,
+                                                         const uint32_t* key, const int32_t* neigh,
+                                                         const int32_t* parent, const float* x, const float* bf,
+                                                         int64_t M, int D, float* pvals, double* partial) {
+    __shared__ double lds[kBlock / 64];
+    const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    double v = 0.0;
+    if (r < n) {
+        const float f = implicit_value(baseD + point_leaf[r], D, px[r], py[r], pz[r], key, neigh, parent, x, bf, M, D);
+        pvals[r] = f;
+        v = double(f);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; w++) t += lds[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+__global__ void k_sum_partials(const double* partial, int64_t n, double* out) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int64_t i = 0; i < n; i++) s += partial[i];
+    *out = s;
+}
+
+// node centre (getNodeCenterAllDepth, main.cu:1403-1420) at the leaf level
+__device__ __forceinline__ void leaf_center(uint32_t k, int D, float c[3]) {
+    c[0] = c[1] = c[2] = 0.5f;
+    float w = 0.25f;
+    for (int i = D - 1; i >= 0; --i) {
+        c[0] = ((k >> (3 * i + 2)) & 1u) ? c[0] + w : c[0] - w;
+        c[1] = ((k >> (3 * i + 1)) & 1u) ? c[1] + w : c[1] - w;
+        c[2] = ((k >> (3 * i)) & 1u) ? c[2] + w : c[2] - w;
+        w = w / 2;
+    }
+}
+
+// corner owners (initVertexOwner, main.cu:1438-1498): corner j = x | y<<1 | z<<2
+__global__ void __launch_bounds__(kBlock) k_corner_owner(const uint32_t* key, const int32_t* neigh, int64_t M,
+                                                         int64_t baseD, int64_t MD, int32_t* own, int32_t* oref) {
+    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (g >= 8 * MD) return;
+    const int64_t t = g >> 3;
+    const int j = int(g & 7);
+    const int64_t i = baseD + t;
+    const int cx = j & 1, cy = (j >> 1) & 1, cz = (j >> 2) & 1;
+    uint32_t best = 0xffffffffu;
+    int64_t owner = i;
+    int jo = j;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int e = 0; e < 2; e++) {
+                const int dx = a ? (2 * cx - 1) : 0, dy = b ? (2 * cy - 1) : 0, dz = e ? (2 * cz - 1) : 0;
+                const int32_t nb = neigh[int64_t(9 * (dx + 1) + 3 * (dy + 1) + (dz + 1)) * M + i];
+                if (nb == -1) continue;
+                const uint32_t kk = key[nb];
+                if (kk < best) {
+                    best = kk;
+                    owner = nb;
+                    jo = (cx ^ a) | ((cy ^ b) << 1) | ((cz ^ e) << 2);
+                }
+            }
+    own[g] = (owner == i) ? 1 : 0;
+    oref[g] = int32_t(8 * (owner - baseD) + jo);
+}
+
+__global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* own, const int32_t* vid, const uint32_t* key,
+                                                         const int32_t* neigh, const int32_t* parent, const float* x,
+                                                         const float* bf, int64_t M, int64_t baseD, int64_t MD, int D,
+                                                         float iso, float* vval) {
+    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (g >= 8 * MD || !own[g]) return;
+    const int64_t t = g >> 3;
+    const int j = int(g & 7);
+    const int64_t i = baseD + t;
+    float c[3];
+    leaf_center(key[i], D, c);
+    const float hw = 1.0f / float(1 << (D + 1));
+    // vertexPos (main.cu:1463-1468)
+    const float vx = c[0] + float(2 * (j & 1) - 1) * hw;
+    const float vy = c[1] + float(2 * ((j & 2) >> 1) - 1) * hw;
+    const float vz = c[2] + float(2 * ((j & 4) >> 2) - 1) * hw;
+    const float f = implicit_value(i, D, vx, vy, vz, key, neigh, parent, x, bf, M, D);
+    vval[vid[g]] = f - iso;
+}
+
+// values of each leaf node's corners in square numbering (vertices[], main.cu:1659-1668)
+__global__ void __launch_bounds__(kBlock) k_corner_square(const int32_t* oref, const int32_t* vid, const float* vval,
+                                                          int64_t MD, float* cval) {
+    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (g >= 8 * MD) return;
+    const int64_t t = g >> 3;
+    const int q = int(g & 7);
+    const int y = (q >> 1) & 1, x = (q & 1) ^ y, z = q >> 2;
+    const int j = x | (y << 1) | (z << 2);
+    cval[g] = vval[vid[oref[8 * t + j]]];
+}
+
+// edge owners (initEdgeArray / maintainEdgeNodePointer, main.cu:1795-2082) and
+// the sign-change test of the owner's corners (generateVexNums, :2457-2479)
+__global__ void __launch_bounds__(kBlock) k_edge_owner(const uint32_t* key, const int32_t* neigh, const float* cval,
+                                                       int64_t M, int64_t baseD, int64_t MD, int32_t* flag,
+                                                       int32_t* eref) {
+    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (g >= 12 * MD) return;
+    const int64_t t = g / 12;
+    const int e = int(g % 12);
+    const int64_t i = baseD + t;
+    const int o = e >> 2;
+    const int a1 = (o == 0) ? 1 : 0, a2 = (o == 2) ? 1 : 2;
+    const int b1 = e & 1, b2 = (e >> 1) & 1;
+    uint32_t best = 0xffffffffu;
+    int64_t owner = i;
+    int eo = e;
+    for (int u = 0; u < 2; u++)
+        for (int v = 0; v < 2; v++) {
+            int dd[3] = {0, 0, 0};
+            dd[a1] = u ? (2 * b1 - 1) : 0;
+            dd[a2] = v ? (2 * b2 - 1) : 0;
+            const int32_t nb = neigh[int64_t(9 * (dd[0] + 1) + 3 * (dd[1] + 1) + (dd[2] + 1)) * M + i];
+            if (nb == -1) continue;
+            const uint32_t kk = key[nb];
+            if (kk < best) {
+                best = kk;
+                owner = nb;
+                eo = (o << 2) | ((b2 ^ v) << 1) | (b1 ^ u);
+            }
+        }
+    eref[g] = int32_t(12 * (owner - baseD) + eo);
+    int f = 0;
+    if (owner == i) {
+        const float f1 = cval[8 * t + c_edge_vertex[e][0]];
+        const float f2 = cval[8 * t + c_edge_vertex[e][1]];
+        f = (f1 * f2 <= 0) ? 1 : 0;
+    }
+    flag[g] = f;
+}
+
+// interpolatePoint (main.cu:2584-2597) for every owned edge with a sign change
+__global__ void __launch_bounds__(kBlock) k_edge_vertex(const int32_t* flag, const int32_t* evid, const uint32_t* key,
+                                                        const float* cval, int64_t baseD, int64_t MD, int D,
+                                                        float* mesh_v) {
+    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (g >= 12 * MD || !flag[g]) return;
+    const int64_t t = g / 12;
+    const int e = int(g % 12);
+    float c[3];
+    leaf_center(key[baseD + t], D, c);
+    const float hw = 1.0f / float(1 << (D + 1));
+    const int q1 = c_edge_vertex[e][0], q2 = c_edge_vertex[e][1];
+    float p1[3], p2[3];
+    {
+        const int y = (q1 >> 1) & 1, x = (q1 & 1) ^ y, z = q1 >> 2;
+        p1[0] = c[0] + float(2 * x - 1) * hw;
+        p1[1] = c[1] + float(2 * y - 1) * hw;
+        p1[2] = c[2] + float(2 * z - 1) * hw;
+    }
+    {
+        const int y = (q2 >> 1) & 1, x = (q2 & 1) ^ y, z = q2 >> 2;
+        p2[0] = c[0] + float(2 * x - 1) * hw;
+        p2[1] = c[1] + float(2 * y - 1) * hw;
+        p2[2] = c[2] + float(2 * z - 1) * hw;
+    }
+    const float f1 = cval[8 * t + q1], f2 = cval[8 * t + q2];
+    const int dim = e >> 2;
+    float out[3] = {p1[0], p1[1], p1[2]};
+    const float pivot = f1 / (f1 - f2);
+    const float another = 1 - pivot;
+    const float s1 = p2[dim] * pivot;
+    const float s2 = p1[dim] * another;
+    out[dim] = s1 + s2;
+    const int64_t v = evid[g];
+    mesh_v[3 * v + 0] = out[0];
+    mesh_v[3 * v + 1] = out[1];
+    mesh_v[3 * v + 2] = out[2];
+}
+
+__device__ __forceinline__ int cube_of(const float* cval, int64_t t) {
+    int cube = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        if (cval[8 * t + q] < 0) cube |= 1 << q;
+    return cube;
+}
+
+__device__ __forceinline__ int tri_count(int cube) {
+    int n = 0;
+    while (n < 15 && c_tri[cube][n] >= 0) n++;
+    return n / 3;
+}
+
+__global__ void __launch_bounds__(kBlock) k_tri_count(const float* cval, int64_t MD, int32_t* ntri) {
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= MD) return;
+    ntri[t] = tri_count(cube_of(cval, t));
+}
+
+__global__ void __launch_bounds__(kBlock) k_tri_emit(const float* cval, const int32_t* toff, const int32_t* eref,
+                                                     const int32_t* evid, int64_t MD, int32_t* mesh_t) {
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= MD) return;
+    const int cube = cube_of(cval, t);
+    const int n = 3 * tri_count(cube);
+    const int64_t o = 3 * int64_t(toff[t]);
+    for (int k = 0; k < n; k++) mesh_t[o + k] = evid[eref[12 * t + c_tri[cube][k]]];
+}
+
+}  // namespace
+
+void compute_isovalue(prg_ctx* c) {
+    ensure_tables(c);
+    StageTimer st(c, PRG_STAGE_ISOVALUE);
+    hipStream_t s = c->stream;
+    const int64_t n = c->N;
+    const unsigned nb = grid_for(n, kBlock);
+    DBuf<double> part, tot;
+    part.reserve(nb);
+    tot.reserve(1);
+    {
+        TimedScope ts(c->kt, "point_values", 24.0 * double(n));
+        k_point_values<<<nb, kBlock, 0, s>>>(c->px.get(), c->py.get(), c->pz.get(), c->point_leaf.get(), n,
+                                             c->bases[c->D], c->key.get(), c->neigh.get(), c->parent.get(),
+                                             c->x.get(), c->d_base_fn.get(), c->M, c->D, c->pvals.reserve(n),
+                                             part.get());
+        PRG_CHECK_LAUNCH();
+    }
+    k_sum_partials<<<1, 64, 0, s>>>(part.get(), nb, tot.get());
+    PRG_CHECK_LAUNCH();
+    double sum = 0.0;
+    PRG_HIP(hipMemcpyAsync(&sum, tot.get(), 8, hipMemcpyDeviceToHost, s));
+    PRG_HIP(hipStreamSynchronize(s));
+    // isoValue = sum / count (main.cu:3494-3496)
+    c->iso = float(sum) / float(n);
+}
+
+void extract_leaf_mesh(prg_ctx* c) {
+    ensure_tables(c);
+    hipStream_t s = c->stream;
+    const int D = c->D;
+    const int64_t M = c->M, MD = c->MD(), baseD = c->bases[D];
+    KernelTimer& kt = c->kt;
+    DBuf<int32_t> own, oref, vid, vtot, eflag, eref, evid, etot, ntri, toff, ttot;
+    DBuf<float> vval;
+    float* CV = c->corner_vals.reserve(8 * MD);
+    int32_t V = 0, NV = 0, NT = 0;
+    {
+        StageTimer st(c, PRG_STAGE_MC_TOPOLOGY);
+        own.reserve(8 * MD);
+        oref.reserve(8 * MD);
+        vid.reserve(8 * MD);
+        vtot.reserve(1);
+        {
+            TimedScope ts(kt, "corner_owner", 8.0 * 64.0 * double(MD));
+            k_corner_owner<<<grid_for(8 * MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->neigh.get(), M, baseD, MD,
+                                                                       own.get(), oref.get());
+            PRG_CHECK_LAUNCH();
+        }
+        scan_exclusive(own.get(), vid.get(), 8 * MD, vtot.get(), s, c->scratch, kt);
+        PRG_HIP(hipMemcpyAsync(&V, vtot.get(), 4, hipMemcpyDeviceToHost, s));
+        PRG_HIP(hipStreamSynchronize(s));
+    }
+    {
+        StageTimer st(c, PRG_STAGE_MC_EVAL);
+        vval.reserve(V);
+        {
+            TimedScope ts(kt, "corner_value", 24.0 * double(V));
+            k_corner_value<<<grid_for(8 * MD, kBlock), kBlock, 0, s>>>(own.get(), vid.get(), c->key.get(),
+                                                                       c->neigh.get(), c->parent.get(), c->x.get(),
+                                                                       c->d_base_fn.get(), M, baseD, MD, D, c->iso,
+                                                                       vval.get());
+            PRG_CHECK_LAUNCH();
+        }
+        k_corner_square<<<grid_for(8 * MD, kBlock), kBlock, 0, s>>>(oref.get(), vid.get(), vval.get(), MD, CV);
+        PRG_CHECK_LAUNCH();
+        eflag.reserve(12 * MD);
+        eref.reserve(12 * MD);
+        evid.reserve(12 * MD);
+        etot.reserve(1);
+        {
+            TimedScope ts(kt, "edge_owner", 12.0 * 40.0 * double(MD));
+            k_edge_owner<<<grid_for(12 * MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->neigh.get(), CV, M, baseD,
+                                                                      MD, eflag.get(), eref.get());
+            PRG_CHECK_LAUNCH();
+        }
+        scan_exclusive(eflag.get(), evid.get(), 12 * MD, etot.get(), s, c->scratch, kt);
+        ntri.reserve(MD);
+        toff.reserve(MD);
+        ttot.reserve(1);
+        k_tri_count<<<grid_for(MD, kBlock), kBlock, 0, s>>>(CV, MD, ntri.get());
+        PRG_CHECK_LAUNCH();
+        scan_exclusive(ntri.get(), toff.get(), MD, ttot.get(), s, c->scratch, kt);
+        PRG_HIP(hipMemcpyAsync(&NV, etot.get(), 4, hipMemcpyDeviceToHost, s));
+        PRG_HIP(hipMemcpyAsync(&NT, ttot.get(), 4, hipMemcpyDeviceToHost, s));
+        PRG_HIP(hipStreamSynchronize(s));
+        float* MV = c->mesh_v.reserve(size_t(3) * NV);
+        int32_t* MT = c->mesh_t.reserve(size_t(3) * NT);
+        k_edge_vertex<<<grid_for(12 * MD, kBlock), kBlock, 0, s>>>(eflag.get(), evid.get(), c->key.get(), CV, baseD,
+                                                                   MD, D, MV);
+        PRG_CHECK_LAUNCH();
+        k_tri_emit<<<grid_for(MD, kBlock), kBlock, 0, s>>>(CV, toff.get(), eref.get(), evid.get(), MD, MT);
+        PRG_CHECK_LAUNCH();
+        PRG_HIP(hipStreamSynchronize(s));
+    }
+    c->mesh_nv = NV;
+    c->mesh_nt = NT;
+}
+
+}  // namespace prg

@@ -1,0 +1,528 @@
+// Octree construction on the GPU (reference: pipelineBuildNodeArray,
+// main.cu:511-841).
+//
+// The reference builds each coarser level with two 8M-slot hash tables and
+// global atomics per depth.  Here every level comes from sorted keys alone:
+//   1. Morton keys of the normalised points (encodePoint, main.cu:132-164) and
+//      a stable radix sort of (key, index) - the order of thrust::sort_by_key
+//      on key<<32|index (main.cu:596-603);
+//   2. run heads + scan give the unique leaves (pidx = run start, pnum = run length);
+//   3. bottom-up, the non-empty nodes of depth d are the blocks of depth d+1:
+//      parent-boundary flags + scan give each non-empty node its block, so its
+//      slot in the sibling-complete level array is 8*block + child slot; the
+//      aggregates (pnum, dnum) are differences of consecutive run starts
+//      because points and leaf-level slots are contiguous per subtree;
+//   4. a per-slot pass fills empty slots (key, running pidx/didx, parent) as
+//      updateEmptyNodeInfo (main.cu:433-490) intends;
+//   5. neighbours top-down with the parent/child lookup of computeNeighbor
+//      (main.cu:492-509).
+#include <algorithm>
+#include <cmath>
+
+#include "ctx.h"
+
+namespace prg {
+namespace {
+
+constexpr int kBlock = 256;
+
+// LUTparent / LUTchild (main.cu:80-99), derived in build order: child slot s,
+// neighbour j -> parent neighbour slot and child slot inside it.
+struct NeighLut {
+    int8_t parent[8][27];
+    int8_t child[8][27];
+};
+__constant__ NeighLut c_lut;
+
+NeighLut make_lut() {
+    NeighLut l;
+    for (int s = 0; s < 8; s++)
+        for (int j = 0; j < 27; j++) {
+            int cc[3] = {(s >> 2) & 1, (s >> 1) & 1, s & 1};
+            int dd[3] = {j / 9 - 1, (j / 3) % 3 - 1, j % 3 - 1};
+            int pj = 0, cs = 0;
+            for (int a = 0; a < 3; a++) {
+                int v = cc[a] + dd[a];
+                int po = v < 0 ? -1 : (v > 1 ? 1 : 0);
+                pj = pj * 3 + (po + 1);
+                cs = (cs << 1) | (v - 2 * po);
+            }
+            l.parent[s][j] = int8_t(pj);
+            l.child[s][j] = int8_t(cs);
+        }
+    return l;
+}
+
+// ---- normalisation (main.cu:530-571) -------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_bbox_partial(const float* xyz, int64_t n, float* part) {
+    __shared__ float smin[3][kBlock], smax[3][kBlock];
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += int64_t(gridDim.x) * kBlock)
+        for (int a = 0; a < 3; a++) {
+            float v = xyz[3 * i + a];
+            mn[a] = fminf(mn[a], v);
+            mx[a] = fmaxf(mx[a], v);
+        }
+    for (int a = 0; a < 3; a++) {
+        smin[a][threadIdx.x] = mn[a];
+        smax[a][threadIdx.x] = mx[a];
+    }
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int a = 0; a < 3; a++) {
+                smin[a][threadIdx.x] = fminf(smin[a][threadIdx.x], smin[a][threadIdx.x + o]);
+                smax[a][threadIdx.x] = fmaxf(smax[a][threadIdx.x], smax[a][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 3; a++) {
+            part[6 * blockIdx.x + a] = smin[a][0];
+            part[6 * blockIdx.x + 3 + a] = smax[a][0];
+        }
+}
+
+__global__ void __launch_bounds__(kBlock) k_normalize(const float* xyz, const float* nxyz, int64_t n, float cx,
+                                                      float cy, float cz, float scale, float nscale, float* px,
+                                                      float* py, float* pz, float* nx, float* ny, float* nz) {
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    px[i] = (xyz[3 * i + 0] - cx) / scale;
+    py[i] = (xyz[3 * i + 1] - cy) / scale;
+    pz[i] = (xyz[3 * i + 2] - cz) / scale;
+    const float a = nxyz[3 * i + 0], b = nxyz[3 * i + 1], c = nxyz[3 * i + 2];
+    float sq = a * a;
+    sq = sq + b * b;
+    sq = sq + c * c;
+    float len = float(sqrt(double(sq)));  // Length(): double sqrt of a float sum (Geometry.inl:29-33)
+    if (len > 1e-6f) len = 1.0f / len;
+    len = len * nscale;
+    nx[i] = a * len;
+    ny[i] = b * len;
+    nz[i] = c * len;
+}
+
+// ---- keys and sort ----------------------------------------------------------------
+__device__ inline uint32_t encode_point(float x, float y, float z, int D) {
+    uint32_t key = 0;
+    float c0 = 0.5f, c1 = 0.5f, c2 = 0.5f, w = 0.25f;
+    for (int i = D - 1; i >= 0; --i) {
+        if (x > c0) {
+            key |= 1u << (3 * i + 2);
+            c0 += w;
+        } else {
+            c0 -= w;
+        }
+        if (y > c1) {
+            key |= 1u << (3 * i + 1);
+            c1 += w;
+        } else {
+            c1 -= w;
+        }
+        if (z > c2) {
+            key |= 1u << (3 * i);
+            c2 += w;
+        } else {
+            c2 -= w;
+        }
+        w *= 0.5f;
+    }
+    return key;
+}
+
+__global__ void __launch_bounds__(kBlock) k_morton(const float* px, const float* py, const float* pz, int64_t n,
+                                                   int D, uint32_t* key, uint32_t* val) {
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    key[i] = encode_point(px[i], py[i], pz[i], D);
+    val[i] = uint32_t(i);
+}
+
+__global__ void __launch_bounds__(kBlock) k_gather6(const uint32_t* perm, int64_t n, const float* a0,
+                                                    const float* a1, const float* a2, const float* a3,
+                                                    const float* a4, const float* a5, float* b0, float* b1,
+                                                    float* b2, float* b3, float* b4, float* b5) {
+    const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t i = perm[r];
+    b0[r] = a0[i];
+    b1[r] = a1[i];
+    b2[r] = a2[i];
+    b3[r] = a3[i];
+    b4[r] = a4[i];
+    b5[r] = a5[i];
+}
+
+// run heads of the sorted keys (generateMark, main.cu:195-211)
+__global__ void __launch_bounds__(kBlock) k_heads(const uint32_t* key, int64_t n, int shift, int32_t* flag) {
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || (key[i] >> shift) != (key[i - 1] >> shift)) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_unique(const uint32_t* key, const int32_t* flag, const int32_t* rank,
+                                                   int64_t n, uint32_t* uk, int32_t* upidx) {
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    uk[rank[i]] = key[i];
+    upidx[rank[i]] = int32_t(i);
+}
+
+// exclusive head count -> index of the run containing each element
+__global__ void __launch_bounds__(kBlock) k_excl_to_incl(int32_t* out, const int32_t* excl, const int32_t* flag,
+                                                         int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i < n) out[i] = excl[i] + flag[i] - 1;
+}
+
+// Non-empty nodes of depth d (compact, NE_d entries, key order) -> their slot
+// in the depth-d level array and the compact arrays of depth d-1.
+__global__ void __launch_bounds__(kBlock) k_level_up(const uint32_t* nk, const int32_t* np, const int32_t* ndi,
+                                                     const int32_t* flag, const int32_t* brank, int64_t ne, int D,
+                                                     int d, int32_t* pos, uint32_t* pk, int32_t* pp, int32_t* pdi) {
+    const int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (q >= ne) return;
+    const int sh = 3 * (D - d);
+    const int32_t b = brank[q];
+    pos[q] = 8 * b + int32_t((nk[q] >> sh) & 7u);
+    if (flag[q]) {
+        pk[b] = nk[q] & ~(7u << sh);
+        pp[b] = np[q];
+        pdi[b] = (d == D) ? 8 * b : ndi[q];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_level_init(int64_t base, int64_t count, bool leaf, int32_t* pnum,
+                                                       int32_t* dnum, int32_t* child_base) {
+    const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (s >= count) return;
+    pnum[base + s] = 0;
+    dnum[base + s] = leaf ? 1 : 0;
+    child_base[base + s] = -1;
+}
+
+// Scatter the non-empty nodes of depth d into their slots.
+__global__ void __launch_bounds__(kBlock) k_level_scatter(const uint32_t* nk, const int32_t* np, const int32_t* ndi,
+                                                          const int32_t* pos, int64_t ne, int64_t npoints,
+                                                          int64_t nleaf_slots, bool leaf, int64_t base,
+                                                          int64_t child_level_base, uint32_t* key, int32_t* pidx,
+                                                          int32_t* pnum, int32_t* didx, int32_t* dnum,
+                                                          int32_t* child_base) {
+    const int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (q >= ne) return;
+    const int64_t g = base + pos[q];
+    key[g] = nk[q];
+    pidx[g] = np[q];
+    pnum[g] = int32_t(((q + 1 < ne) ? int64_t(np[q + 1]) : npoints) - np[q]);
+    if (leaf) {
+        didx[g] = pos[q];
+        dnum[g] = 1;
+    } else {
+        didx[g] = ndi[q];
+        dnum[g] = int32_t(((q + 1 < ne) ? int64_t(ndi[q + 1]) : nleaf_slots) - ndi[q]);
+        child_base[g] = int32_t(child_level_base + 8 * q);
+    }
+}
+
+// Every slot of depth d: key, running pidx / didx inside the sibling block and
+// the parent index (updateEmptyNodeInfo's intent, main.cu:433-490).
+__global__ void __launch_bounds__(kBlock) k_level_fill(const uint32_t* pk, const int32_t* pp, const int32_t* pdi,
+                                                       const int32_t* ppos, int64_t count, int D, int d, int64_t base,
+                                                       int64_t parent_base, uint32_t* key, int32_t* pidx,
+                                                       const int32_t* pnum, int32_t* didx, const int32_t* dnum,
+                                                       int32_t* parent) {
+    const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (s >= count) return;
+    const int64_t b = s >> 3;
+    const int j = int(s & 7);
+    const int64_t g = base + s;
+    key[g] = pk[b] | (uint32_t(j) << (3 * (D - d)));
+    int32_t run_p = pp[b], run_d = pdi[b];
+    for (int k = 0; k < j; k++) {
+        run_p += pnum[base + 8 * b + k];
+        run_d += dnum[base + 8 * b + k];
+    }
+    pidx[g] = run_p;
+    if (d != D) didx[g] = run_d;
+    parent[g] = int32_t(parent_base + ppos[b]);
+}
+
+__global__ void __launch_bounds__(kBlock) k_point_leaf(const int32_t* rank, const int32_t* pos_leaf, int64_t n,
+                                                       int32_t* point_leaf) {
+    const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (r >= n) return;
+    point_leaf[r] = pos_leaf[rank[r]];
+}
+
+__global__ void k_root(uint32_t* key, int32_t* pidx, int32_t* pnum, int32_t* parent, int32_t* child_base,
+                       int32_t* didx, int32_t* dnum, int32_t* neigh, int64_t M, int32_t npoints, int32_t nleaf,
+                       int32_t base1) {
+    const int j = threadIdx.x;
+    if (j < 27) neigh[int64_t(j) * M] = (j == 13) ? 0 : -1;
+    if (j == 0) {
+        key[0] = 0;
+        pidx[0] = 0;
+        pnum[0] = npoints;
+        parent[0] = -1;
+        child_base[0] = base1;
+        didx[0] = 0;
+        dnum[0] = nleaf;
+    }
+}
+
+// computeNeighbor (main.cu:492-509): neighbour j of a depth-d node is child
+// LUTchild[s][j] of its parent's neighbour LUTparent[s][j], when that exists.
+__global__ void __launch_bounds__(kBlock) k_neighbors(const uint32_t* key, const int32_t* parent,
+                                                      const int32_t* child_base, int32_t* neigh, int64_t M,
+                                                      int64_t base, int64_t count, int D, int d) {
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= count) return;
+    const int64_t i = base + t;
+    const int s = int((key[i] >> (3 * (D - d))) & 7u);
+    const int64_t p = parent[i];
+#pragma unroll
+    for (int j = 0; j < 27; j++) {
+        const int32_t np = neigh[int64_t(c_lut.parent[s][j]) * M + p];
+        int32_t v = -1;
+        if (np != -1) {
+            const int32_t cb = child_base[np];
+            if (cb != -1) v = cb + c_lut.child[s][j];
+        }
+        neigh[int64_t(j) * M + i] = v;
+    }
+}
+
+}  // namespace
+
+void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n, bool on_device) {
+    hipStream_t s = c->stream;
+    DBuf<float> hxyz, hn;
+    const float* dxyz = xyz;
+    const float* dn = nxyz;
+    if (!on_device) {
+        PRG_HIP(hipMemcpyAsync(hxyz.reserve(size_t(3 * n)), xyz, size_t(3 * n) * 4, hipMemcpyHostToDevice, s));
+        PRG_HIP(hipMemcpyAsync(hn.reserve(size_t(3 * n)), nxyz, size_t(3 * n) * 4, hipMemcpyHostToDevice, s));
+        dxyz = hxyz.get();
+        dn = hn.get();
+    }
+    const int blocks = int(std::min<int64_t>(1024, (n + kBlock - 1) / kBlock));
+    DBuf<float> part;
+    part.reserve(size_t(6 * blocks));
+    {
+        TimedScope ts(c->kt, "bbox_partial", 12.0 * double(n));
+        k_bbox_partial<<<blocks, kBlock, 0, s>>>(dxyz, n, part.get());
+        PRG_CHECK_LAUNCH();
+    }
+    std::vector<float> hp(size_t(6 * blocks));
+    PRG_HIP(hipMemcpyAsync(hp.data(), part.get(), hp.size() * 4, hipMemcpyDeviceToHost, s));
+    PRG_HIP(hipStreamSynchronize(s));
+    float mn[3], mx[3];
+    for (int a = 0; a < 3; a++) {
+        mn[a] = hp[a];
+        mx[a] = hp[3 + a];
+        for (int b = 1; b < blocks; b++) {
+            mn[a] = std::min(mn[a], hp[6 * b + a]);
+            mx[a] = std::max(mx[a], hp[6 * b + 3 + a]);
+        }
+    }
+    // scale and centre exactly as the reference's float code (main.cu:539-545)
+    float scale = 1.0f, center[3];
+    for (int a = 0; a < 3; a++) {
+        if (!a || scale < mx[a] - mn[a]) scale = float(mx[a] - mn[a]);
+        center[a] = float(mx[a] + mn[a]) / 2;
+    }
+    scale *= 1.25f;
+    for (int a = 0; a < 3; a++) center[a] -= scale / 2;
+    c->N = n;
+    for (int a = 0; a < 3; a++) c->center[a] = center[a];
+    c->scale = scale;
+    {
+        TimedScope ts(c->kt, "normalize", 48.0 * double(n));
+        k_normalize<<<grid_for(n, kBlock), kBlock, 0, s>>>(
+            dxyz, dn, n, center[0], center[1], center[2], scale, float(2 << c->D), c->px.reserve(n),
+            c->py.reserve(n), c->pz.reserve(n), c->nx.reserve(n), c->ny.reserve(n), c->nz.reserve(n));
+        PRG_CHECK_LAUNCH();
+    }
+    PRG_HIP(hipStreamSynchronize(s));  // temporaries are released on return
+}
+
+void build_octree(prg_ctx* c) {
+    static bool lut_ready = false;
+    if (!lut_ready) {
+        NeighLut l = make_lut();
+        PRG_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lut), &l, sizeof(l)));
+        lut_ready = true;
+    }
+    hipStream_t s = c->stream;
+    const int D = c->D;
+    const int64_t n = c->N;
+    KernelTimer& kt = c->kt;
+
+    // 1. keys + stable sort
+    DBuf<uint32_t> key, val, key2, val2;
+    key.reserve(n);
+    val.reserve(n);
+    key2.reserve(n);
+    val2.reserve(n);
+    {
+        TimedScope ts(kt, "morton", 20.0 * double(n));
+        k_morton<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->px.get(), c->py.get(), c->pz.get(), n, D, key.get(),
+                                                         val.get());
+        PRG_CHECK_LAUNCH();
+    }
+    radix_sort_pairs(key.get(), val.get(), key2.get(), val2.get(), n, 3 * D, s, c->scratch, kt);
+    {
+        TimedScope ts(kt, "gather_points", 52.0 * double(n));
+        k_gather6<<<grid_for(n, kBlock), kBlock, 0, s>>>(
+            val.get(), n, c->px.get(), c->py.get(), c->pz.get(), c->nx.get(), c->ny.get(), c->nz.get(),
+            c->tx.reserve(n), c->ty.reserve(n), c->tz.reserve(n), c->tnx.reserve(n), c->tny.reserve(n),
+            c->tnz.reserve(n));
+        PRG_CHECK_LAUNCH();
+    }
+    std::swap(c->px.p, c->tx.p);
+    std::swap(c->px.n, c->tx.n);
+    std::swap(c->py.p, c->ty.p);
+    std::swap(c->py.n, c->ty.n);
+    std::swap(c->pz.p, c->tz.p);
+    std::swap(c->pz.n, c->tz.n);
+    std::swap(c->nx.p, c->tnx.p);
+    std::swap(c->nx.n, c->tnx.n);
+    std::swap(c->ny.p, c->tny.p);
+    std::swap(c->ny.n, c->tny.n);
+    std::swap(c->nz.p, c->tnz.p);
+    std::swap(c->nz.n, c->tnz.n);
+
+    // 2. unique leaves
+    DBuf<int32_t> flag, rank, tot;
+    flag.reserve(n);
+    rank.reserve(n);
+    tot.reserve(1);
+    {
+        TimedScope ts(kt, "heads", 12.0 * double(n));
+        k_heads<<<grid_for(n, kBlock), kBlock, 0, s>>>(key.get(), n, 0, flag.get());
+        PRG_CHECK_LAUNCH();
+    }
+    scan_exclusive(flag.get(), rank.get(), n, tot.get(), s, c->scratch, kt);
+    int32_t U = 0;
+    PRG_HIP(hipMemcpyAsync(&U, tot.get(), 4, hipMemcpyDeviceToHost, s));
+    PRG_HIP(hipStreamSynchronize(s));
+
+    // per-depth compact arrays of non-empty nodes
+    std::vector<std::unique_ptr<DBuf<uint32_t>>> NK(D + 1);
+    std::vector<std::unique_ptr<DBuf<int32_t>>> NP(D + 1), NDI(D + 1), POS(D + 1);
+    std::vector<int64_t> NE(D + 1, 0);
+    for (int d = 0; d <= D; d++) {
+        NK[d].reset(new DBuf<uint32_t>());
+        NP[d].reset(new DBuf<int32_t>());
+        NDI[d].reset(new DBuf<int32_t>());
+        POS[d].reset(new DBuf<int32_t>());
+    }
+    NE[D] = U;
+    NK[D]->reserve(U);
+    NP[D]->reserve(U);
+    NDI[D]->reserve(U);
+    {
+        TimedScope ts(kt, "unique", 16.0 * double(n));
+        k_unique<<<grid_for(n, kBlock), kBlock, 0, s>>>(key.get(), flag.get(), rank.get(), n, NK[D]->get(),
+                                                         NP[D]->get());
+        PRG_CHECK_LAUNCH();
+    }
+    DBuf<int32_t> prank;  // unique-leaf rank of every sorted point
+    prank.reserve(n);
+
+    // 3. levels bottom-up
+    std::vector<int64_t> counts(D + 1, 0);
+    DBuf<int32_t> bflag, brank, btot;
+    btot.reserve(1);
+    for (int d = D; d >= 1; --d) {
+        const int64_t ne = NE[d];
+        bflag.reserve(ne);
+        brank.reserve(ne);
+        {
+            TimedScope ts(kt, "level_heads", 8.0 * double(ne));
+            k_heads<<<grid_for(ne, kBlock), kBlock, 0, s>>>(NK[d]->get(), ne, 3 * (D - d + 1), bflag.get());
+            PRG_CHECK_LAUNCH();
+        }
+        scan_exclusive(bflag.get(), brank.get(), ne, btot.get(), s, c->scratch, kt);
+        int32_t nb = 0;
+        PRG_HIP(hipMemcpyAsync(&nb, btot.get(), 4, hipMemcpyDeviceToHost, s));
+        PRG_HIP(hipStreamSynchronize(s));
+        // brank holds the exclusive count of heads: block index = brank + flag - 1
+        counts[d] = 8 * int64_t(nb);
+        NE[d - 1] = nb;
+        NK[d - 1]->reserve(nb);
+        NP[d - 1]->reserve(nb);
+        NDI[d - 1]->reserve(nb);
+        POS[d]->reserve(ne);
+        {
+            // block index of every non-empty node: heads before it, inclusive, minus one
+            TimedScope ts(kt, "level_up", 28.0 * double(ne));
+            k_excl_to_incl<<<grid_for(ne, kBlock), kBlock, 0, s>>>(brank.get(), brank.get(), bflag.get(), ne);
+            PRG_CHECK_LAUNCH();
+            k_level_up<<<grid_for(ne, kBlock), kBlock, 0, s>>>(NK[d]->get(), NP[d]->get(), NDI[d]->get(),
+                                                                bflag.get(), brank.get(), ne, D, d, POS[d]->get(),
+                                                                NK[d - 1]->get(), NP[d - 1]->get(),
+                                                                NDI[d - 1]->get());
+            PRG_CHECK_LAUNCH();
+        }
+    }
+    counts[0] = 1;
+    // root compact arrays: key 0, pidx 0, didx 0, slot 0
+    POS[0]->reserve(1);
+    PRG_HIP(hipMemsetAsync(POS[0]->get(), 0, 4, s));
+
+    std::vector<int64_t> bases(D + 1, 0);
+    for (int d = 1; d <= D; d++) bases[d] = bases[d - 1] + counts[d - 1];
+    const int64_t M = bases[D] + counts[D];
+    if (M >= (int64_t(1) << 31)) throw Error{PRG_E_ARG, "octree exceeds 2^31 nodes"};
+    c->counts = counts;
+    c->bases = bases;
+    c->M = M;
+    uint32_t* K = c->key.reserve(M);
+    int32_t* PI = c->pidx.reserve(M);
+    int32_t* PN = c->pnum.reserve(M);
+    int32_t* PA = c->parent.reserve(M);
+    int32_t* CB = c->child_base.reserve(M);
+    int32_t* DI = c->didx.reserve(M);
+    int32_t* DN = c->dnum.reserve(M);
+    int32_t* NB = c->neigh.reserve(size_t(27) * M);
+    const int64_t MD = counts[D];
+    {
+        TimedScope ts(kt, "levels_fill", 44.0 * double(M));
+        k_root<<<1, 64, 0, s>>>(K, PI, PN, PA, CB, DI, DN, NB, M, int32_t(n), int32_t(MD), int32_t(bases[1]));
+        PRG_CHECK_LAUNCH();
+        for (int d = 1; d <= D; d++) {
+            const bool leaf = (d == D);
+            k_level_init<<<grid_for(counts[d], kBlock), kBlock, 0, s>>>(bases[d], counts[d], leaf, PN, DN, CB);
+            PRG_CHECK_LAUNCH();
+            k_level_scatter<<<grid_for(NE[d], kBlock), kBlock, 0, s>>>(
+                NK[d]->get(), NP[d]->get(), NDI[d]->get(), POS[d]->get(), NE[d], n, MD, leaf, bases[d],
+                leaf ? 0 : bases[d + 1], K, PI, PN, DI, DN, CB);
+            PRG_CHECK_LAUNCH();
+        }
+        for (int d = 1; d <= D; d++) {
+            k_level_fill<<<grid_for(counts[d], kBlock), kBlock, 0, s>>>(
+                NK[d - 1]->get(), NP[d - 1]->get(), NDI[d - 1]->get(), POS[d - 1]->get(), counts[d], D, d,
+                bases[d], bases[d - 1], K, PI, PN, DI, DN, PA);
+            PRG_CHECK_LAUNCH();
+        }
+    }
+    // point -> leaf-level slot: rank of the run containing each sorted point
+    {
+        TimedScope ts(kt, "point_leaf", 12.0 * double(n));
+        k_excl_to_incl<<<grid_for(n, kBlock), kBlock, 0, s>>>(prank.get(), rank.get(), flag.get(), n);
+        PRG_CHECK_LAUNCH();
+        k_point_leaf<<<grid_for(n, kBlock), kBlock, 0, s>>>(prank.get(), POS[D]->get(), n, c->point_leaf.reserve(n));
+        PRG_CHECK_LAUNCH();
+    }
+    // 4. neighbours, top-down
+    for (int d = 1; d <= D; d++) {
+        TimedScope ts(kt, "neighbors", 128.0 * double(counts[d]));
+        k_neighbors<<<grid_for(counts[d], kBlock), kBlock, 0, s>>>(K, PA, CB, NB, M, bases[d], counts[d], D, d);
+        PRG_CHECK_LAUNCH();
+    }
+    PRG_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace prg
