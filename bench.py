@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark: points/s end-to-end at depth 10 on one MI355X.
+
+Workload: 2,000,000 float32 points on the union of 3 seeded superquadrics
+(noise sigma 0.001, analytic normals), octree depth 10, FP32 CG to 1e-6
+relative residual with FP64 dot products.  A step is the whole pipeline
+(normalisation, octree, divergence, per-depth solves, isovalue, leaf-level
+marching cubes, mesh copied to host memory) starting from points already
+resident in device memory.
+
+Prints one JSON line (rank 0).  One GPU only: --gpus N with N > 1 exits.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def _hip():
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    return hip
+
+
+def workload(args):
+    import poissonrecon_gpu_amd as prg
+
+    if args.workload == "superquadrics_2M_d10":
+        p, n = prg.synth_superquadrics(args.points, 3, 0.001, 3)
+        return p, n, 10
+    if args.workload == "torus_500k_d9":
+        p, n = prg.synth_torus(args.points, 2)
+        return p, n, 9
+    if args.workload == "sphere_36k_d8":
+        p, n = prg.synth_sphere(args.points, 1)
+        return p, n, 8
+    raise SystemExit(f"unknown workload {args.workload}")
+
+
+def run_step(g, dp, dn, n):
+    import poissonrecon_gpu_amd as prg
+
+    g._call("prg_set_points", dp, dn, n, 1)
+    g.build_octree()
+    g.compute_divergence()
+    stats = g.solve()
+    g.isovalue()
+    mesh = g.extract_mesh()
+    return stats, mesh
+
+
+def cpu_baseline(p, n, D, sample):
+    """The CPU oracle (restatement of the reference stages) on a bounded sample."""
+    import oracle
+
+    q, m = p[:sample], n[:sample]
+    o = oracle.Oracle(D)
+    t0 = time.perf_counter()
+    o.set_points(q, m)
+    o.build_octree()
+    o.compute_divergence()
+    o.solve(mode=0)
+    o.isovalue()
+    o.extract_leaf_mesh()
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": sample / dt, "unit": "points/s", "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"first {sample} points of the same cloud at depth {D}, full pipeline to the leaf mesh "
+                      f"({dt:.1f} s, OpenMP threads={oracle.num_threads()})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="superquadrics_2M_d10")
+    ap.add_argument("--points", type=int, default=2_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=400_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-kernels", action="store_true", help="skip the timed steps' extras")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import poissonrecon_gpu_amd as prg
+
+    p, nrm, D = workload(args)
+    N = len(p)
+    hip = _hip()
+    dp, dn = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(dp), p.nbytes) == 0 and hip.hipMalloc(C.byref(dn), nrm.nbytes) == 0
+    assert hip.hipMemcpy(dp, p.ctypes.data, p.nbytes, 1) == 0
+    assert hip.hipMemcpy(dn, nrm.ctypes.data, nrm.nbytes, 1) == 0
+
+    g = prg.Reconstructor(max_depth=D)
+    for _ in range(args.warmup):
+        run_step(g, dp, dn, N)
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats, mesh = run_step(g, dp, dn, N)
+    hip.hipDeviceSynchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    stage_ms = {k: round(v, 3) for k, v in g.stage_times().items()}
+    counts, _ = g.level_info()
+    g.close()
+
+    # one more pass with per-kernel hipEvents (on the context stream) for the roofline
+    gt = prg.Reconstructor(max_depth=D, timing=True)
+    run_step(gt, dp, dn, N)
+    kern = {}
+    for name in ["stencil_spmv_dot", "cg_update", "cg_direction", "vector_field", "div_chunks", "div_leaf",
+                 "radix_downsweep", "radix_upsweep", "point_values", "corner_value", "neighbors", "levels_fill",
+                 "edge_owner", "corner_owner", "scan_exclusive", "gather_points", "morton"]:
+        try:
+            cnt, ms, bpl = gt.kernel_stats(name)
+            kern[name] = {"launches": cnt, "total_ms": round(ms, 4), "avg_us": round(1000 * ms / cnt, 2),
+                          "alg_bytes_per_launch": bpl}
+        except prg.PrgError:
+            pass
+    gt.close()
+    hip.hipFree(dp)
+    hip.hipFree(dn)
+
+    # dominant HBM-bound kernel of the path: the CG SpMV (reads the 27 neighbour planes)
+    dom = "stencil_spmv_dot"
+    k = kern[dom]
+    achieved = k["alg_bytes_per_launch"] / (k["avg_us"] * 1e-6) / 1e9
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+
+    base = None
+    if not args.no_cpu:
+        base = cpu_baseline(p, nrm, D, min(args.cpu_sample, N))
+
+    out = {
+        "metric": "points/s end-to-end at depth 10",
+        "value": round(N / dt, 1),
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000 * dt, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (f64 reductions)",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "points": N, "max_depth": D, "nodes": int(counts.sum()),
+                   "leaf_nodes": int(counts[-1]), "cg_tol": 1e-6, "parallelism": "single GPU"},
+        "stage_ms": stage_ms,
+        "solve_ms": round(sum(s["ms"] for s in stats), 3),
+        "cg_iters": [s["iters"] for s in stats],
+        "mesh": {"vertices": int(len(mesh.vertices)), "triangles": int(len(mesh.triangles))},
+        "kernels": kern,
+        "roofline": roofline,
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -597,9 +597,10 @@ int32_t orc_solve(orc_state* s, int32_t mode, float tol, int32_t max_iters, int3
             double rr = dot_dd(r.data(), r.data(), n);
             p = r;
             const double tol2 = double(tol) * double(tol) * bb;
-            while (bb > 0.0 && it < max_iters && (fixed_iters > 0 ? it < fixed_iters : rr > tol2)) {
+            while (bb > 0.0 && rr != 0.0 && it < max_iters && (fixed_iters > 0 ? it < fixed_iters : rr > tol2)) {
                 spmv_depth(s, d, p.data(), q.data());
                 double pq = dot_dd(p.data(), q.data(), n);
+                if (!(pq > 0.0)) break;  // breakdown once the residual has vanished
                 float af = float(rr / pq);
                 for (int64_t i = 0; i < n; i++) {
                     float t1 = af * p[i];

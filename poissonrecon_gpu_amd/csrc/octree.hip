@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(kBlock) k_level_fill(const uint32_t* pk, const
         run_d += dnum[base + 8 * b + k];
     }
     pidx[g] = run_p;
-    if (d != D) didx[g] = run_d;
+    didx[g] = (d != D) ? run_d : int32_t(s);  // leaf level: every slot is its own first leaf slot
     parent[g] = int32_t(parent_base + ppos[b]);
 }
 

@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(kBlock) k_cg_init(const int32_t* neigh, int64_
         st->rr = RR;
         st->iters = 0;
         const double tol2 = double(tol) * double(tol) * BB;
-        st->done = (BB <= 0.0) || (fixed > 0 ? 0 : (RR <= tol2));
+        st->done = (BB <= 0.0) || (RR == 0.0) || (fixed > 0 ? 0 : (RR <= tol2));
         st->ticket = 0u;
     }
 }
@@ -133,6 +133,8 @@ __global__ void __launch_bounds__(kBlock) k_cg_spmv(const int32_t* neigh, int64_
         const double PQ = sum_partials(partial, gridDim.x);
         st->pq = PQ;
         st->alpha = float(st->rr / PQ);
+        // breakdown (p.Ap <= 0 once the residual has vanished): stop before the update
+        if (!(PQ > 0.0)) st->done = 1;
         st->ticket = 0u;
     }
 }
@@ -144,7 +146,8 @@ __device__ __forceinline__ void finish_iteration(CgState* st, double rr_new, flo
     st->beta = float(rr_new / st->rr);
     st->rr = rr_new;
     const double tol2 = double(tol) * double(tol) * st->bb;
-    const bool stop = (st->iters >= max_iters) || (fixed > 0 ? (st->iters >= fixed) : (rr_new <= tol2));
+    // an exactly zero residual ends the solve in either mode (the next step would divide 0 by 0)
+    const bool stop = (st->iters >= max_iters) || (rr_new == 0.0) || (fixed > 0 ? (st->iters >= fixed) : (rr_new <= tol2));
     st->done = stop ? 1 : 0;
 }
 
