@@ -31,9 +31,26 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 def _hip():
     hip = C.CDLL("libamdhip64.so")
     hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipHostMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint]
     hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
     hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipHostFree.argtypes = [C.c_void_p]
     return hip
+
+
+class PinnedMesh:
+    """Pinned host buffers reused by every step for the mesh copy."""
+
+    def __init__(self, hip, nverts, ntris):
+        self.hip = hip
+        self.vcap, self.tcap = 3 * nverts, 3 * ntris
+        self.v, self.t = C.c_void_p(), C.c_void_p()
+        assert hip.hipHostMalloc(C.byref(self.v), 4 * self.vcap, 0) == 0
+        assert hip.hipHostMalloc(C.byref(self.t), 4 * self.tcap, 0) == 0
+
+    def free(self):
+        self.hip.hipHostFree(self.v)
+        self.hip.hipHostFree(self.t)
 
 
 def workload(args):
@@ -51,7 +68,7 @@ def workload(args):
     raise SystemExit(f"unknown workload {args.workload}")
 
 
-def run_step(g, dp, dn, n):
+def run_step(g, dp, dn, n, pinned=None):
     import poissonrecon_gpu_amd as prg
 
     g._call("prg_set_points", dp, dn, n, 1)
@@ -59,8 +76,12 @@ def run_step(g, dp, dn, n):
     g.compute_divergence()
     stats = g.solve()
     g.isovalue()
-    mesh = g.extract_mesh()
-    return stats, mesh
+    if pinned is None:
+        mesh = g.extract_mesh()
+        return stats, (len(mesh.vertices), len(mesh.triangles))
+    m = prg.PrgMesh()
+    g._call("prg_extract_mesh_into", 0, pinned.v, pinned.vcap, pinned.t, pinned.tcap, C.byref(m))
+    return stats, (m.n_vertices, m.n_triangles)
 
 
 def cpu_baseline(p, n, D, sample):
@@ -92,7 +113,8 @@ def main():
     ap.add_argument("--points", type=int, default=2_000_000)
     ap.add_argument("--cpu-sample", type=int, default=400_000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--profile-kernels", action="store_true", help="skip the timed steps' extras")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="end with a 512 MiB device-to-device copy (a copy kernel of known size for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -109,12 +131,14 @@ def main():
     assert hip.hipMemcpy(dn, nrm.ctypes.data, nrm.nbytes, 1) == 0
 
     g = prg.Reconstructor(max_depth=D)
-    for _ in range(args.warmup):
-        run_step(g, dp, dn, N)
+    _, (nv, nt) = run_step(g, dp, dn, N)  # sizes the pinned mesh buffers
+    pinned = PinnedMesh(hip, int(nv * 1.25) + 1024, int(nt * 1.25) + 1024)
+    for _ in range(max(0, args.warmup - 1)):
+        run_step(g, dp, dn, N, pinned)
     hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        stats, mesh = run_step(g, dp, dn, N)
+        stats, mesh = run_step(g, dp, dn, N, pinned)
     hip.hipDeviceSynchronize()
     dt = (time.perf_counter() - t0) / args.steps
     stage_ms = {k: round(v, 3) for k, v in g.stage_times().items()}
@@ -125,7 +149,8 @@ def main():
     gt = prg.Reconstructor(max_depth=D, timing=True)
     run_step(gt, dp, dn, N)
     kern = {}
-    for name in ["stencil_spmv_dot", "cg_update", "cg_direction", "vector_field", "div_chunks", "div_leaf",
+    for name in ["stencil_spmv_dot", "cg_update", "cg_refresh", "cg_small", "cg_init", "vector_field",
+                 "div_subtree_sums", "div_subtree_finish", "div_from_subtree_sums", "div_leaf", "parent_neighbor_table",
                  "radix_downsweep", "radix_upsweep", "point_values", "corner_value", "neighbors", "levels_fill",
                  "edge_owner", "corner_owner", "scan_exclusive", "gather_points", "morton"]:
         try:
@@ -135,8 +160,17 @@ def main():
         except prg.PrgError:
             pass
     gt.close()
+    pinned.free()
     hip.hipFree(dp)
     hip.hipFree(dn)
+    if args.calibrate_copy:
+        a, b = C.c_void_p(), C.c_void_p()
+        size = 512 << 20
+        assert hip.hipMalloc(C.byref(a), size) == 0 and hip.hipMalloc(C.byref(b), size) == 0
+        assert hip.hipMemcpy(b, a, size, 3) == 0  # device to device
+        hip.hipDeviceSynchronize()
+        hip.hipFree(a)
+        hip.hipFree(b)
 
     # dominant HBM-bound kernel of the path: the CG SpMV (reads the 27 neighbour planes)
     dom = "stencil_spmv_dot"
@@ -144,6 +178,11 @@ def main():
     achieved = k["alg_bytes_per_launch"] / (k["avg_us"] * 1e-6) / 1e9
     roofline = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+    traffic_file = ROOT / "profiles" / f"traffic_{dom}_{args.workload}.json"
+    if traffic_file.exists():
+        t = json.loads(traffic_file.read_text())
+        roofline["traffic"] = t.get("hbm_bytes_per_launch")
+        roofline["traffic_source"] = str(traffic_file.relative_to(ROOT))
 
     base = None
     if not args.no_cpu:
@@ -165,9 +204,9 @@ def main():
         "config": {"workload": args.workload, "points": N, "max_depth": D, "nodes": int(counts.sum()),
                    "leaf_nodes": int(counts[-1]), "cg_tol": 1e-6, "parallelism": "single GPU"},
         "stage_ms": stage_ms,
-        "solve_ms": round(sum(s["ms"] for s in stats), 3),
+        "solve_ms": stage_ms["solve"],
         "cg_iters": [s["iters"] for s in stats],
-        "mesh": {"vertices": int(len(mesh.vertices)), "triangles": int(len(mesh.triangles))},
+        "mesh": {"vertices": int(mesh[0]), "triangles": int(mesh[1])},
         "kernels": kern,
         "roofline": roofline,
         "cpu_baseline": base,

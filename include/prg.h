@@ -187,6 +187,15 @@ prg_status prg_set_isovalue(prg_ctx* ctx, float iso);
 prg_status prg_extract_mesh(prg_ctx* ctx, int32_t subdivide, prg_mesh* out);
 void prg_mesh_release(prg_mesh* mesh);
 
+/* As prg_extract_mesh, but copies into caller-provided host buffers (for
+ * example pinned memory reused across calls): vertices holds vcap floats,
+ * triangles tcap ints.  out receives the counts and, on success, the given
+ * pointers; when a capacity is too small the call returns PRG_E_ARG with the
+ * required counts in out and copies nothing.  prg_mesh_release must not be
+ * called on such a mesh. */
+prg_status prg_extract_mesh_into(prg_ctx* ctx, int32_t subdivide, float* vertices, int64_t vcap, int32_t* triangles,
+                                 int64_t tcap, prg_mesh* out);
+
 /* Whole pipeline: build, divergence, solve, isovalue, mesh. */
 prg_status prg_reconstruct(prg_ctx* ctx, int32_t subdivide, prg_mesh* out, prg_solve_stats* per_depth);
 

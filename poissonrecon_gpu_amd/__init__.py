@@ -88,6 +88,7 @@ SIGNATURES = {
     "prg_set_isovalue": (C.c_int, [_P, C.c_float]),
     "prg_extract_mesh": (C.c_int, [_P, C.c_int32, C.POINTER(PrgMesh)]),
     "prg_mesh_release": (None, [C.POINTER(PrgMesh)]),
+    "prg_extract_mesh_into": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, C.c_int64, C.POINTER(PrgMesh)]),
     "prg_reconstruct": (C.c_int, [_P, C.c_int32, C.POINTER(PrgMesh), C.POINTER(PrgSolveStats)]),
     "prg_stage_times": (C.c_int, [_P, C.POINTER(PrgTimings)]),
     "prg_buffer_size": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64)]),

@@ -113,6 +113,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
     prg_status st = guarded(c, [&]() {
         PRG_HIP(hipSetDevice(cf.device));
         PRG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        PRG_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         PRG_HIP(hipEventCreate(&c->ev_a));
         PRG_HIP(hipEventCreate(&c->ev_b));
         c->kt.enabled = cf.timing != 0;
@@ -131,11 +132,13 @@ void prg_destroy(prg_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->ev_a) (void)hipEventDestroy(c->ev_a);
     if (c->ev_b) (void)hipEventDestroy(c->ev_b);
-    hipStream_t s = c->stream;
+    hipStream_t s = c->stream, s2 = c->stream2;
     delete c;  // releases device buffers
     if (s) (void)hipStreamDestroy(s);
+    if (s2) (void)hipStreamDestroy(s2);
 }
 
 const char* prg_last_error(const prg_ctx* c) { return c ? c->err.c_str() : "null context"; }
@@ -277,6 +280,32 @@ prg_status prg_extract_mesh(prg_ctx* c, int32_t subdivide, prg_mesh* out) {
         PRG_HIP(hipStreamSynchronize(c->stream));
         for (int a = 0; a < 3; a++) out->center[a] = c->center[a];
         out->scale = c->scale;
+        return PRG_OK;
+    });
+}
+
+prg_status prg_extract_mesh_into(prg_ctx* c, int32_t subdivide, float* vertices, int64_t vcap, int32_t* triangles,
+                                 int64_t tcap, prg_mesh* out) {
+    return guarded(c, [&]() {
+        if (!c->have_x) return fail(c, PRG_E_STATE, "prg_solve first");
+        if (!c->iso_ready) return fail(c, PRG_E_STATE, "prg_isovalue or prg_set_isovalue first");
+        if (!out) return fail(c, PRG_E_ARG, "null mesh");
+        if (subdivide) return fail(c, PRG_E_ARG, "coarse-leaf subdivision is not built yet");
+        PRG_HIP(hipSetDevice(c->cfg.device));
+        prg::extract_leaf_mesh(c);
+        std::memset(out, 0, sizeof(*out));
+        out->n_vertices = out->n_leaf_vertices = c->mesh_nv;
+        out->n_triangles = out->n_leaf_triangles = c->mesh_nt;
+        for (int a = 0; a < 3; a++) out->center[a] = c->center[a];
+        out->scale = c->scale;
+        if (3 * c->mesh_nv > vcap || 3 * c->mesh_nt > tcap || (!vertices && c->mesh_nv) || (!triangles && c->mesh_nt))
+            return fail(c, PRG_E_ARG, "mesh buffers too small");
+        prg::StageTimer st(c, PRG_STAGE_MESH_COPY);
+        if (c->mesh_nv) d2h(c, vertices, c->mesh_v.get(), size_t(3 * c->mesh_nv) * 4);
+        if (c->mesh_nt) d2h(c, triangles, c->mesh_t.get(), size_t(3 * c->mesh_nt) * 4);
+        PRG_HIP(hipStreamSynchronize(c->stream));
+        out->vertices = vertices;
+        out->triangles = triangles;
         return PRG_OK;
     });
 }

@@ -6,6 +6,10 @@
 //   nodes      key                    uint32[M]
 //              pidx,pnum,parent,child_base,didx,dnum   int32[M]
 //              neigh                  int32[27][M]  (plane j holds neighbour j of every node)
+//              pnt                    int32[M/8][27] per depth >= 1: for each sibling block, the
+//                                     level-local child base of each of its parent's 27
+//                                     neighbours (-1 if absent); neighbour j of slot s is
+//                                     pnt[b][LUTparent[s][j]] + LUTchild[s][j]
 //   fields     vfield_{x,y,z}         float[M_D]
 //              div, x                 float[M]
 #pragma once
@@ -19,6 +23,7 @@ struct prg_ctx {
     prg_config cfg{};
     int D = 8;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;  // independent work (small-depth solves) overlapping `stream`
     std::string err;
     prg::KernelTimer kt;
     prg::Scratch scratch;
@@ -37,6 +42,8 @@ struct prg_ctx {
     int64_t M = 0;
     prg::DBuf<uint32_t> key;
     prg::DBuf<int32_t> pidx, pnum, parent, child_base, didx, dnum, neigh;
+    prg::DBuf<int32_t> pnt;            // concatenated per depth 1..D, block rows of 27
+    std::vector<int64_t> pnt_off;      // first row of depth d (rows = blocks)
 
     // function space
     std::unique_ptr<prg::FsTables> fs;

@@ -12,10 +12,10 @@
 // reference runs one thread per node (finer depths) and a host loop with
 // three cudaMallocs per node (coarser depths).  Here:
 //   - leaf level: one thread per node, 27 terms;
-//   - coarser depths: the term list of each node is cut into chunks of
-//     kChunk terms, one wave per chunk accumulates in double and reduces with
-//     cross-lane shuffles, and a finishing pass adds each node's chunk partials
-//     in chunk order.  No global atomics; the result is deterministic.
+//   - coarser depths: per-node subtree sums G[a][j] in one streaming pass
+//     (wave-level reductions over chunks of a node's leaves, chunk partials
+//     added in order), then b_i gathers 27 of them.  No global atomics; the
+//     result is deterministic.
 #include <cmath>
 
 #include "ctx.h"
@@ -153,84 +153,142 @@ __device__ __forceinline__ int depth_of(const Bases& B, int D, int64_t i) {
     return d;
 }
 
-// number of terms of every node below the leaf level
-__global__ void __launch_bounds__(kBlock) k_div_terms(const int32_t* neigh, const int32_t* dnum, int64_t M,
-                                                      int64_t n, int32_t* chunks, int64_t* terms) {
-    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (i >= n) return;
-    int64_t L = 0;
-    for (int j = 0; j < 27; j++) {
-        const int32_t nb = neigh[int64_t(j) * M + i];
-        if (nb != -1) L += dnum[nb];
-    }
-    terms[i] = L;
-    chunks[i] = int32_t((L + kChunk - 1) / kChunk);
+// ---- coarser depths: per-node subtree sums G, then a 27-term gather ---------------
+// For a node a of depth d and a relative position j (a = i + delta_j seen from
+// the node i being assembled), the reference's terms involving a's subtree are
+//   floatdot(V_k, (w[(dx+1)s + rx_k], w[(dy+1)s + ry_k], w[(dz+1)s + rz_k]))
+// with r_k the leaf's offset inside a's cell and s = 2^(D-d) - they depend on
+// a's own leaves and on j only.  G[a][j] sums them over a's leaf range (one
+// streaming pass per depth, 27 accumulators per lane), and
+// b_i = sum_j G[neighbour_j(i)][j].  The float terms are the reference's;
+// only the grouping of the double sum differs.
+
+// chunks of a node's own leaf range
+__global__ void __launch_bounds__(kBlock) k_g_chunks(const int32_t* dnum, int64_t n, int32_t* chunks, int32_t* multi) {
+    const int64_t a = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (a >= n) return;
+    const int32_t c = (dnum[a] + kChunk - 1) / kChunk;
+    chunks[a] = c;
+    multi[a] = (c > 1) ? c : 0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_div_chunk_map(const int32_t* chunks, const int32_t* chunk_off, int64_t n,
-                                                          int32_t* chunk_node) {
-    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const int32_t o = chunk_off[i];
-    for (int32_t c = 0; c < chunks[i]; c++) chunk_node[o + c] = int32_t(i);
+__global__ void __launch_bounds__(kBlock) k_g_chunk_map(const int32_t* chunks, const int32_t* chunk_off, int64_t n,
+                                                        int32_t* chunk_node) {
+    const int64_t a = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (a >= n) return;
+    const int32_t o = chunk_off[a];
+    for (int32_t c = 0; c < chunks[a]; c++) chunk_node[o + c] = int32_t(a);
 }
 
-__global__ void __launch_bounds__(kBlock) k_div_chunks(const uint32_t* key, const int32_t* neigh,
-                                                       const int32_t* didx, const int32_t* dnum, const float* vfx,
-                                                       const float* vfy, const float* vfz, const float* table,
-                                                       const int64_t* table_off, const int32_t* chunk_node,
-                                                       const int32_t* chunk_off, const int64_t* terms,
-                                                       const int32_t* d_nchunks, int64_t M, Bases B, int D,
-                                                       double* partial) {
+// one wave per chunk: 27 double accumulators per lane, cross-lane reduction
+__global__ void __launch_bounds__(kBlock) k_g_sums(const uint32_t* key, const int32_t* didx, const int32_t* dnum,
+                                                   const float* vfx, const float* vfy, const float* vfz,
+                                                   const float* table, const int64_t* table_off,
+                                                   const int32_t* chunk_node, const int32_t* chunk_off,
+                                                   const int32_t* chunks, const int32_t* multi_off,
+                                                   const int32_t* d_nchunks, Bases B, int D, double* G,
+                                                   double* partial) {
     const int lane = threadIdx.x & 63;
     const int64_t nchunks = *d_nchunks;
     const int64_t wave0 = (int64_t(blockIdx.x) * kBlock + threadIdx.x) >> 6;
     const int64_t nwaves = (int64_t(gridDim.x) * kBlock) >> 6;
     const int64_t baseD = B.b[D];
     for (int64_t c = wave0; c < nchunks; c += nwaves) {
-        const int64_t i = chunk_node[c];
-        const int64_t k0 = int64_t(c - chunk_off[i]) * kChunk;
-        const int64_t k1 = min(k0 + kChunk, terms[i]);
-        const int d = depth_of(B, D, i);
-        const int64_t s = int64_t(1) << (D - d);
+        const int64_t a = chunk_node[c];
+        const int64_t ci = c - chunk_off[a];
+        const int64_t k0 = int64_t(didx[a]) + ci * kChunk;
+        const int64_t k1 = min(k0 + kChunk, int64_t(didx[a]) + dnum[a]);
+        const int d = depth_of(B, D, a);
+        const int64_t sc = int64_t(1) << (D - d);
+        const uint32_t rmask = uint32_t(sc - 1);
         const float* T = table + table_off[d];
-        const uint32_t ki = key[i];
-        const int64_t oix = compact3(ki >> 2) >> (D - d), oiy = compact3(ki >> 1) >> (D - d),
-                      oiz = compact3(ki) >> (D - d);
-        double acc = 0.0;
-        int64_t prefix = 0;
-        for (int j = 0; j < 27; j++) {
-            const int32_t nb = neigh[int64_t(j) * M + i];
-            if (nb == -1) continue;
-            const int64_t len = dnum[nb];
-            const int64_t lo = max(prefix, k0), hi = min(prefix + len, k1);
-            const int64_t start = didx[nb];
-            for (int64_t t = lo + lane; t < hi; t += 64) {
-                const int64_t leaf = start + (t - prefix);
-                const uint32_t kl = key[baseD + leaf];
-                const int64_t ux = int64_t(compact3(kl >> 2)) - (oix - 1) * s;
-                const int64_t uy = int64_t(compact3(kl >> 1)) - (oiy - 1) * s;
-                const int64_t uz = int64_t(compact3(kl)) - (oiz - 1) * s;
-                const float r = fdot(vfx[leaf], vfy[leaf], vfz[leaf], T[ux], T[uy], T[uz]);
-                acc += double(r);
-            }
-            prefix += len;
-            if (prefix >= k1) break;
+        double acc[27];
+#pragma unroll
+        for (int j = 0; j < 27; j++) acc[j] = 0.0;
+        for (int64_t k = k0 + lane; k < k1; k += 64) {
+            const uint32_t kl = key[baseD + k];
+            const int64_t rx = compact3(kl >> 2) & rmask, ry = compact3(kl >> 1) & rmask, rz = compact3(kl) & rmask;
+            const float wx[3] = {T[rx], T[sc + rx], T[2 * sc + rx]};
+            const float wy[3] = {T[ry], T[sc + ry], T[2 * sc + ry]};
+            const float wz[3] = {T[rz], T[sc + rz], T[2 * sc + rz]};
+            const float vx = vfx[k], vy = vfy[k], vz = vfz[k];
+#pragma unroll
+            for (int j = 0; j < 27; j++) acc[j] += double(fdot(vx, vy, vz, wx[j / 9], wy[(j / 3) % 3], wz[j % 3]));
         }
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-        if (lane == 0) partial[c] = acc;
+        for (int j = 0; j < 27; j++) {
+            double v = acc[j];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+            acc[j] = v;
+        }
+        if (lane == 0) {
+            if (chunks[a] == 1) {
+#pragma unroll
+                for (int j = 0; j < 27; j++) G[27 * a + j] = acc[j];
+            } else {
+                double* P = partial + 27 * (int64_t(multi_off[a]) + ci);
+#pragma unroll
+                for (int j = 0; j < 27; j++) P[j] = acc[j];
+            }
+        }
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_div_finish(const int32_t* chunks, const int32_t* chunk_off,
-                                                       const double* partial, int64_t n, float* div) {
+// nodes spanning several chunks: one workgroup per such node adds its chunk
+// partials (fixed per-thread strides, fixed tree) for all 27 positions
+__global__ void __launch_bounds__(kBlock) k_g_flag_multi(const int32_t* chunks, int64_t n, int32_t* flag) {
+    const int64_t a = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (a < n) flag[a] = chunks[a] > 1 ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_g_list_multi(const int32_t* flag, const int32_t* pos, int64_t n,
+                                                         int32_t* list) {
+    const int64_t a = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (a < n && flag[a]) list[pos[a]] = int32_t(a);
+}
+
+__global__ void __launch_bounds__(kBlock) k_g_finish(const int32_t* list, const int32_t* chunks,
+                                                     const int32_t* multi_off, const double* partial, double* G) {
+    __shared__ double red[kBlock / 64][27];
+    const int64_t a = list[blockIdx.x];
+    const int32_t c = chunks[a];
+    const double* P = partial + 27 * int64_t(multi_off[a]);
+    double acc[27];
+#pragma unroll
+    for (int j = 0; j < 27; j++) acc[j] = 0.0;
+    for (int32_t i = threadIdx.x; i < c; i += kBlock)
+#pragma unroll
+        for (int j = 0; j < 27; j++) acc[j] += P[27 * int64_t(i) + j];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 27; j++) {
+        double v = acc[j];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+        if (lane == 0) red[wave][j] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 27) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; w++) v += red[w][threadIdx.x];
+        G[27 * a + threadIdx.x] = v;
+    }
+}
+
+// b_i = sum_j G[neighbour_j(i)][j] for the nodes below the leaf level
+__global__ void __launch_bounds__(kBlock) k_div_from_g(const int32_t* neigh, const double* G, int64_t M, int64_t n,
+                                                       float* div) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
-    double v = 0.0;
-    const int32_t o = chunk_off[i];
-    for (int32_t c = 0; c < chunks[i]; c++) v += partial[o + c];
-    div[i] = float(v);
+    double val = 0.0;
+#pragma unroll
+    for (int j = 0; j < 27; j++) {
+        const int32_t nb = neigh[int64_t(j) * M + i];
+        if (nb != -1) val += G[27 * int64_t(nb) + j];
+    }
+    div[i] = float(val);
 }
 
 }  // namespace
@@ -281,37 +339,64 @@ void compute_divergence(prg_ctx* c) {
     }
     const int64_t n = baseD;  // nodes of depths 0 .. D-1
     if (n > 0) {
-        DBuf<int32_t> chunks, coff, cnode, ctot;
-        DBuf<int64_t> terms;
-        DBuf<double> partial;
+        DBuf<int32_t> chunks, multi, coff, moff, cnode, ctot, mtot;
+        DBuf<double> G, partial;
         chunks.reserve(n);
+        multi.reserve(n);
         coff.reserve(n);
-        terms.reserve(n);
+        moff.reserve(n);
         ctot.reserve(1);
-        k_div_terms<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->neigh.get(), c->dnum.get(), M, n, chunks.get(),
-                                                           terms.get());
+        mtot.reserve(1);
+        G.reserve(size_t(27) * n);
+        PRG_HIP(hipMemsetAsync(G.get(), 0, size_t(27) * n * 8, s));
+        k_g_chunks<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->dnum.get(), n, chunks.get(), multi.get());
         PRG_CHECK_LAUNCH();
         scan_exclusive(chunks.get(), coff.get(), n, ctot.get(), s, c->scratch, c->kt);
-        int32_t nchunks = 0;
+        scan_exclusive(multi.get(), moff.get(), n, mtot.get(), s, c->scratch, c->kt);
+        int32_t nchunks = 0, nmulti = 0;
         PRG_HIP(hipMemcpyAsync(&nchunks, ctot.get(), 4, hipMemcpyDeviceToHost, s));
+        PRG_HIP(hipMemcpyAsync(&nmulti, mtot.get(), 4, hipMemcpyDeviceToHost, s));
         PRG_HIP(hipStreamSynchronize(s));
         cnode.reserve(nchunks);
-        partial.reserve(nchunks);
-        k_div_chunk_map<<<grid_for(n, kBlock), kBlock, 0, s>>>(chunks.get(), coff.get(), n, cnode.get());
+        partial.reserve(size_t(27) * std::max(1, nmulti));
+        k_g_chunk_map<<<grid_for(n, kBlock), kBlock, 0, s>>>(chunks.get(), coff.get(), n, cnode.get());
         PRG_CHECK_LAUNCH();
         Bases B{};
         for (int d = 0; d <= D; d++) B.b[d] = c->bases[d];
         B.b[D + 1] = M;
-        const int64_t waves = std::max<int64_t>(1, std::min<int64_t>(int64_t(nchunks), 8192));
+        const int64_t waves = std::max<int64_t>(1, std::min<int64_t>(int64_t(nchunks), 16384));
         {
-            TimedScope ts(c->kt, "div_chunks", 16.0 * double(MD) * double(D));
-            k_div_chunks<<<grid_for(waves * 64, kBlock), kBlock, 0, s>>>(
-                c->key.get(), c->neigh.get(), c->didx.get(), c->dnum.get(), VX, VY, VZ, c->d_div_table.get(),
-                toff.get(), cnode.get(), coff.get(), terms.get(), ctot.get(), M, B, D, partial.get());
+            // every leaf slot read once per coarser depth (key + vector field), G written once
+            TimedScope ts(c->kt, "div_subtree_sums", 16.0 * double(MD) * double(D) + 216.0 * double(n));
+            k_g_sums<<<grid_for(waves * 64, kBlock), kBlock, 0, s>>>(
+                c->key.get(), c->didx.get(), c->dnum.get(), VX, VY, VZ, c->d_div_table.get(), toff.get(), cnode.get(),
+                coff.get(), chunks.get(), moff.get(), ctot.get(), B, D, G.get(), partial.get());
             PRG_CHECK_LAUNCH();
         }
-        k_div_finish<<<grid_for(n, kBlock), kBlock, 0, s>>>(chunks.get(), coff.get(), partial.get(), n, DV);
-        PRG_CHECK_LAUNCH();
+        if (nmulti > 0) {
+            DBuf<int32_t> mflag, mpos, mcnt, mlist;
+            mflag.reserve(n);
+            mpos.reserve(n);
+            mcnt.reserve(1);
+            k_g_flag_multi<<<grid_for(n, kBlock), kBlock, 0, s>>>(chunks.get(), n, mflag.get());
+            PRG_CHECK_LAUNCH();
+            scan_exclusive(mflag.get(), mpos.get(), n, mcnt.get(), s, c->scratch, c->kt);
+            int32_t nm = 0;
+            PRG_HIP(hipMemcpyAsync(&nm, mcnt.get(), 4, hipMemcpyDeviceToHost, s));
+            PRG_HIP(hipStreamSynchronize(s));
+            mlist.reserve(nm);
+            k_g_list_multi<<<grid_for(n, kBlock), kBlock, 0, s>>>(mflag.get(), mpos.get(), n, mlist.get());
+            PRG_CHECK_LAUNCH();
+            TimedScope ts(c->kt, "div_subtree_finish", 216.0 * double(nmulti));
+            k_g_finish<<<unsigned(nm), kBlock, 0, s>>>(mlist.get(), chunks.get(), moff.get(), partial.get(), G.get());
+            PRG_CHECK_LAUNCH();
+            PRG_HIP(hipStreamSynchronize(s));
+        }
+        {
+            TimedScope ts(c->kt, "div_from_subtree_sums", 112.0 * double(n) + 216.0 * double(n));
+            k_div_from_g<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->neigh.get(), G.get(), M, n, DV);
+            PRG_CHECK_LAUNCH();
+        }
         PRG_HIP(hipStreamSynchronize(s));
     }
 }

@@ -58,43 +58,72 @@ __device__ __forceinline__ float basis_value(const float* __restrict__ bf, int64
     return res;
 }
 
-// f(pos) restricted to the ancestor chain of node `nd` at depth `d`.
+// Per-depth offsets of the node arrays and of the parent-neighbour tables.
+struct Levels {
+    int64_t base[kMaxDepth + 1];
+    int64_t pnt_off[kMaxDepth + 1];
+};
+
+// f(pos) restricted to the ancestor chain of node `nd` at depth `d`.  At each
+// ancestor only three basis functions per axis are involved (offsets -1, 0,
+// +1 of the ancestor's cell), so 9 evaluations serve the 27 neighbour
+// products; the neighbours come from the ancestor's parent-neighbour row.
+// Values and summation order are those of the reference's loop.
 __device__ float implicit_value(int64_t nd, int d, float px, float py, float pz, const uint32_t* __restrict__ key,
-                                const int32_t* __restrict__ neigh, const int32_t* __restrict__ parent,
-                                const float* __restrict__ x, const float* __restrict__ bf, int64_t M, int D) {
+                                const int32_t* __restrict__ pnt, const Levels& L, const int32_t* __restrict__ parent,
+                                const float* __restrict__ x, const float* __restrict__ bf, int D) {
     float val = 0.0f;
-    while (nd != -1) {
+    while (d > 0) {
         const uint32_t k = key[nd];
-        const int64_t fb = (int64_t(1) << d) - 1;
-        const int64_t ox = fb + (compact3(k >> 2) >> (D - d)), oy = fb + (compact3(k >> 1) >> (D - d)),
-                      oz = fb + (compact3(k) >> (D - d));
+        const int64_t fb = (int64_t(1) << d) - 1, width = int64_t(1) << d;
+        const int64_t ox = compact3(k >> 2) >> (D - d), oy = compact3(k >> 1) >> (D - d),
+                      oz = compact3(k) >> (D - d);
+        float bx[3], by[3], bz[3];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const int64_t u = ox + a - 1, v = oy + a - 1, w = oz + a - 1;
+            bx[a] = (u >= 0 && u < width) ? basis_value(bf, fb + u, px) : 0.0f;
+            by[a] = (v >= 0 && v < width) ? basis_value(bf, fb + v, py) : 0.0f;
+            bz[a] = (w >= 0 && w < width) ? basis_value(bf, fb + w, pz) : 0.0f;
+        }
+        const int64_t t = nd - L.base[d];
+        const int32_t* row = pnt + 27 * (L.pnt_off[d] + (t >> 3));
+        const int s = int(t & 7);
+        const float* xd = x + L.base[d];
+#pragma unroll
         for (int j = 0; j < 27; j++) {
-            const int32_t nb = neigh[int64_t(j) * M + nd];
-            if (nb == -1) continue;
-            const float vx = basis_value(bf, ox + (j / 9 - 1), px);
-            const float vy = basis_value(bf, oy + ((j / 3) % 3 - 1), py);
-            const float vz = basis_value(bf, oz + (j % 3 - 1), pz);
-            float t = x[nb] * vx;
-            t = t * vy;
-            t = t * vz;
-            val = val + t;
+            const int vx = ((s >> 2) & 1) + (j / 9 - 1), vy = ((s >> 1) & 1) + ((j / 3) % 3 - 1),
+                      vz = (s & 1) + (j % 3 - 1);
+            const int32_t cb = row[9 * ((vx >> 1) + 1) + 3 * ((vy >> 1) + 1) + ((vz >> 1) + 1)];
+            if (cb < 0) continue;
+            float tt = xd[cb + (((vx & 1) << 2) | ((vy & 1) << 1) | (vz & 1))] * bx[j / 9];
+            tt = tt * by[(j / 3) % 3];
+            tt = tt * bz[j % 3];
+            val = val + tt;
         }
         nd = parent[nd];
         d--;
+    }
+    // the root's only neighbour is itself (main.cu:803-808)
+    {
+        float tt = x[0] * basis_value(bf, 0, px);
+        tt = tt * basis_value(bf, 0, py);
+        tt = tt * basis_value(bf, 0, pz);
+        val = val + tt;
     }
     return val;
 }
 
 __global__ void __launch_bounds__(kBlock) k_point_values(const float* px, const float* py, const float* pz,
                                                          const int32_t* point_leaf, int64_t n, int64_t baseD,
-                                                         const uint32_t* key, const int32_t* neigh,
+                                                         const uint32_t* key, const int32_t* pnt, Levels L,
                                                          const int32_t* parent, const float* x, const float* bf,
-                                                         int64_t M, int D, float* pvals, double* partial) {
+                                                         int D, float* pvals, double* partial) {
     __shared__ double lds[kBlock / 64];
     const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     double v = 0.0;
     if (r < n) {
-        const float f = implicit_value(baseD + point_leaf[r], D, px[r], py[r], pz[r], key, neigh, parent, x, bf, M, D);
+        const float f = implicit_value(baseD + point_leaf[r], D, px[r], py[r], pz[r], key, pnt, L, parent, x, bf, D);
         pvals[r] = f;
         v = double(f);
     }
@@ -158,12 +187,20 @@ __global__ void __launch_bounds__(kBlock) k_corner_owner(const uint32_t* key, co
     oref[g] = int32_t(8 * (owner - baseD) + jo);
 }
 
-__global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* own, const int32_t* vid, const uint32_t* key,
-                                                         const int32_t* neigh, const int32_t* parent, const float* x,
-                                                         const float* bf, int64_t M, int64_t baseD, int64_t MD, int D,
-                                                         float iso, float* vval) {
+// owned corners, compacted: vlist[vid] = 8 * leaf slot + corner
+__global__ void __launch_bounds__(kBlock) k_corner_list(const int32_t* own, const int32_t* vid, int64_t n8,
+                                                        int32_t* vlist) {
     const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (g >= 8 * MD || !own[g]) return;
+    if (g < n8 && own[g]) vlist[vid[g]] = int32_t(g);
+}
+
+__global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* vlist, int64_t V, const uint32_t* key,
+                                                         const int32_t* pnt, Levels L, const int32_t* parent,
+                                                         const float* x, const float* bf, int64_t baseD, int D,
+                                                         float iso, float* vval) {
+    const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (v >= V) return;
+    const int64_t g = vlist[v];
     const int64_t t = g >> 3;
     const int j = int(g & 7);
     const int64_t i = baseD + t;
@@ -174,8 +211,8 @@ __global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* own, con
     const float vx = c[0] + float(2 * (j & 1) - 1) * hw;
     const float vy = c[1] + float(2 * ((j & 2) >> 1) - 1) * hw;
     const float vz = c[2] + float(2 * ((j & 4) >> 2) - 1) * hw;
-    const float f = implicit_value(i, D, vx, vy, vz, key, neigh, parent, x, bf, M, D);
-    vval[vid[g]] = f - iso;
+    const float f = implicit_value(i, D, vx, vy, vz, key, pnt, L, parent, x, bf, D);
+    vval[v] = f - iso;
 }
 
 // values of each leaf node's corners in square numbering (vertices[], main.cu:1659-1668)
@@ -299,6 +336,15 @@ __global__ void __launch_bounds__(kBlock) k_tri_emit(const float* cval, const in
     for (int k = 0; k < n; k++) mesh_t[o + k] = evid[eref[12 * t + c_tri[cube][k]]];
 }
 
+Levels levels_of(const prg_ctx* c) {
+    Levels L{};
+    for (int d = 0; d <= c->D; d++) {
+        L.base[d] = c->bases[d];
+        L.pnt_off[d] = c->pnt_off[d];
+    }
+    return L;
+}
+
 }  // namespace
 
 void compute_isovalue(prg_ctx* c) {
@@ -313,9 +359,9 @@ void compute_isovalue(prg_ctx* c) {
     {
         TimedScope ts(c->kt, "point_values", 24.0 * double(n));
         k_point_values<<<nb, kBlock, 0, s>>>(c->px.get(), c->py.get(), c->pz.get(), c->point_leaf.get(), n,
-                                             c->bases[c->D], c->key.get(), c->neigh.get(), c->parent.get(),
-                                             c->x.get(), c->d_base_fn.get(), c->M, c->D, c->pvals.reserve(n),
-                                             part.get());
+                                             c->bases[c->D], c->key.get(), c->pnt.get(), levels_of(c),
+                                             c->parent.get(), c->x.get(), c->d_base_fn.get(), c->D,
+                                             c->pvals.reserve(n), part.get());
         PRG_CHECK_LAUNCH();
     }
     k_sum_partials<<<1, 64, 0, s>>>(part.get(), nb, tot.get());
@@ -356,12 +402,15 @@ void extract_leaf_mesh(prg_ctx* c) {
     {
         StageTimer st(c, PRG_STAGE_MC_EVAL);
         vval.reserve(V);
+        DBuf<int32_t> vlist;
+        vlist.reserve(V);
+        k_corner_list<<<grid_for(8 * MD, kBlock), kBlock, 0, s>>>(own.get(), vid.get(), 8 * MD, vlist.get());
+        PRG_CHECK_LAUNCH();
         {
             TimedScope ts(kt, "corner_value", 24.0 * double(V));
-            k_corner_value<<<grid_for(8 * MD, kBlock), kBlock, 0, s>>>(own.get(), vid.get(), c->key.get(),
-                                                                       c->neigh.get(), c->parent.get(), c->x.get(),
-                                                                       c->d_base_fn.get(), M, baseD, MD, D, c->iso,
-                                                                       vval.get());
+            k_corner_value<<<grid_for(V, kBlock), kBlock, 0, s>>>(vlist.get(), V, c->key.get(), c->pnt.get(),
+                                                                  levels_of(c), c->parent.get(), c->x.get(),
+                                                                  c->d_base_fn.get(), baseD, D, c->iso, vval.get());
             PRG_CHECK_LAUNCH();
         }
         k_corner_square<<<grid_for(8 * MD, kBlock), kBlock, 0, s>>>(oref.get(), vid.get(), vval.get(), MD, CV);

@@ -293,6 +293,29 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(const uint32_t* key, const
     }
 }
 
+// Parent-neighbour table of one depth: for sibling block b, the level-local
+// child base of each of the parent's 27 neighbours (-1 where the neighbour
+// does not exist or is empty).  Neighbour j of slot s in the block is
+// pnt[b][LUTparent[s][j]] + LUTchild[s][j] - the same lookup computeNeighbor
+// performs (main.cu:492-509), kept per block so the solver reads 13.5 bytes
+// per node instead of 108.
+__global__ void __launch_bounds__(kBlock) k_parent_neighbor_table(const int32_t* parent, const int32_t* child_base,
+                                                                  const int32_t* neigh, int64_t M, int64_t base,
+                                                                  int64_t nblocks, int32_t* pnt) {
+    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (g >= nblocks * 27) return;
+    const int64_t b = g / 27;
+    const int j = int(g % 27);
+    const int64_t P = parent[base + 8 * b];
+    const int32_t np = neigh[int64_t(j) * M + P];
+    int32_t v = -1;
+    if (np != -1) {
+        const int32_t cb = child_base[np];
+        if (cb != -1) v = int32_t(cb - base);
+    }
+    pnt[g] = v;
+}
+
 }  // namespace
 
 void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n, bool on_device) {
@@ -520,6 +543,17 @@ void build_octree(prg_ctx* c) {
     for (int d = 1; d <= D; d++) {
         TimedScope ts(kt, "neighbors", 128.0 * double(counts[d]));
         k_neighbors<<<grid_for(counts[d], kBlock), kBlock, 0, s>>>(K, PA, CB, NB, M, bases[d], counts[d], D, d);
+        PRG_CHECK_LAUNCH();
+    }
+    // 5. parent-neighbour tables (depths 1..D)
+    c->pnt_off.assign(D + 2, 0);
+    for (int d = 1; d <= D; d++) c->pnt_off[d + 1] = c->pnt_off[d] + counts[d] / 8;
+    int32_t* PT = c->pnt.reserve(size_t(27) * std::max<int64_t>(1, c->pnt_off[D + 1]));
+    for (int d = 1; d <= D; d++) {
+        const int64_t nb = counts[d] / 8;
+        TimedScope ts(kt, "parent_neighbor_table", 120.0 * double(nb));
+        k_parent_neighbor_table<<<grid_for(27 * nb, kBlock), kBlock, 0, s>>>(PA, CB, NB, M, bases[d], nb,
+                                                                           PT + 27 * c->pnt_off[d]);
         PRG_CHECK_LAUNCH();
     }
     PRG_HIP(hipStreamSynchronize(s));
