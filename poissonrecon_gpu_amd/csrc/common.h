@@ -104,6 +104,11 @@ struct KernelTimer {
         if (idx < 0) return;
         PRG_HIP(hipEventRecord(pending[idx].b, stream));
     }
+    // Files a recorded launch under another name (e.g. one that found its
+    // work already finished), before flush().
+    void relabel(int idx, const char* name) {
+        if (idx >= 0 && idx < int(pending.size())) pending[idx].name = name;
+    }
     // Resolves recorded events (synchronises on them).
     void flush() {
         for (auto& p : pending) {

@@ -38,6 +38,11 @@ constexpr int kBlock = 256;
 constexpr int kMaxBlocks = 2048;
 constexpr int kSmallBlock = 1024;
 constexpr int64_t kSmallRows = 4096;
+// Algorithmic bytes per SpMV row under the reference's CSR-free representation
+// (int32 neighbour indices, FP32 values): 108 B of neighbour indices, p_i read,
+// (Ap)_i written.  The kernels here read 27 parent-neighbour entries per sibling
+// block instead (13.5 B per row), so their compulsory traffic is lower.
+constexpr double kSpmvRowBytes = 116.0;
 
 struct Stencil {
     float L[27];
@@ -501,23 +506,29 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             float* pb[2] = {P0.get(), P1.get()};
             PRG_HIP(hipEventRecord(e0, s));
             {
-                TimedScope ts(c->kt, "cg_init", 37.5 * double(n));
+                TimedScope ts(c->kt, "cg_init", kSpmvRowBytes * double(n));
                 k_cg_init<<<nbk, kBlock, 0, s>>>(pnt, n, S, B, Xd, R.get(), pb[0], part.get(), part2.get(), st.get(),
                                                  tol, fixed);
                 PRG_CHECK_LAUNCH();
             }
             int issued = 0;
             int batch = 24;
+            // (timer record, iteration) of this batch's launches: those past the
+            // final iteration return at once and are filed as "cg_idle"
+            std::vector<std::pair<int, int>> launched;
+            const char* spmv_name = d == D ? "stencil_spmv_dot_finest" : "stencil_spmv_dot";
             while (true) {
+                launched.clear();
                 for (int k = 0; k < batch; k++) {
                     const int it = issued + k + 1;  // iteration number, 1-based
                     const bool first = (it == 1);
                     float* pnew = pb[(it - 1) & 1];
                     const float* pold = pb[it & 1];
                     {
-                        // per row: 27/8 parent-neighbour entries (13.5 B), own r and p_{k-1} (8 B),
-                        // p_k and q written (8 B), neighbour r/p gathers counted once (8 B)
-                        TimedScope ts(c->kt, "stencil_spmv_dot", 37.5 * double(n));
+                        // algorithmic bytes of one SpMV row in the reference's representation:
+                        // 27 int32 neighbour indices, p_i read, (Ap)_i written (116 B)
+                        TimedScope ts(c->kt, spmv_name, kSpmvRowBytes * double(n));
+                        launched.emplace_back(ts.idx, it);
                         k_cg_spmv<<<nbk_spmv, kBlock, 0, s>>>(pnt, n, S, R.get(), pold, pnew, Q.get(),
                                                               first ? 1 : 0, part.get(), st.get());
                         PRG_CHECK_LAUNCH();
@@ -525,12 +536,14 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                     const bool do_refresh = (it % refresh) == 0;
                     {
                         TimedScope ts(c->kt, "cg_update", 24.0 * double(n));
+                        launched.emplace_back(ts.idx, it);
                         k_cg_update<<<nbk, kBlock, 0, s>>>(n, pnew, Q.get(), Xd, R.get(), part.get(), st.get(), tol,
                                                            fixed, max_iters, do_refresh ? 1 : 0);
                         PRG_CHECK_LAUNCH();
                     }
                     if (do_refresh) {
-                        TimedScope ts(c->kt, "cg_refresh", 29.5 * double(n));
+                        TimedScope ts(c->kt, "cg_refresh", kSpmvRowBytes * double(n));
+                        launched.emplace_back(ts.idx, it);
                         k_cg_refresh<<<nbk, kBlock, 0, s>>>(pnt, n, S, B, Xd, R.get(), part.get(), st.get(), tol,
                                                             fixed, max_iters);
                         PRG_CHECK_LAUNCH();
@@ -539,11 +552,13 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                 issued += batch;
                 PRG_HIP(hipMemcpyAsync(hst, st.get(), sizeof(CgState), hipMemcpyDeviceToHost, s));
                 PRG_HIP(hipStreamSynchronize(s));
+                for (const auto& l : launched)
+                    if (l.second > hst->iters) c->kt.relabel(l.first, "cg_idle");
                 if (hst->done || issued >= max_iters) break;
                 batch = 6;
             }
             {
-                TimedScope ts(c->kt, "cg_true_residual", 29.5 * double(n));
+                TimedScope ts(c->kt, "cg_true_residual", kSpmvRowBytes * double(n));
                 k_true_residual<<<nbk, kBlock, 0, s>>>(pnt, n, S, B, Xd, part.get(), st.get(), outv.get() + 4 * d);
                 PRG_CHECK_LAUNCH();
             }
