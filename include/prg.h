@@ -196,6 +196,11 @@ void prg_mesh_release(prg_mesh* mesh);
 prg_status prg_extract_mesh_into(prg_ctx* ctx, int32_t subdivide, float* vertices, int64_t vcap, int32_t* triangles,
                                  int64_t tcap, prg_mesh* out);
 
+/* After a prg_extract_mesh* call with subdivide != 0: the number of coarse
+ * leaves that were refined (SubdivideNum, main.cu:3826-3834) and of the
+ * virtual depth-D leaves marching cubes ran over. */
+prg_status prg_subdivision_info(prg_ctx* ctx, int64_t* n_roots, int64_t* n_virtual_leaves);
+
 /* Whole pipeline: build, divergence, solve, isovalue, mesh. */
 prg_status prg_reconstruct(prg_ctx* ctx, int32_t subdivide, prg_mesh* out, prg_solve_stats* per_depth);
 

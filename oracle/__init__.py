@@ -45,6 +45,8 @@ def lib() -> C.CDLL:
             "orc_set_isovalue": (None, [P, C.c_float]),
             "orc_extract_leaf_mesh": (C.c_int32, [P, P, P]),
             "orc_copy_mesh": (C.c_int32, [P, P, P]),
+            "orc_extract_subdivision": (C.c_int32, [P, P, P]),
+            "orc_subdivision_info": (None, [P, P, P]),
             "orc_buffer_size": (C.c_int64, [P, C.c_int32]),
             "orc_read_buffer": (C.c_int32, [P, C.c_int32, P, C.c_int64]),
             "orc_write_buffer": (C.c_int32, [P, C.c_int32, P, C.c_int64]),
@@ -133,6 +135,23 @@ class Oracle:
         t = np.zeros((int(nt[0]), 3), np.int32)
         lib().orc_copy_mesh(self._h, _p(v), _p(t))
         return v, t
+
+    def extract_subdivision(self):
+        """Appends the coarse-leaf subdivision batches; returns the whole mesh."""
+        nv = np.zeros(1, np.int64)
+        nt = np.zeros(1, np.int64)
+        assert lib().orc_extract_subdivision(self._h, _p(nv), _p(nt)) == 0
+        v = np.zeros((int(nv[0]), 3), np.float32)
+        t = np.zeros((int(nt[0]), 3), np.int32)
+        lib().orc_copy_mesh(self._h, _p(v), _p(t))
+        return v, t
+
+    def subdivision_info(self):
+        """(number of subdivided coarse leaves, number of virtual leaves)."""
+        a = np.zeros(1, np.int64)
+        b = np.zeros(1, np.int64)
+        lib().orc_subdivision_info(self._h, _p(a), _p(b))
+        return int(a[0]), int(b[0])
 
     def read_buffer(self, which: int) -> np.ndarray:
         n = lib().orc_buffer_size(self._h, which)

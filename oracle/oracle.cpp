@@ -40,6 +40,13 @@ const int kTriTable[256][16] = {PRG_MC_TRIANGLE_TABLE_DATA};
 // each edge kind.
 const int kEdgeVertex[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 3}, {1, 2},
                                 {4, 7}, {5, 6}, {0, 4}, {1, 5}, {3, 7}, {2, 6}};
+// parentFaceKind (MarchingCubes.cuh:708-717), indexed [sonKey][face] literally
+// (Q22: the rows are laid out with x in bit 0 while sonKey has x in bit 2).
+const int kParentFaceKind[8][6] = {{0, -1, 2, -1, 4, -1}, {-1, 1, 2, -1, 4, -1}, {0, -1, -1, 3, 4, -1},
+                                   {-1, 1, -1, 3, 4, -1}, {0, -1, 2, -1, -1, 5}, {-1, 1, 2, -1, -1, 5},
+                                   {0, -1, -1, 3, -1, 5}, {0, 1, -1, 3, -1, 5}};
+// faceEdges (MarchingCubes.cuh:734-741): the four edge kinds on each face kind.
+const int kFaceEdges[6][4] = {{4, 6, 8, 10}, {5, 7, 9, 11}, {0, 2, 8, 9}, {1, 3, 10, 11}, {0, 1, 4, 5}, {2, 3, 6, 7}};
 // childrenVertexKind (MarchingCubes.cuh:720-723), used literally (Q9).
 const int kChildrenVertexKind[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 
@@ -103,6 +110,16 @@ struct orc_state {
 
     std::vector<float> mesh_v;
     std::vector<int32_t> mesh_t;
+    int64_t leaf_nv = 0, leaf_nt = 0;  // the leaf-level batch's share of mesh_v / mesh_t
+
+    // all-depth marching-cubes state kept by the leaf-level pass for the
+    // coarse-leaf subdivision: node centres, vertices[] (1-based vertex ids,
+    // square numbering), vertex values, and per leaf-level node its cube
+    std::vector<float> node_centers;
+    std::vector<int32_t> vertices;
+    std::vector<float> vval;
+    std::vector<int32_t> leaf_cube;
+    int64_t sub_roots = 0, sub_leaves = 0;
 
     void ensure_fs() {
         if (fs) return;
@@ -827,7 +844,8 @@ int32_t orc_extract_leaf_mesh(orc_state* s, int64_t* n_vertices, int64_t* n_tria
         }
     const int64_t V = int64_t(vnode.size());
     std::vector<float> vpos(3 * V);
-    std::vector<int32_t> vertices(8 * M, 0);
+    std::vector<int32_t>& vertices = s->vertices;
+    vertices.assign(8 * M, 0);
     // maintainVertexNodePointerNonAtomic (main.cu:1640-1681)
 #pragma omp parallel for schedule(dynamic, 1024)
     for (int64_t v = 0; v < V; v++) {
@@ -925,7 +943,8 @@ int32_t orc_extract_leaf_mesh(orc_state* s, int64_t* n_vertices, int64_t* n_tria
         }
     }
     // computeVertexImplicitFunctionValue (main.cu:2259-2326) for every vertex
-    std::vector<float> vval(V);
+    std::vector<float>& vval = s->vval;
+    vval.assign(V, 0.0f);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int64_t v = 0; v < V; v++) {
         const float* pos = &vpos[3 * v];
@@ -984,10 +1003,12 @@ int32_t orc_extract_leaf_mesh(orc_state* s, int64_t* n_vertices, int64_t* n_tria
     }
     // generateTriNums + generateTrianglePos (main.cu:2540-2560,2699-2757)
     s->mesh_t.clear();
+    s->leaf_cube.assign(MD, 0);
     for (int64_t t = 0; t < MD; t++) {
         int cube = 0;
         for (int j = 0; j < 8; j++)
             if (s->leaf_corner_vals[8 * t + j] < 0) cube |= 1 << j;
+        s->leaf_cube[t] = cube;
         const int nt = tri_count(cube);
         for (int k = 0; k < 3 * nt; k++) {
             const int ek = kTriTable[cube][k];
@@ -996,7 +1017,406 @@ int32_t orc_extract_leaf_mesh(orc_state* s, int64_t* n_vertices, int64_t* n_tria
     }
     *n_vertices = nv;
     *n_triangles = int64_t(s->mesh_t.size()) / 3;
+    s->leaf_nv = *n_vertices;
+    s->leaf_nt = *n_triangles;
+    s->node_centers.swap(C);
     return 0;
+}
+
+// ---- A19 coarse-leaf subdivision (main.cu:2957-3217, driver :3799-4561) ------------
+//
+// Restated with the reference's data structures: faces at every depth and their
+// surface-intersection marks, coarse leaves selected by ProcessLeafNodesAtOtherDepth
+// + ifSubdivide, then per group (each root shallower than depth 3 alone, then
+// all roots of one depth 3..D-1 together) an explicit virtual tree of complete
+// subtrees down to D whose neighbours are found through the parents (LUTparent /
+// LUTchild), with the real parent's child pointer redirected to the virtual
+// root, and marching cubes over the virtual leaves.  Each group is one mesh
+// batch appended after the leaf-level batch.
+namespace {
+
+struct VTree {
+    // virtual nodes; global id = M + index
+    std::vector<uint32_t> key;
+    std::vector<int> depth;
+    std::vector<int64_t> parent;
+    std::vector<int64_t> children;  // 8 per node, -1 at depth D
+    std::vector<int64_t> neighs;    // 27 per node
+    std::vector<int64_t> real_root; // real index of the coarse leaf a virtual root replaces, else -1
+    std::vector<int64_t> level_begin, level_end;
+};
+
+// contribution of one real node nb to f at pos (the body of the neighbour loop
+// of computeSubdivideVertexImplicitFunctionValue, main.cu:2346-2372)
+inline float real_contrib(const orc_state* s, int64_t nb, const float* pos, float val) {
+    const int d = s->depth_of[nb];
+    int64_t o[3];
+    cell_offsets(s->key[nb], d, s->D, o);
+    const int64_t fb = (1LL << d) - 1;
+    float vx = orc::cpoly_value(s->base_functions[fb + o[0]], pos[0]);
+    float vy = orc::cpoly_value(s->base_functions[fb + o[1]], pos[1]);
+    float vz = orc::cpoly_value(s->base_functions[fb + o[2]], pos[2]);
+    float t = s->x[nb] * vx;
+    t = t * vy;
+    t = t * vz;
+    return val + t;
+}
+
+// One group of subdivision roots (real node indices, in NodeArray order).
+// Appends the group's vertices and triangles to the mesh.
+void process_group(orc_state* s, const std::vector<int64_t>& roots) {
+    const int D = s->D;
+    const int64_t M = s->M;
+    const int r = s->depth_of[roots[0]];
+    VTree T;
+    std::map<int64_t, int64_t> redirect;  // real parent * 8 + child slot -> virtual id
+    // singleRebuildArray / wholeRebuildArray: levels r..D, each in (root order, Morton) order
+    for (int l = r; l <= D; l++) {
+        T.level_begin.push_back(int64_t(T.key.size()));
+        if (l == r) {
+            for (int64_t q : roots) {
+                const int64_t id = int64_t(T.key.size());
+                T.key.push_back(s->key[q]);
+                T.depth.push_back(r);
+                T.parent.push_back(s->parent[q]);
+                T.real_root.push_back(q);
+                const int son = int((s->key[q] >> (3 * (D - r))) & 7u);
+                redirect[int64_t(s->parent[q]) * 8 + son] = M + id;
+            }
+        } else {
+            const int64_t pb = T.level_begin[l - r - 1], pe = T.level_end[l - r - 1];
+            for (int64_t pi = pb; pi < pe; pi++)
+                for (int k = 0; k < 8; k++) {
+                    T.key.push_back(T.key[pi] | (uint32_t(k) << (3 * (D - l))));
+                    T.depth.push_back(l);
+                    T.parent.push_back(M + pi);
+                    T.real_root.push_back(-1);
+                }
+        }
+        T.level_end.push_back(int64_t(T.key.size()));
+    }
+    const int64_t NV = int64_t(T.key.size());
+    T.children.assign(8 * NV, -1);
+    for (int l = r; l < D; l++) {
+        const int64_t b = T.level_begin[l - r], e = T.level_end[l - r], cb = T.level_end[l - r];
+        for (int64_t i = b; i < e; i++)
+            for (int k = 0; k < 8; k++) T.children[8 * i + k] = M + cb + 8 * (i - b) + k;
+    }
+    auto child_of = [&](int64_t id, int k) -> int64_t {
+        if (id >= M) return T.children[8 * (id - M) + k];
+        auto it = redirect.find(id * 8 + k);
+        if (it != redirect.end()) return it->second;
+        return s->children[8 * id + k];
+    };
+    auto neigh_of = [&](int64_t id, int j) -> int64_t {
+        return id >= M ? T.neighs[27 * (id - M) + j] : int64_t(s->neighs[27 * id + j]);
+    };
+    // computeRebuildNeighbor (main.cu:3157-3217), depth by depth
+    T.neighs.assign(27 * NV, -1);
+    for (int l = r; l <= D; l++)
+        for (int64_t i = T.level_begin[l - r]; i < T.level_end[l - r]; i++) {
+            const int son = int((T.key[i] >> (3 * (D - l))) & 7u);
+            for (int j = 0; j < 27; j++) {
+                const int64_t pn = neigh_of(T.parent[i], kLuts.parent[son][j]);
+                T.neighs[27 * i + j] = (pn == -1) ? -1 : child_of(pn, kLuts.child[son][j]);
+            }
+        }
+    std::vector<float> VC(3 * NV);
+    for (int64_t i = 0; i < NV; i++) node_center(T.key[i], T.depth[i], D, &VC[3 * i]);
+    auto center_of = [&](int64_t id) -> const float* {
+        return id >= M ? &VC[3 * (id - M)] : &s->node_centers[3 * id];
+    };
+    const int64_t lb = T.level_begin[D - r], le = T.level_end[D - r], NL = le - lb;
+    const float hw = 1.0f / float(1 << (D + 1));
+    const float W = 1.0f / float(1 << D);
+    const float W2 = W * W;
+    // initSubdivideVertexOwner (main.cu:1500-1568): virtual owners only
+    std::vector<int64_t> vowner;
+    std::vector<int> vkind;
+    std::vector<float> vpos;
+    for (int64_t t = 0; t < NL; t++) {
+        const int64_t i = lb + t;
+        for (int j = 0; j < 8; j++) {
+            float vp[3];
+            vp[0] = VC[3 * i + 0] + float(2 * (j & 1) - 1) * hw;
+            vp[1] = VC[3 * i + 1] + float(2 * ((j & 2) >> 1) - 1) * hw;
+            vp[2] = VC[3 * i + 2] + float(2 * ((j & 4) >> 2) - 1) * hw;
+            uint32_t best = 0x7fffffffu;
+            int64_t owner = -1;
+            for (int k = 0; k < 27; k++) {
+                const int64_t nb = T.neighs[27 * i + k];
+                if (nb == -1 || nb < M) continue;
+                if (!(sqdist(vp, center_of(nb)) < W2)) continue;
+                if (best > T.key[nb - M]) {
+                    best = T.key[nb - M];
+                    owner = nb;
+                }
+            }
+            if (owner == M + i) {
+                vowner.push_back(owner);
+                vkind.push_back(j);
+                vpos.insert(vpos.end(), vp, vp + 3);
+            }
+        }
+    }
+    const int64_t V = int64_t(vowner.size());
+    // maintainSubdivideVertexNodePointer (main.cu:1683-1740)
+    std::vector<int32_t> vert(8 * NL, 0);
+    for (int64_t v = 0; v < V; v++) {
+        const int64_t o = vowner[v] - M;
+        const float* vp = &vpos[3 * v];
+        for (int k = 0; k < 27; k++) {
+            const int64_t nb = T.neighs[27 * o + k];
+            if (nb == -1) continue;
+            const float* nc = center_of(nb);
+            if (!(sqdist(vp, nc) < W2)) continue;
+            int idx = 0;
+            if (nc[0] - vp[0] < 0) idx |= 1;
+            if (nc[2] - vp[2] < 0) idx |= 4;
+            if (nc[1] - vp[1] < 0) idx += (idx & 1) ? 1 : 3;
+            if (nb >= M) vert[8 * (nb - M - lb) + idx] = int32_t(v + 1);
+        }
+    }
+    // initSubdivideEdgeArray + maintainSubdivideEdgeNodePointer (main.cu:1867-2222)
+    auto edge_center = [&](int64_t i, int j, float ec[3]) {
+        const int orient = j >> 2;
+        const int off[2] = {j & 1, (j & 2) >> 1};
+        int dim = 0;
+        for (int a = 0; a < 3; a++) {
+            int m = 0;
+            if (a != orient) m = 2 * off[dim++] - 1;
+            ec[a] = VC[3 * i + a] + float(m) * hw;
+        }
+    };
+    std::vector<int64_t> eowner;
+    std::vector<int> ekind;
+    for (int64_t t = 0; t < NL; t++) {
+        const int64_t i = lb + t;
+        for (int j = 0; j < 12; j++) {
+            float ec[3];
+            edge_center(i, j, ec);
+            uint32_t best = 0x7fffffffu;
+            int64_t owner = -1;
+            for (int k = 0; k < 27; k++) {
+                const int64_t nb = T.neighs[27 * i + k];
+                if (nb == -1 || nb < M) continue;
+                if (!(sqdist(ec, center_of(nb)) < W2)) continue;
+                if (best > T.key[nb - M]) {
+                    best = T.key[nb - M];
+                    owner = nb;
+                }
+            }
+            if (owner == M + i) {
+                eowner.push_back(owner);
+                ekind.push_back(j);
+            }
+        }
+    }
+    const int64_t E = int64_t(eowner.size());
+    std::vector<int32_t> edg(12 * NL, 0);
+    for (int64_t e = 0; e < E; e++) {
+        const int64_t o = eowner[e] - M;
+        const int j = ekind[e];
+        const int orient = j >> 2;
+        float ec[3];
+        edge_center(o, j, ec);
+        for (int k = 0; k < 27; k++) {
+            const int64_t nb = T.neighs[27 * o + k];
+            if (nb == -1) continue;
+            const float* nc = center_of(nb);
+            if (!(sqdist(ec, nc) < W2)) continue;
+            int idx = orient << 2, dim = 0;
+            for (int a = 0; a < 3; a++)
+                if (a != orient) {
+                    if (nc[a] - ec[a] < 0) idx |= (1 << dim);
+                    ++dim;
+                }
+            if (nb >= M) edg[12 * (nb - M - lb) + idx] = int32_t(e + 1);
+        }
+    }
+    // computeSubdivideVertexImplicitFunctionValue (main.cu:2328-2442): real
+    // neighbours only (virtual coefficients are 0), a virtual root standing
+    // for its coarse leaf
+    std::vector<float> val(V);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t v = 0; v < V; v++) {
+        const float* pos = &vpos[3 * v];
+        float f = 0.0f;
+        int64_t nd = vowner[v];
+        while (nd != -1) {
+            for (int j = 0; j < 27; j++) {
+                int64_t nb = neigh_of(nd, j);
+                if (nb == -1) continue;
+                if (nb >= M && T.real_root[nb - M] >= 0) nb = T.real_root[nb - M];
+                if (nb >= M) continue;
+                f = real_contrib(s, nb, pos, f);
+            }
+            nd = nd >= M ? T.parent[nd - M] : int64_t(s->parent[nd]);
+        }
+        val[v] = f - s->iso;
+    }
+    // generateSubdivideVexNums + scan, generateSubdivideIntersectionPoint
+    std::vector<int32_t> vexAddr(E);
+    int64_t nv = 0;
+    for (int64_t e = 0; e < E; e++) {
+        const int64_t o = eowner[e] - M - lb;
+        const int kind = ekind[e];
+        const float f1 = val[vert[8 * o + kEdgeVertex[kind][0]] - 1];
+        const float f2 = val[vert[8 * o + kEdgeVertex[kind][1]] - 1];
+        vexAddr[e] = int32_t(nv);
+        if (f1 * f2 <= 0) nv++;
+    }
+    if (nv == 0) return;  // main.cu:4063-4071
+    const int64_t voff = int64_t(s->mesh_v.size()) / 3;
+    s->mesh_v.resize(s->mesh_v.size() + 3 * nv);
+    for (int64_t e = 0; e < E; e++) {
+        const int64_t o = eowner[e] - M - lb;
+        const int kind = ekind[e];
+        const int v1 = vert[8 * o + kEdgeVertex[kind][0]] - 1;
+        const int v2 = vert[8 * o + kEdgeVertex[kind][1]] - 1;
+        const float f1 = val[v1], f2 = val[v2];
+        if (!(f1 * f2 <= 0)) continue;
+        const int dim = kind >> 2;
+        float out[3];
+        for (int a = 0; a < 3; a++)
+            if (a != dim) out[a] = vpos[3 * v1 + a];
+        const float pivot = f1 / (f1 - f2);
+        const float another = 1 - pivot;
+        const float t1 = vpos[3 * v2 + dim] * pivot;
+        const float t2 = vpos[3 * v1 + dim] * another;
+        out[dim] = t1 + t2;
+        for (int a = 0; a < 3; a++) s->mesh_v[3 * (voff + vexAddr[e]) + a] = out[a];
+    }
+    // generateTriNums(EasyOctNode) + generateSubdivideTrianglePos, insertTriangle
+    for (int64_t t = 0; t < NL; t++) {
+        int cube = 0;
+        for (int j = 0; j < 8; j++)
+            if (val[vert[8 * t + j] - 1] < 0) cube |= 1 << j;
+        const int nt = tri_count(cube);
+        for (int k = 0; k < 3 * nt; k++)
+            s->mesh_t.push_back(int32_t(voff + vexAddr[edg[12 * t + kTriTable[cube][k]] - 1]));
+    }
+}
+
+}  // namespace
+
+int32_t orc_extract_subdivision(orc_state* s, int64_t* n_vertices, int64_t* n_triangles) {
+    if (s->node_centers.empty()) return 6;  // needs orc_extract_leaf_mesh first
+    const int D = s->D;
+    const int64_t M = s->M, baseD = s->bases[D], MD = s->MD();
+    const std::vector<float>& C = s->node_centers;
+    // initFaceArray (main.cu:2827-2900) over every node; owner = smallest key
+    // among the nodes whose centre is within Width of the face centre
+    std::vector<int64_t> fowner;
+    std::vector<int> fkind, fhpf;
+    for (int64_t i = 0; i < M; i++) {
+        const int d = s->depth_of[i];
+        const float hw = 1.0f / float(1 << (d + 1));
+        const float W = 1.0f / float(1 << d);
+        const float W2 = W * W;
+        const int son = int((s->key[i] >> (3 * (D - d))) & 7u);
+        for (int j = 0; j < 6; j++) {
+            const int o = j >> 1, multi = 2 * (j & 1) - 1;
+            float fc[3] = {C[3 * i], C[3 * i + 1], C[3 * i + 2]};
+            fc[o] += float(multi) * hw;
+            uint32_t best = 0x7fffffffu;
+            int64_t owner = -1;
+            for (int k = 0; k < 27; k++) {
+                const int32_t nb = s->neighs[27 * i + k];
+                if (nb == -1 || !(sqdist(fc, &C[3 * int64_t(nb)]) < W2)) continue;
+                if (best > s->key[nb]) {
+                    best = s->key[nb];
+                    owner = nb;
+                }
+            }
+            if (owner == i) {
+                fowner.push_back(i);
+                fkind.push_back(j);
+                fhpf.push_back(s->parent[i] == -1 ? -1 : (kParentFaceKind[son][j] != -1 ? 1 : -1));
+            }
+        }
+    }
+    // maintainFaceNodePointer (main.cu:2908-2955)
+    const int64_t F = int64_t(fowner.size());
+    std::vector<int32_t> faces(6 * M, 0);
+    for (int64_t f = 0; f < F; f++) {
+        const int64_t i = fowner[f];
+        const int d = s->depth_of[i];
+        const float hw = 1.0f / float(1 << (d + 1));
+        const float W = 1.0f / float(1 << d);
+        const float W2 = W * W;
+        const int j = fkind[f], o = j >> 1, multi = 2 * (j & 1) - 1;
+        float fc[3] = {C[3 * i], C[3 * i + 1], C[3 * i + 2]};
+        fc[o] += float(multi) * hw;
+        for (int k = 0; k < 27; k++) {
+            const int32_t nb = s->neighs[27 * i + k];
+            if (nb == -1) continue;
+            const float* nc = &C[3 * int64_t(nb)];
+            if (!(sqdist(fc, nc) < W2)) continue;
+            int idx = o << 1;
+            if (nc[o] - fc[o] < 0) ++idx;
+            faces[6 * int64_t(nb) + idx] = int32_t(f + 1);
+        }
+    }
+    // hasSurfaceIntersection from the leaf-level triangles (generateTrianglePos, main.cu:2734-2755)
+    std::vector<uint8_t> inter(F, 0);
+    for (int64_t t = 0; t < MD; t++) {
+        const int cube = s->leaf_cube[t];
+        const int nt = tri_count(cube);
+        if (!nt) continue;
+        int has[12] = {0};
+        for (int k = 0; k < 3 * nt; k++) has[kTriTable[cube][k]] = 1;
+        const int64_t i = baseD + t;
+        for (int j = 0; j < 6; j++) {
+            int mark = 0;
+            for (int k = 0; k < 4; k++) mark |= has[kFaceEdges[j][k]];
+            if (!mark) continue;
+            int64_t pnode = s->parent[i];
+            int64_t f = faces[6 * i + j] - 1;
+            inter[f] = 1;
+            while (fhpf[f] != -1) {
+                f = faces[6 * pnode + j] - 1;
+                pnode = s->parent[pnode];
+                inter[f] = 1;
+            }
+        }
+    }
+    // ProcessLeafNodesAtOtherDepth + ifSubdivide (main.cu:2957-2992) over the
+    // nodes above the leaf level (the root, whose corners the reference drops,
+    // always has children and is skipped)
+    std::vector<int64_t> roots;
+    for (int64_t i = 1; i < baseD; i++) {
+        if (s->children[8 * i] != -1) continue;
+        bool tri = false, hit = false;
+        const float v0 = s->vval[s->vertices[8 * i] - 1];
+        const float sign = (v0 < 0) ? -1.0f : 1.0f;
+        for (int j = 1; j < 8 && !tri; j++)
+            if (sign * s->vval[s->vertices[8 * i + j] - 1] < 0) tri = true;
+        for (int j = 0; j < 6 && !hit; j++)
+            if (inter[faces[6 * i + j] - 1]) hit = true;
+        if (tri || hit) roots.push_back(i);
+    }
+    s->sub_roots = int64_t(roots.size());
+    s->sub_leaves = 0;
+    for (int64_t q : roots) s->sub_leaves += int64_t(1) << (3 * (D - s->depth_of[q]));
+    // groups: roots shallower than depth 3 one at a time (main.cu:3896-4206),
+    // then all roots of each depth 3..D-1 together (main.cu:4212-4561)
+    const int kFinerDepth = 3;
+    size_t k = 0;
+    for (; k < roots.size() && s->depth_of[roots[k]] < kFinerDepth; k++) process_group(s, {roots[k]});
+    for (int d = kFinerDepth; d < D; d++) {
+        std::vector<int64_t> g;
+        for (; k < roots.size() && s->depth_of[roots[k]] == d; k++) g.push_back(roots[k]);
+        if (!g.empty()) process_group(s, g);
+    }
+    *n_vertices = int64_t(s->mesh_v.size()) / 3;
+    *n_triangles = int64_t(s->mesh_t.size()) / 3;
+    return 0;
+}
+
+void orc_subdivision_info(orc_state* s, int64_t* n_roots, int64_t* n_virtual_leaves) {
+    *n_roots = s->sub_roots;
+    *n_virtual_leaves = s->sub_leaves;
 }
 
 int32_t orc_copy_mesh(orc_state* s, float* vertices, int32_t* triangles) {

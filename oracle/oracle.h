@@ -40,6 +40,11 @@ void orc_set_isovalue(orc_state* s, float iso);
 /* A17-A18 leaf-level marching cubes (main.cu:3502-3795). */
 int32_t orc_extract_leaf_mesh(orc_state* s, int64_t* n_vertices, int64_t* n_triangles);
 int32_t orc_copy_mesh(orc_state* s, float* vertices, int32_t* triangles);
+/* A19 coarse-leaf subdivision (main.cu:2957-3217, 3799-4561): after
+ * orc_extract_leaf_mesh, appends one batch per group of subdivided coarse
+ * leaves; the counts returned are the whole mesh's. */
+int32_t orc_extract_subdivision(orc_state* s, int64_t* n_vertices, int64_t* n_triangles);
+void orc_subdivision_info(orc_state* s, int64_t* n_roots, int64_t* n_virtual_leaves);
 
 /* Buffers: ids and layouts of prg_buffer (include/prg.h). */
 int64_t orc_buffer_size(orc_state* s, int32_t which);

@@ -89,6 +89,7 @@ SIGNATURES = {
     "prg_extract_mesh": (C.c_int, [_P, C.c_int32, C.POINTER(PrgMesh)]),
     "prg_mesh_release": (None, [C.POINTER(PrgMesh)]),
     "prg_extract_mesh_into": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, C.c_int64, C.POINTER(PrgMesh)]),
+    "prg_subdivision_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "prg_reconstruct": (C.c_int, [_P, C.c_int32, C.POINTER(PrgMesh), C.POINTER(PrgSolveStats)]),
     "prg_stage_times": (C.c_int, [_P, C.POINTER(PrgTimings)]),
     "prg_buffer_size": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64)]),
@@ -259,6 +260,12 @@ class Reconstructor:
                         float(m.scale), m.n_leaf_vertices, m.n_leaf_triangles)
         finally:
             lib().prg_mesh_release(C.byref(m))
+
+    def subdivision_info(self) -> tuple[int, int]:
+        """(coarse leaves refined, virtual leaves) of the last subdivided mesh."""
+        a, b = C.c_int64(), C.c_int64()
+        self._call("prg_subdivision_info", C.byref(a), C.byref(b))
+        return a.value, b.value
 
     # ---- introspection ----
     def level_info(self) -> tuple[np.ndarray, np.ndarray]:

@@ -171,6 +171,7 @@ prg_status prg_build_octree(prg_ctx* c) {
             prg::ensure_tables(c);
         }
         c->have_tree = true;
+        c->tree_gen++;
         c->have_div = c->have_x = c->have_iso = false;
         PRG_HIP(hipMemsetAsync(c->x.reserve(c->M), 0, size_t(c->M) * 4, c->stream));
         PRG_HIP(hipMemsetAsync(c->div.reserve(c->M), 0, size_t(c->M) * 4, c->stream));
@@ -255,31 +256,49 @@ prg_status prg_set_isovalue(prg_ctx* c, float iso) {
     });
 }
 
+namespace {
+
+// Leaf-level marching cubes, then (subdivide != 0) the coarse-leaf
+// subdivision batches; sets the counts of `out`.
+void build_mesh(prg_ctx* c, int32_t subdivide, prg_mesh* out) {
+    PRG_HIP(hipSetDevice(c->cfg.device));
+    prg::extract_leaf_mesh(c);
+    c->sub_nv = c->sub_nt = 0;
+    if (subdivide) prg::extract_subdivision(c);
+    std::memset(out, 0, sizeof(*out));
+    out->n_leaf_vertices = c->mesh_nv;
+    out->n_leaf_triangles = c->mesh_nt;
+    out->n_vertices = c->mesh_nv + c->sub_nv;
+    out->n_triangles = c->mesh_nt + c->sub_nt;
+    for (int a = 0; a < 3; a++) out->center[a] = c->center[a];
+    out->scale = c->scale;
+}
+
+// Copies the leaf-level batch and the subdivision batches, in that order.
+void copy_mesh(prg_ctx* c, float* vertices, int32_t* triangles) {
+    prg::StageTimer st(c, PRG_STAGE_MESH_COPY);
+    if (c->mesh_nv) d2h(c, vertices, c->mesh_v.get(), size_t(3 * c->mesh_nv) * 4);
+    if (c->mesh_nt) d2h(c, triangles, c->mesh_t.get(), size_t(3 * c->mesh_nt) * 4);
+    if (c->sub_nv) d2h(c, vertices + 3 * c->mesh_nv, c->sub_v.get(), size_t(3 * c->sub_nv) * 4);
+    if (c->sub_nt) d2h(c, triangles + 3 * c->mesh_nt, c->sub_t.get(), size_t(3 * c->sub_nt) * 4);
+    PRG_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace
+
 prg_status prg_extract_mesh(prg_ctx* c, int32_t subdivide, prg_mesh* out) {
     return guarded(c, [&]() {
         if (!c->have_x) return fail(c, PRG_E_STATE, "prg_solve first");
         if (!c->iso_ready) return fail(c, PRG_E_STATE, "prg_isovalue or prg_set_isovalue first");
         if (!out) return fail(c, PRG_E_ARG, "null mesh");
-        if (subdivide) return fail(c, PRG_E_ARG, "coarse-leaf subdivision is not built yet");
-        PRG_HIP(hipSetDevice(c->cfg.device));
-        prg::extract_leaf_mesh(c);
-        std::memset(out, 0, sizeof(*out));
-        prg::StageTimer st(c, PRG_STAGE_MESH_COPY);
-        out->n_vertices = c->mesh_nv;
-        out->n_triangles = c->mesh_nt;
-        out->n_leaf_vertices = c->mesh_nv;
-        out->n_leaf_triangles = c->mesh_nt;
-        out->vertices = static_cast<float*>(std::malloc(size_t(3 * c->mesh_nv + 1) * 4));
-        out->triangles = static_cast<int32_t*>(std::malloc(size_t(3 * c->mesh_nt + 1) * 4));
+        build_mesh(c, subdivide, out);
+        out->vertices = static_cast<float*>(std::malloc(size_t(3 * out->n_vertices + 1) * 4));
+        out->triangles = static_cast<int32_t*>(std::malloc(size_t(3 * out->n_triangles + 1) * 4));
         if (!out->vertices || !out->triangles) {
             prg_mesh_release(out);
             return fail(c, PRG_E_NOMEM, "mesh host allocation");
         }
-        if (c->mesh_nv) d2h(c, out->vertices, c->mesh_v.get(), size_t(3 * c->mesh_nv) * 4);
-        if (c->mesh_nt) d2h(c, out->triangles, c->mesh_t.get(), size_t(3 * c->mesh_nt) * 4);
-        PRG_HIP(hipStreamSynchronize(c->stream));
-        for (int a = 0; a < 3; a++) out->center[a] = c->center[a];
-        out->scale = c->scale;
+        copy_mesh(c, out->vertices, out->triangles);
         return PRG_OK;
     });
 }
@@ -290,22 +309,22 @@ prg_status prg_extract_mesh_into(prg_ctx* c, int32_t subdivide, float* vertices,
         if (!c->have_x) return fail(c, PRG_E_STATE, "prg_solve first");
         if (!c->iso_ready) return fail(c, PRG_E_STATE, "prg_isovalue or prg_set_isovalue first");
         if (!out) return fail(c, PRG_E_ARG, "null mesh");
-        if (subdivide) return fail(c, PRG_E_ARG, "coarse-leaf subdivision is not built yet");
-        PRG_HIP(hipSetDevice(c->cfg.device));
-        prg::extract_leaf_mesh(c);
-        std::memset(out, 0, sizeof(*out));
-        out->n_vertices = out->n_leaf_vertices = c->mesh_nv;
-        out->n_triangles = out->n_leaf_triangles = c->mesh_nt;
-        for (int a = 0; a < 3; a++) out->center[a] = c->center[a];
-        out->scale = c->scale;
-        if (3 * c->mesh_nv > vcap || 3 * c->mesh_nt > tcap || (!vertices && c->mesh_nv) || (!triangles && c->mesh_nt))
+        build_mesh(c, subdivide, out);
+        const int64_t nv = out->n_vertices, nt = out->n_triangles;
+        if (3 * nv > vcap || 3 * nt > tcap || (!vertices && nv) || (!triangles && nt))
             return fail(c, PRG_E_ARG, "mesh buffers too small");
-        prg::StageTimer st(c, PRG_STAGE_MESH_COPY);
-        if (c->mesh_nv) d2h(c, vertices, c->mesh_v.get(), size_t(3 * c->mesh_nv) * 4);
-        if (c->mesh_nt) d2h(c, triangles, c->mesh_t.get(), size_t(3 * c->mesh_nt) * 4);
-        PRG_HIP(hipStreamSynchronize(c->stream));
+        copy_mesh(c, vertices, triangles);
         out->vertices = vertices;
         out->triangles = triangles;
+        return PRG_OK;
+    });
+}
+
+prg_status prg_subdivision_info(prg_ctx* c, int64_t* n_roots, int64_t* n_virtual_leaves) {
+    return guarded(c, [&]() {
+        if (!n_roots || !n_virtual_leaves) return fail(c, PRG_E_ARG, "null output");
+        *n_roots = c->sub_roots;
+        *n_virtual_leaves = c->sub_leaves;
         return PRG_OK;
     });
 }

@@ -52,6 +52,19 @@ struct prg_ctx {
     // fields
     prg::DBuf<float> vfx, vfy, vfz, div, x, pvals;
     prg::DBuf<double> dscalars;  // reductions and CG scalars
+    // CG scratch, kept across solves: two concurrent chains of large depths
+    // (r, p0, p1, q, aligned x; FP64 partials; state), the small-depth vectors
+    prg::DBuf<float> cg_vec[2][5];
+    prg::DBuf<double> cg_part[2][4];  // r.r (two slots), p.q, b.b / true-residual partials
+    prg::DBuf<uint8_t> cg_state[2];
+    prg::DBuf<float> cg_small[3];
+    prg::DBuf<double> cg_out;
+    // SpMV tile plans of the large depths, valid for octree `plan_gen`
+    prg::DBuf<int32_t> plan_count, plan_ubase;
+    prg::DBuf<int16_t> plan_rowid;
+    int64_t plan_tile_off[prg::kMaxDepth + 2] = {};
+    int32_t plan_maxcount[prg::kMaxDepth + 1] = {};
+    uint64_t tree_gen = 0, plan_gen = 0;
     float iso = 0.0f;
     bool iso_ready = false;  // iso computed or set by the caller
 
@@ -60,6 +73,11 @@ struct prg_ctx {
     prg::DBuf<float> mesh_v;
     prg::DBuf<int32_t> mesh_t;
     int64_t mesh_nv = 0, mesh_nt = 0;
+    // coarse-leaf subdivision batches (vertex indices continue the leaf-level batch's)
+    prg::DBuf<float> sub_v;
+    prg::DBuf<int32_t> sub_t;
+    int64_t sub_nv = 0, sub_nt = 0, sub_roots = 0, sub_leaves = 0;
+    bool have_sub = false;
 
     // stage timing
     std::array<double, PRG_STAGE_COUNT> stage_ms{};
@@ -79,6 +97,7 @@ void compute_divergence(prg_ctx* c);
 prg_status solve_all(prg_ctx* c, prg_solve_stats* stats);
 void compute_isovalue(prg_ctx* c);
 void extract_leaf_mesh(prg_ctx* c);
+void extract_subdivision(prg_ctx* c);
 
 // Measures one stage with hipEvents on the context stream.
 struct StageTimer {

@@ -19,100 +19,12 @@
 #include <cmath>
 
 #include "ctx.h"
+#include "mc_common.h"
 
 namespace prg {
 namespace {
 
 constexpr int kBlock = 256;
-
-#include "mc_triangle_table.inc"
-__constant__ int8_t c_tri[256][16] = {PRG_MC_TRIANGLE_TABLE_DATA};
-// edgeVertex (MarchingCubes.cuh:693-706): square-numbered corners of each edge kind
-__constant__ int8_t c_edge_vertex[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 3}, {1, 2},
-                                            {4, 7}, {5, 6}, {0, 4}, {1, 5}, {3, 7}, {2, 6}};
-
-__device__ __forceinline__ uint32_t compact3(uint32_t x) {
-    x &= 0x09249249u;
-    x = (x ^ (x >> 2)) & 0x030c30c3u;
-    x = (x ^ (x >> 4)) & 0x0300f00fu;
-    x = (x ^ (x >> 8)) & 0xff0000ffu;
-    x = (x ^ (x >> 16)) & 0x000003ffu;
-    return x;
-}
-
-// value() of baseFunctions[idx] (ConfirmedPPolynomial<3,4>, ConfirmedPPolynomial.cuh:78-91)
-__device__ __forceinline__ float basis_value(const float* __restrict__ bf, int64_t idx, float x) {
-    const float* f = bf + idx * 20;
-    float res = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        if (!(x > f[5 * i])) break;
-        float temp = 1.0f, v = 0.0f;
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            v = v + temp * f[5 * i + 1 + j];
-            temp = temp * x;
-        }
-        res = res + v;
-    }
-    return res;
-}
-
-// Per-depth offsets of the node arrays and of the parent-neighbour tables.
-struct Levels {
-    int64_t base[kMaxDepth + 1];
-    int64_t pnt_off[kMaxDepth + 1];
-};
-
-// f(pos) restricted to the ancestor chain of node `nd` at depth `d`.  At each
-// ancestor only three basis functions per axis are involved (offsets -1, 0,
-// +1 of the ancestor's cell), so 9 evaluations serve the 27 neighbour
-// products; the neighbours come from the ancestor's parent-neighbour row.
-// Values and summation order are those of the reference's loop.
-__device__ float implicit_value(int64_t nd, int d, float px, float py, float pz, const uint32_t* __restrict__ key,
-                                const int32_t* __restrict__ pnt, const Levels& L, const int32_t* __restrict__ parent,
-                                const float* __restrict__ x, const float* __restrict__ bf, int D) {
-    float val = 0.0f;
-    while (d > 0) {
-        const uint32_t k = key[nd];
-        const int64_t fb = (int64_t(1) << d) - 1, width = int64_t(1) << d;
-        const int64_t ox = compact3(k >> 2) >> (D - d), oy = compact3(k >> 1) >> (D - d),
-                      oz = compact3(k) >> (D - d);
-        float bx[3], by[3], bz[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            const int64_t u = ox + a - 1, v = oy + a - 1, w = oz + a - 1;
-            bx[a] = (u >= 0 && u < width) ? basis_value(bf, fb + u, px) : 0.0f;
-            by[a] = (v >= 0 && v < width) ? basis_value(bf, fb + v, py) : 0.0f;
-            bz[a] = (w >= 0 && w < width) ? basis_value(bf, fb + w, pz) : 0.0f;
-        }
-        const int64_t t = nd - L.base[d];
-        const int32_t* row = pnt + 27 * (L.pnt_off[d] + (t >> 3));
-        const int s = int(t & 7);
-        const float* xd = x + L.base[d];
-#pragma unroll
-        for (int j = 0; j < 27; j++) {
-            const int vx = ((s >> 2) & 1) + (j / 9 - 1), vy = ((s >> 1) & 1) + ((j / 3) % 3 - 1),
-                      vz = (s & 1) + (j % 3 - 1);
-            const int32_t cb = row[9 * ((vx >> 1) + 1) + 3 * ((vy >> 1) + 1) + ((vz >> 1) + 1)];
-            if (cb < 0) continue;
-            float tt = xd[cb + (((vx & 1) << 2) | ((vy & 1) << 1) | (vz & 1))] * bx[j / 9];
-            tt = tt * by[(j / 3) % 3];
-            tt = tt * bz[j % 3];
-            val = val + tt;
-        }
-        nd = parent[nd];
-        d--;
-    }
-    // the root's only neighbour is itself (main.cu:803-808)
-    {
-        float tt = x[0] * basis_value(bf, 0, px);
-        tt = tt * basis_value(bf, 0, py);
-        tt = tt * basis_value(bf, 0, pz);
-        val = val + tt;
-    }
-    return val;
-}
 
 __global__ void __launch_bounds__(kBlock) k_point_values(const float* px, const float* py, const float* pz,
                                                          const int32_t* point_leaf, int64_t n, int64_t baseD,
@@ -144,18 +56,6 @@ __global__ void k_sum_partials(const double* partial, int64_t n, double* out) {
     double s = 0.0;
     for (int64_t i = 0; i < n; i++) s += partial[i];
     *out = s;
-}
-
-// node centre (getNodeCenterAllDepth, main.cu:1403-1420) at the leaf level
-__device__ __forceinline__ void leaf_center(uint32_t k, int D, float c[3]) {
-    c[0] = c[1] = c[2] = 0.5f;
-    float w = 0.25f;
-    for (int i = D - 1; i >= 0; --i) {
-        c[0] = ((k >> (3 * i + 2)) & 1u) ? c[0] + w : c[0] - w;
-        c[1] = ((k >> (3 * i + 1)) & 1u) ? c[1] + w : c[1] - w;
-        c[2] = ((k >> (3 * i)) & 1u) ? c[2] + w : c[2] - w;
-        w = w / 2;
-    }
 }
 
 // corner owners (initVertexOwner, main.cu:1438-1498): corner j = x | y<<1 | z<<2
@@ -306,20 +206,6 @@ __global__ void __launch_bounds__(kBlock) k_edge_vertex(const int32_t* flag, con
     mesh_v[3 * v + 2] = out[2];
 }
 
-__device__ __forceinline__ int cube_of(const float* cval, int64_t t) {
-    int cube = 0;
-#pragma unroll
-    for (int q = 0; q < 8; q++)
-        if (cval[8 * t + q] < 0) cube |= 1 << q;
-    return cube;
-}
-
-__device__ __forceinline__ int tri_count(int cube) {
-    int n = 0;
-    while (n < 15 && c_tri[cube][n] >= 0) n++;
-    return n / 3;
-}
-
 __global__ void __launch_bounds__(kBlock) k_tri_count(const float* cval, int64_t MD, int32_t* ntri) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
@@ -336,14 +222,6 @@ __global__ void __launch_bounds__(kBlock) k_tri_emit(const float* cval, const in
     for (int k = 0; k < n; k++) mesh_t[o + k] = evid[eref[12 * t + c_tri[cube][k]]];
 }
 
-Levels levels_of(const prg_ctx* c) {
-    Levels L{};
-    for (int d = 0; d <= c->D; d++) {
-        L.base[d] = c->bases[d];
-        L.pnt_off[d] = c->pnt_off[d];
-    }
-    return L;
-}
 
 }  // namespace
 

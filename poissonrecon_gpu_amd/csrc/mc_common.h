@@ -1,0 +1,179 @@
+// Device helpers shared by the marching-cubes kernels (leaf level and
+// coarse-leaf subdivision): triangle and edge tables, Morton bit handling,
+// basis evaluation and the implicit function over an ancestor chain.
+#pragma once
+
+#include "ctx.h"
+
+namespace prg {
+namespace {
+
+#include "mc_triangle_table.inc"
+__constant__ int8_t c_tri[256][16] = {PRG_MC_TRIANGLE_TABLE_DATA};
+// edgeVertex (MarchingCubes.cuh:693-706): square-numbered corners of each edge kind
+__constant__ int8_t c_edge_vertex[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 3}, {1, 2},
+                                            {4, 7}, {5, 6}, {0, 4}, {1, 5}, {3, 7}, {2, 6}};
+
+__device__ __forceinline__ uint32_t compact3(uint32_t x) {
+    x &= 0x09249249u;
+    x = (x ^ (x >> 2)) & 0x030c30c3u;
+    x = (x ^ (x >> 4)) & 0x0300f00fu;
+    x = (x ^ (x >> 8)) & 0xff0000ffu;
+    x = (x ^ (x >> 16)) & 0x000003ffu;
+    return x;
+}
+
+// value() of baseFunctions[idx] (ConfirmedPPolynomial<3,4>, ConfirmedPPolynomial.cuh:78-91)
+__device__ __forceinline__ float basis_value(const float* __restrict__ bf, int64_t idx, float x) {
+    const float* f = bf + idx * 20;
+    float res = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (!(x > f[5 * i])) break;
+        float temp = 1.0f, v = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            v = v + temp * f[5 * i + 1 + j];
+            temp = temp * x;
+        }
+        res = res + v;
+    }
+    return res;
+}
+
+// Per-depth offsets of the node arrays and of the parent-neighbour tables.
+struct Levels {
+    int64_t base[kMaxDepth + 1];
+    int64_t pnt_off[kMaxDepth + 1];
+};
+
+// Bits 3i of x spread from bits i (i = 0..9): the inverse of compact3.
+__device__ __forceinline__ uint32_t spread3(uint32_t x) {
+    x &= 0x000003ffu;
+    x = (x | (x << 16)) & 0xff0000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// D-level key of the depth-D cell with integer coordinates (x, y, z)
+__device__ __forceinline__ uint32_t cell_key(uint32_t x, uint32_t y, uint32_t z) {
+    return (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+}
+
+// Contribution of node nd (depth d >= 1) and its 27 neighbours to f at
+// (px, py, pz), added to val in the order j = 0..26.  Only three basis
+// functions per axis are involved (offsets -1, 0, +1 of the node's cell), so 9
+// evaluations serve the 27 products; the neighbours come from the node's
+// parent-neighbour row.
+__device__ __forceinline__ float level_contrib(int64_t nd, int d, float px, float py, float pz,
+                                               const uint32_t* __restrict__ key, const int32_t* __restrict__ pnt,
+                                               const Levels& L, const float* __restrict__ x,
+                                               const float* __restrict__ bf, int D, float val) {
+    const uint32_t k = key[nd];
+    const int64_t fb = (int64_t(1) << d) - 1, width = int64_t(1) << d;
+    const int64_t ox = compact3(k >> 2) >> (D - d), oy = compact3(k >> 1) >> (D - d), oz = compact3(k) >> (D - d);
+    float bx[3], by[3], bz[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int64_t u = ox + a - 1, v = oy + a - 1, w = oz + a - 1;
+        bx[a] = (u >= 0 && u < width) ? basis_value(bf, fb + u, px) : 0.0f;
+        by[a] = (v >= 0 && v < width) ? basis_value(bf, fb + v, py) : 0.0f;
+        bz[a] = (w >= 0 && w < width) ? basis_value(bf, fb + w, pz) : 0.0f;
+    }
+    const int64_t t = nd - L.base[d];
+    const int32_t* row = pnt + 27 * (L.pnt_off[d] + (t >> 3));
+    const int s = int(t & 7);
+    const float* xd = x + L.base[d];
+#pragma unroll
+    for (int j = 0; j < 27; j++) {
+        const int vx = ((s >> 2) & 1) + (j / 9 - 1), vy = ((s >> 1) & 1) + ((j / 3) % 3 - 1),
+                  vz = (s & 1) + (j % 3 - 1);
+        const int32_t cb = row[9 * ((vx >> 1) + 1) + 3 * ((vy >> 1) + 1) + ((vz >> 1) + 1)];
+        if (cb < 0) continue;
+        float tt = xd[cb + (((vx & 1) << 2) | ((vy & 1) << 1) | (vz & 1))] * bx[j / 9];
+        tt = tt * by[(j / 3) % 3];
+        tt = tt * bz[j % 3];
+        val = val + tt;
+    }
+    return val;
+}
+
+// f(pos) summed over the ancestor chain of node `nd` at depth `d` (node
+// first), continuing from val; the root's only neighbour is itself
+// (main.cu:803-808).  Values and summation order are those of the
+// reference's loops (main.cu:1353-1374, 2277-2297).
+__device__ __forceinline__ float chain_value(int64_t nd, int d, float px, float py, float pz,
+                                             const uint32_t* __restrict__ key, const int32_t* __restrict__ pnt,
+                                             const Levels& L, const int32_t* __restrict__ parent,
+                                             const float* __restrict__ x, const float* __restrict__ bf, int D,
+                                             float val) {
+    while (d > 0) {
+        val = level_contrib(nd, d, px, py, pz, key, pnt, L, x, bf, D, val);
+        nd = parent[nd];
+        d--;
+    }
+    float tt = x[0] * basis_value(bf, 0, px);
+    tt = tt * basis_value(bf, 0, py);
+    tt = tt * basis_value(bf, 0, pz);
+    return val + tt;
+}
+
+__device__ __forceinline__ float implicit_value(int64_t nd, int d, float px, float py, float pz,
+                                                const uint32_t* __restrict__ key, const int32_t* __restrict__ pnt,
+                                                const Levels& L, const int32_t* __restrict__ parent,
+                                                const float* __restrict__ x, const float* __restrict__ bf, int D) {
+    return chain_value(nd, d, px, py, pz, key, pnt, L, parent, x, bf, D, 0.0f);
+}
+
+// node centre (getNodeCenterAllDepth, main.cu:1403-1420) at the leaf level
+__device__ __forceinline__ void leaf_center(uint32_t k, int D, float c[3]) {
+    c[0] = c[1] = c[2] = 0.5f;
+    float w = 0.25f;
+    for (int i = D - 1; i >= 0; --i) {
+        c[0] = ((k >> (3 * i + 2)) & 1u) ? c[0] + w : c[0] - w;
+        c[1] = ((k >> (3 * i + 1)) & 1u) ? c[1] + w : c[1] - w;
+        c[2] = ((k >> (3 * i)) & 1u) ? c[2] + w : c[2] - w;
+        w = w / 2;
+    }
+}
+
+// node centre (getNodeCenterAllDepth, main.cu:1403-1420) at depth d
+__device__ __forceinline__ void node_center(uint32_t k, int d, int D, float c[3]) {
+    c[0] = c[1] = c[2] = 0.5f;
+    float w = 0.25f;
+    for (int i = D - 1; i >= D - d; --i) {
+        c[0] = ((k >> (3 * i + 2)) & 1u) ? c[0] + w : c[0] - w;
+        c[1] = ((k >> (3 * i + 1)) & 1u) ? c[1] + w : c[1] - w;
+        c[2] = ((k >> (3 * i)) & 1u) ? c[2] + w : c[2] - w;
+        w = w / 2;
+    }
+}
+
+__device__ __forceinline__ int cube_of(const float* cval, int64_t t) {
+    int cube = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        if (cval[8 * t + q] < 0) cube |= 1 << q;
+    return cube;
+}
+
+__device__ __forceinline__ int tri_count(int cube) {
+    int n = 0;
+    while (n < 15 && c_tri[cube][n] >= 0) n++;
+    return n / 3;
+}
+
+inline Levels levels_of(const prg_ctx* c) {
+
+    Levels L{};
+    for (int d = 0; d <= c->D; d++) {
+        L.base[d] = c->bases[d];
+        L.pnt_off[d] = c->pnt_off[d];
+    }
+    return L;
+}
+
+}  // namespace
+}  // namespace prg

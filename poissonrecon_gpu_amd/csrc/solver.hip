@@ -48,11 +48,14 @@ struct Stencil {
     float L[27];
 };
 
+// Scalars of one chain's CG, written only by workgroup 0 of a kernel and read
+// by later kernels (kernel boundaries order them) and by the host.
 struct CgState {
-    double bb, rr, pq;
-    float alpha, beta;
+    double bb;          // b.b
+    double rr;          // r.r at the last stop test
+    double rr_hist[2];  // rr_k in slot k & 1
     int32_t done, iters;
-    uint32_t ticket;
+    uint32_t ticket;    // the final true-residual kernel's last-workgroup ticket
     int32_t pad;
 };
 
@@ -105,8 +108,20 @@ __device__ __forceinline__ double block_allsum(double v, double* lds) {
     return t;
 }
 
+// Sum of the np partials a previous kernel wrote, in a fixed order (fixed
+// per-thread strides, fixed tree): every workgroup that calls it gets the
+// same value, so the CG scalars need no cross-workgroup handshake.
+template <int BS>
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int np, double* lds) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < np; b += BS) s += part[b];
+    return block_allsum<BS>(s, lds);
+}
+
 // Publishes this workgroup's partial; returns true in every thread of the
-// last workgroup to finish, after which `partial` is complete.
+// last workgroup to finish, after which `partial` is complete (used once per
+// solve, by the final true-residual kernel).
+template <int BS>
 __device__ __forceinline__ bool publish(double blk, double* partial, CgState* st, unsigned* flag) {
     if (threadIdx.x == 0) {
         partial[blockIdx.x] = blk;
@@ -120,140 +135,216 @@ __device__ __forceinline__ bool publish(double blk, double* partial, CgState* st
     return last;
 }
 
-// Sum of the partials by the last workgroup: fixed per-thread strides, fixed tree.
-__device__ __forceinline__ double gather_partials(const double* partial, int nb, double* lds) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += kBlock) s += ((volatile const double*)partial)[b];
-    return block_allsum<kBlock>(s, lds);
-}
+constexpr int kVecThreads = 512;  // init / update / refresh / true residual
+constexpr int kSpmvThreads = 512;
+constexpr int kTile = kSpmvThreads / 8;   // sibling blocks per SpMV tile, one row per lane
+constexpr int kTileEntries = 27 * kTile;  // parent-neighbour entries of a tile
+constexpr int kHashSlots = 2048;          // > kTileEntries: linear probing always finds a slot
+static_assert(kHashSlots > kTileEntries, "hash must hold every entry of a tile");
 
-__global__ void __launch_bounds__(kBlock) k_cg_init(const int32_t* pnt, int64_t n, Stencil S, const float* b,
-                                                    const float* x, float* r, float* p, double* partial,
-                                                    double* partial2, CgState* st, float tol, int32_t fixed) {
-    __shared__ double lds[kBlock / 64];
-    __shared__ unsigned flag;
+// r = b - A x0, p = r, xs = x0 (the solve works on the 16-byte-aligned copy
+// xs); partials of r.r (slot 0) and b.b.
+__global__ void __launch_bounds__(kVecThreads) k_cg_init(const int32_t* pnt, int64_t n, Stencil S, const float* b,
+                                                         const float* x, float* xs, float* r, float* p,
+                                                         double* rr_part, double* bb_part, CgState* st) {
+    __shared__ double lds[kVecThreads / 64];
     double bb = 0.0, rr = 0.0;
-    for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < n; t += int64_t(gridDim.x) * kBlock) {
+    for (int64_t t = int64_t(blockIdx.x) * kVecThreads + threadIdx.x; t < n; t += int64_t(gridDim.x) * kVecThreads) {
         const float ax = spmv_pnt(pnt, t, x, S);
         const float rv = b[t] - ax;
+        xs[t] = x[t];
         r[t] = rv;
         p[t] = rv;
         bb += double(b[t]) * double(b[t]);
         rr += double(rv) * double(rv);
     }
-    const double sbb = block_allsum<kBlock>(bb, lds);
-    const double srr = block_allsum<kBlock>(rr, lds);
-    if (threadIdx.x == 0) partial2[blockIdx.x] = sbb;
-    if (publish(srr, partial, st, &flag)) {
-        const double RR = gather_partials(partial, gridDim.x, lds);
-        const double BB = gather_partials(partial2, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            st->bb = BB;
-            st->rr = RR;
-            st->iters = 0;
-            st->beta = 0.0f;
-            const double tol2 = double(tol) * double(tol) * BB;
-            st->done = (BB <= 0.0) || (RR == 0.0) || (fixed > 0 ? 0 : (RR <= tol2));
-            st->ticket = 0u;
-        }
+    const double sbb = block_allsum<kVecThreads>(bb, lds);
+    const double srr = block_allsum<kVecThreads>(rr, lds);
+    if (threadIdx.x == 0) {
+        bb_part[blockIdx.x] = sbb;
+        rr_part[blockIdx.x] = srr;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->done = 0;
+        st->iters = 0;
+        st->ticket = 0u;
     }
 }
 
-// q = A p_k, LDS-staged per sibling block.  Eight lanes own the eight rows of
-// a sibling block; together they load the block's 4x4x4 cell neighbourhood
-// (64 cells reached through the block's 27 parent-neighbour entries), form
-// p_k = r + beta p_{k-1} for each cell once (first != 0: p_1 = r is already
-// in pnew), and then every lane applies its 27-point stencil from LDS in the
-// order j = 0..26.  About 20 global loads per row instead of 81.
-constexpr int kBlkPerWG = kBlock / 8;
-__global__ void __launch_bounds__(kBlock) k_cg_spmv(const int32_t* pnt, int64_t n, Stencil S, const float* r,
-                                                    const float* pold, float* pnew, float* q, int32_t first,
-                                                    double* partial, CgState* st) {
-    __shared__ double lds[kBlock / 64];
-    __shared__ unsigned flag;
-    __shared__ int32_t prow[kBlkPerWG][28];
-    __shared__ float cell[kBlkPerWG][64];
-    if (st->done) return;
-    const float bf = st->beta;
-    const int g = threadIdx.x >> 3, s = threadIdx.x & 7;
-    const int64_t nblk = n >> 3;
-    double pq = 0.0;
-    for (int64_t b0 = int64_t(blockIdx.x) * kBlkPerWG; b0 < nblk; b0 += int64_t(gridDim.x) * kBlkPerWG) {
-        const int64_t blk = b0 + g;
-        const bool valid = blk < nblk;
-        if (valid) {
-#pragma unroll
-            for (int m = 0; m < 4; m++) {
-                const int e = s + 8 * m;
-                if (e < 27) prow[g][e] = pnt[27 * blk + e];
-            }
-        }
-        __syncthreads();
-        if (valid) {
-#pragma unroll
-            for (int m = 0; m < 8; m++) {
-                const int c = s + 8 * m;  // local cell (lx, ly, lz) = (c >> 4, (c >> 2) & 3, c & 3)
-                const int ax = (c >> 4) + 1, ay = ((c >> 2) & 3) + 1, az = (c & 3) + 1;
-                const int32_t cb = prow[g][9 * (ax >> 1) + 3 * (ay >> 1) + (az >> 1)];
-                float v = 0.0f;
-                if (cb >= 0) {
-                    const int64_t idx = cb + (((ax & 1) << 2) | ((ay & 1) << 1) | (az & 1));
-                    v = first ? pnew[idx] : pdir(r, pold, bf, idx);
+// Tile plan of one depth (once per octree): for each tile of kTile sibling
+// blocks, the distinct sibling blocks its rows touch (`ubase`, their first
+// rows) and, per (block, parent-neighbour entry), the slot of that block in
+// the list (`rowid`, -1 if absent).  The parent-neighbour entries of
+// neighbouring rows repeat (about 2 distinct blocks per block), so the SpMV
+// loads each distinct block once per tile as full 32-byte segments.
+__global__ void __launch_bounds__(kSpmvThreads) k_tile_plan(const int32_t* pnt, int64_t nblk, int32_t* count,
+                                                            int32_t* ubase, int16_t* rowid, int32_t* maxcount) {
+    __shared__ int32_t hkey[kHashSlots];
+    __shared__ int16_t hslot[kHashSlots];
+    __shared__ int16_t code[kTileEntries];
+    __shared__ int32_t ucount;
+    const int tid = threadIdx.x;
+    const int64_t tile = blockIdx.x, b0 = tile * kTile;
+    for (int h = tid; h < kHashSlots; h += kSpmvThreads) hkey[h] = -1;
+    if (tid == 0) ucount = 0;
+    __syncthreads();
+    for (int e = tid; e < kTileEntries; e += kSpmvThreads) {
+        int32_t v = -1;
+        if (b0 + e / 27 < nblk) v = pnt[27 * b0 + e];
+        int c = -1;
+        if (v >= 0) {
+            const int32_t key = v >> 3;
+            uint32_t h = (uint32_t(key) * 2654435761u) >> 21;
+            while (true) {
+                const int32_t prev = atomicCAS(&hkey[h], -1, key);
+                if (prev == -1) {
+                    const int id = atomicAdd(&ucount, 1);
+                    hslot[h] = int16_t(id);
+                    ubase[tile * kTileEntries + id] = v;
+                    break;
                 }
-                cell[g][c] = v;
+                if (prev == key) break;
+                h = (h + 1) & (kHashSlots - 1);
             }
+            c = int(h);
+        }
+        code[e] = int16_t(c);
+    }
+    __syncthreads();
+    for (int e = tid; e < kTileEntries; e += kSpmvThreads) {
+        const int c = code[e];
+        rowid[tile * kTileEntries + e] = c >= 0 ? hslot[c] : int16_t(-1);
+    }
+    if (tid == 0) {
+        count[tile] = ucount;
+        atomicMax(maxcount, ucount);
+    }
+}
+
+// CG iteration `it` (1-based), first kernel.  Every workgroup sums the r.r
+// partials of the previous iteration (rr_{it-1}) and applies the stop test
+// (iteration cap, rr == 0, fixed count, rr <= tol^2 b.b); then q = A p_it over
+// tiles, with p_it = r + beta p_{it-1} formed as the cells are loaded (it == 1:
+// p_1 = r is already in pnew), each distinct sibling block of a tile loaded
+// once into LDS (see k_tile_plan), and the rows summed from LDS in the order
+// j = 0..26.  Writes this workgroup's partial of p.q.
+__global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, const int32_t* __restrict__ count,
+                                                          const int32_t* __restrict__ ubase,
+                                                          const int16_t* __restrict__ rowid, const float* r,
+                                                          const float* pold, float* pnew, float* q, int32_t it,
+                                                          const double* rr_part, const double* bb_part, int32_t np,
+                                                          double* pq_part, CgState* st, float tol, int32_t fixed,
+                                                          int32_t max_iters) {
+    extern __shared__ float cellv[];
+    __shared__ int16_t rid[kTileEntries];
+    __shared__ double lds[kSpmvThreads / 64];
+    if (st->done) return;
+    const int k1 = it - 1;
+    const double rr = sum_partials<kSpmvThreads>(rr_part, np, lds);
+    const double bb = (it == 1) ? sum_partials<kSpmvThreads>(bb_part, np, lds) : st->bb;
+    const double tol2 = double(tol) * double(tol) * bb;
+    const bool stop = (bb <= 0.0) || (k1 >= max_iters) || (rr == 0.0) || (fixed > 0 ? (k1 >= fixed) : (rr <= tol2));
+    const float bf = (it == 1) ? 0.0f : float(rr / st->rr_hist[(k1 - 1) & 1]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (it == 1) st->bb = bb;
+        st->rr_hist[k1 & 1] = rr;
+        st->rr = rr;
+        if (stop) {
+            st->done = 1;
+            st->iters = k1;
+        }
+    }
+    if (stop) return;
+    const bool first = (it == 1);
+    const int tid = threadIdx.x;
+    const int g = tid >> 3, s = tid & 7;
+    const int64_t nblk = n >> 3;
+    const int64_t ntiles = (nblk + kTile - 1) / kTile;
+    const int64_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const int64_t tbeg = int64_t(blockIdx.x) * per, tend = tbeg + per < ntiles ? tbeg + per : ntiles;
+    const int sx = (s >> 2) & 1, sy = (s >> 1) & 1, sz = s & 1;
+    double pq = 0.0;
+    for (int64_t tile = tbeg; tile < tend; tile++) {
+        const int16_t* trow = rowid + tile * kTileEntries;
+        const int32_t* tub = ubase + tile * kTileEntries;
+        for (int e = tid; e < kTileEntries; e += kSpmvThreads) rid[e] = trow[e];
+        const int ncell = 8 * count[tile];
+        for (int u = tid; u < ncell; u += kSpmvThreads) {
+            const int64_t idx = int64_t(tub[u >> 3]) + (u & 7);
+            cellv[u] = first ? pnew[idx] : pdir(r, pold, bf, idx);
         }
         __syncthreads();
-        if (valid) {
-            const int sx = (s >> 2) & 1, sy = (s >> 1) & 1, sz = s & 1;
+        const int64_t blk = tile * kTile + g;
+        if (blk < nblk) {
+            const int16_t* rg = rid + 27 * g;
             float out = 0.0f;
 #pragma unroll
             for (int j = 0; j < 27; j++) {
-                const int ax = 2 + sx + (j / 9 - 1), ay = 2 + sy + ((j / 3) % 3 - 1), az = 2 + sz + (j % 3 - 1);
-                if (prow[g][9 * (ax >> 1) + 3 * (ay >> 1) + (az >> 1)] < 0 || S.L[j] == 0.0f) continue;
-                const float prod = S.L[j] * cell[g][(ax - 1) * 16 + (ay - 1) * 4 + (az - 1)];
+                const int vx = sx + (j / 9 - 1), vy = sy + ((j / 3) % 3 - 1), vz = sz + (j % 3 - 1);
+                const int pj = 9 * ((vx >> 1) + 1) + 3 * ((vy >> 1) + 1) + ((vz >> 1) + 1);
+                const int cs = ((vx & 1) << 2) | ((vy & 1) << 1) | (vz & 1);
+                const int id = rg[pj];
+                if (id < 0 || S.L[j] == 0.0f) continue;
+                const float prod = S.L[j] * cellv[8 * id + cs];
                 out = out + prod;
             }
             const int64_t t = 8 * blk + s;
-            const float pt = cell[g][(1 + sx) * 16 + (1 + sy) * 4 + (1 + sz)];
+            const float pt = cellv[8 * rg[13] + s];
             if (!first) pnew[t] = pt;
             q[t] = out;
             pq += double(pt) * double(out);
         }
         __syncthreads();
     }
-    const double sum = block_allsum<kBlock>(pq, lds);
-    if (publish(sum, partial, st, &flag)) {
-        const double PQ = gather_partials(partial, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            st->pq = PQ;
-            st->alpha = float(st->rr / PQ);
-            if (!(PQ > 0.0)) st->done = 1;  // breakdown once the residual has vanished
-            st->ticket = 0u;
-        }
-    }
+    const double sum = block_allsum<kSpmvThreads>(pq, lds);
+    if (tid == 0) pq_part[blockIdx.x] = sum;
 }
 
-__device__ __forceinline__ void finish_iteration(CgState* st, double rr_new, float tol, int32_t fixed,
-                                                 int32_t max_iters, bool count) {
-    if (count) st->iters += 1;
-    st->beta = float(rr_new / st->rr);
-    st->rr = rr_new;
-    const double tol2 = double(tol) * double(tol) * st->bb;
-    const bool stop =
-        (st->iters >= max_iters) || (rr_new == 0.0) || (fixed > 0 ? (st->iters >= fixed) : (rr_new <= tol2));
-    st->done = stop ? 1 : 0;
-}
-
-__global__ void __launch_bounds__(kBlock) k_cg_update(int64_t n, const float* p, const float* q, float* x, float* r,
-                                                      double* partial, CgState* st, float tol, int32_t fixed,
-                                                      int32_t max_iters, int32_t refresh_next) {
-    __shared__ double lds[kBlock / 64];
-    __shared__ unsigned flag;
+// CG iteration `it`, second kernel: alpha = rr_{it-1} / p.q (every workgroup
+// sums the p.q partials; p.q <= 0 after the residual vanished stops the
+// solve), x += alpha p, r -= alpha q on 16-byte-aligned vectors four rows
+// per lane, and this workgroup's partial of r.r (slot it & 1) unless r is
+// recomputed next.
+__global__ void __launch_bounds__(kVecThreads) k_cg_update(int64_t n, const float* p, const float* q, float* x, float* r,
+                                                           int32_t it, const double* pq_part, int32_t npq,
+                                                           double* rr_part, int32_t refresh_next, CgState* st) {
+    __shared__ double lds[kVecThreads / 64];
     if (st->done) return;
-    const float a = st->alpha;
+    const double PQ = sum_partials<kVecThreads>(pq_part, npq, lds);
+    if (!(PQ > 0.0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->done = 1;
+            st->iters = it - 1;
+        }
+        return;
+    }
+    const float a = float(st->rr_hist[(it - 1) & 1] / PQ);
     double rr = 0.0;
-    for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < n; t += int64_t(gridDim.x) * kBlock) {
+    const int64_t n4 = n >> 2;
+    const float4* p4 = reinterpret_cast<const float4*>(p);
+    const float4* q4 = reinterpret_cast<const float4*>(q);
+    float4* x4 = reinterpret_cast<float4*>(x);
+    float4* r4 = reinterpret_cast<float4*>(r);
+    for (int64_t t = int64_t(blockIdx.x) * kVecThreads + threadIdx.x; t < n4; t += int64_t(gridDim.x) * kVecThreads) {
+        const float4 pv = p4[t], qv = q4[t];
+        float4 xv = x4[t], rv = r4[t];
+        xv.x = xv.x + a * pv.x;
+        xv.y = xv.y + a * pv.y;
+        xv.z = xv.z + a * pv.z;
+        xv.w = xv.w + a * pv.w;
+        rv.x = rv.x - a * qv.x;
+        rv.y = rv.y - a * qv.y;
+        rv.z = rv.z - a * qv.z;
+        rv.w = rv.w - a * qv.w;
+        x4[t] = xv;
+        r4[t] = rv;
+        rr += double(rv.x) * double(rv.x);
+        rr += double(rv.y) * double(rv.y);
+        rr += double(rv.z) * double(rv.z);
+        rr += double(rv.w) * double(rv.w);
+    }
+    for (int64_t t = 4 * n4 + int64_t(blockIdx.x) * kVecThreads + threadIdx.x; t < n;
+         t += int64_t(gridDim.x) * kVecThreads) {
         const float t1 = a * p[t];
         x[t] = x[t] + t1;
         const float t2 = a * q[t];
@@ -261,66 +352,56 @@ __global__ void __launch_bounds__(kBlock) k_cg_update(int64_t n, const float* p,
         r[t] = rv;
         rr += double(rv) * double(rv);
     }
-    const double sum = refresh_next ? 0.0 : block_allsum<kBlock>(rr, lds);
-    if (publish(sum, partial, st, &flag)) {
-        if (refresh_next) {
-            // the residual is recomputed next; only count the iteration
-            if (threadIdx.x == 0) {
-                st->iters += 1;
-                st->ticket = 0u;
-            }
-            return;
-        }
-        const double RR = gather_partials(partial, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            finish_iteration(st, RR, tol, fixed, max_iters, true);
-            st->ticket = 0u;
-        }
+    if (!refresh_next) {
+        const double sum = block_allsum<kVecThreads>(rr, lds);
+        if (threadIdx.x == 0) rr_part[blockIdx.x] = sum;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->iters = it;
 }
 
-__global__ void __launch_bounds__(kBlock) k_cg_refresh(const int32_t* pnt, int64_t n, Stencil S, const float* b,
-                                                       const float* x, float* r, double* partial, CgState* st,
-                                                       float tol, int32_t fixed, int32_t max_iters) {
-    __shared__ double lds[kBlock / 64];
-    __shared__ unsigned flag;
+// r = b - A x (every `refresh` iterations) and its r.r partial (slot it & 1)
+__global__ void __launch_bounds__(kVecThreads) k_cg_refresh(const int32_t* pnt, int64_t n, Stencil S, const float* b,
+                                                            const float* x, float* r, double* rr_part, CgState* st) {
+    __shared__ double lds[kVecThreads / 64];
     if (st->done) return;
     double rr = 0.0;
-    for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < n; t += int64_t(gridDim.x) * kBlock) {
+    for (int64_t t = int64_t(blockIdx.x) * kVecThreads + threadIdx.x; t < n; t += int64_t(gridDim.x) * kVecThreads) {
         const float ax = spmv_pnt(pnt, t, x, S);
         const float rv = b[t] - ax;
         r[t] = rv;
         rr += double(rv) * double(rv);
     }
-    const double sum = block_allsum<kBlock>(rr, lds);
-    if (publish(sum, partial, st, &flag)) {
-        const double RR = gather_partials(partial, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            finish_iteration(st, RR, tol, fixed, max_iters, false);
-            st->ticket = 0u;
-        }
-    }
+    const double sum = block_allsum<kVecThreads>(rr, lds);
+    if (threadIdx.x == 0) rr_part[blockIdx.x] = sum;
 }
 
-// ||b - A x|| / ||b|| of the final iterate (FP32 SpMV, FP64 norm)
-__global__ void __launch_bounds__(kBlock) k_true_residual(const int32_t* pnt, int64_t n, Stencil S, const float* b,
-                                                          const float* x, double* partial, CgState* st,
-                                                          double* out) {
-    __shared__ double lds[kBlock / 64];
+// ||b - A x|| / ||b|| of the final iterate (FP32 SpMV, FP64 norm), the
+// recursive residual of the last iteration (its r.r partials, np of them, in
+// slot iters & 1) and the iteration count.
+__global__ void __launch_bounds__(kVecThreads) k_true_residual(const int32_t* pnt, int64_t n, Stencil S, const float* b,
+                                                               const float* x, double* partial,
+                                                               const double* rr_slots, int32_t np, int32_t slot_stride,
+                                                               CgState* st, double* out) {
+    __shared__ double lds[kVecThreads / 64];
     __shared__ unsigned flag;
     double rr = 0.0;
-    for (int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x; t < n; t += int64_t(gridDim.x) * kBlock) {
+    for (int64_t t = int64_t(blockIdx.x) * kVecThreads + threadIdx.x; t < n; t += int64_t(gridDim.x) * kVecThreads) {
         const float ax = spmv_pnt(pnt, t, x, S);
         const double rv = double(b[t]) - double(ax);
         rr += rv * rv;
     }
-    const double sum = block_allsum<kBlock>(rr, lds);
-    if (publish(sum, partial, st, &flag)) {
-        const double RR = gather_partials(partial, gridDim.x, lds);
+    const double sum = block_allsum<kVecThreads>(rr, lds);
+    if (publish<kVecThreads>(sum, partial, st, &flag)) {
+        double s = 0.0;
+        for (int w = threadIdx.x; w < int(gridDim.x); w += kVecThreads) s += ((volatile const double*)partial)[w];
+        const double RR = block_allsum<kVecThreads>(s, lds);
+        const int32_t iters = st->iters;
+        const double rec = sum_partials<kVecThreads>(rr_slots + int64_t(iters & 1) * slot_stride, np, lds);
         if (threadIdx.x == 0) {
-            out[0] = (st->bb > 0.0) ? sqrt(RR / st->bb) : 0.0;
-            out[1] = (st->bb > 0.0) ? sqrt(st->rr / st->bb) : 0.0;
-            out[2] = double(st->iters);
+            const double bb = st->bb;
+            out[0] = (bb > 0.0) ? sqrt(RR / bb) : 0.0;
+            out[1] = (bb > 0.0) ? sqrt(rec / bb) : 0.0;
+            out[2] = double(iters);
             st->ticket = 0u;
         }
     }
@@ -431,145 +512,278 @@ __global__ void __launch_bounds__(kSmallBlock) k_cg_small(SmallJobs jobs, const 
 
 }  // namespace
 
+namespace {
+
+// One chain of large depths solved one after another on one stream.  The
+// finest depth forms one chain and the other large depths the second, so the
+// two overlap (the per-depth systems are independent).
+struct Chain {
+    hipStream_t s = nullptr;
+    std::vector<int> depths;
+    size_t next = 0;
+    int d = -1;       // depth in progress (-1: none)
+    int issued = 0;   // iterations queued for depth d
+    int batch = 24;
+    bool waiting = false;
+    float *R = nullptr, *P[2] = {nullptr, nullptr}, *Q = nullptr, *XS = nullptr;
+    double *rr = nullptr, *pq = nullptr, *bb = nullptr;  // partials (r.r: two slots of kMaxWG)
+    CgState* st = nullptr;
+    CgState* hst = nullptr;  // pinned host copy of st
+    hipEvent_t ev = nullptr;  // marks the state copy
+    std::vector<std::pair<int, int>> launched;
+};
+
+constexpr int kMaxWG = 1024;       // workgroups of the vector kernels (their partial count)
+constexpr int kMaxSpmvWG = 2048;   // workgroups of the SpMV
+
+int64_t tiles_of(int64_t n) { return (n / 8 + kTile - 1) / kTile; }
+
+// Tile plans of every depth above kSmallRows rows, once per octree.
+void ensure_plans(prg_ctx* c) {
+    if (c->plan_gen == c->tree_gen) return;
+    hipStream_t s = c->stream;
+    const int D = c->D;
+    int64_t off = 0;
+    for (int d = 0; d <= D; d++) {
+        c->plan_tile_off[d] = off;
+        if (c->counts[d] > kSmallRows) off += tiles_of(c->counts[d]);
+    }
+    c->plan_tile_off[D + 1] = off;
+    int32_t* cnt = c->plan_count.reserve(off + 1);
+    int32_t* ub = c->plan_ubase.reserve(size_t(off) * kTileEntries + 1);
+    int16_t* rw = c->plan_rowid.reserve(size_t(off) * kTileEntries + 1);
+    DBuf<int32_t> mx;
+    mx.reserve(D + 1);
+    PRG_HIP(hipMemsetAsync(mx.get(), 0, size_t(D + 1) * 4, s));
+    for (int d = 0; d <= D; d++) {
+        const int64_t nt = c->plan_tile_off[d + 1] - c->plan_tile_off[d];
+        if (c->counts[d] <= kSmallRows || nt == 0) continue;
+        const int64_t t0 = c->plan_tile_off[d];
+        TimedScope ts(c->kt, "spmv_tile_plan", 13.5 * double(c->counts[d]));
+        k_tile_plan<<<unsigned(nt), kSpmvThreads, 0, s>>>(c->pnt.get() + 27 * c->pnt_off[d], c->counts[d] / 8,
+                                                           cnt + t0, ub + t0 * kTileEntries, rw + t0 * kTileEntries,
+                                                           mx.get() + d);
+        PRG_CHECK_LAUNCH();
+    }
+    PRG_HIP(hipMemcpyAsync(c->plan_maxcount, mx.get(), size_t(D + 1) * 4, hipMemcpyDeviceToHost, s));
+    PRG_HIP(hipStreamSynchronize(s));
+    c->plan_gen = c->tree_gen;
+}
+
+}  // namespace
+
 prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     ensure_tables(c);
     StageTimer stage(c, PRG_STAGE_SOLVE);
     hipStream_t s = c->stream;
     const int D = c->D;
     const int64_t M = c->M;
+    ensure_plans(c);
     float* X = c->x.reserve(M);
     if (c->cfg.warm_start == PRG_WARM_NONE) PRG_HIP(hipMemsetAsync(X, 0, size_t(M) * 4, s));
-    DBuf<float> R, P0, P1, Q, Rs, Ps, Qs;
-    R.reserve(M);
-    P0.reserve(M);
-    P1.reserve(M);
-    Q.reserve(M);
-    DBuf<double> part, part2, outv;
-    part.reserve(kMaxBlocks);
-    part2.reserve(kMaxBlocks);
-    outv.reserve(size_t(4) * (D + 1));
-    DBuf<CgState> st;
-    st.reserve(1);
-    PRG_HIP(hipMemsetAsync(st.get(), 0, sizeof(CgState), s));
-    PRG_HIP(hipMemsetAsync(outv.get(), 0, size_t(4) * (D + 1) * 8, s));
+    double* outv = c->cg_out.reserve(size_t(4) * (D + 1));
+    PRG_HIP(hipMemsetAsync(outv, 0, size_t(4) * (D + 1) * 8, s));
     const float tol = c->cfg.cg_tol;
     const int32_t fixed = c->cfg.cg_fixed_iters, max_iters = c->cfg.cg_max_iters;
     const int32_t refresh = c->cfg.cg_refresh > 0 ? c->cfg.cg_refresh : (1 << 30);
-    std::vector<double> ms(D + 1, 0.0);
-    hipEvent_t e0, e1, es0, es1;
-    PRG_HIP(hipEventCreate(&e0));
-    PRG_HIP(hipEventCreate(&e1));
-    PRG_HIP(hipEventCreate(&es0));
-    PRG_HIP(hipEventCreate(&es1));
+    // With per-kernel timing on, everything runs on the context stream, one
+    // chain after the other with the finest depth last, so each launch's
+    // events time that launch alone.
+    const bool serial = c->kt.enabled;
+
+    // small depths: one launch, one workgroup each, on the second stream
+    SmallJobs jobs{};
+    for (int d = 0; d <= D; d++)
+        if (c->counts[d] <= kSmallRows) {
+            jobs.depth[jobs.count] = d;
+            jobs.base[jobs.count] = c->bases[d];
+            jobs.n[jobs.count] = c->counts[d];
+            jobs.pnt_row[jobs.count] = d == 0 ? -1 : c->pnt_off[d];
+            jobs.count++;
+        }
+    Chain ch[2];
+    for (int d = 0; d <= D; d++)
+        if (c->counts[d] > kSmallRows) ch[d == D ? 0 : 1].depths.push_back(d);
+    ch[0].s = s;
+    ch[1].s = serial ? s : c->stream2;
+
+    std::vector<hipEvent_t> evs;
+    auto new_event = [&]() {
+        hipEvent_t e;
+        PRG_HIP(hipEventCreate(&e));
+        evs.push_back(e);
+        return e;
+    };
+    std::vector<hipEvent_t> dbeg(D + 1, nullptr), dend(D + 1, nullptr);
     CgState* hst = nullptr;
-    PRG_HIP(hipHostMalloc(&hst, sizeof(CgState), hipHostMallocDefault));
-    prg_status result = PRG_OK;
+    std::vector<double> ms(D + 1, 0.0);
     std::vector<double> ov(size_t(4) * (D + 1), 0.0);
+    prg_status result = PRG_OK;
+    auto stencil_of = [&](int d) {
+        Stencil S;
+        for (int j = 0; j < 27; j++) S.L[j] = c->fs->stencil[size_t(d) * 27 + j];
+        return S;
+    };
+    auto vec_grid = [&](int64_t n) {
+        return int32_t(std::min<int64_t>(kMaxWG, (n + 4 * kVecThreads - 1) / (4 * kVecThreads)));
+    };
     try {
-        // small depths: one launch, one workgroup each
-        SmallJobs jobs{};
-        for (int d = 0; d <= D; d++)
-            if (c->counts[d] <= kSmallRows) {
-                jobs.depth[jobs.count] = d;
-                jobs.base[jobs.count] = c->bases[d];
-                jobs.n[jobs.count] = c->counts[d];
-                jobs.pnt_row[jobs.count] = d == 0 ? -1 : c->pnt_off[d];
-                jobs.count++;
-            }
-        // The small depths run on the second stream, overlapping the large
-        // ones; they get their own scratch vectors (indexed by global node).
+        PRG_HIP(hipHostMalloc(&hst, 2 * sizeof(CgState), hipHostMallocDefault));
+        hipEvent_t es0 = new_event(), es1 = new_event();
+        PRG_HIP(hipEventRecord(es0, s));
+        PRG_HIP(hipStreamWaitEvent(c->stream2, es0, 0));
         if (jobs.count) {
             const int64_t small_end = jobs.base[jobs.count - 1] + jobs.n[jobs.count - 1];
-            Rs.reserve(small_end);
-            Ps.reserve(small_end);
-            Qs.reserve(small_end);
-            hipStream_t s2 = c->stream2;
-            PRG_HIP(hipEventRecord(es0, s));
-            PRG_HIP(hipStreamWaitEvent(s2, es0, 0));
-            k_cg_small<<<jobs.count, kSmallBlock, 0, s2>>>(jobs, c->d_stencil.get(), c->pnt.get(), c->div.get(), X,
-                                                           Rs.get(), Ps.get(), Qs.get(), tol, fixed, max_iters, refresh,
-                                                           outv.get());
+            float* Rs = c->cg_small[0].reserve(small_end);
+            float* Ps = c->cg_small[1].reserve(small_end);
+            float* Qs = c->cg_small[2].reserve(small_end);
+            k_cg_small<<<jobs.count, kSmallBlock, 0, c->stream2>>>(jobs, c->d_stencil.get(), c->pnt.get(),
+                                                                   c->div.get(), X, Rs, Ps, Qs, tol, fixed,
+                                                                   max_iters, refresh, outv);
             PRG_CHECK_LAUNCH();
-            PRG_HIP(hipEventRecord(es1, s2));
+            PRG_HIP(hipEventRecord(es1, c->stream2));
         }
-        for (int d = 0; d <= D; d++) {
+        for (int k = 0; k < 2; k++) {
+            Chain& h = ch[k];
+            int64_t nmax = 0;
+            for (int d : h.depths) nmax = std::max(nmax, c->counts[d]);
+            if (!nmax) continue;
+            h.R = c->cg_vec[k][0].reserve(nmax);
+            h.P[0] = c->cg_vec[k][1].reserve(nmax);
+            h.P[1] = c->cg_vec[k][2].reserve(nmax);
+            h.Q = c->cg_vec[k][3].reserve(nmax);
+            h.XS = c->cg_vec[k][4].reserve(nmax);
+            h.rr = c->cg_part[k][0].reserve(2 * kMaxWG);
+            h.pq = c->cg_part[k][1].reserve(kMaxSpmvWG);
+            h.bb = c->cg_part[k][2].reserve(std::max(kMaxWG, kMaxSpmvWG));
+            h.st = reinterpret_cast<CgState*>(c->cg_state[k].reserve(sizeof(CgState)));
+            h.hst = hst + k;
+            h.ev = new_event();
+            PRG_HIP(hipMemsetAsync(h.st, 0, sizeof(CgState), h.s));
+        }
+
+        auto begin_depth = [&](Chain& h) {
+            const int d = h.depths[h.next++];
+            h.d = d;
+            h.issued = 0;
+            h.batch = 24;
             const int64_t n = c->counts[d], base = c->bases[d];
-            if (n <= kSmallRows) continue;
-            Stencil S;
-            for (int j = 0; j < 27; j++) S.L[j] = c->fs->stencil[size_t(d) * 27 + j];
-            const unsigned nbk = unsigned(std::min<int64_t>(kMaxBlocks, (n + kBlock - 1) / kBlock));
-            // the LDS-staged SpMV walks sibling blocks: 32 per workgroup
-            const unsigned nbk_spmv = unsigned(std::min<int64_t>(kMaxBlocks, (n / 8 + kBlkPerWG - 1) / kBlkPerWG));
+            dbeg[d] = new_event();
+            dend[d] = new_event();
+            PRG_HIP(hipEventRecord(dbeg[d], h.s));
+            TimedScope ts(c->kt, "cg_init", kSpmvRowBytes * double(n));
+            k_cg_init<<<vec_grid(n), kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], n, stencil_of(d),
+                                                            c->div.get() + base, X + base, h.XS, h.R, h.P[0], h.rr,
+                                                            h.bb, h.st);
+            PRG_CHECK_LAUNCH();
+        };
+        auto issue_batch = [&](Chain& h) {
+            const int d = h.d;
+            const int64_t n = c->counts[d];
+            const Stencil S = stencil_of(d);
+            const int32_t nv = vec_grid(n);
+            const int64_t ntiles = tiles_of(n);
+            const int32_t ns = int32_t(std::min<int64_t>(kMaxSpmvWG, ntiles));
+            const size_t lds = size_t(8) * 4 * std::max(1, c->plan_maxcount[d]);
+            const int64_t t0 = c->plan_tile_off[d];
+            const int32_t* cnt = c->plan_count.get() + t0;
+            const int32_t* ub = c->plan_ubase.get() + t0 * kTileEntries;
+            const int16_t* rw = c->plan_rowid.get() + t0 * kTileEntries;
             const int32_t* pnt = c->pnt.get() + 27 * c->pnt_off[d];
-            const float* B = c->div.get() + base;
-            float* Xd = X + base;
-            float* pb[2] = {P0.get(), P1.get()};
-            PRG_HIP(hipEventRecord(e0, s));
-            {
-                TimedScope ts(c->kt, "cg_init", kSpmvRowBytes * double(n));
-                k_cg_init<<<nbk, kBlock, 0, s>>>(pnt, n, S, B, Xd, R.get(), pb[0], part.get(), part2.get(), st.get(),
-                                                 tol, fixed);
-                PRG_CHECK_LAUNCH();
-            }
-            int issued = 0;
-            int batch = 24;
-            // (timer record, iteration) of this batch's launches: those past the
-            // final iteration return at once and are filed as "cg_idle"
-            std::vector<std::pair<int, int>> launched;
+            const float* B = c->div.get() + c->bases[d];
             const char* spmv_name = d == D ? "stencil_spmv_dot_finest" : "stencil_spmv_dot";
-            while (true) {
-                launched.clear();
-                for (int k = 0; k < batch; k++) {
-                    const int it = issued + k + 1;  // iteration number, 1-based
-                    const bool first = (it == 1);
-                    float* pnew = pb[(it - 1) & 1];
-                    const float* pold = pb[it & 1];
-                    {
-                        // algorithmic bytes of one SpMV row in the reference's representation:
-                        // 27 int32 neighbour indices, p_i read, (Ap)_i written (116 B)
-                        TimedScope ts(c->kt, spmv_name, kSpmvRowBytes * double(n));
-                        launched.emplace_back(ts.idx, it);
-                        k_cg_spmv<<<nbk_spmv, kBlock, 0, s>>>(pnt, n, S, R.get(), pold, pnew, Q.get(),
-                                                              first ? 1 : 0, part.get(), st.get());
-                        PRG_CHECK_LAUNCH();
-                    }
-                    const bool do_refresh = (it % refresh) == 0;
-                    {
-                        TimedScope ts(c->kt, "cg_update", 24.0 * double(n));
-                        launched.emplace_back(ts.idx, it);
-                        k_cg_update<<<nbk, kBlock, 0, s>>>(n, pnew, Q.get(), Xd, R.get(), part.get(), st.get(), tol,
-                                                           fixed, max_iters, do_refresh ? 1 : 0);
-                        PRG_CHECK_LAUNCH();
-                    }
-                    if (do_refresh) {
-                        TimedScope ts(c->kt, "cg_refresh", kSpmvRowBytes * double(n));
-                        launched.emplace_back(ts.idx, it);
-                        k_cg_refresh<<<nbk, kBlock, 0, s>>>(pnt, n, S, B, Xd, R.get(), part.get(), st.get(), tol,
-                                                            fixed, max_iters);
-                        PRG_CHECK_LAUNCH();
-                    }
+            h.launched.clear();
+            for (int k = 0; k < h.batch; k++) {
+                const int it = h.issued + k + 1;  // iteration number, 1-based
+                float* pnew = h.P[(it - 1) & 1];
+                const float* pold = h.P[it & 1];
+                double* rr_prev = h.rr + ((it - 1) & 1) * kMaxWG;
+                double* rr_next = h.rr + (it & 1) * kMaxWG;
+                {
+                    // algorithmic bytes of one SpMV row in the reference's representation:
+                    // 27 int32 neighbour indices, p_i read, (Ap)_i written (116 B)
+                    TimedScope ts(c->kt, spmv_name, kSpmvRowBytes * double(n));
+                    h.launched.emplace_back(ts.idx, it);
+                    k_cg_spmv<<<ns, kSpmvThreads, lds, h.s>>>(n, S, cnt, ub, rw, h.R, pold, pnew, h.Q, it, rr_prev,
+                                                              h.bb, nv, h.pq, h.st, tol, fixed, max_iters);
+                    PRG_CHECK_LAUNCH();
                 }
-                issued += batch;
-                PRG_HIP(hipMemcpyAsync(hst, st.get(), sizeof(CgState), hipMemcpyDeviceToHost, s));
-                PRG_HIP(hipStreamSynchronize(s));
-                for (const auto& l : launched)
-                    if (l.second > hst->iters) c->kt.relabel(l.first, "cg_idle");
-                if (hst->done || issued >= max_iters) break;
-                batch = 6;
+                const bool do_refresh = (it % refresh) == 0;
+                {
+                    TimedScope ts(c->kt, "cg_update", 24.0 * double(n));
+                    h.launched.emplace_back(ts.idx, it);
+                    k_cg_update<<<nv, kVecThreads, 0, h.s>>>(n, pnew, h.Q, h.XS, h.R, it, h.pq, ns, rr_next,
+                                                             do_refresh ? 1 : 0, h.st);
+                    PRG_CHECK_LAUNCH();
+                }
+                if (do_refresh) {
+                    TimedScope ts(c->kt, "cg_refresh", kSpmvRowBytes * double(n));
+                    h.launched.emplace_back(ts.idx, it);
+                    k_cg_refresh<<<nv, kVecThreads, 0, h.s>>>(pnt, n, S, B, h.XS, h.R, rr_next, h.st);
+                    PRG_CHECK_LAUNCH();
+                }
             }
+            h.issued += h.batch;
+            PRG_HIP(hipMemcpyAsync(h.hst, h.st, sizeof(CgState), hipMemcpyDeviceToHost, h.s));
+            PRG_HIP(hipEventRecord(h.ev, h.s));
+            h.waiting = true;
+        };
+        auto end_depth = [&](Chain& h) {
+            const int d = h.d;
+            const int64_t n = c->counts[d], base = c->bases[d];
+            const int32_t nv = vec_grid(n);
             {
                 TimedScope ts(c->kt, "cg_true_residual", kSpmvRowBytes * double(n));
-                k_true_residual<<<nbk, kBlock, 0, s>>>(pnt, n, S, B, Xd, part.get(), st.get(), outv.get() + 4 * d);
+                k_true_residual<<<nv, kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], n, stencil_of(d),
+                                                             c->div.get() + base, h.XS, h.bb, h.rr, nv, kMaxWG, h.st,
+                                                             outv + 4 * d);
                 PRG_CHECK_LAUNCH();
             }
-            PRG_HIP(hipEventRecord(e1, s));
-            PRG_HIP(hipEventSynchronize(e1));
-            float t = 0.0f;
-            PRG_HIP(hipEventElapsedTime(&t, e0, e1));
-            ms[d] = t;
+            PRG_HIP(hipMemcpyAsync(X + base, h.XS, size_t(n) * 4, hipMemcpyDeviceToDevice, h.s));
+            PRG_HIP(hipEventRecord(dend[d], h.s));
+            h.d = -1;
+        };
+
+        // Each chain: init, batches of iterations with a state read-back after
+        // each, true residual; the host polls the chains so neither waits for
+        // the other.  Serial mode runs chain 1 to completion before chain 0.
+        int first_chain = 0;
+        if (serial && !ch[1].depths.empty()) first_chain = 1;
+        for (int k = 0; k < 2; k++) {
+            Chain& h = ch[k];
+            if (h.depths.empty() || (serial && k != first_chain)) continue;
+            begin_depth(h);
+            issue_batch(h);
+        }
+        while (true) {
+            bool active = false;
+            for (int k = 0; k < 2; k++) {
+                Chain& h = ch[k];
+                if (!h.waiting) continue;
+                active = true;
+                const hipError_t q = hipEventQuery(h.ev);
+                if (q == hipErrorNotReady) continue;
+                PRG_HIP(q);
+                h.waiting = false;
+                for (const auto& l : h.launched)
+                    if (l.second > h.hst->iters) c->kt.relabel(l.first, "cg_idle");
+                if (h.hst->done || h.issued >= max_iters) {
+                    end_depth(h);
+                    if (h.next < h.depths.size()) {
+                        begin_depth(h);
+                        issue_batch(h);
+                    } else if (serial && k == 1 && !ch[0].depths.empty()) {
+                        begin_depth(ch[0]);
+                        issue_batch(ch[0]);
+                    }
+                } else {
+                    h.batch = 6;
+                    issue_batch(h);
+                }
+            }
+            if (!active) break;
         }
         if (jobs.count) {
-            PRG_HIP(hipStreamWaitEvent(s, es1, 0));
             PRG_HIP(hipEventSynchronize(es1));
             float t = 0.0f;
             PRG_HIP(hipEventElapsedTime(&t, es0, es1));
@@ -577,16 +791,28 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             c->kt.recs["cg_small"].launches += 1;
             c->kt.recs["cg_small"].total_ms += t;
         }
-        PRG_HIP(hipMemcpyAsync(ov.data(), outv.get(), ov.size() * 8, hipMemcpyDeviceToHost, s));
+        for (int d = 0; d <= D; d++)
+            if (dbeg[d]) {
+                PRG_HIP(hipEventSynchronize(dend[d]));
+                float t = 0.0f;
+                PRG_HIP(hipEventElapsedTime(&t, dbeg[d], dend[d]));
+                ms[d] = t;
+            }
+        // the context stream continues after both chains and the small solves
+        hipEvent_t ej = new_event();
+        PRG_HIP(hipEventRecord(ej, c->stream2));
+        PRG_HIP(hipStreamWaitEvent(s, ej, 0));
+        PRG_HIP(hipMemcpyAsync(ov.data(), outv, ov.size() * 8, hipMemcpyDeviceToHost, s));
         PRG_HIP(hipStreamSynchronize(s));
     } catch (...) {
         (void)hipStreamSynchronize(c->stream2);
-        (void)hipHostFree(hst);
-        for (hipEvent_t e : {e0, e1, es0, es1}) (void)hipEventDestroy(e);
+        (void)hipStreamSynchronize(s);
+        if (hst) (void)hipHostFree(hst);
+        for (hipEvent_t e : evs) (void)hipEventDestroy(e);
         throw;
     }
     (void)hipHostFree(hst);
-    for (hipEvent_t e : {e0, e1, es0, es1}) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     for (int d = 0; d <= D; d++) {
         const int32_t iters = int32_t(ov[4 * d + 2]);
         if (stats) {
