@@ -108,6 +108,94 @@ def cpu_baseline(p, n, D, sample):
                       f"({dt:.1f} s, OpenMP threads={oracle.num_threads()})"}
 
 
+def learned_leg(args):
+    """Learned warm start vs plain CG at equal residual (1e-6 relative at every depth).
+
+    Training and held-out shapes: unions of 1-4 seeded superquadrics
+    (seeds 1000 + k), 200,000 points each, depth 9.  The network works on the
+    64^3 grid of depth 6; its training pairs (divergence, solution) come from
+    the solver itself.  Solve ms is the solve stage's wall time, including the
+    network's inference for the learned runs."""
+    import poissonrecon_gpu_amd as prg
+
+    D, gd = 9, 6
+    ntr, nte = args.learn_train, args.learn_test
+    g = prg.Reconstructor(max_depth=D)
+    B, M, X, S = [], [], [], []
+    t0 = time.perf_counter()
+    for k in range(ntr + nte):
+        p, n = prg.synth_superquadrics(args.learn_points, 1 + k % 4, 0.001, 1000 + k)
+        g.set_points(p, n)
+        g.build_octree()
+        g.compute_divergence()
+        g.solve()
+        b, rms = g.dense_grid(gd, 0)
+        B.append(b)
+        M.append(g.dense_grid(gd, 2)[0])
+        X.append(g.dense_grid(gd, 1)[0])
+        S.append(rms)
+    g.close()
+    t_data = time.perf_counter() - t0
+    B, M, X, S = np.stack(B), np.stack(M), np.stack(X), np.array(S, np.float32)
+    net = prg.UNet(grid=1 << gd, seed=1)
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(0)
+    losses = []
+    for ep in range(args.learn_epochs):
+        order = rng.permutation(ntr)
+        lr = args.learn_lr * (0.5 ** (ep // max(1, args.learn_epochs // 3)))
+        el = []
+        for i in range(0, ntr, args.learn_batch):
+            idx = order[i:i + args.learn_batch]
+            el.append(net.train_step(B[idx], M[idx], X[idx], S[idx], lr))
+        losses.append(float(np.mean(el)))
+    t_train = time.perf_counter() - t0
+    # held-out: plain and learned solves of the same systems
+    plain = prg.Reconstructor(max_depth=D)
+    warm = prg.Reconstructor(max_depth=D, warm_start=prg.WARM_LEARNED)
+    warm.set_warm_start_net(net)
+    res = {"plain": [], "learned": []}
+    for k in range(ntr, ntr + nte):
+        p, n = prg.synth_superquadrics(args.learn_points, 1 + k % 4, 0.001, 1000 + k)
+        for name, r in (("plain", plain), ("learned", warm)):
+            r.set_points(p, n)
+            r.build_octree()
+            r.compute_divergence()
+            r.solve()  # warm-up of this shape's buffers
+            st = r.solve()
+            res[name].append((r.stage_times()["solve"], st[gd]["iters"], sum(s["iters"] for s in st),
+                              max(s["true_rel_residual"] for s in st)))
+    plain.close()
+    warm.close()
+    # held-out prediction error at depth 6 (relative L2 on the nodes)
+    errs = []
+    for k in range(ntr, ntr + nte):
+        pred = net.predict(B[k], M[k], float(S[k]))
+        m = M[k] > 0
+        errs.append(float(np.linalg.norm((pred - X[k])[m]) / max(np.linalg.norm(X[k][m]), 1e-30)))
+    net.close()
+    mp = np.array(res["plain"])
+    ml = np.array(res["learned"])
+    return {
+        "protocol": f"{ntr} training + {nte} held-out unions of 1-4 superquadrics, {args.learn_points} points, "
+                    f"depth {D}, network on the 64^3 grid of depth {gd}, {args.learn_epochs} epochs, batch "
+                    f"{args.learn_batch}, Adam lr {args.learn_lr}",
+        "solve_ms_plain": round(float(mp[:, 0].mean()), 3),
+        "solve_ms_learned": round(float(ml[:, 0].mean()), 3),
+        "speedup": round(float(mp[:, 0].mean() / ml[:, 0].mean()), 4),
+        "depth6_iters_plain": round(float(mp[:, 1].mean()), 2),
+        "depth6_iters_learned": round(float(ml[:, 1].mean()), 2),
+        "total_iters_plain": round(float(mp[:, 2].mean()), 2),
+        "total_iters_learned": round(float(ml[:, 2].mean()), 2),
+        "max_true_rel_residual_plain": float(mp[:, 3].max()),
+        "max_true_rel_residual_learned": float(ml[:, 3].max()),
+        "heldout_depth6_prediction_rel_l2": round(float(np.mean(errs)), 4),
+        "train_loss_by_epoch": [round(x, 5) for x in losses],
+        "data_s": round(t_data, 1),
+        "train_s": round(t_train, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +205,13 @@ def main():
     ap.add_argument("--points", type=int, default=2_000_000)
     ap.add_argument("--cpu-sample", type=int, default=400_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--learn-train", type=int, default=512)
+    ap.add_argument("--learn-test", type=int, default=64)
+    ap.add_argument("--learn-points", type=int, default=200_000)
+    ap.add_argument("--learn-epochs", type=int, default=6)
+    ap.add_argument("--learn-batch", type=int, default=8)
+    ap.add_argument("--learn-lr", type=float, default=1e-3)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="end with a 512 MiB device-to-device copy (a copy kernel of known size for counter scaling)")
     args = ap.parse_args()
@@ -195,12 +290,14 @@ def main():
         roofline["traffic"] = t.get("hbm_bytes_per_launch")
         roofline["traffic_source"] = str(traffic_file.relative_to(ROOT))
 
+    learned = None if args.no_learned else learned_leg(args)
+
     base = None
     if not args.no_cpu:
         base = cpu_baseline(p, nrm, D, min(args.cpu_sample, N))
 
     out = {
-        "metric": "points/s end-to-end at depth 10",
+        "metric": "points/s end-to-end at depth 10; solve ms learned vs plain CG at equal residual",
         "value": round(N / dt, 1),
         "unit": "points/s",
         "n_gpus": 1,
@@ -221,6 +318,7 @@ def main():
         "kernels": kern,
         "roofline": roofline,
         "cpu_baseline": base,
+        "learned_warm_start": learned,
     }
     print(json.dumps(out))
 

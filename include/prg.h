@@ -35,9 +35,9 @@ typedef enum prg_status {
 } prg_status;
 
 typedef enum prg_warm_start {
-    PRG_WARM_NONE = 0,    /* x0 = 0 at every depth (reference behaviour, main.cu:1237-1329) */
-    PRG_WARM_PROLONG = 1, /* x0 at depth d+1 = B-spline prolongation of x at depth d          */
-    PRG_WARM_LEARNED = 2  /* coarse-grid U-Net prediction, then prolongation                 */
+    PRG_WARM_NONE = 0,   /* x0 = 0 at every depth (reference behaviour, main.cu:1237-1329)        */
+    PRG_WARM_LEARNED = 2 /* the attached U-Net's prediction at its grid's depth (prg_unet.h);
+                            every other depth from zero                                          */
 } prg_warm_start;
 
 /* Runtime configuration; replaces the compile-time #defines of main.cu:29-78. */

@@ -38,7 +38,7 @@ BUFFER_DTYPES = {
 STAGES = ["normalize", "octree", "tables", "vector_field", "divergence", "solve", "isovalue",
           "mc_topology", "mc_eval", "subdivide", "mesh_copy"]
 
-WARM_NONE, WARM_PROLONG, WARM_LEARNED = 0, 1, 2
+WARM_NONE, WARM_LEARNED = 0, 2
 
 
 class PrgConfig(C.Structure):
@@ -102,6 +102,18 @@ SIGNATURES = {
     "prg_read_points": (C.c_int, [C.c_char_p, C.POINTER(_F32P), C.POINTER(_F32P), C.POINTER(C.c_int64)]),
     "prg_points_release": (None, [_F32P, _F32P]),
     "prg_write_mesh_ply": (C.c_int, [C.c_char_p, C.POINTER(PrgMesh)]),
+    "prg_unet_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.c_int32, C.c_uint64]),
+    "prg_unet_destroy": (None, [_P]),
+    "prg_unet_last_error": (C.c_char_p, [_P]),
+    "prg_unet_num_params": (C.c_int64, [_P]),
+    "prg_unet_grid": (C.c_int32, [_P]),
+    "prg_unet_get_params": (C.c_int, [_P, _P, C.c_int64]),
+    "prg_unet_set_params": (C.c_int, [_P, _P, C.c_int64]),
+    "prg_unet_predict": (C.c_int, [_P, _P, _P, C.c_float, _P]),
+    "prg_unet_gradients": (C.c_int, [_P, _P, _P, _P, C.c_float, _P, C.POINTER(C.c_double)]),
+    "prg_unet_train_step": (C.c_int, [_P, _P, _P, _P, _P, C.c_int32, C.c_float, C.POINTER(C.c_double)]),
+    "prg_dense_grid": (C.c_int, [_P, C.c_int32, C.c_int32, _P, C.POINTER(C.c_float)]),
+    "prg_set_warm_start_net": (C.c_int, [_P, _P]),
     "prg_synth_sphere": (C.c_int, [C.c_int64, C.c_uint64, _P, _P]),
     "prg_synth_torus": (C.c_int, [C.c_int64, C.c_uint64, _P, _P]),
     "prg_synth_superquadrics": (C.c_int, [C.c_int64, C.c_int32, C.c_float, C.c_uint64, _P, _P]),
@@ -267,6 +279,18 @@ class Reconstructor:
         self._call("prg_subdivision_info", C.byref(a), C.byref(b))
         return a.value, b.value
 
+    def dense_grid(self, depth: int, which: int) -> tuple[np.ndarray, float]:
+        """Dense (G, G, G) grid of one depth (0 divergence, 1 solution, 2 node mask), and
+        the divergence's root-mean-square over the depth's nodes."""
+        G = 1 << depth
+        out = np.zeros((G, G, G), np.float32)
+        rms = C.c_float()
+        self._call("prg_dense_grid", depth, which, _ptr(out), C.byref(rms))
+        return out, rms.value
+
+    def set_warm_start_net(self, net: "UNet | None") -> None:
+        self._call("prg_set_warm_start_net", net._h if net is not None else None)
+
     # ---- introspection ----
     def level_info(self) -> tuple[np.ndarray, np.ndarray]:
         counts = (C.c_int64 * (self.max_depth + 1))()
@@ -328,3 +352,68 @@ def write_mesh_ply(path: str, mesh: Mesh) -> None:
     m.center[:] = list(mesh.center)
     m.scale = mesh.scale
     _check(lib().prg_write_mesh_ply(path.encode(), C.byref(m)), f"prg_write_mesh_ply({path})")
+
+
+class UNet:
+    """The learned warm start's network (include/prg_unet.h), trained and run on the GPU."""
+
+    def __init__(self, grid: int = 64, device: int = 0, seed: int = 1):
+        self._h = C.c_void_p()
+        st = lib().prg_unet_create(C.byref(self._h), device, grid, seed)
+        if st != OK:
+            raise PrgError(st, "prg_unet_create")
+        self.grid = grid
+
+    def close(self) -> None:
+        if self._h:
+            lib().prg_unet_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _call(self, name: str, *args) -> None:
+        st = getattr(lib(), name)(self._h, *args)
+        if st != OK:
+            raise PrgError(st, f"{name}: {lib().prg_unet_last_error(self._h).decode()}")
+
+    @property
+    def num_params(self) -> int:
+        return int(lib().prg_unet_num_params(self._h))
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.num_params, np.float32)
+        self._call("prg_unet_get_params", _ptr(out), out.size)
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        self._call("prg_unet_set_params", _ptr(p), p.size)
+
+    def predict(self, div_grid: np.ndarray, mask_grid: np.ndarray, scale: float) -> np.ndarray:
+        b = np.ascontiguousarray(div_grid, np.float32)
+        m = np.ascontiguousarray(mask_grid, np.float32)
+        out = np.zeros_like(b)
+        self._call("prg_unet_predict", _ptr(b), _ptr(m), C.c_float(scale), _ptr(out))
+        return out
+
+    def gradients(self, div_grid, mask_grid, target_grid, scale: float) -> tuple[float, np.ndarray]:
+        b = np.ascontiguousarray(div_grid, np.float32)
+        m = np.ascontiguousarray(mask_grid, np.float32)
+        t = np.ascontiguousarray(target_grid, np.float32)
+        g = np.zeros(self.num_params, np.float32)
+        loss = C.c_double()
+        self._call("prg_unet_gradients", _ptr(b), _ptr(m), _ptr(t), C.c_float(scale), _ptr(g), C.byref(loss))
+        return loss.value, g
+
+    def train_step(self, div_grids, mask_grids, target_grids, scales, lr: float) -> float:
+        b = np.ascontiguousarray(div_grids, np.float32)
+        m = np.ascontiguousarray(mask_grids, np.float32)
+        t = np.ascontiguousarray(target_grids, np.float32)
+        s = np.ascontiguousarray(scales, np.float32)
+        loss = C.c_double()
+        self._call("prg_unet_train_step", _ptr(b), _ptr(m), _ptr(t), _ptr(s), len(s), C.c_float(lr), C.byref(loss))
+        return loss.value

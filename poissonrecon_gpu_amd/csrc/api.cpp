@@ -103,6 +103,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
     const prg_config& cf = cfg ? *cfg : c0;
     if (cf.max_depth < 2 || cf.max_depth > prg::kMaxDepth) return PRG_E_ARG;
     if (!(cf.cg_tol >= 0.0f) || cf.cg_max_iters < 1) return PRG_E_ARG;
+    if (cf.warm_start != PRG_WARM_NONE && cf.warm_start != PRG_WARM_LEARNED) return PRG_E_ARG;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PRG_E_NODEVICE;
     if (cf.device < 0 || cf.device >= ndev) return PRG_E_ARG;

@@ -58,6 +58,16 @@ struct DBuf {
     T* get() const { return p; }
 };
 
+// A typed view of one slot of a context's buffer pool: grow-only device
+// memory kept for the context's lifetime, so a repeated stage allocates
+// nothing.  Each function uses its own slot numbers.
+template <class T>
+struct PoolBuf {
+    DBuf<uint8_t>* b;
+    T* reserve(size_t count) { return reinterpret_cast<T*>(b->reserve((count ? count : 1) * sizeof(T))); }
+    T* get() const { return reinterpret_cast<T*>(b->get()); }
+};
+
 // ---- launch helpers --------------------------------------------------------------
 inline unsigned grid_for(int64_t n, int block) {
     int64_t g = (n + block - 1) / block;

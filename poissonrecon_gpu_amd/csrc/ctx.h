@@ -19,6 +19,8 @@
 
 #include "common.h"
 
+struct prg_unet;
+
 struct prg_ctx {
     prg_config cfg{};
     int D = 8;
@@ -65,6 +67,9 @@ struct prg_ctx {
     int64_t plan_tile_off[prg::kMaxDepth + 2] = {};
     int32_t plan_maxcount[prg::kMaxDepth + 1] = {};
     uint64_t tree_gen = 0, plan_gen = 0;
+    prg_unet* warm_net = nullptr;  // learned warm start (not owned)
+    // per-stage temporaries (slots: octree 0-59, divergence 60-75, marching cubes 76-95)
+    prg::DBuf<uint8_t> pool[128];
     float iso = 0.0f;
     bool iso_ready = false;  // iso computed or set by the caller
 
@@ -98,6 +103,9 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats);
 void compute_isovalue(prg_ctx* c);
 void extract_leaf_mesh(prg_ctx* c);
 void extract_subdivision(prg_ctx* c);
+// learned warm start: writes the network's prediction for depth `depth` into
+// the solution vector x (device), on the context stream
+void unet_warm_start(prg_ctx* c, int depth);
 
 // Measures one stage with hipEvents on the context stream.
 struct StageTimer {

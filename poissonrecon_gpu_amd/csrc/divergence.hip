@@ -328,7 +328,7 @@ void compute_divergence(prg_ctx* c) {
     }
     StageTimer st(c, PRG_STAGE_DIVERGENCE);
     float* DV = c->div.reserve(M);
-    DBuf<int64_t> toff;
+    PoolBuf<int64_t> toff{&c->pool[60]};
     PRG_HIP(hipMemcpyAsync(toff.reserve(c->fs->div_off.size()), c->fs->div_off.data(), c->fs->div_off.size() * 8,
                            hipMemcpyHostToDevice, s));
     {
@@ -339,8 +339,9 @@ void compute_divergence(prg_ctx* c) {
     }
     const int64_t n = baseD;  // nodes of depths 0 .. D-1
     if (n > 0) {
-        DBuf<int32_t> chunks, multi, coff, moff, cnode, ctot, mtot;
-        DBuf<double> G, partial;
+        PoolBuf<int32_t> chunks{&c->pool[61]}, multi{&c->pool[62]}, coff{&c->pool[63]}, moff{&c->pool[64]},
+            cnode{&c->pool[65]}, ctot{&c->pool[66]}, mtot{&c->pool[67]};
+        PoolBuf<double> G{&c->pool[68]}, partial{&c->pool[69]};
         chunks.reserve(n);
         multi.reserve(n);
         coff.reserve(n);
@@ -374,7 +375,7 @@ void compute_divergence(prg_ctx* c) {
             PRG_CHECK_LAUNCH();
         }
         if (nmulti > 0) {
-            DBuf<int32_t> mflag, mpos, mcnt, mlist;
+            PoolBuf<int32_t> mflag{&c->pool[70]}, mpos{&c->pool[71]}, mcnt{&c->pool[72]}, mlist{&c->pool[73]};
             mflag.reserve(n);
             mpos.reserve(n);
             mcnt.reserve(1);

@@ -231,7 +231,7 @@ void compute_isovalue(prg_ctx* c) {
     hipStream_t s = c->stream;
     const int64_t n = c->N;
     const unsigned nb = grid_for(n, kBlock);
-    DBuf<double> part, tot;
+    PoolBuf<double> part{&c->pool[76]}, tot{&c->pool[77]};
     part.reserve(nb);
     tot.reserve(1);
     {
@@ -257,8 +257,10 @@ void extract_leaf_mesh(prg_ctx* c) {
     const int D = c->D;
     const int64_t M = c->M, MD = c->MD(), baseD = c->bases[D];
     KernelTimer& kt = c->kt;
-    DBuf<int32_t> own, oref, vid, vtot, eflag, eref, evid, etot, ntri, toff, ttot;
-    DBuf<float> vval;
+    PoolBuf<int32_t> own{&c->pool[78]}, oref{&c->pool[79]}, vid{&c->pool[80]}, vtot{&c->pool[81]},
+        eflag{&c->pool[82]}, eref{&c->pool[83]}, evid{&c->pool[84]}, etot{&c->pool[85]}, ntri{&c->pool[86]},
+        toff{&c->pool[87]}, ttot{&c->pool[88]};
+    PoolBuf<float> vval{&c->pool[89]};
     float* CV = c->corner_vals.reserve(8 * MD);
     int32_t V = 0, NV = 0, NT = 0;
     {
@@ -280,7 +282,7 @@ void extract_leaf_mesh(prg_ctx* c) {
     {
         StageTimer st(c, PRG_STAGE_MC_EVAL);
         vval.reserve(V);
-        DBuf<int32_t> vlist;
+        PoolBuf<int32_t> vlist{&c->pool[90]};
         vlist.reserve(V);
         k_corner_list<<<grid_for(8 * MD, kBlock), kBlock, 0, s>>>(own.get(), vid.get(), 8 * MD, vlist.get());
         PRG_CHECK_LAUNCH();

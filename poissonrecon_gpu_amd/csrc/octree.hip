@@ -320,7 +320,7 @@ __global__ void __launch_bounds__(kBlock) k_parent_neighbor_table(const int32_t*
 
 void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n, bool on_device) {
     hipStream_t s = c->stream;
-    DBuf<float> hxyz, hn;
+    PoolBuf<float> hxyz{&c->pool[0]}, hn{&c->pool[1]};
     const float* dxyz = xyz;
     const float* dn = nxyz;
     if (!on_device) {
@@ -330,7 +330,7 @@ void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n
         dn = hn.get();
     }
     const int blocks = int(std::min<int64_t>(1024, (n + kBlock - 1) / kBlock));
-    DBuf<float> part;
+    PoolBuf<float> part{&c->pool[2]};
     part.reserve(size_t(6 * blocks));
     {
         TimedScope ts(c->kt, "bbox_partial", 12.0 * double(n));
@@ -383,7 +383,7 @@ void build_octree(prg_ctx* c) {
     KernelTimer& kt = c->kt;
 
     // 1. keys + stable sort
-    DBuf<uint32_t> key, val, key2, val2;
+    PoolBuf<uint32_t> key{&c->pool[3]}, val{&c->pool[4]}, key2{&c->pool[5]}, val2{&c->pool[6]};
     key.reserve(n);
     val.reserve(n);
     key2.reserve(n);
@@ -417,7 +417,7 @@ void build_octree(prg_ctx* c) {
     std::swap(c->nz.n, c->tnz.n);
 
     // 2. unique leaves
-    DBuf<int32_t> flag, rank, tot;
+    PoolBuf<int32_t> flag{&c->pool[7]}, rank{&c->pool[8]}, tot{&c->pool[9]};
     flag.reserve(n);
     rank.reserve(n);
     tot.reserve(1);
@@ -432,14 +432,14 @@ void build_octree(prg_ctx* c) {
     PRG_HIP(hipStreamSynchronize(s));
 
     // per-depth compact arrays of non-empty nodes
-    std::vector<std::unique_ptr<DBuf<uint32_t>>> NK(D + 1);
-    std::vector<std::unique_ptr<DBuf<int32_t>>> NP(D + 1), NDI(D + 1), POS(D + 1);
+    std::vector<std::unique_ptr<PoolBuf<uint32_t>>> NK(D + 1);
+    std::vector<std::unique_ptr<PoolBuf<int32_t>>> NP(D + 1), NDI(D + 1), POS(D + 1);
     std::vector<int64_t> NE(D + 1, 0);
     for (int d = 0; d <= D; d++) {
-        NK[d].reset(new DBuf<uint32_t>());
-        NP[d].reset(new DBuf<int32_t>());
-        NDI[d].reset(new DBuf<int32_t>());
-        POS[d].reset(new DBuf<int32_t>());
+        NK[d].reset(new PoolBuf<uint32_t>{&c->pool[16 + 4 * d]});
+        NP[d].reset(new PoolBuf<int32_t>{&c->pool[17 + 4 * d]});
+        NDI[d].reset(new PoolBuf<int32_t>{&c->pool[18 + 4 * d]});
+        POS[d].reset(new PoolBuf<int32_t>{&c->pool[19 + 4 * d]});
     }
     NE[D] = U;
     NK[D]->reserve(U);
@@ -451,12 +451,12 @@ void build_octree(prg_ctx* c) {
                                                          NP[D]->get());
         PRG_CHECK_LAUNCH();
     }
-    DBuf<int32_t> prank;  // unique-leaf rank of every sorted point
+    PoolBuf<int32_t> prank{&c->pool[10]};  // unique-leaf rank of every sorted point
     prank.reserve(n);
 
     // 3. levels bottom-up
     std::vector<int64_t> counts(D + 1, 0);
-    DBuf<int32_t> bflag, brank, btot;
+    PoolBuf<int32_t> bflag{&c->pool[11]}, brank{&c->pool[12]}, btot{&c->pool[13]};
     btot.reserve(1);
     for (int d = D; d >= 1; --d) {
         const int64_t ne = NE[d];

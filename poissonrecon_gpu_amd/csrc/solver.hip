@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "../../include/prg_unet.h"
 #include "ctx.h"
 
 namespace prg {
@@ -237,6 +238,7 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, 
                                                           int32_t max_iters) {
     extern __shared__ float cellv[];
     __shared__ int16_t rid[kTileEntries];
+    __shared__ float grid[kTile][64];  // each block's 4x4x4 neighbourhood, (x, y, z) = (c >> 4, (c >> 2) & 3, c & 3)
     __shared__ double lds[kSpmvThreads / 64];
     if (st->done) return;
     const int k1 = it - 1;
@@ -262,7 +264,8 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, 
     const int64_t ntiles = (nblk + kTile - 1) / kTile;
     const int64_t per = (ntiles + gridDim.x - 1) / gridDim.x;
     const int64_t tbeg = int64_t(blockIdx.x) * per, tend = tbeg + per < ntiles ? tbeg + per : ntiles;
-    const int sx = (s >> 2) & 1, sy = (s >> 1) & 1, sz = s & 1;
+    // this lane's row sits at (1 + sx, 1 + sy, 1 + sz) of its block's grid
+    const int c0 = 16 * ((s >> 2) & 1) + 4 * ((s >> 1) & 1) + (s & 1);
     double pq = 0.0;
     for (int64_t tile = tbeg; tile < tend; tile++) {
         const int16_t* trow = rowid + tile * kTileEntries;
@@ -274,22 +277,30 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, 
             cellv[u] = first ? pnew[idx] : pdir(r, pold, bf, idx);
         }
         __syncthreads();
+        // lanes of block g fill its 4x4x4 grid: cell (x, y, z) lies in parent
+        // neighbour (x >> 1, y >> 1, z >> 1) at child slot (x & 1, y & 1, z & 1)
+        // of the grid whose origin is the block's (-1, -1, -1) cell
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+            const int cc = s + 8 * m;
+            const int ax = (cc >> 4) + 1, ay = ((cc >> 2) & 3) + 1, az = (cc & 3) + 1;
+            const int id = rid[27 * g + 9 * (ax >> 1) + 3 * (ay >> 1) + (az >> 1)];
+            grid[g][cc] = id < 0 ? 0.0f : cellv[8 * id + (((ax & 1) << 2) | ((ay & 1) << 1) | (az & 1))];
+        }
+        __syncthreads();
         const int64_t blk = tile * kTile + g;
         if (blk < nblk) {
-            const int16_t* rg = rid + 27 * g;
+            // absent neighbours hold +0: adding L*0 leaves a float sum
+            // unchanged, so the result equals the sum over present terms
+            const float* gc = grid[g] + c0;
             float out = 0.0f;
 #pragma unroll
             for (int j = 0; j < 27; j++) {
-                const int vx = sx + (j / 9 - 1), vy = sy + ((j / 3) % 3 - 1), vz = sz + (j % 3 - 1);
-                const int pj = 9 * ((vx >> 1) + 1) + 3 * ((vy >> 1) + 1) + ((vz >> 1) + 1);
-                const int cs = ((vx & 1) << 2) | ((vy & 1) << 1) | (vz & 1);
-                const int id = rg[pj];
-                if (id < 0 || S.L[j] == 0.0f) continue;
-                const float prod = S.L[j] * cellv[8 * id + cs];
+                const float prod = S.L[j] * gc[16 * (j / 9) + 4 * ((j / 3) % 3) + (j % 3)];
                 out = out + prod;
             }
             const int64_t t = 8 * blk + s;
-            const float pt = cellv[8 * rg[13] + s];
+            const float pt = gc[16 + 4 + 1];
             if (!first) pnew[t] = pt;
             q[t] = out;
             pq += double(pt) * double(out);
@@ -580,7 +591,14 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     const int64_t M = c->M;
     ensure_plans(c);
     float* X = c->x.reserve(M);
-    if (c->cfg.warm_start == PRG_WARM_NONE) PRG_HIP(hipMemsetAsync(X, 0, size_t(M) * 4, s));
+    PRG_HIP(hipMemsetAsync(X, 0, size_t(M) * 4, s));
+    if (c->cfg.warm_start == PRG_WARM_LEARNED) {
+        // the network predicts the depth of its grid; the other depths start from zero
+        if (!c->warm_net) throw Error{PRG_E_STATE, "PRG_WARM_LEARNED needs prg_set_warm_start_net"};
+        int g = 0;
+        while ((int64_t(1) << g) < prg_unet_grid(c->warm_net)) g++;
+        if (g <= D && c->counts[g] > 0) unet_warm_start(c, g);
+    }
     double* outv = c->cg_out.reserve(size_t(4) * (D + 1));
     PRG_HIP(hipMemsetAsync(outv, 0, size_t(4) * (D + 1) * 8, s));
     const float tol = c->cfg.cg_tol;
