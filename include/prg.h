@@ -204,6 +204,29 @@ prg_status prg_subdivision_info(prg_ctx* ctx, int64_t* n_roots, int64_t* n_virtu
 /* Whole pipeline: build, divergence, solve, isovalue, mesh. */
 prg_status prg_reconstruct(prg_ctx* ctx, int32_t subdivide, prg_mesh* out, prg_solve_stats* per_depth);
 
+/* ---- many clouds in one pass ---------------------------------------------
+ * The reference reconstructs one cloud per process run (main.cu:3247-4573).
+ * prg_reconstruct_batch runs a list of independent clouds: `concurrency`
+ * worker threads each own a context on `device` and take the next cloud from
+ * a queue ordered by decreasing points x depth, so several clouds are in
+ * flight on the GPU at once.  Each cloud's mesh is written to its item
+ * (release with prg_mesh_release); a failed cloud reports its status there
+ * and the others continue. */
+typedef struct prg_batch_item {
+    const float* xyz;       /* 3 * n floats, host memory                         */
+    const float* nxyz;      /* 3 * n floats, host memory                         */
+    int64_t n;
+    int32_t max_depth;      /* 2 .. 10                                           */
+    int32_t status;         /* out: prg_status of this cloud                     */
+    prg_mesh mesh;          /* out                                                */
+    double ms;              /* out: wall time of this cloud on its worker        */
+    double max_true_rel_residual; /* out: largest per-depth ||b - Ax|| / ||b||    */
+    int32_t total_iters;    /* out: CG iterations summed over depths             */
+} prg_batch_item;
+
+prg_status prg_reconstruct_batch(prg_batch_item* items, int64_t count, const prg_config* config,
+                                 int32_t concurrency, int32_t subdivide);
+
 /* ---- introspection ----------------------------------------------------- */
 
 prg_status prg_stage_times(prg_ctx* ctx, prg_timings* out);

@@ -59,6 +59,12 @@ class PrgMesh(C.Structure):
                 ("n_leaf_vertices", C.c_int64), ("n_leaf_triangles", C.c_int64)]
 
 
+class PrgBatchItem(C.Structure):
+    _fields_ = [("xyz", C.c_void_p), ("nxyz", C.c_void_p), ("n", C.c_int64), ("max_depth", C.c_int32),
+                ("status", C.c_int32), ("mesh", PrgMesh), ("ms", C.c_double),
+                ("max_true_rel_residual", C.c_double), ("total_iters", C.c_int32)]
+
+
 class PrgTimings(C.Structure):
     _fields_ = [("ms", C.c_double * 11)]
 
@@ -102,6 +108,8 @@ SIGNATURES = {
     "prg_read_points": (C.c_int, [C.c_char_p, C.POINTER(_F32P), C.POINTER(_F32P), C.POINTER(C.c_int64)]),
     "prg_points_release": (None, [_F32P, _F32P]),
     "prg_write_mesh_ply": (C.c_int, [C.c_char_p, C.POINTER(PrgMesh)]),
+    "prg_reconstruct_batch": (C.c_int, [C.POINTER(PrgBatchItem), C.c_int64, C.POINTER(PrgConfig), C.c_int32,
+                                         C.c_int32]),
     "prg_unet_create": (C.c_int, [C.POINTER(_P), C.c_int32, C.c_int32, C.c_uint64]),
     "prg_unet_destroy": (None, [_P]),
     "prg_unet_last_error": (C.c_char_p, [_P]),
@@ -417,3 +425,37 @@ class UNet:
         loss = C.c_double()
         self._call("prg_unet_train_step", _ptr(b), _ptr(m), _ptr(t), _ptr(s), len(s), C.c_float(lr), C.byref(loss))
         return loss.value
+
+
+def reconstruct_batch(clouds, depths, concurrency: int = 4, subdivide: bool = False, device: int = 0,
+                      cg_tol: float = 1e-6, return_meshes: bool = False):
+    """Reconstructs independent clouds [(xyz, nxyz), ...] in one pass
+    (prg_reconstruct_batch).  Returns per-cloud dicts: status, mesh sizes, ms,
+    the largest per-depth true relative residual and the CG iterations."""
+    n = len(clouds)
+    items = (PrgBatchItem * n)()
+    keep = []
+    for i, ((p, q), d) in enumerate(zip(clouds, depths)):
+        p = np.ascontiguousarray(p, np.float32)
+        q = np.ascontiguousarray(q, np.float32)
+        keep.append((p, q))
+        items[i].xyz = p.ctypes.data
+        items[i].nxyz = q.ctypes.data
+        items[i].n = len(p)
+        items[i].max_depth = int(d)
+    cfg = PrgConfig()
+    lib().prg_config_default(C.byref(cfg))
+    cfg.device, cfg.cg_tol = device, cg_tol
+    st = lib().prg_reconstruct_batch(items, n, C.byref(cfg), concurrency, int(subdivide))
+    out = []
+    for it in items:
+        r = dict(status=int(it.status), vertices=int(it.mesh.n_vertices), triangles=int(it.mesh.n_triangles),
+                 ms=float(it.ms), max_true_rel_residual=float(it.max_true_rel_residual),
+                 total_iters=int(it.total_iters))
+        if return_meshes and it.mesh.vertices:
+            nv, nt = it.mesh.n_vertices, it.mesh.n_triangles
+            r["mesh_vertices"] = np.ctypeslib.as_array(it.mesh.vertices, shape=(max(nv, 1) * 3,))[: 3 * nv].reshape(-1, 3).copy()
+            r["mesh_triangles"] = np.ctypeslib.as_array(it.mesh.triangles, shape=(max(nt, 1) * 3,))[: 3 * nt].reshape(-1, 3).copy()
+        out.append(r)
+        lib().prg_mesh_release(C.byref(it.mesh))
+    return st, out

@@ -66,7 +66,7 @@ def build_library(jobs: int = 8) -> Path:
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, *DEVICE, "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, *DEVICE, "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lpthread"])
     os.replace(tmp, LIB)
     return LIB
 

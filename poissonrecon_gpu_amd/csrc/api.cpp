@@ -117,6 +117,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         PRG_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         PRG_HIP(hipEventCreate(&c->ev_a));
         PRG_HIP(hipEventCreate(&c->ev_b));
+        PRG_HIP(hipHostMalloc(&c->transform_host, 16, hipHostMallocDefault));
         c->kt.enabled = cf.timing != 0;
         c->kt.stream = c->stream;
         return PRG_OK;
@@ -136,6 +137,7 @@ void prg_destroy(prg_ctx* c) {
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->ev_a) (void)hipEventDestroy(c->ev_a);
     if (c->ev_b) (void)hipEventDestroy(c->ev_b);
+    if (c->transform_host) (void)hipHostFree(c->transform_host);
     hipStream_t s = c->stream, s2 = c->stream2;
     delete c;  // releases device buffers
     if (s) (void)hipStreamDestroy(s);
@@ -259,6 +261,15 @@ prg_status prg_set_isovalue(prg_ctx* c, float iso) {
 
 namespace {
 
+// The host copy of the normalisation transform, once its copy has landed.
+void resolve_transform(prg_ctx* c) {
+    if (!c->transform_pending) return;
+    PRG_HIP(hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; a++) c->center[a] = c->transform_host[a];
+    c->scale = c->transform_host[3];
+    c->transform_pending = false;
+}
+
 // Leaf-level marching cubes, then (subdivide != 0) the coarse-leaf
 // subdivision batches; sets the counts of `out`.
 void build_mesh(prg_ctx* c, int32_t subdivide, prg_mesh* out) {
@@ -267,6 +278,7 @@ void build_mesh(prg_ctx* c, int32_t subdivide, prg_mesh* out) {
     c->sub_nv = c->sub_nt = 0;
     if (subdivide) prg::extract_subdivision(c);
     std::memset(out, 0, sizeof(*out));
+    resolve_transform(c);
     out->n_leaf_vertices = c->mesh_nv;
     out->n_leaf_triangles = c->mesh_nt;
     out->n_vertices = c->mesh_nv + c->sub_nv;
@@ -464,6 +476,7 @@ prg_status prg_kernel_stats_reset(prg_ctx* c) {
 prg_status prg_get_transform(prg_ctx* c, float* center3, float* scale) {
     return guarded(c, [&]() {
         if (!c->have_points) return fail(c, PRG_E_STATE, "prg_set_points first");
+        resolve_transform(c);
         if (center3)
             for (int a = 0; a < 3; a++) center3[a] = c->center[a];
         if (scale) *scale = c->scale;

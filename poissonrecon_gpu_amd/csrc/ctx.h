@@ -68,6 +68,11 @@ struct prg_ctx {
     int32_t plan_maxcount[prg::kMaxDepth + 1] = {};
     uint64_t tree_gen = 0, plan_gen = 0;
     prg_unet* warm_net = nullptr;  // learned warm start (not owned)
+    // normalisation transform (centre x, y, z, scale) computed on the device;
+    // the host copy is read at the next synchronisation
+    prg::DBuf<float> transform_dev;
+    float* transform_host = nullptr;  // pinned, 4 floats
+    bool transform_pending = false;
     // per-stage temporaries (slots: octree 0-59, divergence 60-75, marching cubes 76-95)
     prg::DBuf<uint8_t> pool[128];
     float iso = 0.0f;

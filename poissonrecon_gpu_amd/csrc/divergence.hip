@@ -164,10 +164,12 @@ __device__ __forceinline__ int depth_of(const Bases& B, int D, int64_t i) {
 // only the grouping of the double sum differs.
 
 // chunks of a node's own leaf range
-__global__ void __launch_bounds__(kBlock) k_g_chunks(const int32_t* dnum, int64_t n, int32_t* chunks, int32_t* multi) {
+__global__ void __launch_bounds__(kBlock) k_g_chunks(const int32_t* dnum, int64_t n, int64_t first_skipped,
+                                                     int32_t* chunks, int32_t* multi) {
     const int64_t a = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (a >= n) return;
-    const int32_t c = (dnum[a] + kChunk - 1) / kChunk;
+    // nodes from first_skipped on (the depth just above the leaf level) are summed by k_g_parent_sums
+    const int32_t c = a >= first_skipped ? 0 : (dnum[a] + kChunk - 1) / kChunk;
     chunks[a] = c;
     multi[a] = (c > 1) ? c : 0;
 }
@@ -187,7 +189,7 @@ __global__ void __launch_bounds__(kBlock) k_g_sums(const uint32_t* key, const in
                                                    const int32_t* chunk_node, const int32_t* chunk_off,
                                                    const int32_t* chunks, const int32_t* multi_off,
                                                    const int32_t* d_nchunks, Bases B, int D, double* G,
-                                                   double* partial) {
+                                                   int64_t gstride, double* partial) {
     const int lane = threadIdx.x & 63;
     const int64_t nchunks = *d_nchunks;
     const int64_t wave0 = (int64_t(blockIdx.x) * kBlock + threadIdx.x) >> 6;
@@ -225,7 +227,7 @@ __global__ void __launch_bounds__(kBlock) k_g_sums(const uint32_t* key, const in
         if (lane == 0) {
             if (chunks[a] == 1) {
 #pragma unroll
-                for (int j = 0; j < 27; j++) G[27 * a + j] = acc[j];
+                for (int j = 0; j < 27; j++) G[j * gstride + a] = acc[j];
             } else {
                 double* P = partial + 27 * (int64_t(multi_off[a]) + ci);
 #pragma unroll
@@ -249,7 +251,8 @@ __global__ void __launch_bounds__(kBlock) k_g_list_multi(const int32_t* flag, co
 }
 
 __global__ void __launch_bounds__(kBlock) k_g_finish(const int32_t* list, const int32_t* chunks,
-                                                     const int32_t* multi_off, const double* partial, double* G) {
+                                                     const int32_t* multi_off, const double* partial, double* G,
+                                                     int64_t gstride) {
     __shared__ double red[kBlock / 64][27];
     const int64_t a = list[blockIdx.x];
     const int32_t c = chunks[a];
@@ -273,20 +276,64 @@ __global__ void __launch_bounds__(kBlock) k_g_finish(const int32_t* list, const 
         double v = 0.0;
 #pragma unroll
         for (int w = 0; w < kBlock / 64; w++) v += red[w][threadIdx.x];
-        G[27 * a + threadIdx.x] = v;
+        G[threadIdx.x * gstride + a] = v;
     }
 }
 
-// b_i = sum_j G[neighbour_j(i)][j] for the nodes below the leaf level
-__global__ void __launch_bounds__(kBlock) k_div_from_g(const int32_t* neigh, const double* G, int64_t M, int64_t n,
-                                                       float* div) {
+// Depth D-1: every node's leaf range is its own sibling block of 8 leaf-level
+// slots (or empty), so one lane per slot computes its 27 terms and the 8 lanes
+// of a block add them with a fixed butterfly (xor 1, 2, 4: every lane ends
+// with the same sums); lane s stores positions j = s, s + 8, s + 16, s + 24.
+__global__ void __launch_bounds__(kBlock) k_g_parent_sums(const uint32_t* key, const int32_t* parent,
+                                                          const float* vfx, const float* vfy, const float* vfz,
+                                                          const float* T, int64_t baseD, int64_t MD, double* G,
+                                                          int64_t gstride) {
+    const int64_t k = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (k >= MD) return;  // MD is a multiple of 8 and kBlock of 8: blocks never straddle
+    const uint32_t kl = key[baseD + k];
+    const int rx = (kl >> 2) & 1, ry = (kl >> 1) & 1, rz = kl & 1;  // offset inside the parent's cell (s = 2)
+    const float wx[3] = {T[rx], T[2 + rx], T[4 + rx]};
+    const float wy[3] = {T[ry], T[2 + ry], T[4 + ry]};
+    const float wz[3] = {T[rz], T[2 + rz], T[4 + rz]};
+    const float vx = vfx[k], vy = vfy[k], vz = vfz[k];
+    double acc[27];
+#pragma unroll
+    for (int j = 0; j < 27; j++) acc[j] = double(fdot(vx, vy, vz, wx[j / 9], wy[(j / 3) % 3], wz[j % 3]));
+#pragma unroll
+    for (int m = 1; m < 8; m <<= 1)
+#pragma unroll
+        for (int j = 0; j < 27; j++) acc[j] += __shfl_xor(acc[j], m, 64);
+    const int64_t a = parent[baseD + k];
+    const int sl = int(k & 7);
+#pragma unroll
+    for (int j = 0; j < 27; j++)
+        if ((j & 7) == sl) G[j * gstride + a] = acc[j];
+}
+
+// b_i = sum_j G[neighbour_j(i)][j] for the nodes above the leaf level; the
+// neighbours come from the parent-neighbour rows (depth >= 1), the root's
+// only neighbour is itself.
+__global__ void __launch_bounds__(kBlock) k_div_from_g(const int32_t* pnt, const int64_t* pnt_off, Bases B, int D,
+                                                       const double* G, int64_t gstride, int64_t n, float* div) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     double val = 0.0;
+    if (i == 0) {
+        val += G[13 * gstride + 0];
+    } else {
+        const int d = depth_of(B, D, i);
+        const int64_t t = i - B.b[d];
+        const int32_t* row = pnt + 27 * (pnt_off[d] + (t >> 3));
+        const int s = int(t & 7);
 #pragma unroll
-    for (int j = 0; j < 27; j++) {
-        const int32_t nb = neigh[int64_t(j) * M + i];
-        if (nb != -1) val += G[27 * int64_t(nb) + j];
+        for (int j = 0; j < 27; j++) {
+            const int vx = ((s >> 2) & 1) + (j / 9 - 1), vy = ((s >> 1) & 1) + ((j / 3) % 3 - 1),
+                      vz = (s & 1) + (j % 3 - 1);
+            const int32_t cb = row[9 * ((vx >> 1) + 1) + 3 * ((vy >> 1) + 1) + ((vz >> 1) + 1)];
+            if (cb < 0) continue;
+            const int64_t nb = B.b[d] + cb + (((vx & 1) << 2) | ((vy & 1) << 1) | (vz & 1));
+            val += G[j * gstride + nb];
+        }
     }
     div[i] = float(val);
 }
@@ -350,7 +397,15 @@ void compute_divergence(prg_ctx* c) {
         mtot.reserve(1);
         G.reserve(size_t(27) * n);
         PRG_HIP(hipMemsetAsync(G.get(), 0, size_t(27) * n * 8, s));
-        k_g_chunks<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->dnum.get(), n, chunks.get(), multi.get());
+        const int64_t first_skipped = D >= 1 ? c->bases[D - 1] : n;
+        if (D >= 1) {
+            TimedScope ts(c->kt, "div_parent_sums", 16.0 * double(MD) + 216.0 * double(c->counts[D - 1]));
+            k_g_parent_sums<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->parent.get(), VX, VY, VZ,
+                                                                   c->d_div_table.get() + c->fs->div_off[D - 1],
+                                                                   baseD, MD, G.get(), n);
+            PRG_CHECK_LAUNCH();
+        }
+        k_g_chunks<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->dnum.get(), n, first_skipped, chunks.get(), multi.get());
         PRG_CHECK_LAUNCH();
         scan_exclusive(chunks.get(), coff.get(), n, ctot.get(), s, c->scratch, c->kt);
         scan_exclusive(multi.get(), moff.get(), n, mtot.get(), s, c->scratch, c->kt);
@@ -371,7 +426,7 @@ void compute_divergence(prg_ctx* c) {
             TimedScope ts(c->kt, "div_subtree_sums", 16.0 * double(MD) * double(D) + 216.0 * double(n));
             k_g_sums<<<grid_for(waves * 64, kBlock), kBlock, 0, s>>>(
                 c->key.get(), c->didx.get(), c->dnum.get(), VX, VY, VZ, c->d_div_table.get(), toff.get(), cnode.get(),
-                coff.get(), chunks.get(), moff.get(), ctot.get(), B, D, G.get(), partial.get());
+                coff.get(), chunks.get(), moff.get(), ctot.get(), B, D, G.get(), n, partial.get());
             PRG_CHECK_LAUNCH();
         }
         if (nmulti > 0) {
@@ -389,13 +444,16 @@ void compute_divergence(prg_ctx* c) {
             k_g_list_multi<<<grid_for(n, kBlock), kBlock, 0, s>>>(mflag.get(), mpos.get(), n, mlist.get());
             PRG_CHECK_LAUNCH();
             TimedScope ts(c->kt, "div_subtree_finish", 216.0 * double(nmulti));
-            k_g_finish<<<unsigned(nm), kBlock, 0, s>>>(mlist.get(), chunks.get(), moff.get(), partial.get(), G.get());
+            k_g_finish<<<unsigned(nm), kBlock, 0, s>>>(mlist.get(), chunks.get(), moff.get(), partial.get(), G.get(),
+                                                       n);
             PRG_CHECK_LAUNCH();
             PRG_HIP(hipStreamSynchronize(s));
         }
         {
-            TimedScope ts(c->kt, "div_from_subtree_sums", 112.0 * double(n) + 216.0 * double(n));
-            k_div_from_g<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->neigh.get(), G.get(), M, n, DV);
+            PoolBuf<int64_t> poff{&c->pool[74]};
+            PRG_HIP(hipMemcpyAsync(poff.reserve(D + 2), c->pnt_off.data(), size_t(D + 2) * 8, hipMemcpyHostToDevice, s));
+            TimedScope ts(c->kt, "div_from_subtree_sums", 13.5 * double(n) + 216.0 * double(n) + 4.0 * double(n));
+            k_div_from_g<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->pnt.get(), poff.get(), B, D, G.get(), n, n, DV);
             PRG_CHECK_LAUNCH();
         }
         PRG_HIP(hipStreamSynchronize(s));

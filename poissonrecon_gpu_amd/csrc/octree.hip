@@ -83,11 +83,36 @@ __global__ void __launch_bounds__(kBlock) k_bbox_partial(const float* xyz, int64
         }
 }
 
-__global__ void __launch_bounds__(kBlock) k_normalize(const float* xyz, const float* nxyz, int64_t n, float cx,
-                                                      float cy, float cz, float scale, float nscale, float* px,
-                                                      float* py, float* pz, float* nx, float* ny, float* nz) {
+// Scale and centre from the block minima / maxima, in the reference's float
+// operations (main.cu:539-545): tr = (centre x, y, z, scale).
+__global__ void k_bbox_final(const float* part, int nb, float* tr) {
+    if (threadIdx.x != 0) return;
+    float mn[3], mx[3];
+    for (int a = 0; a < 3; a++) {
+        mn[a] = part[a];
+        mx[a] = part[3 + a];
+        for (int b = 1; b < nb; b++) {
+            mn[a] = fminf(mn[a], part[6 * b + a]);
+            mx[a] = fmaxf(mx[a], part[6 * b + 3 + a]);
+        }
+    }
+    float scale = 1.0f, center[3];
+    for (int a = 0; a < 3; a++) {
+        const float ext = mx[a] - mn[a];
+        if (!a || scale < ext) scale = ext;
+        center[a] = (mx[a] + mn[a]) / 2;
+    }
+    scale = scale * 1.25f;
+    for (int a = 0; a < 3; a++) tr[a] = center[a] - scale / 2;
+    tr[3] = scale;
+}
+
+__global__ void __launch_bounds__(kBlock) k_normalize(const float* xyz, const float* nxyz, int64_t n,
+                                                      const float* tr, float nscale, float* px, float* py, float* pz,
+                                                      float* nx, float* ny, float* nz) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
+    const float cx = tr[0], cy = tr[1], cz = tr[2], scale = tr[3];
     px[i] = (xyz[3 * i + 0] - cx) / scale;
     py[i] = (xyz[3 * i + 1] - cy) / scale;
     pz[i] = (xyz[3 * i + 2] - cz) / scale;
@@ -161,6 +186,15 @@ __global__ void __launch_bounds__(kBlock) k_heads(const uint32_t* key, int64_t n
     flag[i] = (i == 0 || (key[i] >> shift) != (key[i - 1] >> shift)) ? 1 : 0;
 }
 
+// the same over the first *count of ub entries (0 beyond)
+__global__ void __launch_bounds__(kBlock) k_heads_dev(const uint32_t* key, const int32_t* count, int64_t ub, int shift,
+                                                      int32_t* flag) {
+    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (i >= ub) return;
+    const int64_t n = *count;
+    flag[i] = (i < n && (i == 0 || (key[i] >> shift) != (key[i - 1] >> shift))) ? 1 : 0;
+}
+
 __global__ void __launch_bounds__(kBlock) k_unique(const uint32_t* key, const int32_t* flag, const int32_t* rank,
                                                    int64_t n, uint32_t* uk, int32_t* upidx) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -179,10 +213,11 @@ __global__ void __launch_bounds__(kBlock) k_excl_to_incl(int32_t* out, const int
 // Non-empty nodes of depth d (compact, NE_d entries, key order) -> their slot
 // in the depth-d level array and the compact arrays of depth d-1.
 __global__ void __launch_bounds__(kBlock) k_level_up(const uint32_t* nk, const int32_t* np, const int32_t* ndi,
-                                                     const int32_t* flag, const int32_t* brank, int64_t ne, int D,
-                                                     int d, int32_t* pos, uint32_t* pk, int32_t* pp, int32_t* pdi) {
+                                                     const int32_t* flag, const int32_t* brank, const int32_t* ne_dev,
+                                                     int64_t ub, int D, int d, int32_t* pos, uint32_t* pk,
+                                                     int32_t* pp, int32_t* pdi) {
     const int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (q >= ne) return;
+    if (q >= ub || q >= *ne_dev) return;
     const int sh = 3 * (D - d);
     const int32_t b = brank[q];
     pos[q] = 8 * b + int32_t((nk[q] >> sh) & 7u);
@@ -337,37 +372,22 @@ void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n
         k_bbox_partial<<<blocks, kBlock, 0, s>>>(dxyz, n, part.get());
         PRG_CHECK_LAUNCH();
     }
-    std::vector<float> hp(size_t(6 * blocks));
-    PRG_HIP(hipMemcpyAsync(hp.data(), part.get(), hp.size() * 4, hipMemcpyDeviceToHost, s));
-    PRG_HIP(hipStreamSynchronize(s));
-    float mn[3], mx[3];
-    for (int a = 0; a < 3; a++) {
-        mn[a] = hp[a];
-        mx[a] = hp[3 + a];
-        for (int b = 1; b < blocks; b++) {
-            mn[a] = std::min(mn[a], hp[6 * b + a]);
-            mx[a] = std::max(mx[a], hp[6 * b + 3 + a]);
-        }
+    {
+        k_bbox_final<<<1, 64, 0, s>>>(part.get(), blocks, c->transform_dev.reserve(4));
+        PRG_CHECK_LAUNCH();
     }
-    // scale and centre exactly as the reference's float code (main.cu:539-545)
-    float scale = 1.0f, center[3];
-    for (int a = 0; a < 3; a++) {
-        if (!a || scale < mx[a] - mn[a]) scale = float(mx[a] - mn[a]);
-        center[a] = float(mx[a] + mn[a]) / 2;
-    }
-    scale *= 1.25f;
-    for (int a = 0; a < 3; a++) center[a] -= scale / 2;
     c->N = n;
-    for (int a = 0; a < 3; a++) c->center[a] = center[a];
-    c->scale = scale;
     {
         TimedScope ts(c->kt, "normalize", 48.0 * double(n));
         k_normalize<<<grid_for(n, kBlock), kBlock, 0, s>>>(
-            dxyz, dn, n, center[0], center[1], center[2], scale, float(2 << c->D), c->px.reserve(n),
-            c->py.reserve(n), c->pz.reserve(n), c->nx.reserve(n), c->ny.reserve(n), c->nz.reserve(n));
+            dxyz, dn, n, c->transform_dev.get(), float(2 << c->D), c->px.reserve(n), c->py.reserve(n),
+            c->pz.reserve(n), c->nx.reserve(n), c->ny.reserve(n), c->nz.reserve(n));
         PRG_CHECK_LAUNCH();
     }
-    PRG_HIP(hipStreamSynchronize(s));  // temporaries are released on return
+    // the host copy of (centre, scale) arrives with the next synchronisation
+    PRG_HIP(hipMemcpyAsync(c->transform_host, c->transform_dev.get(), 16, hipMemcpyDeviceToHost, s));
+    c->transform_pending = true;
+    if (!on_device) PRG_HIP(hipStreamSynchronize(s));  // the staging copies must finish before the caller reuses its arrays
 }
 
 void build_octree(prg_ctx* c) {
@@ -426,10 +446,20 @@ void build_octree(prg_ctx* c) {
         k_heads<<<grid_for(n, kBlock), kBlock, 0, s>>>(key.get(), n, 0, flag.get());
         PRG_CHECK_LAUNCH();
     }
-    scan_exclusive(flag.get(), rank.get(), n, tot.get(), s, c->scratch, kt);
+    PoolBuf<int32_t> ne_dev{&c->pool[14]};  // non-empty nodes per depth
+    ne_dev.reserve(D + 1);
+    scan_exclusive(flag.get(), rank.get(), n, ne_dev.get() + D, s, c->scratch, kt);
     int32_t U = 0;
-    PRG_HIP(hipMemcpyAsync(&U, tot.get(), 4, hipMemcpyDeviceToHost, s));
+    PRG_HIP(hipMemcpyAsync(&U, ne_dev.get() + D, 4, hipMemcpyDeviceToHost, s));
     PRG_HIP(hipStreamSynchronize(s));
+    if (c->transform_pending) {
+        for (int a = 0; a < 3; a++) c->center[a] = c->transform_host[a];
+        c->scale = c->transform_host[3];
+        c->transform_pending = false;
+    }
+    // upper bounds of the non-empty node counts: min(U, 8^d)
+    std::vector<int64_t> ub(D + 1);
+    for (int d = 0; d <= D; d++) ub[d] = std::min<int64_t>(U, int64_t(1) << (3 * d));
 
     // per-depth compact arrays of non-empty nodes
     std::vector<std::unique_ptr<PoolBuf<uint32_t>>> NK(D + 1);
@@ -454,41 +484,42 @@ void build_octree(prg_ctx* c) {
     PoolBuf<int32_t> prank{&c->pool[10]};  // unique-leaf rank of every sorted point
     prank.reserve(n);
 
-    // 3. levels bottom-up
+    // 3. levels bottom-up: sizes stay on the device (upper bounds size the
+    // launches); one read-back after the loop
     std::vector<int64_t> counts(D + 1, 0);
-    PoolBuf<int32_t> bflag{&c->pool[11]}, brank{&c->pool[12]}, btot{&c->pool[13]};
-    btot.reserve(1);
+    PoolBuf<int32_t> bflag{&c->pool[11]}, brank{&c->pool[12]};
+    bflag.reserve(ub[D]);
+    brank.reserve(ub[D]);
     for (int d = D; d >= 1; --d) {
-        const int64_t ne = NE[d];
-        bflag.reserve(ne);
-        brank.reserve(ne);
         {
-            TimedScope ts(kt, "level_heads", 8.0 * double(ne));
-            k_heads<<<grid_for(ne, kBlock), kBlock, 0, s>>>(NK[d]->get(), ne, 3 * (D - d + 1), bflag.get());
+            TimedScope ts(kt, "level_heads", 8.0 * double(ub[d]));
+            k_heads_dev<<<grid_for(ub[d], kBlock), kBlock, 0, s>>>(NK[d]->get(), ne_dev.get() + d, ub[d],
+                                                                   3 * (D - d + 1), bflag.get());
             PRG_CHECK_LAUNCH();
         }
-        scan_exclusive(bflag.get(), brank.get(), ne, btot.get(), s, c->scratch, kt);
-        int32_t nb = 0;
-        PRG_HIP(hipMemcpyAsync(&nb, btot.get(), 4, hipMemcpyDeviceToHost, s));
-        PRG_HIP(hipStreamSynchronize(s));
-        // brank holds the exclusive count of heads: block index = brank + flag - 1
-        counts[d] = 8 * int64_t(nb);
-        NE[d - 1] = nb;
-        NK[d - 1]->reserve(nb);
-        NP[d - 1]->reserve(nb);
-        NDI[d - 1]->reserve(nb);
-        POS[d]->reserve(ne);
+        scan_exclusive(bflag.get(), brank.get(), ub[d], ne_dev.get() + d - 1, s, c->scratch, kt);
+        NK[d - 1]->reserve(ub[d - 1]);
+        NP[d - 1]->reserve(ub[d - 1]);
+        NDI[d - 1]->reserve(ub[d - 1]);
+        POS[d]->reserve(ub[d]);
         {
             // block index of every non-empty node: heads before it, inclusive, minus one
-            TimedScope ts(kt, "level_up", 28.0 * double(ne));
-            k_excl_to_incl<<<grid_for(ne, kBlock), kBlock, 0, s>>>(brank.get(), brank.get(), bflag.get(), ne);
+            TimedScope ts(kt, "level_up", 28.0 * double(ub[d]));
+            k_excl_to_incl<<<grid_for(ub[d], kBlock), kBlock, 0, s>>>(brank.get(), brank.get(), bflag.get(), ub[d]);
             PRG_CHECK_LAUNCH();
-            k_level_up<<<grid_for(ne, kBlock), kBlock, 0, s>>>(NK[d]->get(), NP[d]->get(), NDI[d]->get(),
-                                                                bflag.get(), brank.get(), ne, D, d, POS[d]->get(),
-                                                                NK[d - 1]->get(), NP[d - 1]->get(),
-                                                                NDI[d - 1]->get());
+            k_level_up<<<grid_for(ub[d], kBlock), kBlock, 0, s>>>(NK[d]->get(), NP[d]->get(), NDI[d]->get(),
+                                                                   bflag.get(), brank.get(), ne_dev.get() + d, ub[d],
+                                                                   D, d, POS[d]->get(), NK[d - 1]->get(),
+                                                                   NP[d - 1]->get(), NDI[d - 1]->get());
             PRG_CHECK_LAUNCH();
         }
+    }
+    {
+        std::vector<int32_t> hne(D + 1);
+        PRG_HIP(hipMemcpyAsync(hne.data(), ne_dev.get(), size_t(D + 1) * 4, hipMemcpyDeviceToHost, s));
+        PRG_HIP(hipStreamSynchronize(s));
+        for (int d = 0; d <= D; d++) NE[d] = hne[d];
+        for (int d = 1; d <= D; d++) counts[d] = 8 * NE[d - 1];
     }
     counts[0] = 1;
     // root compact arrays: key 0, pidx 0, didx 0, slot 0
