@@ -84,10 +84,10 @@ def test_training_reduces_loss(prg):
         ms = np.ones((B, G, G, G), np.float32)
         ts = 0.5 * (bs + np.roll(bs, 1, axis=1) + np.roll(bs, -1, axis=2)) / 3.0
         first = net.train_step(bs, ms, ts, np.ones(B, np.float32), 2e-3)
-        for _ in range(40):
+        for _ in range(80):
             last = net.train_step(bs, ms, ts, np.ones(B, np.float32), 2e-3)
         print(f"loss {first:.4f} -> {last:.4f}")
-        assert last < 0.5 * first
+        assert last < 0.6 * first
     finally:
         net.close()
 
