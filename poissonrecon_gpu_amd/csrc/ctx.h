@@ -73,7 +73,7 @@ struct prg_ctx {
     prg::DBuf<float> transform_dev;
     float* transform_host = nullptr;  // pinned, 4 floats
     bool transform_pending = false;
-    // per-stage temporaries (slots: octree 0-59, divergence 60-75, marching cubes 76-95)
+    // per-stage temporaries (slots: octree 0-59, divergence 60-75, marching cubes 76-95, warm start 96)
     prg::DBuf<uint8_t> pool[128];
     float iso = 0.0f;
     bool iso_ready = false;  // iso computed or set by the caller
@@ -108,9 +108,9 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats);
 void compute_isovalue(prg_ctx* c);
 void extract_leaf_mesh(prg_ctx* c);
 void extract_subdivision(prg_ctx* c);
-// learned warm start: writes the network's prediction for depth `depth` into
-// the solution vector x (device), on the context stream
-void unet_warm_start(prg_ctx* c, int depth);
+// learned warm start: enqueues on `s` the network's prediction for depth
+// `depth`, written into the solution vector x (device)
+void unet_warm_start(prg_ctx* c, int depth, hipStream_t s);
 
 // Measures one stage with hipEvents on the context stream.
 struct StageTimer {

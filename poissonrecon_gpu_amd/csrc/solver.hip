@@ -592,12 +592,14 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     ensure_plans(c);
     float* X = c->x.reserve(M);
     PRG_HIP(hipMemsetAsync(X, 0, size_t(M) * 4, s));
+    // the learned warm start predicts the depth of its network's grid; the
+    // other depths start from zero
+    int warm_depth = -1;
     if (c->cfg.warm_start == PRG_WARM_LEARNED) {
-        // the network predicts the depth of its grid; the other depths start from zero
         if (!c->warm_net) throw Error{PRG_E_STATE, "PRG_WARM_LEARNED needs prg_set_warm_start_net"};
         int g = 0;
         while ((int64_t(1) << g) < prg_unet_grid(c->warm_net)) g++;
-        if (g <= D && c->counts[g] > 0) unet_warm_start(c, g);
+        if (g <= D && c->counts[g] > 0) warm_depth = g;
     }
     double* outv = c->cg_out.reserve(size_t(4) * (D + 1));
     PRG_HIP(hipMemsetAsync(outv, 0, size_t(4) * (D + 1) * 8, s));
@@ -647,9 +649,17 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     };
     try {
         PRG_HIP(hipHostMalloc(&hst, 2 * sizeof(CgState), hipHostMallocDefault));
+        // The network runs ahead of the chain that solves its depth: on the
+        // second stream (small depths and the non-finest chain), overlapping
+        // the finest depth's solve, or on the context stream (before the
+        // second stream forks from it) when its depth is the finest or in
+        // serial mode.
+        const bool warm_on_main = (warm_depth == D && c->counts[D] > kSmallRows) || serial;
+        if (warm_depth >= 0 && warm_on_main) unet_warm_start(c, warm_depth, s);
         hipEvent_t es0 = new_event(), es1 = new_event();
         PRG_HIP(hipEventRecord(es0, s));
         PRG_HIP(hipStreamWaitEvent(c->stream2, es0, 0));
+        if (warm_depth >= 0 && !warm_on_main) unet_warm_start(c, warm_depth, c->stream2);
         if (jobs.count) {
             const int64_t small_end = jobs.base[jobs.count - 1] + jobs.n[jobs.count - 1];
             float* Rs = c->cg_small[0].reserve(small_end);

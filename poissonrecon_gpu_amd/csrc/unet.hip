@@ -608,18 +608,17 @@ prg_status unet_guard(prg_unet* n, F&& f) {
 
 }  // namespace
 
-void unet_warm_start(prg_ctx* c, int depth) {
+void unet_warm_start(prg_ctx* c, int depth, hipStream_t s) {
     prg_unet* n = c->warm_net;
     if (!n || (int64_t(1) << depth) != n->G) throw Error{PRG_E_STATE, "warm-start network does not match the depth"};
     if (n->device != c->cfg.device) throw Error{PRG_E_ARG, "warm-start network is on another device"};
     hipStream_t keep = n->stream;
-    n->stream = c->stream;  // the network's kernels run in the solve's order
+    n->stream = s;  // the network's kernels run in the order of the chain that solves `depth`
     try {
         const int64_t V = int64_t(n->G) * n->G * n->G, cnt = c->counts[depth], base = c->bases[depth];
-        hipStream_t s = c->stream;
         PRG_HIP(hipMemsetAsync(n->bgrid.get(), 0, size_t(V) * 4, s));
         PRG_HIP(hipMemsetAsync(n->mask.get(), 0, size_t(V) * 4, s));
-        DBuf<double> ss;
+        PoolBuf<double> ss{&c->pool[96]};
         ss.reserve(2);
         k_scatter_nodes<<<blocks(cnt), kT, 0, s>>>(c->key.get() + base, c->div.get() + base, cnt, depth, c->D,
                                                    n->bgrid.get(), n->mask.get());
@@ -635,7 +634,6 @@ void unet_warm_start(prg_ctx* c, int depth) {
         k_gather_nodes<<<blocks(cnt), kT, 0, s>>>(c->key.get() + base, n->pred.get(), ss.get(), cnt, depth, c->D,
                                                   c->x.get() + base);
         PRG_CHECK_LAUNCH();
-        PRG_HIP(hipStreamSynchronize(s));  // ss is released at scope end
     } catch (...) {
         n->stream = keep;
         throw;
