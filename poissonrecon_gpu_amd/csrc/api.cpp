@@ -115,6 +115,8 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         PRG_HIP(hipSetDevice(cf.device));
         PRG_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         PRG_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        PRG_HIP(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
+        PRG_HIP(hipStreamCreateWithFlags(&c->stream4, hipStreamNonBlocking));
         PRG_HIP(hipEventCreate(&c->ev_a));
         PRG_HIP(hipEventCreate(&c->ev_b));
         PRG_HIP(hipHostMalloc(&c->transform_host, 16, hipHostMallocDefault));
@@ -134,14 +136,15 @@ void prg_destroy(prg_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->cfg.device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    for (hipStream_t x : {c->stream2, c->stream3, c->stream4})
+        if (x) (void)hipStreamSynchronize(x);
     if (c->ev_a) (void)hipEventDestroy(c->ev_a);
     if (c->ev_b) (void)hipEventDestroy(c->ev_b);
     if (c->transform_host) (void)hipHostFree(c->transform_host);
-    hipStream_t s = c->stream, s2 = c->stream2;
+    hipStream_t ss[4] = {c->stream, c->stream2, c->stream3, c->stream4};
     delete c;  // releases device buffers
-    if (s) (void)hipStreamDestroy(s);
-    if (s2) (void)hipStreamDestroy(s2);
+    for (hipStream_t x : ss)
+        if (x) (void)hipStreamDestroy(x);
 }
 
 const char* prg_last_error(const prg_ctx* c) { return c ? c->err.c_str() : "null context"; }

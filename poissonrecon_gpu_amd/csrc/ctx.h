@@ -25,7 +25,9 @@ struct prg_ctx {
     prg_config cfg{};
     int D = 8;
     hipStream_t stream = nullptr;
-    hipStream_t stream2 = nullptr;  // independent work (small-depth solves) overlapping `stream`
+    hipStream_t stream2 = nullptr;  // independent work overlapping `stream` (CG chains, small-depth solves)
+    hipStream_t stream3 = nullptr;
+    hipStream_t stream4 = nullptr;
     std::string err;
     prg::KernelTimer kt;
     prg::Scratch scratch;
@@ -54,11 +56,11 @@ struct prg_ctx {
     // fields
     prg::DBuf<float> vfx, vfy, vfz, div, x, pvals;
     prg::DBuf<double> dscalars;  // reductions and CG scalars
-    // CG scratch, kept across solves: two concurrent chains of large depths
+    // CG scratch, kept across solves: three concurrent chains of large depths
     // (r, p0, p1, q, aligned x; FP64 partials; state), the small-depth vectors
-    prg::DBuf<float> cg_vec[2][5];
-    prg::DBuf<double> cg_part[2][4];  // r.r (two slots), p.q, b.b / true-residual partials
-    prg::DBuf<uint8_t> cg_state[2];
+    prg::DBuf<float> cg_vec[3][5];
+    prg::DBuf<double> cg_part[3][4];  // r.r (two slots), p.q, b.b / true-residual partials
+    prg::DBuf<uint8_t> cg_state[3];
     prg::DBuf<float> cg_small[3];
     prg::DBuf<double> cg_out;
     // SpMV tile plans of the large depths, valid for octree `plan_gen`

@@ -144,16 +144,18 @@ constexpr int kHashSlots = 2048;          // > kTileEntries: linear probing alwa
 static_assert(kHashSlots > kTileEntries, "hash must hold every entry of a tile");
 
 // r = b - A x0, p = r, xs = x0 (the solve works on the 16-byte-aligned copy
-// xs); partials of r.r (slot 0) and b.b.
+// xs); partials of r.r (slot 0) and b.b.  zero_x: x0 = 0 is known, and A x0
+// is +0 (a float sum of signed zeros started at +0), so r = b - 0 without the
+// SpMV.
 __global__ void __launch_bounds__(kVecThreads) k_cg_init(const int32_t* pnt, int64_t n, Stencil S, const float* b,
-                                                         const float* x, float* xs, float* r, float* p,
-                                                         double* rr_part, double* bb_part, CgState* st) {
+                                                         const float* x, int32_t zero_x, float* xs, float* r,
+                                                         float* p, double* rr_part, double* bb_part, CgState* st) {
     __shared__ double lds[kVecThreads / 64];
     double bb = 0.0, rr = 0.0;
     for (int64_t t = int64_t(blockIdx.x) * kVecThreads + threadIdx.x; t < n; t += int64_t(gridDim.x) * kVecThreads) {
-        const float ax = spmv_pnt(pnt, t, x, S);
+        const float ax = zero_x ? 0.0f : spmv_pnt(pnt, t, x, S);
         const float rv = b[t] - ax;
-        xs[t] = x[t];
+        xs[t] = zero_x ? 0.0f : x[t];
         r[t] = rv;
         p[t] = rv;
         bb += double(b[t]) * double(b[t]);
@@ -611,7 +613,7 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     // events time that launch alone.
     const bool serial = c->kt.enabled;
 
-    // small depths: one launch, one workgroup each, on the second stream
+    // small depths: one launch, one workgroup each, on a stream of their own
     SmallJobs jobs{};
     for (int d = 0; d <= D; d++)
         if (c->counts[d] <= kSmallRows) {
@@ -621,11 +623,17 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             jobs.pnt_row[jobs.count] = d == 0 ? -1 : c->pnt_off[d];
             jobs.count++;
         }
-    Chain ch[2];
+    // chain 0: the finest depth; chain 1: the next one; chain 2: the other
+    // large depths, coarse to fine
+    constexpr int kChains = 3;
+    Chain ch[kChains];
+    auto chain_of = [&](int d) { return d == D ? 0 : (d == D - 1 ? 1 : 2); };
     for (int d = 0; d <= D; d++)
-        if (c->counts[d] > kSmallRows) ch[d == D ? 0 : 1].depths.push_back(d);
+        if (c->counts[d] > kSmallRows) ch[chain_of(d)].depths.push_back(d);
     ch[0].s = s;
     ch[1].s = serial ? s : c->stream2;
+    ch[2].s = serial ? s : c->stream3;
+    hipStream_t small_s = serial ? s : c->stream4;
 
     std::vector<hipEvent_t> evs;
     auto new_event = [&]() {
@@ -648,30 +656,29 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
         return int32_t(std::min<int64_t>(kMaxWG, (n + 4 * kVecThreads - 1) / (4 * kVecThreads)));
     };
     try {
-        PRG_HIP(hipHostMalloc(&hst, 2 * sizeof(CgState), hipHostMallocDefault));
-        // The network runs ahead of the chain that solves its depth: on the
-        // second stream (small depths and the non-finest chain), overlapping
-        // the finest depth's solve, or on the context stream (before the
-        // second stream forks from it) when its depth is the finest or in
-        // serial mode.
-        const bool warm_on_main = (warm_depth == D && c->counts[D] > kSmallRows) || serial;
-        if (warm_depth >= 0 && warm_on_main) unet_warm_start(c, warm_depth, s);
+        PRG_HIP(hipHostMalloc(&hst, kChains * sizeof(CgState), hipHostMallocDefault));
+        // The other streams fork from the context stream here.  The network
+        // runs ahead of the work that solves its depth, on that work's
+        // stream, overlapping the other chains.
         hipEvent_t es0 = new_event(), es1 = new_event();
         PRG_HIP(hipEventRecord(es0, s));
-        PRG_HIP(hipStreamWaitEvent(c->stream2, es0, 0));
-        if (warm_depth >= 0 && !warm_on_main) unet_warm_start(c, warm_depth, c->stream2);
+        for (hipStream_t x : {c->stream2, c->stream3, c->stream4}) PRG_HIP(hipStreamWaitEvent(x, es0, 0));
+        if (warm_depth >= 0)
+            unet_warm_start(c, warm_depth, c->counts[warm_depth] <= kSmallRows ? small_s : ch[chain_of(warm_depth)].s);
         if (jobs.count) {
             const int64_t small_end = jobs.base[jobs.count - 1] + jobs.n[jobs.count - 1];
             float* Rs = c->cg_small[0].reserve(small_end);
             float* Ps = c->cg_small[1].reserve(small_end);
             float* Qs = c->cg_small[2].reserve(small_end);
-            k_cg_small<<<jobs.count, kSmallBlock, 0, c->stream2>>>(jobs, c->d_stencil.get(), c->pnt.get(),
-                                                                   c->div.get(), X, Rs, Ps, Qs, tol, fixed,
-                                                                   max_iters, refresh, outv);
+            es0 = new_event();
+            PRG_HIP(hipEventRecord(es0, small_s));
+            k_cg_small<<<jobs.count, kSmallBlock, 0, small_s>>>(jobs, c->d_stencil.get(), c->pnt.get(),
+                                                                c->div.get(), X, Rs, Ps, Qs, tol, fixed, max_iters,
+                                                                refresh, outv);
             PRG_CHECK_LAUNCH();
-            PRG_HIP(hipEventRecord(es1, c->stream2));
+            PRG_HIP(hipEventRecord(es1, small_s));
         }
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < kChains; k++) {
             Chain& h = ch[k];
             int64_t nmax = 0;
             for (int d : h.depths) nmax = std::max(nmax, c->counts[d]);
@@ -701,8 +708,8 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             PRG_HIP(hipEventRecord(dbeg[d], h.s));
             TimedScope ts(c->kt, "cg_init", kSpmvRowBytes * double(n));
             k_cg_init<<<vec_grid(n), kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], n, stencil_of(d),
-                                                            c->div.get() + base, X + base, h.XS, h.R, h.P[0], h.rr,
-                                                            h.bb, h.st);
+                                                            c->div.get() + base, X + base, d == warm_depth ? 0 : 1,
+                                                            h.XS, h.R, h.P[0], h.rr, h.bb, h.st);
             PRG_CHECK_LAUNCH();
         };
         auto issue_batch = [&](Chain& h) {
@@ -773,19 +780,26 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
         };
 
         // Each chain: init, batches of iterations with a state read-back after
-        // each, true residual; the host polls the chains so neither waits for
-        // the other.  Serial mode runs chain 1 to completion before chain 0.
-        int first_chain = 0;
-        if (serial && !ch[1].depths.empty()) first_chain = 1;
-        for (int k = 0; k < 2; k++) {
-            Chain& h = ch[k];
-            if (h.depths.empty() || (serial && k != first_chain)) continue;
-            begin_depth(h);
-            issue_batch(h);
+        // each, true residual; the host polls the chains so none waits for
+        // another.  Serial mode runs the chains one after another, the finest
+        // depth's last.
+        std::vector<int> serial_order;
+        for (int k = kChains - 1; k >= 0; k--)
+            if (!ch[k].depths.empty()) serial_order.push_back(k);
+        size_t serial_next = 0;
+        auto start_chain = [&](int k) {
+            begin_depth(ch[k]);
+            issue_batch(ch[k]);
+        };
+        if (serial) {
+            if (!serial_order.empty()) start_chain(serial_order[serial_next++]);
+        } else {
+            for (int k = 0; k < kChains; k++)
+                if (!ch[k].depths.empty()) start_chain(k);
         }
         while (true) {
             bool active = false;
-            for (int k = 0; k < 2; k++) {
+            for (int k = 0; k < kChains; k++) {
                 Chain& h = ch[k];
                 if (!h.waiting) continue;
                 active = true;
@@ -800,9 +814,8 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                     if (h.next < h.depths.size()) {
                         begin_depth(h);
                         issue_batch(h);
-                    } else if (serial && k == 1 && !ch[0].depths.empty()) {
-                        begin_depth(ch[0]);
-                        issue_batch(ch[0]);
+                    } else if (serial && serial_next < serial_order.size()) {
+                        start_chain(serial_order[serial_next++]);
                     }
                 } else {
                     h.batch = 6;
@@ -826,14 +839,16 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                 PRG_HIP(hipEventElapsedTime(&t, dbeg[d], dend[d]));
                 ms[d] = t;
             }
-        // the context stream continues after both chains and the small solves
-        hipEvent_t ej = new_event();
-        PRG_HIP(hipEventRecord(ej, c->stream2));
-        PRG_HIP(hipStreamWaitEvent(s, ej, 0));
+        // the context stream continues after every chain and the small solves
+        for (hipStream_t x : {c->stream2, c->stream3, c->stream4}) {
+            hipEvent_t ej = new_event();
+            PRG_HIP(hipEventRecord(ej, x));
+            PRG_HIP(hipStreamWaitEvent(s, ej, 0));
+        }
         PRG_HIP(hipMemcpyAsync(ov.data(), outv, ov.size() * 8, hipMemcpyDeviceToHost, s));
         PRG_HIP(hipStreamSynchronize(s));
     } catch (...) {
-        (void)hipStreamSynchronize(c->stream2);
+        for (hipStream_t x : {c->stream2, c->stream3, c->stream4}) (void)hipStreamSynchronize(x);
         (void)hipStreamSynchronize(s);
         if (hst) (void)hipHostFree(hst);
         for (hipEvent_t e : evs) (void)hipEventDestroy(e);
