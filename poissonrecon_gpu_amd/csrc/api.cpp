@@ -120,6 +120,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         PRG_HIP(hipEventCreate(&c->ev_a));
         PRG_HIP(hipEventCreate(&c->ev_b));
         PRG_HIP(hipHostMalloc(&c->transform_host, 16, hipHostMallocDefault));
+        PRG_HIP(hipHostMalloc(&c->cg_host, 256, hipHostMallocDefault));
         c->kt.enabled = cf.timing != 0;
         c->kt.stream = c->stream;
         return PRG_OK;
@@ -141,6 +142,8 @@ void prg_destroy(prg_ctx* c) {
     if (c->ev_a) (void)hipEventDestroy(c->ev_a);
     if (c->ev_b) (void)hipEventDestroy(c->ev_b);
     if (c->transform_host) (void)hipHostFree(c->transform_host);
+    if (c->cg_host) (void)hipHostFree(c->cg_host);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     hipStream_t ss[4] = {c->stream, c->stream2, c->stream3, c->stream4};
     delete c;  // releases device buffers
     for (hipStream_t x : ss)

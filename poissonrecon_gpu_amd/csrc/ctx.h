@@ -94,6 +94,8 @@ struct prg_ctx {
     // stage timing
     std::array<double, PRG_STAGE_COUNT> stage_ms{};
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    std::vector<hipEvent_t> ev_pool;  // the solver's events, reused across solves
+    void* cg_host = nullptr;          // pinned: the CG chains' state read-backs (256 bytes)
 
     int64_t MD() const { return counts.empty() ? 0 : counts[D]; }
     int64_t base_of(int d) const { return bases[d]; }

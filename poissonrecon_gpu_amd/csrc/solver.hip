@@ -630,17 +630,27 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     auto chain_of = [&](int d) { return d == D ? 0 : (d == D - 1 ? 1 : 2); };
     for (int d = 0; d <= D; d++)
         if (c->counts[d] > kSmallRows) ch[chain_of(d)].depths.push_back(d);
+    // the warm-started depth goes last in its chain, so the chain's other
+    // depths overlap the network's inference
+    if (warm_depth >= 0 && c->counts[warm_depth] > kSmallRows) {
+        auto& v = ch[chain_of(warm_depth)].depths;
+        v.erase(std::find(v.begin(), v.end(), warm_depth));
+        v.push_back(warm_depth);
+    }
     ch[0].s = s;
     ch[1].s = serial ? s : c->stream2;
     ch[2].s = serial ? s : c->stream3;
     hipStream_t small_s = serial ? s : c->stream4;
 
-    std::vector<hipEvent_t> evs;
+    // events come from the context's pool (created once, reused by every solve)
+    size_t ev_used = 0;
     auto new_event = [&]() {
-        hipEvent_t e;
-        PRG_HIP(hipEventCreate(&e));
-        evs.push_back(e);
-        return e;
+        if (ev_used == c->ev_pool.size()) {
+            hipEvent_t e;
+            PRG_HIP(hipEventCreate(&e));
+            c->ev_pool.push_back(e);
+        }
+        return c->ev_pool[ev_used++];
     };
     std::vector<hipEvent_t> dbeg(D + 1, nullptr), dend(D + 1, nullptr);
     CgState* hst = nullptr;
@@ -656,15 +666,22 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
         return int32_t(std::min<int64_t>(kMaxWG, (n + 4 * kVecThreads - 1) / (4 * kVecThreads)));
     };
     try {
-        PRG_HIP(hipHostMalloc(&hst, kChains * sizeof(CgState), hipHostMallocDefault));
+        static_assert(kChains * sizeof(CgState) <= 256, "chain states exceed the pinned read-back buffer");
+        hst = static_cast<CgState*>(c->cg_host);
         // The other streams fork from the context stream here.  The network
         // runs ahead of the work that solves its depth, on that work's
         // stream, overlapping the other chains.
         hipEvent_t es0 = new_event(), es1 = new_event();
         PRG_HIP(hipEventRecord(es0, s));
         for (hipStream_t x : {c->stream2, c->stream3, c->stream4}) PRG_HIP(hipStreamWaitEvent(x, es0, 0));
-        if (warm_depth >= 0)
-            unet_warm_start(c, warm_depth, c->counts[warm_depth] <= kSmallRows ? small_s : ch[chain_of(warm_depth)].s);
+        // The network runs on the small depths' stream, ahead of their solve;
+        // the warm-started depth's chain waits for it (event ew).
+        hipEvent_t ew = nullptr;
+        if (warm_depth >= 0) {
+            unet_warm_start(c, warm_depth, small_s);
+            ew = new_event();
+            PRG_HIP(hipEventRecord(ew, small_s));
+        }
         if (jobs.count) {
             const int64_t small_end = jobs.base[jobs.count - 1] + jobs.n[jobs.count - 1];
             float* Rs = c->cg_small[0].reserve(small_end);
@@ -705,6 +722,7 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             const int64_t n = c->counts[d], base = c->bases[d];
             dbeg[d] = new_event();
             dend[d] = new_event();
+            if (d == warm_depth && ew) PRG_HIP(hipStreamWaitEvent(h.s, ew, 0));
             PRG_HIP(hipEventRecord(dbeg[d], h.s));
             TimedScope ts(c->kt, "cg_init", kSpmvRowBytes * double(n));
             k_cg_init<<<vec_grid(n), kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], n, stencil_of(d),
@@ -850,12 +868,8 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     } catch (...) {
         for (hipStream_t x : {c->stream2, c->stream3, c->stream4}) (void)hipStreamSynchronize(x);
         (void)hipStreamSynchronize(s);
-        if (hst) (void)hipHostFree(hst);
-        for (hipEvent_t e : evs) (void)hipEventDestroy(e);
         throw;
     }
-    (void)hipHostFree(hst);
-    for (hipEvent_t e : evs) (void)hipEventDestroy(e);
     for (int d = 0; d <= D; d++) {
         const int32_t iters = int32_t(ov[4 * d + 2]);
         if (stats) {

@@ -48,55 +48,176 @@ __device__ __forceinline__ int64_t nb_voxel(int64_t v, int t, int G) {
     return (int64_t(nx) * G + ny) * G + nz;
 }
 
-// out[v, co] = act(bias[co] + sum_t sum_ci bf16(in[v + off_t, ci]) * w[t][co][ci])
-// One wave per 16 voxels, all output channels (16 * NT).  MFMA lane layout
-// (16x16x16): lane l holds A[l % 16][4 (l / 16) + e], B[4 (l / 16) + e][l % 16]
-// and D[4 (l / 16) + e][l % 16], e = 0..3.
-template <int NT>
-__global__ void __launch_bounds__(kT) k_conv3(const float* __restrict__ in, int cin, const short* __restrict__ w,
-                                             const float* __restrict__ bias, int G, int relu,
-                                             float* __restrict__ out) {
-    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int64_t V = int64_t(G) * G * G;
-    const int64_t mt = int64_t(blockIdx.x) * 4 + wave;
-    if (mt * 16 >= V) return;
-    const int i = l & 15, kq = l >> 4;
-    const int64_t v = mt * 16 + i;
-    const int cout = 16 * NT;
-    f4 acc[NT];
+// Input of a convolution: channels [0, clo) come from `lo` (grid G / 2,
+// nearest upsampling), channels [clo, clo + cskip) from `skip` (grid G); clo
+// is 0 for a plain input.  Channel counts are 16, 32 or 64 per source.
+struct ConvIn {
+    const float* lo;
+    int clo;
+    const float* skip;
+    int cskip;
+};
+
+__device__ __forceinline__ int log2i(int v) { return 31 - __builtin_clz(v); }
+
+// a / d for 0 <= a < 2^20 and 1 <= d < 2^10 (inv = 1.0f / d): the float
+// estimate is within one of the quotient and is corrected
+__device__ __forceinline__ int small_div(int a, int d, float inv) {
+    int q = int(float(a) * inv);
+    if (q * d > a) q--;
+    if ((q + 1) * d <= a) q++;
+    return q;
+}
+
+constexpr int kLoadBatch = 8;  // halo loads in flight per thread
+
+// One source of the halo: HX x HY x HZ voxels x c channels into LDS channels
+// [choff, choff + c); HALF reads the source at G / 2 (nearest upsampling).
+// Each thread issues kLoadBatch independent loads before it converts and
+// stores any of them.
+template <bool HALF>
+__device__ __forceinline__ void load_halo(s4* halo4, int H, int HY, int HZ, const float* __restrict__ src, int c,
+                                          int choff, int ox, int oy, int oz, int G) {
+    const int shift = log2i(c >> 2);
+    const int total = H << shift;
+    const int Gs = HALF ? G >> 1 : G;
+    const float invz = 1.0f / float(HZ), invy = 1.0f / float(HY);
+    for (int base = threadIdx.x; base < total; base += kT * kLoadBatch) {
+        float4 v[kLoadBatch];
+        int dst[kLoadBatch];
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++) acc[nt] = f4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int t = 0; t < 27; t++) {
-        const int64_t nv = v < V ? nb_voxel(v, t, G) : -1;
-        for (int c0 = 0; c0 < cin; c0 += 16) {
-            s4 a;
-            if (nv >= 0) {
-                const float4 f = *reinterpret_cast<const float4*>(in + nv * cin + c0 + 4 * kq);
-                a[0] = bf(f.x);
-                a[1] = bf(f.y);
-                a[2] = bf(f.z);
-                a[3] = bf(f.w);
-            } else {
-                a = s4{0, 0, 0, 0};
+        for (int k = 0; k < kLoadBatch; k++) {
+            const int it = base + k * kT;
+            v[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            dst[k] = -1;
+            if (it < total) {
+                const int h = it >> shift, c4 = it & ((1 << shift) - 1);
+                const int hxy = small_div(h, HZ, invz), hz = h - hxy * HZ;
+                const int hx = small_div(hxy, HY, invy), hy = hxy - hx * HY;
+                const int gx = ox + hx - 1, gy = oy + hy - 1, gz = oz + hz - 1;
+                if (gx >= 0 && gy >= 0 && gz >= 0 && gx < G && gy < G && gz < G) {
+                    const int64_t vox = HALF ? (int64_t(gx >> 1) * Gs + (gy >> 1)) * Gs + (gz >> 1)
+                                             : (int64_t(gx) * G + gy) * G + gz;
+                    v[k] = *reinterpret_cast<const float4*>(src + vox * c + 4 * c4);
+                }
+                const int ch = choff + 4 * c4;
+                dst[k] = ((ch >> 4) * H + h) * 4 + ((ch >> 2) & 3);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kLoadBatch; k++)
+            if (dst[k] >= 0) halo4[dst[k]] = s4{bf(v[k].x), bf(v[k].y), bf(v[k].z), bf(v[k].w)};
+    }
+}
+
+// out[v, co] = act(bias[co] + sum_t sum_ci bf16(in[v + off_t, ci]) * w[t][co][ci])
+//
+// One workgroup per brick of BX x BY x BZ output voxels (x, y, z; z fastest).
+// The brick's input halo, (BX + 2)(BY + 2)(BZ + 2) voxels, is staged in LDS
+// as BF16, 16-channel slice after slice: lds[s][h][16]; each wave loads whole
+// halo rows (contiguous in memory).  The brick's voxels in (x, y, z) order
+// form M-tiles of 16; wave w computes M-tiles [MA (w % MG), MA (w % MG) + MA)
+// for N-tiles [NA (w / MG), NA (w / MG) + NA) (16 output channels each).
+// MFMA lane layout (16x16x16): lane l holds A[l % 16][4 (l / 16) + e],
+// B[4 (l / 16) + e][l % 16] and D[4 (l / 16) + e][l % 16], e = 0..3.  K runs
+// over slices, then taps; a slice's 27 weight fragments are loaded together.
+template <int MA, int NA>
+__global__ void __launch_bounds__(kT) k_conv3(ConvIn src, const short* __restrict__ w, const float* __restrict__ bias,
+                                             int G, int BX, int BY, int BZ, int MG, int relu, float* __restrict__ out,
+                                             int cout) {
+    extern __shared__ s4 halo4[];
+    const short* halo = reinterpret_cast<const short*>(halo4);
+    const int cin = src.clo + src.cskip;
+    const int HY = BY + 2, HZ = BZ + 2, HYZ = HY * HZ, H = (BX + 2) * HYZ;
+    const int nbz = G / BZ, nby = G / BY;
+    int b = blockIdx.x;
+    const int oz = (b % nbz) * BZ;
+    b /= nbz;
+    const int oy = (b % nby) * BY;
+    const int ox = (b / nby) * BX;
+    const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int i = l & 15, kq = l >> 4;
+    const int mg = wave % MG, ng = wave / MG;
+    const int NT = cout >> 4, S = cin >> 4, nvox = BX * BY * BZ;
+    const bool active = (ng + 1) * NA <= NT;
+    // the first slice's weight fragments are in flight while the halo loads
+    const short* wl = w + int64_t((active ? ng : 0) * NA * 16 + i) * cin + 4 * kq;
+    s4 wr[27][NA];
+#pragma unroll
+    for (int t = 0; t < 27; t++)
+#pragma unroll
+        for (int n = 0; n < NA; n++) wr[t][n] = *reinterpret_cast<const s4*>(wl + (int64_t(t) * cout + n * 16) * cin);
+    if (src.clo) load_halo<true>(halo4, H, HY, HZ, src.lo, src.clo, 0, ox, oy, oz, G);
+    load_halo<false>(halo4, H, HY, HZ, src.skip, src.cskip, src.clo, ox, oy, oz, G);
+    __syncthreads();
+    if (!active) return;  // idle wave; no barrier follows
+    // lanes past the brick's voxels (grids under 16^3) read the brick's first
+    // voxel, whose 27 neighbours are all in the halo, and use zeros
+    int hb[MA];
+    bool ok[MA];
+#pragma unroll
+    for (int m = 0; m < MA; m++) {
+        const int q = (mg * MA + m) * 16 + i;
+        ok[m] = q < nvox;
+        const int qx = q / (BY * BZ), qy = (q / BZ) % BY, qz = q % BZ;
+        hb[m] = ok[m] ? (qx + 1) * HYZ + (qy + 1) * HZ + qz + 1 : HYZ + HZ + 1;
+    }
+    // three partial sums over alternating taps keep independent MFMAs in flight
+    f4 acc[3][MA][NA];
+#pragma unroll
+    for (int x = 0; x < 3; x++)
+#pragma unroll
+        for (int m = 0; m < MA; m++)
+#pragma unroll
+            for (int n = 0; n < NA; n++) acc[x][m][n] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+    const s4 zero4 = s4{0, 0, 0, 0};
+    for (int sl = 0; sl < S; sl++) {
+        const short* hs = halo + sl * H * 16 + 4 * kq;
+        // next slice's weights (the last slice re-reads its own: in bounds, unused)
+        const int nsl = sl + 1 < S ? sl + 1 : sl;
+#pragma unroll
+        for (int g = 0; g < 3; g++) {
+            // the nine taps with dx = g - 1: their A fragments first, then the MFMAs
+            s4 a[9][MA];
+#pragma unroll
+            for (int j = 0; j < 9; j++) {
+                const int toff = (g - 1) * HYZ + (j / 3 - 1) * HZ + (j % 3 - 1);
+#pragma unroll
+                for (int m = 0; m < MA; m++) a[j][m] = *reinterpret_cast<const s4*>(hs + (hb[m] + toff) * 16);
             }
 #pragma unroll
-            for (int nt = 0; nt < NT; nt++) {
-                const s4 b = *reinterpret_cast<const s4*>(w + (int64_t(t) * cout + nt * 16 + i) * cin + c0 + 4 * kq);
-                acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
+            for (int j = 0; j < 9; j++)
+#pragma unroll
+                for (int m = 0; m < MA; m++) a[j][m] = ok[m] ? a[j][m] : zero4;
+#pragma unroll
+            for (int j = 0; j < 9; j++) {
+                const int t = 9 * g + j;
+#pragma unroll
+                for (int n = 0; n < NA; n++) {
+#pragma unroll
+                    for (int m = 0; m < MA; m++)
+                        acc[j % 3][m][n] =
+                            __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[j][m], wr[t][n], acc[j % 3][m][n], 0, 0, 0);
+                    wr[t][n] = *reinterpret_cast<const s4*>(wl + (int64_t(t) * cout + n * 16) * cin + nsl * 16);
+                }
             }
         }
     }
 #pragma unroll
-    for (int nt = 0; nt < NT; nt++)
+    for (int m = 0; m < MA; m++)
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const int64_t vo = mt * 16 + 4 * kq + e;
-            if (vo >= V) continue;
-            const int co = nt * 16 + i;
-            float val = acc[nt][e];
-            if (bias) val = val + bias[co];
-            if (relu) val = val > 0.0f ? val : 0.0f;
-            out[vo * cout + co] = val;
+            const int q = (mg * MA + m) * 16 + 4 * kq + e;
+            if (q >= nvox) continue;
+            const int gx = ox + q / (BY * BZ), gy = oy + (q / BZ) % BY, gz = oz + q % BZ;
+            float* o = out + ((int64_t(gx) * G + gy) * G + gz) * cout + ng * NA * 16 + i;
+#pragma unroll
+            for (int n = 0; n < NA; n++) {
+                float val = (acc[0][m][n][e] + acc[1][m][n][e]) + acc[2][m][n][e];
+                if (bias) val = val + bias[(ng * NA + n) * 16 + i];
+                if (relu) val = val > 0.0f ? val : 0.0f;
+                o[n * 16] = val;
+            }
         }
 }
 
@@ -403,7 +524,11 @@ unsigned blocks(int64_t n) { return unsigned((n + kT - 1) / kT); }
 struct UNetConv {
     int cin, cout, G;
     int64_t woff, boff;
-    const float* in;
+    const float* in;  // the input tensor (cin channels) the weight gradient reads
+    const float* lo;  // decoder convolutions: the upsampled part of the input (clo channels, grid G / 2)
+    int clo;          // 0 for a plain input
+    const float* skip;
+    float* cat;  // decoder convolutions: `in`, written only for the backward pass
     float* out;
     DBuf<short> wb, wf;
 };
@@ -434,20 +559,64 @@ namespace {
 const int kConvDef[14][3] = {{16, 16, 0}, {16, 16, 0}, {16, 32, 1}, {32, 32, 1}, {32, 64, 2}, {64, 64, 2}, {64, 64, 3},
                              {64, 64, 3}, {128, 64, 2}, {64, 64, 2}, {96, 32, 1}, {32, 32, 1}, {48, 16, 0}, {16, 16, 0}};
 
-void launch_conv(hipStream_t s, const float* in, int cin, const short* w, const float* bias, int G, int relu, float* out,
-                 int cout) {
-    const int64_t V = int64_t(G) * G * G;
-    const unsigned nb = unsigned((V / 16 + 3) / 4 + 1);
-    switch (cout / 16) {
-        case 1: k_conv3<1><<<nb, kT, 0, s>>>(in, cin, w, bias, G, relu, out); break;
-        case 2: k_conv3<2><<<nb, kT, 0, s>>>(in, cin, w, bias, G, relu, out); break;
-        case 3: k_conv3<3><<<nb, kT, 0, s>>>(in, cin, w, bias, G, relu, out); break;
-        case 4: k_conv3<4><<<nb, kT, 0, s>>>(in, cin, w, bias, G, relu, out); break;
-        case 6: k_conv3<6><<<nb, kT, 0, s>>>(in, cin, w, bias, G, relu, out); break;
-        case 8: k_conv3<8><<<nb, kT, 0, s>>>(in, cin, w, bias, G, relu, out); break;
+// Brick and wave split of a convolution with output grid G and NT = cout / 16
+// N-tiles: few voxels and many channels per workgroup on the small grids, so
+// every grid gives the chip enough workgroups.
+struct ConvPlan {
+    int BX, BY, BZ, MA, NA, MG;
+};
+
+ConvPlan conv_plan(int G, int NT) {
+    ConvPlan p{};
+    p.BZ = G < 16 ? G : 16;
+    int bxy = 1;  // BX = BY = bxy before small-grid widening
+    switch (NT) {
+        case 1: bxy = int64_t(G) * G * G >= int64_t(256) * 1024 ? 4 : 2; p.MA = bxy == 4 ? 4 : 1; p.NA = 1; break;
+        case 2: p.MA = 1; p.NA = 1; break;
+        case 3: bxy = 2; p.MA = 1; p.NA = 3; break;
+        case 4: p.MA = 1; p.NA = 1; break;
+        case 6: p.MA = 1; p.NA = 3; break;
+        case 8: p.MA = 1; p.NA = 2; break;
         default: throw Error{PRG_E_ARG, "unsupported channel count"};
     }
-    PRG_CHECK_LAUNCH();
+    p.BX = bxy < G ? bxy : G;
+    p.BY = bxy < G ? bxy : G;
+    if (NT == 2 || NT == 6) p.BY = 2 < G ? 2 : G;
+    // at least one full M-tile of 16 voxels where the grid has them
+    while (p.BX * p.BY * p.BZ < 16 && p.BY < G) p.BY *= 2;
+    while (p.BX * p.BY * p.BZ < 16 && p.BX < G) p.BX *= 2;
+    const int MT = (p.BX * p.BY * p.BZ + 15) / 16;
+    if (MT % p.MA || NT % p.NA) throw Error{PRG_E_ARG, "convolution plan does not tile"};
+    p.MG = MT / p.MA;
+    if (p.MG * (NT / p.NA) > kT / 64) throw Error{PRG_E_ARG, "convolution plan needs more waves"};
+    if (G % p.BX || G % p.BY || G % p.BZ) throw Error{PRG_E_ARG, "convolution brick does not divide the grid"};
+    return p;
+}
+
+bool pow2_channels(int c) { return c == 16 || c == 32 || c == 64; }
+
+void launch_conv(hipStream_t s, ConvIn src, const short* w, const float* bias, int G, int relu, float* out, int cout) {
+    const int cin = src.clo + src.cskip;
+    if (!pow2_channels(src.cskip) || (src.clo && !pow2_channels(src.clo)) || cout % 16 || cout <= 0 ||
+        (src.clo && G < 2))
+        throw Error{PRG_E_ARG, "unsupported convolution channels"};
+    const ConvPlan p = conv_plan(G, cout / 16);
+    const size_t lds = size_t(p.BX + 2) * (p.BY + 2) * (p.BZ + 2) * cin * 2;
+    if (lds > 64 * 1024) throw Error{PRG_E_ARG, "convolution halo exceeds 64 KiB of LDS"};
+    const unsigned nb = unsigned((G / p.BX) * (G / p.BY) * (G / p.BZ));
+    const int MA = p.MA, NA = p.NA;
+#define PRG_CONV(ma, na)                                                                                         \
+    if (MA == ma && NA == na) {                                                                                   \
+        k_conv3<ma, na><<<nb, kT, lds, s>>>(src, w, bias, G, p.BX, p.BY, p.BZ, p.MG, relu, out, cout);             \
+        PRG_CHECK_LAUNCH();                                                                                       \
+        return;                                                                                                   \
+    }
+    PRG_CONV(4, 1)
+    PRG_CONV(1, 1)
+    PRG_CONV(1, 2)
+    PRG_CONV(1, 3)
+#undef PRG_CONV
+    throw Error{PRG_E_ARG, "no convolution kernel for the plan"};
 }
 
 void pack_all(prg_unet* n) {
@@ -464,14 +633,11 @@ void forward(prg_unet* n) {
     auto& C = n->conv;
     auto run = [&](int k) {
         auto& L = C[k];
-        launch_conv(s, L.in, L.cin, L.wb.get(), n->params.get() + L.boff, L.G, 1, L.out, L.cout);
+        launch_conv(s, ConvIn{L.lo, L.clo, L.skip, L.cin - L.clo}, L.wb.get(), n->params.get() + L.boff, L.G, 1,
+                    L.out, L.cout);
     };
     auto pool = [&](const float* in, int g, int c, float* out) {
         k_pool<<<blocks(int64_t(g / 2) * (g / 2) * (g / 2) * c), kT, 0, s>>>(in, g, c, out);
-        PRG_CHECK_LAUNCH();
-    };
-    auto upc = [&](const float* lo, int c, const float* skip, int sk, int g, float* out) {
-        k_up_concat<<<blocks(int64_t(g) * g * g * (c + sk)), kT, 0, s>>>(lo, c, skip, sk, g, out);
         PRG_CHECK_LAUNCH();
     };
     run(0);
@@ -485,13 +651,10 @@ void forward(prg_unet* n) {
     pool(C[5].out, G / 4, 64, n->p3.get());
     run(6);
     run(7);
-    upc(C[7].out, 64, C[5].out, 64, G / 4, n->c2.get());
     run(8);
     run(9);
-    upc(C[9].out, 64, C[3].out, 32, G / 2, n->c1.get());
     run(10);
     run(11);
-    upc(C[11].out, 32, C[1].out, 16, G, n->c0.get());
     run(12);
     run(13);
     const int64_t V = int64_t(G) * G * G;
@@ -509,6 +672,11 @@ void conv_back(prg_unet* n, int k, float* g, float* gin) {
     PRG_CHECK_LAUNCH();
     k_bias_grad<<<L.cout, kT, 0, s>>>(g, V, L.cout, n->grads.get() + L.boff);
     PRG_CHECK_LAUNCH();
+    if (L.clo) {
+        // the decoder input the forward pass read through its fused loader
+        k_up_concat<<<blocks(V * L.cin), kT, 0, s>>>(L.lo, L.clo, L.skip, L.cin - L.clo, L.G, L.cat);
+        PRG_CHECK_LAUNCH();
+    }
     const int nchunk = int((V + kWChunk - 1) / kWChunk);
     const int64_t nw = int64_t(27) * L.cout * L.cin;
     const int64_t jobs = int64_t(nchunk) * 27 * (L.cout / 16) * (L.cin / 16);
@@ -516,7 +684,7 @@ void conv_back(prg_unet* n, int k, float* g, float* gin) {
     PRG_CHECK_LAUNCH();
     k_sum_chunks<<<blocks(nw), kT, 0, s>>>(n->part.get(), nchunk, nw, n->grads.get() + L.woff);
     PRG_CHECK_LAUNCH();
-    if (gin) launch_conv(s, g, L.cout, L.wf.get(), nullptr, L.G, 0, gin, L.cin);
+    if (gin) launch_conv(s, ConvIn{nullptr, 0, g, L.cout}, L.wf.get(), nullptr, L.G, 0, gin, L.cin);
 }
 
 void backward(prg_unet* n) {
@@ -715,6 +883,22 @@ prg_status prg_unet_create(prg_unet** out, int32_t device, int32_t grid, uint64_
         C[11].in = C[10].out;
         C[12].in = n->c0.get();
         C[13].in = C[12].out;
+        for (int k = 0; k < 14; k++) {
+            C[k].lo = nullptr;
+            C[k].clo = 0;
+            C[k].skip = C[k].in;
+            C[k].cat = nullptr;
+        }
+        // decoder inputs: concat(upsampled coarser output, encoder skip)
+        const int dec[3][3] = {{8, 7, 5}, {10, 9, 3}, {12, 11, 1}};
+        float* cats[3] = {n->c2.get(), n->c1.get(), n->c0.get()};
+        for (int j = 0; j < 3; j++) {
+            auto& L = C[dec[j][0]];
+            L.lo = C[dec[j][1]].out;
+            L.clo = C[dec[j][1]].cout;
+            L.skip = C[dec[j][2]].out;
+            L.cat = cats[j];
+        }
         // He-normal initial weights (splitmix64 + Box-Muller), zero biases
         std::vector<float> h(size_t(n->nparams), 0.0f);
         uint64_t st = seed;
