@@ -58,33 +58,54 @@ __global__ void k_sum_partials(const double* partial, int64_t n, double* out) {
     *out = s;
 }
 
-// corner owners (initVertexOwner, main.cu:1438-1498): corner j = x | y<<1 | z<<2
+// corner owners (initVertexOwner, main.cu:1438-1498): corner j = x | y<<1 | z<<2.
+// One thread per leaf slot: the slot's 26 neighbours and their keys are read
+// once and serve its 8 corners.  Corner j is owned by the smallest key among
+// the (up to 8) leaf slots sharing it; own[8 t + j] marks slot t's own
+// corners and oref[8 t + j] = 8 * (owner slot) + the corner's number there.
 __global__ void __launch_bounds__(kBlock) k_corner_owner(const uint32_t* key, const int32_t* neigh, int64_t M,
                                                          int64_t baseD, int64_t MD, int32_t* own, int32_t* oref) {
-    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (g >= 8 * MD) return;
-    const int64_t t = g >> 3;
-    const int j = int(g & 7);
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= MD) return;
     const int64_t i = baseD + t;
-    const int cx = j & 1, cy = (j >> 1) & 1, cz = (j >> 2) & 1;
-    uint32_t best = 0xffffffffu;
-    int64_t owner = i;
-    int jo = j;
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++)
-            for (int e = 0; e < 2; e++) {
-                const int dx = a ? (2 * cx - 1) : 0, dy = b ? (2 * cy - 1) : 0, dz = e ? (2 * cz - 1) : 0;
-                const int32_t nb = neigh[int64_t(9 * (dx + 1) + 3 * (dy + 1) + (dz + 1)) * M + i];
-                if (nb == -1) continue;
-                const uint32_t kk = key[nb];
-                if (kk < best) {
-                    best = kk;
-                    owner = nb;
-                    jo = (cx ^ a) | ((cy ^ b) << 1) | ((cz ^ e) << 2);
+    int32_t nb[27];
+    uint32_t kk[27];
+#pragma unroll
+    for (int dir = 0; dir < 27; dir++) {
+        nb[dir] = neigh[int64_t(dir) * M + i];
+        kk[dir] = nb[dir] == -1 ? 0xffffffffu : key[nb[dir]];
+    }
+    int32_t ow[8], orf[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int cx = j & 1, cy = (j >> 1) & 1, cz = (j >> 2) & 1;
+        uint32_t best = 0xffffffffu;
+        int64_t owner = i;
+        int jo = j;
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const int dx = a ? (2 * cx - 1) : 0, dy = b ? (2 * cy - 1) : 0, dz = e ? (2 * cz - 1) : 0;
+                    const int dir = 9 * (dx + 1) + 3 * (dy + 1) + (dz + 1);
+                    if (nb[dir] == -1) continue;
+                    if (kk[dir] < best) {
+                        best = kk[dir];
+                        owner = nb[dir];
+                        jo = (cx ^ a) | ((cy ^ b) << 1) | ((cz ^ e) << 2);
+                    }
                 }
-            }
-    own[g] = (owner == i) ? 1 : 0;
-    oref[g] = int32_t(8 * (owner - baseD) + jo);
+        ow[j] = (owner == i) ? 1 : 0;
+        orf[j] = int32_t(8 * (owner - baseD) + jo);
+    }
+    int4* o4 = reinterpret_cast<int4*>(own + 8 * t);
+    int4* r4 = reinterpret_cast<int4*>(oref + 8 * t);
+    o4[0] = make_int4(ow[0], ow[1], ow[2], ow[3]);
+    o4[1] = make_int4(ow[4], ow[5], ow[6], ow[7]);
+    r4[0] = make_int4(orf[0], orf[1], orf[2], orf[3]);
+    r4[1] = make_int4(orf[4], orf[5], orf[6], orf[7]);
 }
 
 // owned corners, compacted: vlist[vid] = 8 * leaf slot + corner
@@ -94,25 +115,49 @@ __global__ void __launch_bounds__(kBlock) k_corner_list(const int32_t* own, cons
     if (g < n8 && own[g]) vlist[vid[g]] = int32_t(g);
 }
 
+// the corner basis table (corner_basis_entry) for depths 0..D
+__global__ void __launch_bounds__(kBlock) k_corner_basis(const float* bf, int D, float* tab) {
+    const int64_t G = int64_t(1) << D;
+    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (g >= int64_t(D + 1) * G * 2) return;
+    const int cx = int(g & 1);
+    const int64_t ix = (g >> 1) % G;
+    const int d = int((g >> 1) / G);
+    float v[4];
+    corner_basis_entry(bf, D, d, ix, cx, v);
+    reinterpret_cast<float4*>(tab)[g] = make_float4(v[0], v[1], v[2], v[3]);
+}
+
+// f(corner) - iso for the owned corners (computeVertexImplicitFunctionValue,
+// main.cu:2259-2326): the chain sum of implicit_value at the corner's position,
+// with the basis values read from the corner basis table
 __global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* vlist, int64_t V, const uint32_t* key,
                                                          const int32_t* pnt, Levels L, const int32_t* parent,
-                                                         const float* x, const float* bf, int64_t baseD, int D,
+                                                         const float* x, const float4* tab, int64_t baseD, int D,
                                                          float iso, float* vval) {
     const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (v >= V) return;
     const int64_t g = vlist[v];
-    const int64_t t = g >> 3;
     const int j = int(g & 7);
-    const int64_t i = baseD + t;
-    float c[3];
-    leaf_center(key[i], D, c);
-    const float hw = 1.0f / float(1 << (D + 1));
-    // vertexPos (main.cu:1463-1468)
-    const float vx = c[0] + float(2 * (j & 1) - 1) * hw;
-    const float vy = c[1] + float(2 * ((j & 2) >> 1) - 1) * hw;
-    const float vz = c[2] + float(2 * ((j & 4) >> 2) - 1) * hw;
-    const float f = implicit_value(i, D, vx, vy, vz, key, pnt, L, parent, x, bf, D);
-    vval[v] = f - iso;
+    int64_t nd = baseD + (g >> 3);
+    const uint32_t k = key[nd];
+    const int64_t G = int64_t(1) << D;
+    // table rows of this corner's three coordinates at depth 0
+    const int64_t rx = 2 * int64_t(compact3(k >> 2)) + (j & 1);
+    const int64_t ry = 2 * int64_t(compact3(k >> 1)) + ((j >> 1) & 1);
+    const int64_t rz = 2 * int64_t(compact3(k)) + ((j >> 2) & 1);
+    float val = 0.0f;
+    for (int d = D; d > 0; d--) {
+        const int64_t o = 2 * G * d;
+        const float4 tx = tab[o + rx], ty = tab[o + ry], tz = tab[o + rz];
+        const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
+        val = level_sum(nd - L.base[d], d, bx, by, bz, pnt, L, x, val);
+        nd = parent[nd];
+    }
+    float tt = x[0] * tab[rx].y;
+    tt = tt * tab[ry].y;
+    tt = tt * tab[rz].y;
+    vval[v] = (val + tt) - iso;
 }
 
 // values of each leaf node's corners in square numbering (vertices[], main.cu:1659-1668)
@@ -271,8 +316,8 @@ void extract_leaf_mesh(prg_ctx* c) {
         vtot.reserve(1);
         {
             TimedScope ts(kt, "corner_owner", 8.0 * 64.0 * double(MD));
-            k_corner_owner<<<grid_for(8 * MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->neigh.get(), M, baseD, MD,
-                                                                       own.get(), oref.get());
+            k_corner_owner<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->neigh.get(), M, baseD, MD,
+                                                                   own.get(), oref.get());
             PRG_CHECK_LAUNCH();
         }
         scan_exclusive(own.get(), vid.get(), 8 * MD, vtot.get(), s, c->scratch, kt);
@@ -286,11 +331,16 @@ void extract_leaf_mesh(prg_ctx* c) {
         vlist.reserve(V);
         k_corner_list<<<grid_for(8 * MD, kBlock), kBlock, 0, s>>>(own.get(), vid.get(), 8 * MD, vlist.get());
         PRG_CHECK_LAUNCH();
+        PoolBuf<float> btab{&c->pool[91]};
+        const int64_t nbt = int64_t(D + 1) * (int64_t(1) << D) * 2;
+        btab.reserve(4 * nbt);
+        k_corner_basis<<<grid_for(nbt, kBlock), kBlock, 0, s>>>(c->d_base_fn.get(), D, btab.get());
+        PRG_CHECK_LAUNCH();
         {
             TimedScope ts(kt, "corner_value", 24.0 * double(V));
-            k_corner_value<<<grid_for(V, kBlock), kBlock, 0, s>>>(vlist.get(), V, c->key.get(), c->pnt.get(),
-                                                                  levels_of(c), c->parent.get(), c->x.get(),
-                                                                  c->d_base_fn.get(), baseD, D, c->iso, vval.get());
+            k_corner_value<<<grid_for(V, kBlock), kBlock, 0, s>>>(
+                vlist.get(), V, c->key.get(), c->pnt.get(), levels_of(c), c->parent.get(), c->x.get(),
+                reinterpret_cast<const float4*>(btab.get()), baseD, D, c->iso, vval.get());
             PRG_CHECK_LAUNCH();
         }
         k_corner_square<<<grid_for(8 * MD, kBlock), kBlock, 0, s>>>(oref.get(), vid.get(), vval.get(), MD, CV);

@@ -62,11 +62,65 @@ __device__ __forceinline__ uint32_t cell_key(uint32_t x, uint32_t y, uint32_t z)
     return (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
 }
 
+// Contribution of node nd (depth d >= 1, slot t in its depth) and its 27
+// neighbours to f, added to val in the order j = 0..26, given the basis
+// values of the node's cell offsets -1, 0, +1 along each axis (bx, by, bz);
+// the neighbours come from the node's parent-neighbour row.
+__device__ __forceinline__ float level_sum(int64_t t, int d, const float bx[3], const float by[3], const float bz[3],
+                                           const int32_t* __restrict__ pnt, const Levels& L,
+                                           const float* __restrict__ x, float val) {
+    const int32_t* row = pnt + 27 * (L.pnt_off[d] + (t >> 3));
+    const int s = int(t & 7);
+    const int sx = (s >> 2) & 1, sy = (s >> 1) & 1, sz = s & 1;
+    const float* xd = x + L.base[d];
+    // Neighbour (dx, dy, dz) of child slot s lies in parent neighbour
+    // ((sx + dx) >> 1) + 1 along x: sx for dx = -1, sx + 1 for dx = +1 and,
+    // for dx = 0, sx + 1 - sx.  So 8 row entries serve the 27 taps; E[a][b][c]
+    // is entry (sx + a, sy + b, sz + c), and the dx = 0 (dy, dz = 0) taps pick
+    // a = 1 - sx (b = 1 - sy, c = 1 - sz).
+    int32_t E[2][2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) E[a][b][c] = row[9 * (sx + a) + 3 * (sy + b) + (sz + c)];
+    int32_t F[3][2][2];  // x resolved: F[dx + 1][b][c]
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            F[0][b][c] = E[0][b][c];
+            F[1][b][c] = sx ? E[0][b][c] : E[1][b][c];
+            F[2][b][c] = E[1][b][c];
+        }
+    int32_t Gy[3][3][2];  // y resolved
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            Gy[a][0][c] = F[a][0][c];
+            Gy[a][1][c] = sy ? F[a][0][c] : F[a][1][c];
+            Gy[a][2][c] = F[a][1][c];
+        }
+#pragma unroll
+    for (int j = 0; j < 27; j++) {
+        const int jx = j / 9, jy = (j / 3) % 3, jz = j % 3;
+        const int vx = sx + (jx - 1), vy = sy + (jy - 1), vz = sz + (jz - 1);
+        const int32_t cb = jz == 0 ? Gy[jx][jy][0] : (jz == 2 ? Gy[jx][jy][1] : (sz ? Gy[jx][jy][0] : Gy[jx][jy][1]));
+        if (cb < 0) continue;
+        float tt = xd[cb + (((vx & 1) << 2) | ((vy & 1) << 1) | (vz & 1))] * bx[j / 9];
+        tt = tt * by[(j / 3) % 3];
+        tt = tt * bz[j % 3];
+        val = val + tt;
+    }
+    return val;
+}
+
 // Contribution of node nd (depth d >= 1) and its 27 neighbours to f at
 // (px, py, pz), added to val in the order j = 0..26.  Only three basis
 // functions per axis are involved (offsets -1, 0, +1 of the node's cell), so 9
-// evaluations serve the 27 products; the neighbours come from the node's
-// parent-neighbour row.
+// evaluations serve the 27 products.
 __device__ __forceinline__ float level_contrib(int64_t nd, int d, float px, float py, float pz,
                                                const uint32_t* __restrict__ key, const int32_t* __restrict__ pnt,
                                                const Levels& L, const float* __restrict__ x,
@@ -82,22 +136,35 @@ __device__ __forceinline__ float level_contrib(int64_t nd, int d, float px, floa
         by[a] = (v >= 0 && v < width) ? basis_value(bf, fb + v, py) : 0.0f;
         bz[a] = (w >= 0 && w < width) ? basis_value(bf, fb + w, pz) : 0.0f;
     }
-    const int64_t t = nd - L.base[d];
-    const int32_t* row = pnt + 27 * (L.pnt_off[d] + (t >> 3));
-    const int s = int(t & 7);
-    const float* xd = x + L.base[d];
-#pragma unroll
-    for (int j = 0; j < 27; j++) {
-        const int vx = ((s >> 2) & 1) + (j / 9 - 1), vy = ((s >> 1) & 1) + ((j / 3) % 3 - 1),
-                  vz = (s & 1) + (j % 3 - 1);
-        const int32_t cb = row[9 * ((vx >> 1) + 1) + 3 * ((vy >> 1) + 1) + ((vz >> 1) + 1)];
-        if (cb < 0) continue;
-        float tt = xd[cb + (((vx & 1) << 2) | ((vy & 1) << 1) | (vz & 1))] * bx[j / 9];
-        tt = tt * by[(j / 3) % 3];
-        tt = tt * bz[j % 3];
-        val = val + tt;
+    return level_sum(nd - L.base[d], d, bx, by, bz, pnt, L, x, val);
+}
+
+// Basis values at the corners of the leaf level.  A corner's coordinate along
+// one axis is its leaf's centre (offset ix at depth D) moved half a cell to
+// side cx (0 low, 1 high), and the leaf's ancestor at depth d has offset
+// ix >> (D - d), so every value level_contrib needs at a corner is
+//   tab[((d 2^D + ix) 2 + cx) 4 + a] = B_{2^d - 1 + (ix >> (D - d)) + a - 1}(coordinate)
+// (0 outside the depth's range; a = 0..2, slot 3 unused).  The same table
+// serves all three axes: leaf centres and basis functions are the same per
+// axis.  Depth 0 holds the root's basis in slot a = 1.  Built with the same
+// float operations as leaf_center and basis_value, so the values are those
+// the direct evaluation computes.
+__device__ __forceinline__ void corner_basis_entry(const float* __restrict__ bf, int D, int d, int64_t ix, int cx,
+                                                   float out[4]) {
+    float c = 0.5f, w = 0.25f;
+    for (int i = D - 1; i >= 0; --i) {
+        c = ((ix >> i) & 1) ? c + w : c - w;
+        w = w / 2;
     }
-    return val;
+    const float hw = 1.0f / float(1 << (D + 1));
+    const float p = c + float(2 * cx - 1) * hw;
+    const int64_t fb = (int64_t(1) << d) - 1, width = int64_t(1) << d, o = ix >> (D - d);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int64_t u = o + a - 1;
+        out[a] = (u >= 0 && u < width) ? basis_value(bf, fb + u, p) : 0.0f;
+    }
+    out[3] = 0.0f;
 }
 
 // f(pos) summed over the ancestor chain of node `nd` at depth `d` (node
