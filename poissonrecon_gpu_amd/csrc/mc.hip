@@ -161,94 +161,151 @@ __global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* vlist, i
 }
 
 // values of each leaf node's corners in square numbering (vertices[], main.cu:1659-1668)
+// values of each leaf node's corners in square numbering (vertices[],
+// main.cu:1659-1668): one thread per leaf slot
 __global__ void __launch_bounds__(kBlock) k_corner_square(const int32_t* oref, const int32_t* vid, const float* vval,
                                                           int64_t MD, float* cval) {
-    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (g >= 8 * MD) return;
-    const int64_t t = g >> 3;
-    const int q = int(g & 7);
-    const int y = (q >> 1) & 1, x = (q & 1) ^ y, z = q >> 2;
-    const int j = x | (y << 1) | (z << 2);
-    cval[g] = vval[vid[oref[8 * t + j]]];
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= MD) return;
+    const int4* o4 = reinterpret_cast<const int4*>(oref + 8 * t);
+    const int4 oa = o4[0], ob = o4[1];
+    const int32_t o[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
+    float v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int y = (q >> 1) & 1, x = (q & 1) ^ y, z = q >> 2;
+        v[q] = vval[vid[o[x | (y << 1) | (z << 2)]]];
+    }
+    float4* c4 = reinterpret_cast<float4*>(cval + 8 * t);
+    c4[0] = make_float4(v[0], v[1], v[2], v[3]);
+    c4[1] = make_float4(v[4], v[5], v[6], v[7]);
 }
 
 // edge owners (initEdgeArray / maintainEdgeNodePointer, main.cu:1795-2082) and
-// the sign-change test of the owner's corners (generateVexNums, :2457-2479)
+// the sign-change test of the owner's corners (generateVexNums, :2457-2479).
+// One thread per leaf slot.  Corner values are per corner, so an edge's
+// sign-change test gives the same answer from every slot sharing it; a slot
+// none of whose 12 edges changes sign flags nothing and skips the ownership
+// search (its eref entries are never read: only edges of cubes with
+// triangles are, and those change sign).
 __global__ void __launch_bounds__(kBlock) k_edge_owner(const uint32_t* key, const int32_t* neigh, const float* cval,
                                                        int64_t M, int64_t baseD, int64_t MD, int32_t* flag,
                                                        int32_t* eref) {
-    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (g >= 12 * MD) return;
-    const int64_t t = g / 12;
-    const int e = int(g % 12);
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= MD) return;
     const int64_t i = baseD + t;
-    const int o = e >> 2;
-    const int a1 = (o == 0) ? 1 : 0, a2 = (o == 2) ? 1 : 2;
-    const int b1 = e & 1, b2 = (e >> 1) & 1;
-    uint32_t best = 0xffffffffu;
-    int64_t owner = i;
-    int eo = e;
-    for (int u = 0; u < 2; u++)
-        for (int v = 0; v < 2; v++) {
-            int dd[3] = {0, 0, 0};
-            dd[a1] = u ? (2 * b1 - 1) : 0;
-            dd[a2] = v ? (2 * b2 - 1) : 0;
-            const int32_t nb = neigh[int64_t(9 * (dd[0] + 1) + 3 * (dd[1] + 1) + (dd[2] + 1)) * M + i];
-            if (nb == -1) continue;
-            const uint32_t kk = key[nb];
-            if (kk < best) {
-                best = kk;
-                owner = nb;
-                eo = (o << 2) | ((b2 ^ v) << 1) | (b1 ^ u);
-            }
-        }
-    eref[g] = int32_t(12 * (owner - baseD) + eo);
-    int f = 0;
-    if (owner == i) {
-        const float f1 = cval[8 * t + c_edge_vertex[e][0]];
-        const float f2 = cval[8 * t + c_edge_vertex[e][1]];
-        f = (f1 * f2 <= 0) ? 1 : 0;
+    const float4* c4 = reinterpret_cast<const float4*>(cval + 8 * t);
+    const float4 ca = c4[0], cb = c4[1];
+    const float cv[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+    bool sc[12];
+    bool any = false;
+#pragma unroll
+    for (int e = 0; e < 12; e++) {
+        sc[e] = cv[c_edge_vertex_h[e][0]] * cv[c_edge_vertex_h[e][1]] <= 0;
+        any = any || sc[e];
     }
-    flag[g] = f;
+    int4* f4 = reinterpret_cast<int4*>(flag + 12 * t);
+    if (!any) {
+        f4[0] = f4[1] = f4[2] = make_int4(0, 0, 0, 0);
+        return;
+    }
+    // the 27 neighbours and their keys (13 is the slot itself)
+    int32_t nb[27];
+    uint32_t kk[27];
+#pragma unroll
+    for (int dir = 0; dir < 27; dir++) {
+        const int ddx = dir / 9 - 1, ddy = (dir / 3) % 3 - 1, ddz = dir % 3 - 1;
+        if (ddx != 0 && ddy != 0 && ddz != 0) {  // corner neighbours share no edge
+            nb[dir] = -1;
+            kk[dir] = 0xffffffffu;
+            continue;
+        }
+        nb[dir] = neigh[int64_t(dir) * M + i];
+        kk[dir] = nb[dir] == -1 ? 0xffffffffu : key[nb[dir]];
+    }
+    int32_t fl[12], er[12];
+#pragma unroll
+    for (int e = 0; e < 12; e++) {
+        const int o = e >> 2;
+        const int a1 = (o == 0) ? 1 : 0, a2 = (o == 2) ? 1 : 2;
+        const int b1 = e & 1, b2 = (e >> 1) & 1;
+        uint32_t best = 0xffffffffu;
+        int64_t owner = i;
+        int eo = e;
+#pragma unroll
+        for (int u = 0; u < 2; u++)
+#pragma unroll
+            for (int v = 0; v < 2; v++) {
+                int dd[3] = {0, 0, 0};
+                dd[a1] = u ? (2 * b1 - 1) : 0;
+                dd[a2] = v ? (2 * b2 - 1) : 0;
+                const int dir = 9 * (dd[0] + 1) + 3 * (dd[1] + 1) + (dd[2] + 1);
+                if (nb[dir] == -1) continue;
+                if (kk[dir] < best) {
+                    best = kk[dir];
+                    owner = nb[dir];
+                    eo = (o << 2) | ((b2 ^ v) << 1) | (b1 ^ u);
+                }
+            }
+        er[e] = int32_t(12 * (owner - baseD) + eo);
+        fl[e] = (owner == i && sc[e]) ? 1 : 0;
+    }
+    f4[0] = make_int4(fl[0], fl[1], fl[2], fl[3]);
+    f4[1] = make_int4(fl[4], fl[5], fl[6], fl[7]);
+    f4[2] = make_int4(fl[8], fl[9], fl[10], fl[11]);
+    int4* r4 = reinterpret_cast<int4*>(eref + 12 * t);
+    r4[0] = make_int4(er[0], er[1], er[2], er[3]);
+    r4[1] = make_int4(er[4], er[5], er[6], er[7]);
+    r4[2] = make_int4(er[8], er[9], er[10], er[11]);
 }
 
-// interpolatePoint (main.cu:2584-2597) for every owned edge with a sign change
+// interpolatePoint (main.cu:2584-2597) for every owned edge with a sign
+// change; one thread per leaf slot
 __global__ void __launch_bounds__(kBlock) k_edge_vertex(const int32_t* flag, const int32_t* evid, const uint32_t* key,
                                                         const float* cval, int64_t baseD, int64_t MD, int D,
                                                         float* mesh_v) {
-    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (g >= 12 * MD || !flag[g]) return;
-    const int64_t t = g / 12;
-    const int e = int(g % 12);
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= MD) return;
+    const int4* f4 = reinterpret_cast<const int4*>(flag + 12 * t);
+    const int4 fa = f4[0], fb = f4[1], fc = f4[2];
+    const int32_t fl[12] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w, fc.x, fc.y, fc.z, fc.w};
+    bool any = false;
+#pragma unroll
+    for (int e = 0; e < 12; e++) any = any || fl[e] != 0;
+    if (!any) return;
     float c[3];
     leaf_center(key[baseD + t], D, c);
     const float hw = 1.0f / float(1 << (D + 1));
-    const int q1 = c_edge_vertex[e][0], q2 = c_edge_vertex[e][1];
-    float p1[3], p2[3];
-    {
-        const int y = (q1 >> 1) & 1, x = (q1 & 1) ^ y, z = q1 >> 2;
-        p1[0] = c[0] + float(2 * x - 1) * hw;
-        p1[1] = c[1] + float(2 * y - 1) * hw;
-        p1[2] = c[2] + float(2 * z - 1) * hw;
+#pragma unroll
+    for (int e = 0; e < 12; e++) {
+        if (!fl[e]) continue;
+        const int q1 = c_edge_vertex_h[e][0], q2 = c_edge_vertex_h[e][1];
+        float p1[3], p2[3];
+        {
+            const int y = (q1 >> 1) & 1, x = (q1 & 1) ^ y, z = q1 >> 2;
+            p1[0] = c[0] + float(2 * x - 1) * hw;
+            p1[1] = c[1] + float(2 * y - 1) * hw;
+            p1[2] = c[2] + float(2 * z - 1) * hw;
+        }
+        {
+            const int y = (q2 >> 1) & 1, x = (q2 & 1) ^ y, z = q2 >> 2;
+            p2[0] = c[0] + float(2 * x - 1) * hw;
+            p2[1] = c[1] + float(2 * y - 1) * hw;
+            p2[2] = c[2] + float(2 * z - 1) * hw;
+        }
+        const float f1 = cval[8 * t + q1], f2 = cval[8 * t + q2];
+        const int dim = e >> 2;
+        float out[3] = {p1[0], p1[1], p1[2]};
+        const float pivot = f1 / (f1 - f2);
+        const float another = 1 - pivot;
+        const float s1 = p2[dim] * pivot;
+        const float s2 = p1[dim] * another;
+        out[dim] = s1 + s2;
+        const int64_t v = evid[12 * t + e];
+        mesh_v[3 * v + 0] = out[0];
+        mesh_v[3 * v + 1] = out[1];
+        mesh_v[3 * v + 2] = out[2];
     }
-    {
-        const int y = (q2 >> 1) & 1, x = (q2 & 1) ^ y, z = q2 >> 2;
-        p2[0] = c[0] + float(2 * x - 1) * hw;
-        p2[1] = c[1] + float(2 * y - 1) * hw;
-        p2[2] = c[2] + float(2 * z - 1) * hw;
-    }
-    const float f1 = cval[8 * t + q1], f2 = cval[8 * t + q2];
-    const int dim = e >> 2;
-    float out[3] = {p1[0], p1[1], p1[2]};
-    const float pivot = f1 / (f1 - f2);
-    const float another = 1 - pivot;
-    const float s1 = p2[dim] * pivot;
-    const float s2 = p1[dim] * another;
-    out[dim] = s1 + s2;
-    const int64_t v = evid[g];
-    mesh_v[3 * v + 0] = out[0];
-    mesh_v[3 * v + 1] = out[1];
-    mesh_v[3 * v + 2] = out[2];
 }
 
 __global__ void __launch_bounds__(kBlock) k_tri_count(const float* cval, int64_t MD, int32_t* ntri) {
@@ -343,7 +400,7 @@ void extract_leaf_mesh(prg_ctx* c) {
                 reinterpret_cast<const float4*>(btab.get()), baseD, D, c->iso, vval.get());
             PRG_CHECK_LAUNCH();
         }
-        k_corner_square<<<grid_for(8 * MD, kBlock), kBlock, 0, s>>>(oref.get(), vid.get(), vval.get(), MD, CV);
+        k_corner_square<<<grid_for(MD, kBlock), kBlock, 0, s>>>(oref.get(), vid.get(), vval.get(), MD, CV);
         PRG_CHECK_LAUNCH();
         eflag.reserve(12 * MD);
         eref.reserve(12 * MD);
@@ -351,8 +408,8 @@ void extract_leaf_mesh(prg_ctx* c) {
         etot.reserve(1);
         {
             TimedScope ts(kt, "edge_owner", 12.0 * 40.0 * double(MD));
-            k_edge_owner<<<grid_for(12 * MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->neigh.get(), CV, M, baseD,
-                                                                      MD, eflag.get(), eref.get());
+            k_edge_owner<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->neigh.get(), CV, M, baseD, MD,
+                                                                 eflag.get(), eref.get());
             PRG_CHECK_LAUNCH();
         }
         scan_exclusive(eflag.get(), evid.get(), 12 * MD, etot.get(), s, c->scratch, kt);
@@ -367,8 +424,8 @@ void extract_leaf_mesh(prg_ctx* c) {
         PRG_HIP(hipStreamSynchronize(s));
         float* MV = c->mesh_v.reserve(size_t(3) * NV);
         int32_t* MT = c->mesh_t.reserve(size_t(3) * NT);
-        k_edge_vertex<<<grid_for(12 * MD, kBlock), kBlock, 0, s>>>(eflag.get(), evid.get(), c->key.get(), CV, baseD,
-                                                                   MD, D, MV);
+        k_edge_vertex<<<grid_for(MD, kBlock), kBlock, 0, s>>>(eflag.get(), evid.get(), c->key.get(), CV, baseD, MD,
+                                                              D, MV);
         PRG_CHECK_LAUNCH();
         k_tri_emit<<<grid_for(MD, kBlock), kBlock, 0, s>>>(CV, toff.get(), eref.get(), evid.get(), MD, MT);
         PRG_CHECK_LAUNCH();

@@ -13,6 +13,9 @@ __constant__ int8_t c_tri[256][16] = {PRG_MC_TRIANGLE_TABLE_DATA};
 // edgeVertex (MarchingCubes.cuh:693-706): square-numbered corners of each edge kind
 __constant__ int8_t c_edge_vertex[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 3}, {1, 2},
                                             {4, 7}, {5, 6}, {0, 4}, {1, 5}, {3, 7}, {2, 6}};
+// the same table as compile-time constants, for fully unrolled per-slot loops
+__device__ constexpr int8_t c_edge_vertex_h[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 3}, {1, 2},
+                                                      {4, 7}, {5, 6}, {0, 4}, {1, 5}, {3, 7}, {2, 6}};
 
 __device__ __forceinline__ uint32_t compact3(uint32_t x) {
     x &= 0x09249249u;
