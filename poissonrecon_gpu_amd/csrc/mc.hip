@@ -51,11 +51,21 @@ __global__ void __launch_bounds__(kBlock) k_point_values(const float* px, const 
     }
 }
 
-__global__ void k_sum_partials(const double* partial, int64_t n, double* out) {
-    if (threadIdx.x != 0) return;
+// sum of n partials in a fixed order: contiguous runs per thread, then a tree
+__global__ void __launch_bounds__(1024) k_sum_partials(const double* partial, int64_t n, double* out) {
+    __shared__ double red[1024];
+    const int tid = threadIdx.x;
+    const int64_t per = (n + 1023) / 1024;
+    const int64_t b = int64_t(tid) * per, e = b + per < n ? b + per : n;
     double s = 0.0;
-    for (int64_t i = 0; i < n; i++) s += partial[i];
-    *out = s;
+    for (int64_t i = b; i < e; i++) s += partial[i];
+    red[tid] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) *out = red[0];
 }
 
 // corner owners (initVertexOwner, main.cu:1438-1498): corner j = x | y<<1 | z<<2.
@@ -344,7 +354,7 @@ void compute_isovalue(prg_ctx* c) {
                                              c->pvals.reserve(n), part.get());
         PRG_CHECK_LAUNCH();
     }
-    k_sum_partials<<<1, 64, 0, s>>>(part.get(), nb, tot.get());
+    k_sum_partials<<<1, 1024, 0, s>>>(part.get(), nb, tot.get());
     PRG_CHECK_LAUNCH();
     double sum = 0.0;
     PRG_HIP(hipMemcpyAsync(&sum, tot.get(), 8, hipMemcpyDeviceToHost, s));

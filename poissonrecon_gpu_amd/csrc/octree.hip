@@ -85,16 +85,30 @@ __global__ void __launch_bounds__(kBlock) k_bbox_partial(const float* xyz, int64
 
 // Scale and centre from the block minima / maxima, in the reference's float
 // operations (main.cu:539-545): tr = (centre x, y, z, scale).
-__global__ void k_bbox_final(const float* part, int nb, float* tr) {
-    if (threadIdx.x != 0) return;
+__global__ void __launch_bounds__(1024) k_bbox_final(const float* part, int nb, float* tr) {
+    // minima and maxima are exact in any order: one partial per thread, then a tree
+    __shared__ float red[6][1024];
+    const int tid = threadIdx.x;
+    for (int a = 0; a < 6; a++) red[a][tid] = tid < nb ? part[6 * tid + a] : part[a];
+    for (int b = tid + 1024; b < nb; b += 1024)
+        for (int a = 0; a < 3; a++) {
+            red[a][tid] = fminf(red[a][tid], part[6 * b + a]);
+            red[3 + a][tid] = fmaxf(red[3 + a][tid], part[6 * b + 3 + a]);
+        }
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (tid < o)
+            for (int a = 0; a < 3; a++) {
+                red[a][tid] = fminf(red[a][tid], red[a][tid + o]);
+                red[3 + a][tid] = fmaxf(red[3 + a][tid], red[3 + a][tid + o]);
+            }
+        __syncthreads();
+    }
+    if (tid != 0) return;
     float mn[3], mx[3];
     for (int a = 0; a < 3; a++) {
-        mn[a] = part[a];
-        mx[a] = part[3 + a];
-        for (int b = 1; b < nb; b++) {
-            mn[a] = fminf(mn[a], part[6 * b + a]);
-            mx[a] = fmaxf(mx[a], part[6 * b + 3 + a]);
-        }
+        mn[a] = red[a][0];
+        mx[a] = red[3 + a][0];
     }
     float scale = 1.0f, center[3];
     for (int a = 0; a < 3; a++) {
@@ -316,16 +330,16 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(const uint32_t* key, const
     const int64_t i = base + t;
     const int s = int((key[i] >> (3 * (D - d))) & 7u);
     const int64_t p = parent[i];
+    // The parent's rows (depth d - 1) and the rows written here (depth d) are
+    // disjoint: all loads are issued before the first store, so the 27
+    // lookups overlap instead of waiting on each other.
+    int32_t np[27], cb[27];
 #pragma unroll
-    for (int j = 0; j < 27; j++) {
-        const int32_t np = neigh[int64_t(c_lut.parent[s][j]) * M + p];
-        int32_t v = -1;
-        if (np != -1) {
-            const int32_t cb = child_base[np];
-            if (cb != -1) v = cb + c_lut.child[s][j];
-        }
-        neigh[int64_t(j) * M + i] = v;
-    }
+    for (int j = 0; j < 27; j++) np[j] = neigh[int64_t(c_lut.parent[s][j]) * M + p];
+#pragma unroll
+    for (int j = 0; j < 27; j++) cb[j] = np[j] != -1 ? child_base[np[j]] : -1;
+#pragma unroll
+    for (int j = 0; j < 27; j++) neigh[int64_t(j) * M + i] = cb[j] != -1 ? cb[j] + c_lut.child[s][j] : -1;
 }
 
 // Parent-neighbour table of one depth: for sibling block b, the level-local
@@ -373,7 +387,7 @@ void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n
         PRG_CHECK_LAUNCH();
     }
     {
-        k_bbox_final<<<1, 64, 0, s>>>(part.get(), blocks, c->transform_dev.reserve(4));
+        k_bbox_final<<<1, 1024, 0, s>>>(part.get(), blocks, c->transform_dev.reserve(4));
         PRG_CHECK_LAUNCH();
     }
     c->N = n;
