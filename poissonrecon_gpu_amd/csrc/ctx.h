@@ -95,6 +95,7 @@ struct prg_ctx {
     std::array<double, PRG_STAGE_COUNT> stage_ms{};
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     std::vector<hipEvent_t> ev_pool;  // the solver's events, reused across solves
+    int num_cu = 0;                   // compute units of the device
     void* cg_host = nullptr;          // pinned: the CG chains' state read-backs (256 bytes)
 
     int64_t MD() const { return counts.empty() ? 0 : counts[D]; }

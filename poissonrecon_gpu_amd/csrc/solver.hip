@@ -141,6 +141,7 @@ constexpr int kSpmvThreads = 512;
 constexpr int kTile = kSpmvThreads / 8;   // sibling blocks per SpMV tile, one row per lane
 constexpr int kTileEntries = 27 * kTile;  // parent-neighbour entries of a tile
 constexpr int kHashSlots = 2048;          // > kTileEntries: linear probing always finds a slot
+constexpr int kGridRow = 72;              // floats per block grid row in the SpMV (64 cells + bank padding)
 static_assert(kHashSlots > kTileEntries, "hash must hold every entry of a tile");
 
 // r = b - A x0, p = r, xs = x0 (the solve works on the 16-byte-aligned copy
@@ -231,6 +232,12 @@ __global__ void __launch_bounds__(kSpmvThreads) k_tile_plan(const int32_t* pnt, 
 // p_1 = r is already in pnew), each distinct sibling block of a tile loaded
 // once into LDS (see k_tile_plan), and the rows summed from LDS in the order
 // j = 0..26.  Writes this workgroup's partial of p.q.
+//
+// The tile loop is software-pipelined two tiles deep: while tile t is summed,
+// the cell values of tile t + 1 and the plan (count, row ids, block bases) of
+// tile t + 2 are in flight in registers.  K = cells per thread covers the
+// depth's largest tile (8 * count <= K * kSpmvThreads).
+template <int K>
 __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, const int32_t* __restrict__ count,
                                                           const int32_t* __restrict__ ubase,
                                                           const int16_t* __restrict__ rowid, const float* r,
@@ -238,10 +245,17 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, 
                                                           const double* rr_part, const double* bb_part, int32_t np,
                                                           double* pq_part, CgState* st, float tol, int32_t fixed,
                                                           int32_t max_iters) {
+    static_assert(K * kSpmvThreads <= 8 * kTileEntries, "cells per thread beyond a tile's plan");
+    constexpr int kRidWords = kTileEntries / 2;  // row ids as 32-bit words
     extern __shared__ float cellv[];
-    __shared__ int16_t rid[kTileEntries];
-    __shared__ float grid[kTile][64];  // each block's 4x4x4 neighbourhood, (x, y, z) = (c >> 4, (c >> 2) & 3, c & 3)
+    __shared__ int32_t rid32[kRidWords];
+    // each block's 4x4x4 neighbourhood, (x, y, z) = (c >> 4, (c >> 2) & 3, c & 3);
+    // rows padded to 72 floats: the 8 blocks of a wave start 8 banks apart, so
+    // a stencil read (ds_read2 over 64 lanes) meets at most 2 lanes per bank
+    // and the fill writes none (64-float rows put all 8 blocks in one bank set)
+    __shared__ float grid[kTile][kGridRow];
     __shared__ double lds[kSpmvThreads / 64];
+    const int16_t* rid = reinterpret_cast<const int16_t*>(rid32);
     if (st->done) return;
     const int k1 = it - 1;
     const double rr = sum_partials<kSpmvThreads>(rr_part, np, lds);
@@ -268,16 +282,51 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, 
     const int64_t tbeg = int64_t(blockIdx.x) * per, tend = tbeg + per < ntiles ? tbeg + per : ntiles;
     // this lane's row sits at (1 + sx, 1 + sy, 1 + sz) of its block's grid
     const int c0 = 16 * ((s >> 2) & 1) + 4 * ((s >> 1) & 1) + (s & 1);
+    // plan of one tile: cell count, row-id words, block base of each of this thread's cells
+    struct Plan {
+        int32_t cnt;
+        int32_t rw[2];
+        int32_t ub[K];
+    };
+    auto load_plan = [&](int64_t tile, Plan& P) {
+        if (tile >= tend) return;
+        P.cnt = count[tile];
+        const int32_t* w = reinterpret_cast<const int32_t*>(rowid + tile * kTileEntries);
+        P.rw[0] = w[tid];
+        P.rw[1] = tid + kSpmvThreads < kRidWords ? w[tid + kSpmvThreads] : 0;
+        const int32_t* tub = ubase + tile * kTileEntries;
+#pragma unroll
+        for (int k = 0; k < K; k++) P.ub[k] = tub[(tid + k * kSpmvThreads) >> 3];  // in bounds; masked by cnt later
+    };
+    auto load_cells = [&](int64_t tile, const Plan& P, float v[K]) {
+        if (tile >= tend) return;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int u = tid + k * kSpmvThreads;
+            if (u < 8 * P.cnt) {
+                const int64_t idx = int64_t(P.ub[k]) + (u & 7);
+                v[k] = first ? pnew[idx] : pdir(r, pold, bf, idx);
+            }
+        }
+    };
+    Plan pa{}, pb{};
+    float va[K];
+    load_plan(tbeg, pa);
+    load_cells(tbeg, pa, va);
+    load_plan(tbeg + 1, pb);
     double pq = 0.0;
     for (int64_t tile = tbeg; tile < tend; tile++) {
-        const int16_t* trow = rowid + tile * kTileEntries;
-        const int32_t* tub = ubase + tile * kTileEntries;
-        for (int e = tid; e < kTileEntries; e += kSpmvThreads) rid[e] = trow[e];
-        const int ncell = 8 * count[tile];
-        for (int u = tid; u < ncell; u += kSpmvThreads) {
-            const int64_t idx = int64_t(tub[u >> 3]) + (u & 7);
-            cellv[u] = first ? pnew[idx] : pdir(r, pold, bf, idx);
+        // tile: values in va, plan in pa; tile + 1: plan in pb
+        rid32[tid] = pa.rw[0];
+        if (tid + kSpmvThreads < kRidWords) rid32[tid + kSpmvThreads] = pa.rw[1];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int u = tid + k * kSpmvThreads;
+            if (u < 8 * pa.cnt) cellv[u] = va[k];
         }
+        load_cells(tile + 1, pb, va);  // va is free again: tile + 1's values
+        pa = pb;
+        load_plan(tile + 2, pb);
         __syncthreads();
         // lanes of block g fill its 4x4x4 grid: cell (x, y, z) lies in parent
         // neighbour (x >> 1, y >> 1, z >> 1) at child slot (x & 1, y & 1, z & 1)
@@ -547,7 +596,31 @@ struct Chain {
 };
 
 constexpr int kMaxWG = 1024;       // workgroups of the vector kernels (their partial count)
-constexpr int kMaxSpmvWG = 2048;   // workgroups of the SpMV
+constexpr int kMaxSpmvWG = 2048;   // workgroups of the SpMV (its partial count)
+
+// SpMV instance: cells per thread covering the depth's largest tile.  The
+// launch is one round of workgroups, two per CU, each pipelining a run of
+// tiles (four per CU measured the same at depth 10).
+int spmv_instance(int kcells) { return kcells <= 2 ? 0 : kcells <= 4 ? 1 : kcells <= 8 ? 2 : 3; }
+
+constexpr int kSpmvWGPerCU = 2;
+
+void launch_spmv(int inst, int32_t ns, size_t lds, hipStream_t s, int64_t n, Stencil S, const int32_t* cnt,
+                 const int32_t* ub, const int16_t* rw, const float* r, const float* pold, float* pnew, float* q,
+                 int32_t it, const double* rr_part, const double* bb_part, int32_t np, double* pq_part, CgState* st,
+                 float tol, int32_t fixed, int32_t max_iters) {
+#define PRG_SPMV(KC)                                                                                            \
+    k_cg_spmv<KC><<<ns, kSpmvThreads, lds, s>>>(n, S, cnt, ub, rw, r, pold, pnew, q, it, rr_part, bb_part, np, \
+                                                pq_part, st, tol, fixed, max_iters)
+    switch (inst) {
+        case 0: PRG_SPMV(2); break;
+        case 1: PRG_SPMV(4); break;
+        case 2: PRG_SPMV(8); break;
+        default: PRG_SPMV(27); break;
+    }
+#undef PRG_SPMV
+    PRG_CHECK_LAUNCH();
+}
 
 int64_t tiles_of(int64_t n) { return (n / 8 + kTile - 1) / kTile; }
 
@@ -736,8 +809,10 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             const Stencil S = stencil_of(d);
             const int32_t nv = vec_grid(n);
             const int64_t ntiles = tiles_of(n);
-            const int32_t ns = int32_t(std::min<int64_t>(kMaxSpmvWG, ntiles));
             const size_t lds = size_t(8) * 4 * std::max(1, c->plan_maxcount[d]);
+            const int inst = spmv_instance((8 * std::max(1, c->plan_maxcount[d]) + kSpmvThreads - 1) / kSpmvThreads);
+            const int32_t ns =
+                int32_t(std::min<int64_t>({int64_t(kMaxSpmvWG), int64_t(kSpmvWGPerCU) * std::max(1, c->num_cu), ntiles}));
             const int64_t t0 = c->plan_tile_off[d];
             const int32_t* cnt = c->plan_count.get() + t0;
             const int32_t* ub = c->plan_ubase.get() + t0 * kTileEntries;
@@ -757,9 +832,8 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                     // 27 int32 neighbour indices, p_i read, (Ap)_i written (116 B)
                     TimedScope ts(c->kt, spmv_name, kSpmvRowBytes * double(n));
                     h.launched.emplace_back(ts.idx, it);
-                    k_cg_spmv<<<ns, kSpmvThreads, lds, h.s>>>(n, S, cnt, ub, rw, h.R, pold, pnew, h.Q, it, rr_prev,
-                                                              h.bb, nv, h.pq, h.st, tol, fixed, max_iters);
-                    PRG_CHECK_LAUNCH();
+                    launch_spmv(inst, ns, lds, h.s, n, S, cnt, ub, rw, h.R, pold, pnew, h.Q, it, rr_prev, h.bb, nv,
+                                h.pq, h.st, tol, fixed, max_iters);
                 }
                 const bool do_refresh = (it % refresh) == 0;
                 {
