@@ -19,6 +19,7 @@
 #include <cmath>
 
 #include "ctx.h"
+#include "pnt.h"
 
 namespace prg {
 namespace {
@@ -129,22 +130,88 @@ __device__ __forceinline__ float fdot(float vx, float vy, float vz, float ux, fl
     return r;
 }
 
-// leaf level: 27 terms per node
-__global__ void __launch_bounds__(kBlock) k_div_leaf(const int32_t* neigh, const float* vfx, const float* vfy,
-                                                     const float* vfz, const float* T, int64_t M, int64_t baseD,
-                                                     int64_t MD, float* div) {
+// leaf level: 27 terms per node, the neighbours from the leaf level's
+// parent-neighbour rows
+__global__ void __launch_bounds__(kBlock) k_div_leaf(const int32_t* pntD, const float* vfx, const float* vfy,
+                                                     const float* vfz, const float* T, int64_t baseD, int64_t MD,
+                                                     float* div) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
-    const int64_t i = baseD + t;
+    int32_t nb[27];
+    pnt_neighbours(pntD + 27 * (t >> 3), int(t & 7), nb);
     double val = 0.0;
+#pragma unroll
     for (int j = 0; j < 27; j++) {
-        const int32_t nb = neigh[int64_t(j) * M + i];
-        if (nb == -1) continue;
-        const int64_t l = nb - baseD;
+        if (nb[j] < 0) continue;
+        const int64_t l = nb[j];
         const float r = fdot(vfx[l], vfy[l], vfz[l], T[j / 9], T[(j / 3) % 3], T[j % 3]);
         val += double(r);
     }
-    div[i] = float(val);
+    div[baseD + t] = float(val);
+}
+
+// The 27 terms fdot(v, (wx[j / 9], wy[(j / 3) % 3], wz[j % 3])) from the 9
+// distinct products: each term is still ((0 + vx wx) + vy wy) + vz wz in
+// float, the operations fdot performs.
+__device__ __forceinline__ void terms27(float vx, float vy, float vz, const float wx[3], const float wy[3],
+                                       const float wz[3], float r[27]) {
+    float sx[3], sxy[3][3], pz[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        sx[a] = 0.0f + vx * wx[a];
+        pz[a] = vz * wz[a];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) sxy[a][b] = sx[a] + vy * wy[b];
+#pragma unroll
+    for (int j = 0; j < 27; j++) r[j] = sxy[j / 9][(j / 3) % 3] + pz[j % 3];
+}
+
+// Sums of 32 per-lane doubles over the 64 lanes of a wave in a fixed order
+// (a reduce-scatter: each halving step exchanges half of the values still
+// held), 32 shuffles instead of 6 per value.  Returns the total of entry
+// q(lane) = 16 b5 + 8 b4 + 4 b3 + 2 b2 + b1 (b_i = bit i of the lane); lanes
+// L and L ^ 1 hold the same entry.
+__device__ __forceinline__ double wave_reduce_scatter32(double v[32], int lane) {
+    const bool h5 = (lane >> 5) & 1, h4 = (lane >> 4) & 1, h3 = (lane >> 3) & 1, h2 = (lane >> 2) & 1,
+               h1 = (lane >> 1) & 1;
+    double k16[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const double give = h5 ? v[i] : v[16 + i];
+        const double keep = h5 ? v[16 + i] : v[i];
+        k16[i] = keep + __shfl_xor(give, 32, 64);
+    }
+    double k8[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const double give = h4 ? k16[i] : k16[8 + i];
+        const double keep = h4 ? k16[8 + i] : k16[i];
+        k8[i] = keep + __shfl_xor(give, 16, 64);
+    }
+    double k4[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double give = h3 ? k8[i] : k8[4 + i];
+        const double keep = h3 ? k8[4 + i] : k8[i];
+        k4[i] = keep + __shfl_xor(give, 8, 64);
+    }
+    double k2[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const double give = h2 ? k4[i] : k4[2 + i];
+        const double keep = h2 ? k4[2 + i] : k4[i];
+        k2[i] = keep + __shfl_xor(give, 4, 64);
+    }
+    const double give = h1 ? k2[0] : k2[1];
+    const double keep = h1 ? k2[1] : k2[0];
+    const double k1 = keep + __shfl_xor(give, 2, 64);
+    // lanes L and L ^ 1 hold the same entry's two halves; the lower lane's
+    // part comes first in the sum
+    const double other = __shfl_xor(k1, 1, 64);
+    return (lane & 1) ? other + k1 : k1 + other;
 }
 
 __device__ __forceinline__ int depth_of(const Bases& B, int D, int64_t i) {
@@ -174,12 +241,22 @@ __global__ void __launch_bounds__(kBlock) k_g_chunks(const int32_t* dnum, int64_
     multi[a] = (c > 1) ? c : 0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_g_chunk_map(const int32_t* chunks, const int32_t* chunk_off, int64_t n,
+// node of each chunk: the last node whose chunk offset is <= c (the
+// exclusive scan of chunk counts is non-decreasing; nodes without chunks
+// share their successor's offset, so the last one is the chunk's owner)
+__global__ void __launch_bounds__(kBlock) k_g_chunk_map(const int32_t* chunk_off, int64_t n, const int32_t* d_nchunks,
                                                         int32_t* chunk_node) {
-    const int64_t a = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (a >= n) return;
-    const int32_t o = chunk_off[a];
-    for (int32_t c = 0; c < chunks[a]; c++) chunk_node[o + c] = int32_t(a);
+    const int64_t c = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (c >= *d_nchunks) return;
+    int64_t lo = 0, hi = n;  // invariant: chunk_off[lo] <= c, and hi is past the answer
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (chunk_off[mid] <= c)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    chunk_node[c] = int32_t(lo);
 }
 
 // one wave per chunk: 27 double accumulators per lane, cross-lane reduction
@@ -204,35 +281,28 @@ __global__ void __launch_bounds__(kBlock) k_g_sums(const uint32_t* key, const in
         const int64_t sc = int64_t(1) << (D - d);
         const uint32_t rmask = uint32_t(sc - 1);
         const float* T = table + table_off[d];
-        double acc[27];
+        double acc[32];
 #pragma unroll
-        for (int j = 0; j < 27; j++) acc[j] = 0.0;
+        for (int j = 0; j < 32; j++) acc[j] = 0.0;
         for (int64_t k = k0 + lane; k < k1; k += 64) {
             const uint32_t kl = key[baseD + k];
             const int64_t rx = compact3(kl >> 2) & rmask, ry = compact3(kl >> 1) & rmask, rz = compact3(kl) & rmask;
             const float wx[3] = {T[rx], T[sc + rx], T[2 * sc + rx]};
             const float wy[3] = {T[ry], T[sc + ry], T[2 * sc + ry]};
             const float wz[3] = {T[rz], T[sc + rz], T[2 * sc + rz]};
-            const float vx = vfx[k], vy = vfy[k], vz = vfz[k];
+            float r[27];
+            terms27(vfx[k], vfy[k], vfz[k], wx, wy, wz, r);
 #pragma unroll
-            for (int j = 0; j < 27; j++) acc[j] += double(fdot(vx, vy, vz, wx[j / 9], wy[(j / 3) % 3], wz[j % 3]));
+            for (int j = 0; j < 27; j++) acc[j] += double(r[j]);
         }
-#pragma unroll
-        for (int j = 0; j < 27; j++) {
-            double v = acc[j];
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-            acc[j] = v;
-        }
-        if (lane == 0) {
-            if (chunks[a] == 1) {
-#pragma unroll
-                for (int j = 0; j < 27; j++) G[j * gstride + a] = acc[j];
-            } else {
-                double* P = partial + 27 * (int64_t(multi_off[a]) + ci);
-#pragma unroll
-                for (int j = 0; j < 27; j++) P[j] = acc[j];
-            }
+        const double tot = wave_reduce_scatter32(acc, lane);
+        const int q = 16 * ((lane >> 5) & 1) + 8 * ((lane >> 4) & 1) + 4 * ((lane >> 3) & 1) + 2 * ((lane >> 2) & 1) +
+                      ((lane >> 1) & 1);
+        if (!(lane & 1) && q < 27) {
+            if (chunks[a] == 1)
+                G[q * gstride + a] = tot;
+            else
+                partial[27 * (int64_t(multi_off[a]) + ci) + q] = tot;
         }
     }
 }
@@ -295,10 +365,11 @@ __global__ void __launch_bounds__(kBlock) k_g_parent_sums(const uint32_t* key, c
     const float wx[3] = {T[rx], T[2 + rx], T[4 + rx]};
     const float wy[3] = {T[ry], T[2 + ry], T[4 + ry]};
     const float wz[3] = {T[rz], T[2 + rz], T[4 + rz]};
-    const float vx = vfx[k], vy = vfy[k], vz = vfz[k];
+    float r[27];
+    terms27(vfx[k], vfy[k], vfz[k], wx, wy, wz, r);
     double acc[27];
 #pragma unroll
-    for (int j = 0; j < 27; j++) acc[j] = double(fdot(vx, vy, vz, wx[j / 9], wy[(j / 3) % 3], wz[j % 3]));
+    for (int j = 0; j < 27; j++) acc[j] = double(r[j]);
 #pragma unroll
     for (int m = 1; m < 8; m <<= 1)
 #pragma unroll
@@ -380,8 +451,8 @@ void compute_divergence(prg_ctx* c) {
                            hipMemcpyHostToDevice, s));
     {
         TimedScope ts(c->kt, "div_leaf", 124.0 * double(MD));
-        k_div_leaf<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->neigh.get(), VX, VY, VZ,
-                                                           c->d_div_table.get() + c->fs->div_off[D], M, baseD, MD, DV);
+        k_div_leaf<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->pnt.get() + 27 * c->pnt_off[D], VX, VY, VZ,
+                                                           c->d_div_table.get() + c->fs->div_off[D], baseD, MD, DV);
         PRG_CHECK_LAUNCH();
     }
     const int64_t n = baseD;  // nodes of depths 0 .. D-1
@@ -415,7 +486,7 @@ void compute_divergence(prg_ctx* c) {
         PRG_HIP(hipStreamSynchronize(s));
         cnode.reserve(nchunks);
         partial.reserve(size_t(27) * std::max(1, nmulti));
-        k_g_chunk_map<<<grid_for(n, kBlock), kBlock, 0, s>>>(chunks.get(), coff.get(), n, cnode.get());
+        k_g_chunk_map<<<grid_for(std::max(1, nchunks), kBlock), kBlock, 0, s>>>(coff.get(), n, ctot.get(), cnode.get());
         PRG_CHECK_LAUNCH();
         Bases B{};
         for (int d = 0; d <= D; d++) B.b[d] = c->bases[d];

@@ -4,6 +4,7 @@
 #pragma once
 
 #include "ctx.h"
+#include "pnt.h"
 
 namespace prg {
 namespace {
@@ -73,46 +74,13 @@ __device__ __forceinline__ float level_sum(int64_t t, int d, const float bx[3], 
                                            const int32_t* __restrict__ pnt, const Levels& L,
                                            const float* __restrict__ x, float val) {
     const int32_t* row = pnt + 27 * (L.pnt_off[d] + (t >> 3));
-    const int s = int(t & 7);
-    const int sx = (s >> 2) & 1, sy = (s >> 1) & 1, sz = s & 1;
     const float* xd = x + L.base[d];
-    // Neighbour (dx, dy, dz) of child slot s lies in parent neighbour
-    // ((sx + dx) >> 1) + 1 along x: sx for dx = -1, sx + 1 for dx = +1 and,
-    // for dx = 0, sx + 1 - sx.  So 8 row entries serve the 27 taps; E[a][b][c]
-    // is entry (sx + a, sy + b, sz + c), and the dx = 0 (dy, dz = 0) taps pick
-    // a = 1 - sx (b = 1 - sy, c = 1 - sz).
-    int32_t E[2][2][2];
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++)
-#pragma unroll
-            for (int c = 0; c < 2; c++) E[a][b][c] = row[9 * (sx + a) + 3 * (sy + b) + (sz + c)];
-    int32_t F[3][2][2];  // x resolved: F[dx + 1][b][c]
-#pragma unroll
-    for (int b = 0; b < 2; b++)
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            F[0][b][c] = E[0][b][c];
-            F[1][b][c] = sx ? E[0][b][c] : E[1][b][c];
-            F[2][b][c] = E[1][b][c];
-        }
-    int32_t Gy[3][3][2];  // y resolved
-#pragma unroll
-    for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-            Gy[a][0][c] = F[a][0][c];
-            Gy[a][1][c] = sy ? F[a][0][c] : F[a][1][c];
-            Gy[a][2][c] = F[a][1][c];
-        }
+    int32_t nb[27];
+    pnt_neighbours(row, int(t & 7), nb);
 #pragma unroll
     for (int j = 0; j < 27; j++) {
-        const int jx = j / 9, jy = (j / 3) % 3, jz = j % 3;
-        const int vx = sx + (jx - 1), vy = sy + (jy - 1), vz = sz + (jz - 1);
-        const int32_t cb = jz == 0 ? Gy[jx][jy][0] : (jz == 2 ? Gy[jx][jy][1] : (sz ? Gy[jx][jy][0] : Gy[jx][jy][1]));
-        if (cb < 0) continue;
-        float tt = xd[cb + (((vx & 1) << 2) | ((vy & 1) << 1) | (vz & 1))] * bx[j / 9];
+        if (nb[j] < 0) continue;
+        float tt = xd[nb[j]] * bx[j / 9];
         tt = tt * by[(j / 3) % 3];
         tt = tt * bz[j % 3];
         val = val + tt;
