@@ -69,6 +69,7 @@ struct prg_ctx {
     int64_t plan_tile_off[prg::kMaxDepth + 2] = {};
     int32_t plan_maxcount[prg::kMaxDepth + 1] = {};
     uint64_t tree_gen = 0, plan_gen = 0;
+    bool neigh_leaf_ready = false;  // the leaf level's 27 neighbour planes are built on demand
     prg_unet* warm_net = nullptr;  // learned warm start (not owned)
     // normalisation transform (centre x, y, z, scale) computed on the device;
     // the host copy is read at the next synchronisation
@@ -107,6 +108,9 @@ namespace prg {
 // Stage entry points implemented in the .hip files.
 void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n, bool on_device);
 void build_octree(prg_ctx* c);
+// the leaf level's 27-neighbour planes (kernels use the parent-neighbour rows;
+// these serve level views, buffer reads and the subdivision)
+void ensure_leaf_neighbours(prg_ctx* c);
 void ensure_tables(prg_ctx* c);
 void compute_divergence(prg_ctx* c);
 prg_status solve_all(prg_ctx* c, prg_solve_stats* stats);

@@ -86,12 +86,11 @@ __device__ __forceinline__ float shifted_value(const BaseMD& bm, float t, float 
     return res;
 }
 
-__global__ void __launch_bounds__(kBlock) k_vector_field(const uint32_t* key, const int32_t* neigh,
+__global__ void __launch_bounds__(kBlock) k_vector_field(const uint32_t* key, const int32_t* pntD,
                                                          const int32_t* pidx, const int32_t* pnum, const float* px,
                                                          const float* py, const float* pz, const float* nx,
-                                                         const float* ny, const float* nz, int64_t M, int64_t baseD,
-                                                         int64_t MD, int D, BaseMD bm, float* vfx, float* vfy,
-                                                         float* vfz) {
+                                                         const float* ny, const float* nz, int64_t baseD, int64_t MD,
+                                                         int D, BaseMD bm, float* vfx, float* vfy, float* vfz) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
     const int64_t i = baseD + t;
@@ -101,9 +100,11 @@ __global__ void __launch_bounds__(kBlock) k_vector_field(const uint32_t* key, co
     const float ocy = float((0.5 + double(compact3(k >> 1))) * double(width));
     const float ocz = float((0.5 + double(compact3(k))) * double(width));
     float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    int32_t nbr[27];
+    pnt_neighbours(pntD + 27 * (t >> 3), int(t & 7), nbr);
     for (int j = 0; j < 27; j++) {
-        const int32_t nb = neigh[int64_t(j) * M + i];
-        if (nb == -1) continue;
+        if (nbr[j] < 0) continue;
+        const int64_t nb = baseD + nbr[j];
         const int32_t p0 = pidx[nb], pn = pnum[nb];
         for (int32_t q = p0; q < p0 + pn; q++) {
             const float fx = shifted_value(bm, px[q], ocx);
@@ -440,8 +441,8 @@ void compute_divergence(prg_ctx* c) {
         StageTimer st(c, PRG_STAGE_VECTOR_FIELD);
         TimedScope ts(c->kt, "vector_field", 12.0 * double(MD) + 116.0 * double(MD) + 24.0 * double(c->N));
         k_vector_field<<<grid_for(MD, kBlock), kBlock, 0, s>>>(
-            c->key.get(), c->neigh.get(), c->pidx.get(), c->pnum.get(), c->px.get(), c->py.get(), c->pz.get(),
-            c->nx.get(), c->ny.get(), c->nz.get(), M, baseD, MD, D, bm, VX, VY, VZ);
+            c->key.get(), c->pnt.get() + 27 * c->pnt_off[D], c->pidx.get(), c->pnum.get(), c->px.get(), c->py.get(),
+            c->pz.get(), c->nx.get(), c->ny.get(), c->nz.get(), baseD, MD, D, bm, VX, VY, VZ);
         PRG_CHECK_LAUNCH();
     }
     StageTimer st(c, PRG_STAGE_DIVERGENCE);

@@ -73,16 +73,17 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const double* partial, in
 // once and serve its 8 corners.  Corner j is owned by the smallest key among
 // the (up to 8) leaf slots sharing it; own[8 t + j] marks slot t's own
 // corners and oref[8 t + j] = 8 * (owner slot) + the corner's number there.
-__global__ void __launch_bounds__(kBlock) k_corner_owner(const uint32_t* key, const int32_t* neigh, int64_t M,
-                                                         int64_t baseD, int64_t MD, int32_t* own, int32_t* oref) {
+__global__ void __launch_bounds__(kBlock) k_corner_owner(const uint32_t* key, const int32_t* pntD, int64_t baseD,
+                                                         int64_t MD, int32_t* own, int32_t* oref) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
     const int64_t i = baseD + t;
     int32_t nb[27];
     uint32_t kk[27];
+    pnt_neighbours(pntD + 27 * (t >> 3), int(t & 7), nb);
 #pragma unroll
     for (int dir = 0; dir < 27; dir++) {
-        nb[dir] = neigh[int64_t(dir) * M + i];
+        nb[dir] = nb[dir] < 0 ? -1 : int32_t(baseD + nb[dir]);
         kk[dir] = nb[dir] == -1 ? 0xffffffffu : key[nb[dir]];
     }
     int32_t ow[8], orf[8];
@@ -198,9 +199,8 @@ __global__ void __launch_bounds__(kBlock) k_corner_square(const int32_t* oref, c
 // none of whose 12 edges changes sign flags nothing and skips the ownership
 // search (its eref entries are never read: only edges of cubes with
 // triangles are, and those change sign).
-__global__ void __launch_bounds__(kBlock) k_edge_owner(const uint32_t* key, const int32_t* neigh, const float* cval,
-                                                       int64_t M, int64_t baseD, int64_t MD, int32_t* flag,
-                                                       int32_t* eref) {
+__global__ void __launch_bounds__(kBlock) k_edge_owner(const uint32_t* key, const int32_t* pntD, const float* cval,
+                                                       int64_t baseD, int64_t MD, int32_t* flag, int32_t* eref) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
     const int64_t i = baseD + t;
@@ -219,18 +219,15 @@ __global__ void __launch_bounds__(kBlock) k_edge_owner(const uint32_t* key, cons
         f4[0] = f4[1] = f4[2] = make_int4(0, 0, 0, 0);
         return;
     }
-    // the 27 neighbours and their keys (13 is the slot itself)
+    // the 27 neighbours and their keys (13 is the slot itself); corner
+    // neighbours share no edge and are not looked at
     int32_t nb[27];
     uint32_t kk[27];
+    pnt_neighbours(pntD + 27 * (t >> 3), int(t & 7), nb);
 #pragma unroll
     for (int dir = 0; dir < 27; dir++) {
         const int ddx = dir / 9 - 1, ddy = (dir / 3) % 3 - 1, ddz = dir % 3 - 1;
-        if (ddx != 0 && ddy != 0 && ddz != 0) {  // corner neighbours share no edge
-            nb[dir] = -1;
-            kk[dir] = 0xffffffffu;
-            continue;
-        }
-        nb[dir] = neigh[int64_t(dir) * M + i];
+        nb[dir] = (nb[dir] < 0 || (ddx != 0 && ddy != 0 && ddz != 0)) ? -1 : int32_t(baseD + nb[dir]);
         kk[dir] = nb[dir] == -1 ? 0xffffffffu : key[nb[dir]];
     }
     int32_t fl[12], er[12];
@@ -383,8 +380,8 @@ void extract_leaf_mesh(prg_ctx* c) {
         vtot.reserve(1);
         {
             TimedScope ts(kt, "corner_owner", 8.0 * 64.0 * double(MD));
-            k_corner_owner<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->neigh.get(), M, baseD, MD,
-                                                                   own.get(), oref.get());
+            k_corner_owner<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->pnt.get() + 27 * c->pnt_off[D],
+                                                                   baseD, MD, own.get(), oref.get());
             PRG_CHECK_LAUNCH();
         }
         scan_exclusive(own.get(), vid.get(), 8 * MD, vtot.get(), s, c->scratch, kt);
@@ -418,8 +415,8 @@ void extract_leaf_mesh(prg_ctx* c) {
         etot.reserve(1);
         {
             TimedScope ts(kt, "edge_owner", 12.0 * 40.0 * double(MD));
-            k_edge_owner<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->neigh.get(), CV, M, baseD, MD,
-                                                                 eflag.get(), eref.get());
+            k_edge_owner<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->pnt.get() + 27 * c->pnt_off[D], CV,
+                                                                 baseD, MD, eflag.get(), eref.get());
             PRG_CHECK_LAUNCH();
         }
         scan_exclusive(eflag.get(), evid.get(), 12 * MD, etot.get(), s, c->scratch, kt);

@@ -404,6 +404,16 @@ void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n
     if (!on_device) PRG_HIP(hipStreamSynchronize(s));  // the staging copies must finish before the caller reuses its arrays
 }
 
+void ensure_leaf_neighbours(prg_ctx* c) {
+    if (c->neigh_leaf_ready) return;
+    const int D = c->D;
+    TimedScope ts(c->kt, "neighbors", 128.0 * double(c->counts[D]));
+    k_neighbors<<<grid_for(c->counts[D], kBlock), kBlock, 0, c->stream>>>(
+        c->key.get(), c->parent.get(), c->child_base.get(), c->neigh.get(), c->M, c->bases[D], c->counts[D], D, D);
+    PRG_CHECK_LAUNCH();
+    c->neigh_leaf_ready = true;
+}
+
 void build_octree(prg_ctx* c) {
     static bool lut_ready = false;
     if (!lut_ready) {
@@ -584,8 +594,10 @@ void build_octree(prg_ctx* c) {
         k_point_leaf<<<grid_for(n, kBlock), kBlock, 0, s>>>(prank.get(), POS[D]->get(), n, c->point_leaf.reserve(n));
         PRG_CHECK_LAUNCH();
     }
-    // 4. neighbours, top-down
-    for (int d = 1; d <= D; d++) {
+    // 4. neighbours, top-down, down to the level above the leaves: the leaf
+    // level's planes (27 x 4 B x M_D, the largest) are built on demand
+    c->neigh_leaf_ready = (D == 0);
+    for (int d = 1; d < D; d++) {
         TimedScope ts(kt, "neighbors", 128.0 * double(counts[d]));
         k_neighbors<<<grid_for(counts[d], kBlock), kBlock, 0, s>>>(K, PA, CB, NB, M, bases[d], counts[d], D, d);
         PRG_CHECK_LAUNCH();
