@@ -73,8 +73,9 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const double* partial, in
 // once and serve its 8 corners.  Corner j is owned by the smallest key among
 // the (up to 8) leaf slots sharing it; own[8 t + j] marks slot t's own
 // corners and oref[8 t + j] = 8 * (owner slot) + the corner's number there.
+// cmask[t] has bit j set for the corners slot t owns, ccount[t] counts them.
 __global__ void __launch_bounds__(kBlock) k_corner_owner(const uint32_t* key, const int32_t* pntD, int64_t baseD,
-                                                         int64_t MD, int32_t* own, int32_t* oref) {
+                                                         int64_t MD, int32_t* cmask, int32_t* ccount, int32_t* oref) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
     const int64_t i = baseD + t;
@@ -111,19 +112,35 @@ __global__ void __launch_bounds__(kBlock) k_corner_owner(const uint32_t* key, co
         ow[j] = (owner == i) ? 1 : 0;
         orf[j] = int32_t(8 * (owner - baseD) + jo);
     }
-    int4* o4 = reinterpret_cast<int4*>(own + 8 * t);
+    int m = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) m |= ow[j] << j;
+    cmask[t] = m;
+    ccount[t] = __popc(m);
     int4* r4 = reinterpret_cast<int4*>(oref + 8 * t);
-    o4[0] = make_int4(ow[0], ow[1], ow[2], ow[3]);
-    o4[1] = make_int4(ow[4], ow[5], ow[6], ow[7]);
     r4[0] = make_int4(orf[0], orf[1], orf[2], orf[3]);
     r4[1] = make_int4(orf[4], orf[5], orf[6], orf[7]);
 }
 
 // owned corners, compacted: vlist[vid] = 8 * leaf slot + corner
-__global__ void __launch_bounds__(kBlock) k_corner_list(const int32_t* own, const int32_t* vid, int64_t n8,
+// Owned corners are numbered in (slot, corner) order: cbase is the exclusive
+// scan of the per-slot counts, and a corner's rank inside its slot counts the
+// owned corners before it.
+__device__ __forceinline__ int32_t corner_id(const int32_t* cbase, const int32_t* cmask, int32_t g) {
+    const int32_t t = g >> 3;
+    return cbase[t] + __popc(cmask[t] & ((1 << (g & 7)) - 1));
+}
+
+// owned corners, compacted: vlist[id] = 8 * leaf slot + corner
+__global__ void __launch_bounds__(kBlock) k_corner_list(const int32_t* cmask, const int32_t* cbase, int64_t MD,
                                                         int32_t* vlist) {
-    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (g < n8 && own[g]) vlist[vid[g]] = int32_t(g);
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= MD) return;
+    const int m = cmask[t];
+    int32_t r = cbase[t];
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        if ((m >> j) & 1) vlist[r++] = int32_t(8 * t + j);
 }
 
 // the corner basis table (corner_basis_entry) for depths 0..D
@@ -174,8 +191,9 @@ __global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* vlist, i
 // values of each leaf node's corners in square numbering (vertices[], main.cu:1659-1668)
 // values of each leaf node's corners in square numbering (vertices[],
 // main.cu:1659-1668): one thread per leaf slot
-__global__ void __launch_bounds__(kBlock) k_corner_square(const int32_t* oref, const int32_t* vid, const float* vval,
-                                                          int64_t MD, float* cval) {
+__global__ void __launch_bounds__(kBlock) k_corner_square(const int32_t* oref, const int32_t* cmask,
+                                                          const int32_t* cbase, const float* vval, int64_t MD,
+                                                          float* cval) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
     const int4* o4 = reinterpret_cast<const int4*>(oref + 8 * t);
@@ -185,7 +203,7 @@ __global__ void __launch_bounds__(kBlock) k_corner_square(const int32_t* oref, c
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const int y = (q >> 1) & 1, x = (q & 1) ^ y, z = q >> 2;
-        v[q] = vval[vid[o[x | (y << 1) | (z << 2)]]];
+        v[q] = vval[corner_id(cbase, cmask, o[x | (y << 1) | (z << 2)])];
     }
     float4* c4 = reinterpret_cast<float4*>(cval + 8 * t);
     c4[0] = make_float4(v[0], v[1], v[2], v[3]);
@@ -200,7 +218,8 @@ __global__ void __launch_bounds__(kBlock) k_corner_square(const int32_t* oref, c
 // search (its eref entries are never read: only edges of cubes with
 // triangles are, and those change sign).
 __global__ void __launch_bounds__(kBlock) k_edge_owner(const uint32_t* key, const int32_t* pntD, const float* cval,
-                                                       int64_t baseD, int64_t MD, int32_t* flag, int32_t* eref) {
+                                                       int64_t baseD, int64_t MD, int32_t* emask, int32_t* ecount,
+                                                       int32_t* eref) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
     const int64_t i = baseD + t;
@@ -214,9 +233,9 @@ __global__ void __launch_bounds__(kBlock) k_edge_owner(const uint32_t* key, cons
         sc[e] = cv[c_edge_vertex_h[e][0]] * cv[c_edge_vertex_h[e][1]] <= 0;
         any = any || sc[e];
     }
-    int4* f4 = reinterpret_cast<int4*>(flag + 12 * t);
     if (!any) {
-        f4[0] = f4[1] = f4[2] = make_int4(0, 0, 0, 0);
+        emask[t] = 0;
+        ecount[t] = 0;
         return;
     }
     // the 27 neighbours and their keys (13 is the slot itself); corner
@@ -257,9 +276,11 @@ __global__ void __launch_bounds__(kBlock) k_edge_owner(const uint32_t* key, cons
         er[e] = int32_t(12 * (owner - baseD) + eo);
         fl[e] = (owner == i && sc[e]) ? 1 : 0;
     }
-    f4[0] = make_int4(fl[0], fl[1], fl[2], fl[3]);
-    f4[1] = make_int4(fl[4], fl[5], fl[6], fl[7]);
-    f4[2] = make_int4(fl[8], fl[9], fl[10], fl[11]);
+    int m = 0;
+#pragma unroll
+    for (int e = 0; e < 12; e++) m |= fl[e] << e;
+    emask[t] = m;
+    ecount[t] = __popc(m);
     int4* r4 = reinterpret_cast<int4*>(eref + 12 * t);
     r4[0] = make_int4(er[0], er[1], er[2], er[3]);
     r4[1] = make_int4(er[4], er[5], er[6], er[7]);
@@ -268,24 +289,27 @@ __global__ void __launch_bounds__(kBlock) k_edge_owner(const uint32_t* key, cons
 
 // interpolatePoint (main.cu:2584-2597) for every owned edge with a sign
 // change; one thread per leaf slot
-__global__ void __launch_bounds__(kBlock) k_edge_vertex(const int32_t* flag, const int32_t* evid, const uint32_t* key,
+// Mesh vertices are numbered in (slot, edge kind) order: ebase is the
+// exclusive scan of the per-slot counts of owned sign-change edges.
+__device__ __forceinline__ int32_t edge_id(const int32_t* ebase, const int32_t* emask, int32_t r) {
+    const int32_t t = r / 12;
+    return ebase[t] + __popc(emask[t] & ((1 << (r - 12 * t)) - 1));
+}
+
+__global__ void __launch_bounds__(kBlock) k_edge_vertex(const int32_t* emask, const int32_t* ebase, const uint32_t* key,
                                                         const float* cval, int64_t baseD, int64_t MD, int D,
                                                         float* mesh_v) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
-    const int4* f4 = reinterpret_cast<const int4*>(flag + 12 * t);
-    const int4 fa = f4[0], fb = f4[1], fc = f4[2];
-    const int32_t fl[12] = {fa.x, fa.y, fa.z, fa.w, fb.x, fb.y, fb.z, fb.w, fc.x, fc.y, fc.z, fc.w};
-    bool any = false;
-#pragma unroll
-    for (int e = 0; e < 12; e++) any = any || fl[e] != 0;
-    if (!any) return;
+    const int m = emask[t];
+    if (!m) return;
+    int64_t v = ebase[t];
     float c[3];
     leaf_center(key[baseD + t], D, c);
     const float hw = 1.0f / float(1 << (D + 1));
 #pragma unroll
     for (int e = 0; e < 12; e++) {
-        if (!fl[e]) continue;
+        if (!((m >> e) & 1)) continue;
         const int q1 = c_edge_vertex_h[e][0], q2 = c_edge_vertex_h[e][1];
         float p1[3], p2[3];
         {
@@ -308,10 +332,10 @@ __global__ void __launch_bounds__(kBlock) k_edge_vertex(const int32_t* flag, con
         const float s1 = p2[dim] * pivot;
         const float s2 = p1[dim] * another;
         out[dim] = s1 + s2;
-        const int64_t v = evid[12 * t + e];
         mesh_v[3 * v + 0] = out[0];
         mesh_v[3 * v + 1] = out[1];
         mesh_v[3 * v + 2] = out[2];
+        v++;
     }
 }
 
@@ -322,13 +346,14 @@ __global__ void __launch_bounds__(kBlock) k_tri_count(const float* cval, int64_t
 }
 
 __global__ void __launch_bounds__(kBlock) k_tri_emit(const float* cval, const int32_t* toff, const int32_t* eref,
-                                                     const int32_t* evid, int64_t MD, int32_t* mesh_t) {
+                                                     const int32_t* emask, const int32_t* ebase, int64_t MD,
+                                                     int32_t* mesh_t) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
     const int cube = cube_of(cval, t);
     const int n = 3 * tri_count(cube);
     const int64_t o = 3 * int64_t(toff[t]);
-    for (int k = 0; k < n; k++) mesh_t[o + k] = evid[eref[12 * t + c_tri[cube][k]]];
+    for (int k = 0; k < n; k++) mesh_t[o + k] = edge_id(ebase, emask, eref[12 * t + c_tri[cube][k]]);
 }
 
 
@@ -366,25 +391,28 @@ void extract_leaf_mesh(prg_ctx* c) {
     const int D = c->D;
     const int64_t M = c->M, MD = c->MD(), baseD = c->bases[D];
     KernelTimer& kt = c->kt;
-    PoolBuf<int32_t> own{&c->pool[78]}, oref{&c->pool[79]}, vid{&c->pool[80]}, vtot{&c->pool[81]},
-        eflag{&c->pool[82]}, eref{&c->pool[83]}, evid{&c->pool[84]}, etot{&c->pool[85]}, ntri{&c->pool[86]},
-        toff{&c->pool[87]}, ttot{&c->pool[88]};
+    // per leaf slot: owned-corner mask and count, their scan (corner ids);
+    // owned sign-change edge mask and count, their scan (vertex ids)
+    PoolBuf<int32_t> cmask{&c->pool[78]}, oref{&c->pool[79]}, cbase{&c->pool[80]}, vtot{&c->pool[81]},
+        emask{&c->pool[82]}, eref{&c->pool[83]}, ebase{&c->pool[84]}, etot{&c->pool[85]}, ntri{&c->pool[86]},
+        toff{&c->pool[87]}, ttot{&c->pool[88]}, ccount{&c->pool[92]}, ecount{&c->pool[93]};
     PoolBuf<float> vval{&c->pool[89]};
     float* CV = c->corner_vals.reserve(8 * MD);
     int32_t V = 0, NV = 0, NT = 0;
     {
         StageTimer st(c, PRG_STAGE_MC_TOPOLOGY);
-        own.reserve(8 * MD);
+        cmask.reserve(MD);
+        ccount.reserve(MD);
         oref.reserve(8 * MD);
-        vid.reserve(8 * MD);
+        cbase.reserve(MD);
         vtot.reserve(1);
         {
             TimedScope ts(kt, "corner_owner", 8.0 * 64.0 * double(MD));
             k_corner_owner<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->pnt.get() + 27 * c->pnt_off[D],
-                                                                   baseD, MD, own.get(), oref.get());
+                                                                   baseD, MD, cmask.get(), ccount.get(), oref.get());
             PRG_CHECK_LAUNCH();
         }
-        scan_exclusive(own.get(), vid.get(), 8 * MD, vtot.get(), s, c->scratch, kt);
+        scan_exclusive(ccount.get(), cbase.get(), MD, vtot.get(), s, c->scratch, kt);
         PRG_HIP(hipMemcpyAsync(&V, vtot.get(), 4, hipMemcpyDeviceToHost, s));
         PRG_HIP(hipStreamSynchronize(s));
     }
@@ -393,7 +421,7 @@ void extract_leaf_mesh(prg_ctx* c) {
         vval.reserve(V);
         PoolBuf<int32_t> vlist{&c->pool[90]};
         vlist.reserve(V);
-        k_corner_list<<<grid_for(8 * MD, kBlock), kBlock, 0, s>>>(own.get(), vid.get(), 8 * MD, vlist.get());
+        k_corner_list<<<grid_for(MD, kBlock), kBlock, 0, s>>>(cmask.get(), cbase.get(), MD, vlist.get());
         PRG_CHECK_LAUNCH();
         PoolBuf<float> btab{&c->pool[91]};
         const int64_t nbt = int64_t(D + 1) * (int64_t(1) << D) * 2;
@@ -407,19 +435,21 @@ void extract_leaf_mesh(prg_ctx* c) {
                 reinterpret_cast<const float4*>(btab.get()), baseD, D, c->iso, vval.get());
             PRG_CHECK_LAUNCH();
         }
-        k_corner_square<<<grid_for(MD, kBlock), kBlock, 0, s>>>(oref.get(), vid.get(), vval.get(), MD, CV);
+        k_corner_square<<<grid_for(MD, kBlock), kBlock, 0, s>>>(oref.get(), cmask.get(), cbase.get(), vval.get(), MD,
+                                                                CV);
         PRG_CHECK_LAUNCH();
-        eflag.reserve(12 * MD);
+        emask.reserve(MD);
+        ecount.reserve(MD);
         eref.reserve(12 * MD);
-        evid.reserve(12 * MD);
+        ebase.reserve(MD);
         etot.reserve(1);
         {
             TimedScope ts(kt, "edge_owner", 12.0 * 40.0 * double(MD));
             k_edge_owner<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->pnt.get() + 27 * c->pnt_off[D], CV,
-                                                                 baseD, MD, eflag.get(), eref.get());
+                                                                 baseD, MD, emask.get(), ecount.get(), eref.get());
             PRG_CHECK_LAUNCH();
         }
-        scan_exclusive(eflag.get(), evid.get(), 12 * MD, etot.get(), s, c->scratch, kt);
+        scan_exclusive(ecount.get(), ebase.get(), MD, etot.get(), s, c->scratch, kt);
         ntri.reserve(MD);
         toff.reserve(MD);
         ttot.reserve(1);
@@ -431,10 +461,11 @@ void extract_leaf_mesh(prg_ctx* c) {
         PRG_HIP(hipStreamSynchronize(s));
         float* MV = c->mesh_v.reserve(size_t(3) * NV);
         int32_t* MT = c->mesh_t.reserve(size_t(3) * NT);
-        k_edge_vertex<<<grid_for(MD, kBlock), kBlock, 0, s>>>(eflag.get(), evid.get(), c->key.get(), CV, baseD, MD,
+        k_edge_vertex<<<grid_for(MD, kBlock), kBlock, 0, s>>>(emask.get(), ebase.get(), c->key.get(), CV, baseD, MD,
                                                               D, MV);
         PRG_CHECK_LAUNCH();
-        k_tri_emit<<<grid_for(MD, kBlock), kBlock, 0, s>>>(CV, toff.get(), eref.get(), evid.get(), MD, MT);
+        k_tri_emit<<<grid_for(MD, kBlock), kBlock, 0, s>>>(CV, toff.get(), eref.get(), emask.get(), ebase.get(), MD,
+                                                           MT);
         PRG_CHECK_LAUNCH();
         PRG_HIP(hipStreamSynchronize(s));
     }
