@@ -120,7 +120,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         PRG_HIP(hipEventCreate(&c->ev_a));
         PRG_HIP(hipEventCreate(&c->ev_b));
         PRG_HIP(hipHostMalloc(&c->transform_host, 16, hipHostMallocDefault));
-        PRG_HIP(hipHostMalloc(&c->cg_host, 256, hipHostMallocDefault));
+        PRG_HIP(hipHostMalloc(&c->cg_host, 4096, hipHostMallocDefault));
         PRG_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, cf.device));
         c->kt.enabled = cf.timing != 0;
         c->kt.stream = c->stream;

@@ -27,7 +27,7 @@ struct prg_ctx {
     hipStream_t stream = nullptr;
     hipStream_t stream2 = nullptr;  // independent work overlapping `stream` (CG chains, small-depth solves)
     hipStream_t stream3 = nullptr;
-    hipStream_t stream4 = nullptr;
+    hipStream_t stream4 = nullptr;  // the warm-started depth (after the network's inference)
     std::string err;
     prg::KernelTimer kt;
     prg::Scratch scratch;
@@ -61,6 +61,7 @@ struct prg_ctx {
     prg::DBuf<float> cg_vec[3][5];
     prg::DBuf<double> cg_part[3][4];  // r.r (two slots), p.q, b.b / true-residual partials
     prg::DBuf<uint8_t> cg_state[3];
+    prg::DBuf<uint8_t> cg_seg;  // the solver's per-depth segment table
     prg::DBuf<float> cg_small[3];
     prg::DBuf<double> cg_out;
     // SpMV tile plans of the large depths, valid for octree `plan_gen`
@@ -97,7 +98,7 @@ struct prg_ctx {
     hipEvent_t ev_a = nullptr, ev_b = nullptr;
     std::vector<hipEvent_t> ev_pool;  // the solver's events, reused across solves
     int num_cu = 0;                   // compute units of the device
-    void* cg_host = nullptr;          // pinned: the CG chains' state read-backs (256 bytes)
+    void* cg_host = nullptr;          // pinned, 4 KiB: CG state read-backs and the segment table's staging
 
     int64_t MD() const { return counts.empty() ? 0 : counts[D]; }
     int64_t base_of(int d) const { return bases[d]; }

@@ -60,6 +60,30 @@ struct CgState {
     int32_t pad;
 };
 
+// One depth of a chain: its rows, its workgroup ranges in the chain's SpMV and
+// vector launches, its tile plan, vectors (p double-buffered), FP64 partials
+// and CG state, and its stencil.  A chain's kernels serve all its depths in
+// one launch; a workgroup finds its depth from the ranges (wave-uniform).
+struct Seg {
+    int64_t n;
+    int32_t wg0, nwg;    // SpMV workgroups [wg0, wg0 + nwg)
+    int32_t vwg0, nvwg;  // vector-kernel workgroups [vwg0, vwg0 + nvwg)
+    const int32_t* cnt;  // tile plan
+    const int32_t* ub;
+    const int16_t* rw;
+    float *R, *P0, *P1, *Q, *XS;
+    double *rr, *pq, *bb;  // r.r (two slots of kMaxWG), p.q, b.b partials
+    CgState* st;
+    float L[27];
+};
+
+__device__ __forceinline__ int seg_of(const Seg* __restrict__ segs, int nseg, int b, bool vec) {
+    int k = 0;
+    for (int i = 1; i < nseg; i++)
+        if (b >= (vec ? segs[i].vwg0 : segs[i].wg0)) k = i;
+    return k;
+}
+
 // neighbour j (compile-time after unrolling) of slot s in the block whose
 // parent-neighbour row is `row`; -1 if absent.
 __device__ __forceinline__ int32_t pnt_neighbor(const int32_t* __restrict__ row, int s, int j) {
@@ -137,6 +161,8 @@ __device__ __forceinline__ bool publish(double blk, double* partial, CgState* st
 }
 
 constexpr int kVecThreads = 512;  // init / update / refresh / true residual
+constexpr int kMaxWG = 1024;       // workgroups of a depth in the vector kernels (its partial count)
+constexpr int kMaxSpmvWG = 2048;   // workgroups of a depth in the SpMV (its partial count)
 constexpr int kSpmvThreads = 512;
 constexpr int kTile = kSpmvThreads / 8;   // sibling blocks per SpMV tile, one row per lane
 constexpr int kTileEntries = 27 * kTile;  // parent-neighbour entries of a tile
@@ -238,13 +264,8 @@ __global__ void __launch_bounds__(kSpmvThreads) k_tile_plan(const int32_t* pnt, 
 // tile t + 2 are in flight in registers.  K = cells per thread covers the
 // depth's largest tile (8 * count <= K * kSpmvThreads).
 template <int K>
-__global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, const int32_t* __restrict__ count,
-                                                          const int32_t* __restrict__ ubase,
-                                                          const int16_t* __restrict__ rowid, const float* r,
-                                                          const float* pold, float* pnew, float* q, int32_t it,
-                                                          const double* rr_part, const double* bb_part, int32_t np,
-                                                          double* pq_part, CgState* st, float tol, int32_t fixed,
-                                                          int32_t max_iters) {
+__global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(const Seg* __restrict__ segs, int32_t nseg, int32_t it,
+                                                          float tol, int32_t fixed, int32_t max_iters) {
     static_assert(K * kSpmvThreads <= 8 * kTileEntries, "cells per thread beyond a tile's plan");
     constexpr int kRidWords = kTileEntries / 2;  // row ids as 32-bit words
     extern __shared__ float cellv[];
@@ -256,14 +277,29 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, 
     __shared__ float grid[kTile][kGridRow];
     __shared__ double lds[kSpmvThreads / 64];
     const int16_t* rid = reinterpret_cast<const int16_t*>(rid32);
+    const Seg& sg = segs[seg_of(segs, nseg, blockIdx.x, false)];
+    CgState* st = sg.st;
     if (st->done) return;
+    const int64_t n = sg.n;
+    const int32_t* __restrict__ count = sg.cnt;
+    const int32_t* __restrict__ ubase = sg.ub;
+    const int16_t* __restrict__ rowid = sg.rw;
+    const float* r = sg.R;
+    float* pnew = ((it - 1) & 1) ? sg.P1 : sg.P0;
+    const float* pold = (it & 1) ? sg.P1 : sg.P0;
+    float* q = sg.Q;
+    const double* rr_part = sg.rr + ((it - 1) & 1) * kMaxWG;
+    const double* bb_part = sg.bb;
+    const int32_t np = sg.nvwg;
+    double* pq_part = sg.pq;
+    const int bid = int(blockIdx.x) - sg.wg0, nblocks = sg.nwg;
     const int k1 = it - 1;
     const double rr = sum_partials<kSpmvThreads>(rr_part, np, lds);
     const double bb = (it == 1) ? sum_partials<kSpmvThreads>(bb_part, np, lds) : st->bb;
     const double tol2 = double(tol) * double(tol) * bb;
     const bool stop = (bb <= 0.0) || (k1 >= max_iters) || (rr == 0.0) || (fixed > 0 ? (k1 >= fixed) : (rr <= tol2));
     const float bf = (it == 1) ? 0.0f : float(rr / st->rr_hist[(k1 - 1) & 1]);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
         if (it == 1) st->bb = bb;
         st->rr_hist[k1 & 1] = rr;
         st->rr = rr;
@@ -278,8 +314,8 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, 
     const int g = tid >> 3, s = tid & 7;
     const int64_t nblk = n >> 3;
     const int64_t ntiles = (nblk + kTile - 1) / kTile;
-    const int64_t per = (ntiles + gridDim.x - 1) / gridDim.x;
-    const int64_t tbeg = int64_t(blockIdx.x) * per, tend = tbeg + per < ntiles ? tbeg + per : ntiles;
+    const int64_t per = (ntiles + nblocks - 1) / nblocks;
+    const int64_t tbeg = int64_t(bid) * per, tend = tbeg + per < ntiles ? tbeg + per : ntiles;
     // this lane's row sits at (1 + sx, 1 + sy, 1 + sz) of its block's grid
     const int c0 = 16 * ((s >> 2) & 1) + 4 * ((s >> 1) & 1) + (s & 1);
     // plan of one tile: cell count, row-id words, block base of each of this thread's cells
@@ -347,7 +383,7 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, 
             float out = 0.0f;
 #pragma unroll
             for (int j = 0; j < 27; j++) {
-                const float prod = S.L[j] * gc[16 * (j / 9) + 4 * ((j / 3) % 3) + (j % 3)];
+                const float prod = sg.L[j] * gc[16 * (j / 9) + 4 * ((j / 3) % 3) + (j % 3)];
                 out = out + prod;
             }
             const int64_t t = 8 * blk + s;
@@ -359,7 +395,7 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, 
         __syncthreads();
     }
     const double sum = block_allsum<kSpmvThreads>(pq, lds);
-    if (tid == 0) pq_part[blockIdx.x] = sum;
+    if (tid == 0) pq_part[bid] = sum;
 }
 
 // CG iteration `it`, second kernel: alpha = rr_{it-1} / p.q (every workgroup
@@ -367,14 +403,23 @@ __global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(int64_t n, Stencil S, 
 // solve), x += alpha p, r -= alpha q on 16-byte-aligned vectors four rows
 // per lane, and this workgroup's partial of r.r (slot it & 1) unless r is
 // recomputed next.
-__global__ void __launch_bounds__(kVecThreads) k_cg_update(int64_t n, const float* p, const float* q, float* x, float* r,
-                                                           int32_t it, const double* pq_part, int32_t npq,
-                                                           double* rr_part, int32_t refresh_next, CgState* st) {
+__global__ void __launch_bounds__(kVecThreads) k_cg_update(const Seg* __restrict__ segs, int32_t nseg, int32_t it,
+                                                           int32_t refresh) {
     __shared__ double lds[kVecThreads / 64];
+    const Seg& sg = segs[seg_of(segs, nseg, blockIdx.x, true)];
+    CgState* st = sg.st;
     if (st->done) return;
-    const double PQ = sum_partials<kVecThreads>(pq_part, npq, lds);
+    const int64_t n = sg.n;
+    const float* p = ((it - 1) & 1) ? sg.P1 : sg.P0;
+    const float* q = sg.Q;
+    float* x = sg.XS;
+    float* r = sg.R;
+    double* rr_part = sg.rr + (it & 1) * kMaxWG;
+    const bool refresh_next = (it % refresh) == 0;
+    const int bid = int(blockIdx.x) - sg.vwg0, nblocks = sg.nvwg;
+    const double PQ = sum_partials<kVecThreads>(sg.pq, sg.nwg, lds);
     if (!(PQ > 0.0)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (bid == 0 && threadIdx.x == 0) {
             st->done = 1;
             st->iters = it - 1;
         }
@@ -387,7 +432,7 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(int64_t n, const floa
     const float4* q4 = reinterpret_cast<const float4*>(q);
     float4* x4 = reinterpret_cast<float4*>(x);
     float4* r4 = reinterpret_cast<float4*>(r);
-    for (int64_t t = int64_t(blockIdx.x) * kVecThreads + threadIdx.x; t < n4; t += int64_t(gridDim.x) * kVecThreads) {
+    for (int64_t t = int64_t(bid) * kVecThreads + threadIdx.x; t < n4; t += int64_t(nblocks) * kVecThreads) {
         const float4 pv = p4[t], qv = q4[t];
         float4 xv = x4[t], rv = r4[t];
         xv.x = xv.x + a * pv.x;
@@ -405,8 +450,7 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(int64_t n, const floa
         rr += double(rv.z) * double(rv.z);
         rr += double(rv.w) * double(rv.w);
     }
-    for (int64_t t = 4 * n4 + int64_t(blockIdx.x) * kVecThreads + threadIdx.x; t < n;
-         t += int64_t(gridDim.x) * kVecThreads) {
+    for (int64_t t = 4 * n4 + int64_t(bid) * kVecThreads + threadIdx.x; t < n; t += int64_t(nblocks) * kVecThreads) {
         const float t1 = a * p[t];
         x[t] = x[t] + t1;
         const float t2 = a * q[t];
@@ -416,9 +460,9 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_update(int64_t n, const floa
     }
     if (!refresh_next) {
         const double sum = block_allsum<kVecThreads>(rr, lds);
-        if (threadIdx.x == 0) rr_part[blockIdx.x] = sum;
+        if (threadIdx.x == 0) rr_part[bid] = sum;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->iters = it;
+    if (bid == 0 && threadIdx.x == 0) st->iters = it;
 }
 
 // r = b - A x (every `refresh` iterations) and its r.r partial (slot it & 1)
@@ -576,42 +620,39 @@ __global__ void __launch_bounds__(kSmallBlock) k_cg_small(SmallJobs jobs, const 
 
 namespace {
 
-// One chain of large depths solved one after another on one stream.  The
-// finest depth forms one chain and the other large depths the second, so the
-// two overlap (the per-depth systems are independent).
+// A chain: a set of large depths solved together on one stream.  One SpMV and
+// one update launch per iteration serve all of them (the per-depth systems are
+// independent; each depth has its own scalars, partials and stop flag).
 struct Chain {
     hipStream_t s = nullptr;
     std::vector<int> depths;
-    size_t next = 0;
-    int d = -1;       // depth in progress (-1: none)
-    int issued = 0;   // iterations queued for depth d
+    int seg0 = 0;                // its first entry in the solve's segment table
+    int32_t nwg = 0, nvwg = 0;   // workgroups of its SpMV and vector launches
+    int inst = 0;                // SpMV instance (cells per thread)
+    size_t lds = 0;              // SpMV dynamic LDS
+    int issued = 0;              // iterations queued
     int batch = 24;
     bool waiting = false;
-    float *R = nullptr, *P[2] = {nullptr, nullptr}, *Q = nullptr, *XS = nullptr;
-    double *rr = nullptr, *pq = nullptr, *bb = nullptr;  // partials (r.r: two slots of kMaxWG)
-    CgState* st = nullptr;
-    CgState* hst = nullptr;  // pinned host copy of st
-    hipEvent_t ev = nullptr;  // marks the state copy
+    CgState* st = nullptr;       // device states, one per depth
+    CgState* hst = nullptr;      // pinned host copies
+    hipEvent_t ev = nullptr;     // marks the state copy
     std::vector<std::pair<int, int>> launched;
 };
 
-constexpr int kMaxWG = 1024;       // workgroups of the vector kernels (their partial count)
-constexpr int kMaxSpmvWG = 2048;   // workgroups of the SpMV (its partial count)
+constexpr int kMaxSegs = 12;         // depths per solve above kSmallRows rows (D <= 10 has at most 10)
+constexpr size_t kHostStates = 1024;  // pinned read-back area: states first, then the segment table
 
-// SpMV instance: cells per thread covering the depth's largest tile.  The
-// launch is one round of workgroups, two per CU, each pipelining a run of
-// tiles (four per CU measured the same at depth 10).
+// SpMV instance: cells per thread covering the largest tile of the chain's
+// depths.  A chain's launch is one round of workgroups, two per CU, shared
+// among its depths by tile count, each pipelining a run of tiles (four per CU
+// measured the same at depth 10).
 int spmv_instance(int kcells) { return kcells <= 2 ? 0 : kcells <= 4 ? 1 : kcells <= 8 ? 2 : 3; }
 
 constexpr int kSpmvWGPerCU = 2;
 
-void launch_spmv(int inst, int32_t ns, size_t lds, hipStream_t s, int64_t n, Stencil S, const int32_t* cnt,
-                 const int32_t* ub, const int16_t* rw, const float* r, const float* pold, float* pnew, float* q,
-                 int32_t it, const double* rr_part, const double* bb_part, int32_t np, double* pq_part, CgState* st,
+void launch_spmv(int inst, int32_t nwg, size_t lds, hipStream_t s, const Seg* segs, int32_t nseg, int32_t it,
                  float tol, int32_t fixed, int32_t max_iters) {
-#define PRG_SPMV(KC)                                                                                            \
-    k_cg_spmv<KC><<<ns, kSpmvThreads, lds, s>>>(n, S, cnt, ub, rw, r, pold, pnew, q, it, rr_part, bb_part, np, \
-                                                pq_part, st, tol, fixed, max_iters)
+#define PRG_SPMV(KC) k_cg_spmv<KC><<<nwg, kSpmvThreads, lds, s>>>(segs, nseg, it, tol, fixed, max_iters)
     switch (inst) {
         case 0: PRG_SPMV(2); break;
         case 1: PRG_SPMV(4); break;
@@ -696,24 +737,18 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             jobs.pnt_row[jobs.count] = d == 0 ? -1 : c->pnt_off[d];
             jobs.count++;
         }
-    // chain 0: the finest depth; chain 1: the next one; chain 2: the other
-    // large depths, coarse to fine
+    // chain 0: the finest depth; chain 1: every other large depth, together;
+    // chain 2: a warm-started depth (other than the finest), after the
+    // network's inference on its own stream
     constexpr int kChains = 3;
     Chain ch[kChains];
-    auto chain_of = [&](int d) { return d == D ? 0 : (d == D - 1 ? 1 : 2); };
+    const bool warm_large = warm_depth >= 0 && c->counts[warm_depth] > kSmallRows;
     for (int d = 0; d <= D; d++)
-        if (c->counts[d] > kSmallRows) ch[chain_of(d)].depths.push_back(d);
-    // the warm-started depth goes last in its chain, so the chain's other
-    // depths overlap the network's inference
-    if (warm_depth >= 0 && c->counts[warm_depth] > kSmallRows) {
-        auto& v = ch[chain_of(warm_depth)].depths;
-        v.erase(std::find(v.begin(), v.end(), warm_depth));
-        v.push_back(warm_depth);
-    }
+        if (c->counts[d] > kSmallRows) ch[d == D ? 0 : (warm_large && d == warm_depth ? 2 : 1)].depths.push_back(d);
     ch[0].s = s;
     ch[1].s = serial ? s : c->stream2;
-    ch[2].s = serial ? s : c->stream3;
-    hipStream_t small_s = serial ? s : c->stream4;
+    ch[2].s = serial ? s : c->stream4;
+    hipStream_t small_s = serial ? s : c->stream3;
 
     // events come from the context's pool (created once, reused by every solve)
     size_t ev_used = 0;
@@ -726,7 +761,6 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
         return c->ev_pool[ev_used++];
     };
     std::vector<hipEvent_t> dbeg(D + 1, nullptr), dend(D + 1, nullptr);
-    CgState* hst = nullptr;
     std::vector<double> ms(D + 1, 0.0);
     std::vector<double> ov(size_t(4) * (D + 1), 0.0);
     prg_status result = PRG_OK;
@@ -738,22 +772,99 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     auto vec_grid = [&](int64_t n) {
         return int32_t(std::min<int64_t>(kMaxWG, (n + 4 * kVecThreads - 1) / (4 * kVecThreads)));
     };
+
+    // segment table: chain 0's depth, then chain 1's depths
+    int nsegs = 0;
+    for (auto& h : ch) nsegs += int(h.depths.size());
+    if (nsegs > kMaxSegs) throw Error{PRG_E_ARG, "too many large depths"};
+    static_assert(kMaxSegs * sizeof(CgState) <= kHostStates, "chain states exceed their pinned area");
+    static_assert(kHostStates + kMaxSegs * sizeof(Seg) <= 4096, "segment table exceeds the pinned staging area");
+    CgState* hst = static_cast<CgState*>(c->cg_host);
+    Seg* hseg = reinterpret_cast<Seg*>(static_cast<uint8_t*>(c->cg_host) + kHostStates);
+    Seg* dseg = reinterpret_cast<Seg*>(c->cg_seg.reserve(std::max<size_t>(1, size_t(nsegs)) * sizeof(Seg)));
+    {
+        int si = 0;
+        for (int k = 0; k < kChains; k++) {
+            Chain& h = ch[k];
+            h.seg0 = si;
+            const int cnt = int(h.depths.size());
+            if (!cnt) continue;
+            // vectors: one 64-float-aligned segment per depth
+            std::vector<int64_t> voff(cnt + 1, 0);
+            int64_t tiles_all = 0;
+            for (int i = 0; i < cnt; i++) {
+                voff[i + 1] = voff[i] + ((c->counts[h.depths[i]] + 63) / 64) * 64;
+                tiles_all += tiles_of(c->counts[h.depths[i]]);
+            }
+            float* V[5];
+            for (int v = 0; v < 5; v++) V[v] = c->cg_vec[k][v].reserve(size_t(voff[cnt]));
+            double* rr = c->cg_part[k][0].reserve(size_t(cnt) * 2 * kMaxWG);
+            double* pq = c->cg_part[k][1].reserve(size_t(cnt) * kMaxSpmvWG);
+            double* bb = c->cg_part[k][2].reserve(size_t(cnt) * std::max(kMaxWG, kMaxSpmvWG));
+            h.st = reinterpret_cast<CgState*>(c->cg_state[k].reserve(size_t(cnt) * sizeof(CgState)));
+            h.hst = hst + si;
+            h.ev = new_event();
+            // the chain's SpMV workgroups: one round (two per CU), shared by tile count
+            const int64_t budget = int64_t(kSpmvWGPerCU) * std::max(1, c->num_cu);
+            int32_t wg = 0, vwg = 0, kc = 1;
+            size_t lds = 0;
+            for (int i = 0; i < cnt; i++) {
+                const int d = h.depths[i];
+                const int64_t n = c->counts[d], nt = tiles_of(n);
+                Seg& g = hseg[si + i];
+                g.n = n;
+                g.nwg = int32_t(std::min<int64_t>(
+                    {int64_t(kMaxSpmvWG), nt, std::max<int64_t>(1, (budget * nt + tiles_all - 1) / tiles_all)}));
+                g.wg0 = wg;
+                wg += g.nwg;
+                g.nvwg = vec_grid(n);
+                g.vwg0 = vwg;
+                vwg += g.nvwg;
+                const int64_t t0 = c->plan_tile_off[d];
+                g.cnt = c->plan_count.get() + t0;
+                g.ub = c->plan_ubase.get() + t0 * kTileEntries;
+                g.rw = c->plan_rowid.get() + t0 * kTileEntries;
+                g.R = V[0] + voff[i];
+                g.P0 = V[1] + voff[i];
+                g.P1 = V[2] + voff[i];
+                g.Q = V[3] + voff[i];
+                g.XS = V[4] + voff[i];
+                g.rr = rr + size_t(i) * 2 * kMaxWG;
+                g.pq = pq + size_t(i) * kMaxSpmvWG;
+                g.bb = bb + size_t(i) * std::max(kMaxWG, kMaxSpmvWG);
+                g.st = h.st + i;
+                const Stencil S = stencil_of(d);
+                for (int j = 0; j < 27; j++) g.L[j] = S.L[j];
+                const int mc = std::max(1, c->plan_maxcount[d]);
+                kc = std::max(kc, (8 * mc + kSpmvThreads - 1) / kSpmvThreads);
+                lds = std::max(lds, size_t(8) * 4 * mc);
+            }
+            h.nwg = wg;
+            h.nvwg = vwg;
+            h.inst = spmv_instance(kc);
+            h.lds = lds;
+            si += cnt;
+        }
+    }
     try {
-        static_assert(kChains * sizeof(CgState) <= 256, "chain states exceed the pinned read-back buffer");
-        hst = static_cast<CgState*>(c->cg_host);
-        // The other streams fork from the context stream here.  The network
-        // runs ahead of the work that solves its depth, on that work's
-        // stream, overlapping the other chains.
+        // the segment table and zeroed states, then the other streams fork
+        // from the context stream
+        if (nsegs) PRG_HIP(hipMemcpyAsync(dseg, hseg, size_t(nsegs) * sizeof(Seg), hipMemcpyHostToDevice, s));
+        for (auto& h : ch)
+            if (!h.depths.empty()) PRG_HIP(hipMemsetAsync(h.st, 0, h.depths.size() * sizeof(CgState), s));
         hipEvent_t es0 = new_event(), es1 = new_event();
         PRG_HIP(hipEventRecord(es0, s));
         for (hipStream_t x : {c->stream2, c->stream3, c->stream4}) PRG_HIP(hipStreamWaitEvent(x, es0, 0));
-        // The network runs on the small depths' stream, ahead of their solve;
-        // the warm-started depth's chain waits for it (event ew).
+        // The network runs ahead of the solve of its depth: on that depth's
+        // chain stream when the depth is large (the finest depth's chain or
+        // chain 2), else on the small depths' stream.  A chain on the same
+        // stream simply starts after it.
         hipEvent_t ew = nullptr;
         if (warm_depth >= 0) {
-            unet_warm_start(c, warm_depth, small_s);
+            hipStream_t ws = !warm_large ? small_s : (warm_depth == D ? ch[0].s : ch[2].s);
+            unet_warm_start(c, warm_depth, ws);
             ew = new_event();
-            PRG_HIP(hipEventRecord(ew, small_s));
+            PRG_HIP(hipEventRecord(ew, ws));
         }
         if (jobs.count) {
             const int64_t small_end = jobs.base[jobs.count - 1] + jobs.n[jobs.count - 1];
@@ -768,119 +879,91 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             PRG_CHECK_LAUNCH();
             PRG_HIP(hipEventRecord(es1, small_s));
         }
-        for (int k = 0; k < kChains; k++) {
-            Chain& h = ch[k];
-            int64_t nmax = 0;
-            for (int d : h.depths) nmax = std::max(nmax, c->counts[d]);
-            if (!nmax) continue;
-            h.R = c->cg_vec[k][0].reserve(nmax);
-            h.P[0] = c->cg_vec[k][1].reserve(nmax);
-            h.P[1] = c->cg_vec[k][2].reserve(nmax);
-            h.Q = c->cg_vec[k][3].reserve(nmax);
-            h.XS = c->cg_vec[k][4].reserve(nmax);
-            h.rr = c->cg_part[k][0].reserve(2 * kMaxWG);
-            h.pq = c->cg_part[k][1].reserve(kMaxSpmvWG);
-            h.bb = c->cg_part[k][2].reserve(std::max(kMaxWG, kMaxSpmvWG));
-            h.st = reinterpret_cast<CgState*>(c->cg_state[k].reserve(sizeof(CgState)));
-            h.hst = hst + k;
-            h.ev = new_event();
-            PRG_HIP(hipMemsetAsync(h.st, 0, sizeof(CgState), h.s));
-        }
 
-        auto begin_depth = [&](Chain& h) {
-            const int d = h.depths[h.next++];
-            h.d = d;
+        auto begin_chain = [&](Chain& h) {
+            if (ew && std::find(h.depths.begin(), h.depths.end(), warm_depth) != h.depths.end())
+                PRG_HIP(hipStreamWaitEvent(h.s, ew, 0));
+            for (size_t i = 0; i < h.depths.size(); i++) {
+                const int d = h.depths[i];
+                const Seg& g = hseg[h.seg0 + i];
+                const int64_t n = c->counts[d], base = c->bases[d];
+                dbeg[d] = new_event();
+                dend[d] = new_event();
+                PRG_HIP(hipEventRecord(dbeg[d], h.s));
+                TimedScope ts(c->kt, "cg_init", kSpmvRowBytes * double(n));
+                k_cg_init<<<g.nvwg, kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], n, stencil_of(d),
+                                                          c->div.get() + base, X + base, d == warm_depth ? 0 : 1,
+                                                          g.XS, g.R, g.P0, g.rr, g.bb, g.st);
+                PRG_CHECK_LAUNCH();
+            }
             h.issued = 0;
             h.batch = 24;
-            const int64_t n = c->counts[d], base = c->bases[d];
-            dbeg[d] = new_event();
-            dend[d] = new_event();
-            if (d == warm_depth && ew) PRG_HIP(hipStreamWaitEvent(h.s, ew, 0));
-            PRG_HIP(hipEventRecord(dbeg[d], h.s));
-            TimedScope ts(c->kt, "cg_init", kSpmvRowBytes * double(n));
-            k_cg_init<<<vec_grid(n), kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], n, stencil_of(d),
-                                                            c->div.get() + base, X + base, d == warm_depth ? 0 : 1,
-                                                            h.XS, h.R, h.P[0], h.rr, h.bb, h.st);
-            PRG_CHECK_LAUNCH();
         };
         auto issue_batch = [&](Chain& h) {
-            const int d = h.d;
-            const int64_t n = c->counts[d];
-            const Stencil S = stencil_of(d);
-            const int32_t nv = vec_grid(n);
-            const int64_t ntiles = tiles_of(n);
-            const size_t lds = size_t(8) * 4 * std::max(1, c->plan_maxcount[d]);
-            const int inst = spmv_instance((8 * std::max(1, c->plan_maxcount[d]) + kSpmvThreads - 1) / kSpmvThreads);
-            const int32_t ns =
-                int32_t(std::min<int64_t>({int64_t(kMaxSpmvWG), int64_t(kSpmvWGPerCU) * std::max(1, c->num_cu), ntiles}));
-            const int64_t t0 = c->plan_tile_off[d];
-            const int32_t* cnt = c->plan_count.get() + t0;
-            const int32_t* ub = c->plan_ubase.get() + t0 * kTileEntries;
-            const int16_t* rw = c->plan_rowid.get() + t0 * kTileEntries;
-            const int32_t* pnt = c->pnt.get() + 27 * c->pnt_off[d];
-            const float* B = c->div.get() + c->bases[d];
-            const char* spmv_name = d == D ? "stencil_spmv_dot_finest" : "stencil_spmv_dot";
+            const int32_t cnt = int32_t(h.depths.size());
+            int64_t rows = 0;
+            for (int d : h.depths) rows += c->counts[d];
+            const char* spmv_name = h.depths.back() == D ? "stencil_spmv_dot_finest" : "stencil_spmv_dot";
             h.launched.clear();
             for (int k = 0; k < h.batch; k++) {
                 const int it = h.issued + k + 1;  // iteration number, 1-based
-                float* pnew = h.P[(it - 1) & 1];
-                const float* pold = h.P[it & 1];
-                double* rr_prev = h.rr + ((it - 1) & 1) * kMaxWG;
-                double* rr_next = h.rr + (it & 1) * kMaxWG;
                 {
                     // algorithmic bytes of one SpMV row in the reference's representation:
                     // 27 int32 neighbour indices, p_i read, (Ap)_i written (116 B)
-                    TimedScope ts(c->kt, spmv_name, kSpmvRowBytes * double(n));
+                    TimedScope ts(c->kt, spmv_name, kSpmvRowBytes * double(rows));
                     h.launched.emplace_back(ts.idx, it);
-                    launch_spmv(inst, ns, lds, h.s, n, S, cnt, ub, rw, h.R, pold, pnew, h.Q, it, rr_prev, h.bb, nv,
-                                h.pq, h.st, tol, fixed, max_iters);
+                    launch_spmv(h.inst, h.nwg, h.lds, h.s, dseg + h.seg0, cnt, it, tol, fixed, max_iters);
                 }
-                const bool do_refresh = (it % refresh) == 0;
                 {
-                    TimedScope ts(c->kt, "cg_update", 24.0 * double(n));
+                    TimedScope ts(c->kt, "cg_update", 24.0 * double(rows));
                     h.launched.emplace_back(ts.idx, it);
-                    k_cg_update<<<nv, kVecThreads, 0, h.s>>>(n, pnew, h.Q, h.XS, h.R, it, h.pq, ns, rr_next,
-                                                             do_refresh ? 1 : 0, h.st);
+                    k_cg_update<<<h.nvwg, kVecThreads, 0, h.s>>>(dseg + h.seg0, cnt, it, refresh);
                     PRG_CHECK_LAUNCH();
                 }
-                if (do_refresh) {
-                    TimedScope ts(c->kt, "cg_refresh", kSpmvRowBytes * double(n));
-                    h.launched.emplace_back(ts.idx, it);
-                    k_cg_refresh<<<nv, kVecThreads, 0, h.s>>>(pnt, n, S, B, h.XS, h.R, rr_next, h.st);
-                    PRG_CHECK_LAUNCH();
+                if ((it % refresh) == 0) {
+                    for (int i = 0; i < cnt; i++) {
+                        const int d = h.depths[i];
+                        const Seg& g = hseg[h.seg0 + i];
+                        TimedScope ts(c->kt, "cg_refresh", kSpmvRowBytes * double(g.n));
+                        h.launched.emplace_back(ts.idx, it);
+                        k_cg_refresh<<<g.nvwg, kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], g.n,
+                                                                     stencil_of(d), c->div.get() + c->bases[d], g.XS,
+                                                                     g.R, g.rr + (it & 1) * kMaxWG, g.st);
+                        PRG_CHECK_LAUNCH();
+                    }
                 }
             }
             h.issued += h.batch;
-            PRG_HIP(hipMemcpyAsync(h.hst, h.st, sizeof(CgState), hipMemcpyDeviceToHost, h.s));
+            PRG_HIP(hipMemcpyAsync(h.hst, h.st, size_t(cnt) * sizeof(CgState), hipMemcpyDeviceToHost, h.s));
             PRG_HIP(hipEventRecord(h.ev, h.s));
             h.waiting = true;
         };
-        auto end_depth = [&](Chain& h) {
-            const int d = h.d;
-            const int64_t n = c->counts[d], base = c->bases[d];
-            const int32_t nv = vec_grid(n);
-            {
-                TimedScope ts(c->kt, "cg_true_residual", kSpmvRowBytes * double(n));
-                k_true_residual<<<nv, kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], n, stencil_of(d),
-                                                             c->div.get() + base, h.XS, h.bb, h.rr, nv, kMaxWG, h.st,
-                                                             outv + 4 * d);
-                PRG_CHECK_LAUNCH();
+        auto end_chain = [&](Chain& h) {
+            for (size_t i = 0; i < h.depths.size(); i++) {
+                const int d = h.depths[i];
+                const Seg& g = hseg[h.seg0 + i];
+                const int64_t n = c->counts[d], base = c->bases[d];
+                {
+                    TimedScope ts(c->kt, "cg_true_residual", kSpmvRowBytes * double(n));
+                    k_true_residual<<<g.nvwg, kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], n,
+                                                                    stencil_of(d), c->div.get() + base, g.XS, g.bb,
+                                                                    g.rr, g.nvwg, kMaxWG, g.st, outv + 4 * d);
+                    PRG_CHECK_LAUNCH();
+                }
+                PRG_HIP(hipMemcpyAsync(X + base, g.XS, size_t(n) * 4, hipMemcpyDeviceToDevice, h.s));
+                PRG_HIP(hipEventRecord(dend[d], h.s));
             }
-            PRG_HIP(hipMemcpyAsync(X + base, h.XS, size_t(n) * 4, hipMemcpyDeviceToDevice, h.s));
-            PRG_HIP(hipEventRecord(dend[d], h.s));
-            h.d = -1;
         };
 
         // Each chain: init, batches of iterations with a state read-back after
-        // each, true residual; the host polls the chains so none waits for
-        // another.  Serial mode runs the chains one after another, the finest
-        // depth's last.
+        // each, true residuals; the host polls the chains so none waits for
+        // another.  Serial mode runs chain 2, then 1, the finest depth last.
         std::vector<int> serial_order;
         for (int k = kChains - 1; k >= 0; k--)
             if (!ch[k].depths.empty()) serial_order.push_back(k);
         size_t serial_next = 0;
         auto start_chain = [&](int k) {
-            begin_depth(ch[k]);
+            begin_chain(ch[k]);
             issue_batch(ch[k]);
         };
         if (serial) {
@@ -899,16 +982,17 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                 if (q == hipErrorNotReady) continue;
                 PRG_HIP(q);
                 h.waiting = false;
+                bool all_done = true;
+                int32_t last = 0;
+                for (size_t i = 0; i < h.depths.size(); i++) {
+                    all_done = all_done && (h.hst[i].done || h.issued >= max_iters);
+                    last = std::max(last, h.hst[i].iters);
+                }
                 for (const auto& l : h.launched)
-                    if (l.second > h.hst->iters) c->kt.relabel(l.first, "cg_idle");
-                if (h.hst->done || h.issued >= max_iters) {
-                    end_depth(h);
-                    if (h.next < h.depths.size()) {
-                        begin_depth(h);
-                        issue_batch(h);
-                    } else if (serial && serial_next < serial_order.size()) {
-                        start_chain(serial_order[serial_next++]);
-                    }
+                    if (l.second > last && all_done) c->kt.relabel(l.first, "cg_idle");
+                if (all_done) {
+                    end_chain(h);
+                    if (serial && serial_next < serial_order.size()) start_chain(serial_order[serial_next++]);
                 } else {
                     h.batch = 6;
                     issue_batch(h);
