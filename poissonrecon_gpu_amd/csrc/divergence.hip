@@ -86,30 +86,58 @@ __device__ __forceinline__ float shifted_value(const BaseMD& bm, float t, float 
     return res;
 }
 
-__global__ void __launch_bounds__(kBlock) k_vector_field(const uint32_t* key, const int32_t* pntD,
-                                                         const int32_t* pidx, const int32_t* pnum, const float* px,
-                                                         const float* py, const float* pz, const float* nx,
+// centre coordinate of the depth-D cell with offset o along one axis, as the
+// reference computes it (BinaryNode::CenterAndWidth through double)
+__device__ __forceinline__ float cell_centre(int64_t o, float width) {
+    return float((0.5 + double(o)) * double(width));
+}
+
+// Per sorted point q and axis a, the weights of the three cells around its
+// leaf along a: W[9 q + 3 a + k] = value(BaseFunctionMaxDepth.shift(p_a), c)
+// with c the centre of the leaf's offset + k - 1.  Every slot that gathers q
+// (the leaf's 27 neighbours) needs one of them per axis; computing them once
+// per point instead of once per (slot, point) pair does the same float
+// operations.
+__global__ void __launch_bounds__(kBlock) k_point_weights(const uint32_t* key, const int32_t* point_leaf,
+                                                          const float* px, const float* py, const float* pz,
+                                                          int64_t n, int64_t baseD, int D, BaseMD bm, float* W) {
+    const int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (q >= n) return;
+    const uint32_t k = key[baseD + point_leaf[q]];
+    const float width = float(1.0 / double(1 << D));
+    const int64_t o[3] = {compact3(k >> 2), compact3(k >> 1), compact3(k)};
+    const float p[3] = {px[q], py[q], pz[q]};
+    float w[9];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int kk = 0; kk < 3; kk++) w[3 * a + kk] = shifted_value(bm, p[a], cell_centre(o[a] + kk - 1, width));
+#pragma unroll
+    for (int i = 0; i < 9; i++) W[9 * q + i] = w[i];
+}
+
+// V_t = sum over t's neighbours j (in order) and their points (in order) of
+// w * n_q, w = wx * wy * wz (computeVectorField, main.cu:913-964).  Slot t
+// sits at offset -delta_j from neighbour j's leaf, so its weights are the
+// point's W entries k = 1 - delta_j.
+__global__ void __launch_bounds__(kBlock) k_vector_field(const int32_t* pntD, const int32_t* pidx,
+                                                         const int32_t* pnum, const float* W, const float* nx,
                                                          const float* ny, const float* nz, int64_t baseD, int64_t MD,
-                                                         int D, BaseMD bm, float* vfx, float* vfy, float* vfz) {
+                                                         float* vfx, float* vfy, float* vfz) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
-    const int64_t i = baseD + t;
-    const uint32_t k = key[i];
-    const float width = float(1.0 / double(1 << D));
-    const float ocx = float((0.5 + double(compact3(k >> 2))) * double(width));
-    const float ocy = float((0.5 + double(compact3(k >> 1))) * double(width));
-    const float ocz = float((0.5 + double(compact3(k))) * double(width));
     float vx = 0.0f, vy = 0.0f, vz = 0.0f;
     int32_t nbr[27];
     pnt_neighbours(pntD + 27 * (t >> 3), int(t & 7), nbr);
+#pragma unroll
     for (int j = 0; j < 27; j++) {
         if (nbr[j] < 0) continue;
         const int64_t nb = baseD + nbr[j];
         const int32_t p0 = pidx[nb], pn = pnum[nb];
+        const int kx = 2 - j / 9, ky = 2 - (j / 3) % 3, kz = 2 - j % 3;  // 1 - delta
         for (int32_t q = p0; q < p0 + pn; q++) {
-            const float fx = shifted_value(bm, px[q], ocx);
-            const float fy = shifted_value(bm, py[q], ocy);
-            const float fz = shifted_value(bm, pz[q], ocz);
+            const float* wq = W + 9 * int64_t(q);
+            const float fx = wq[kx], fy = wq[3 + ky], fz = wq[6 + kz];
             float w = fx * fy;
             w = w * fz;
             vx = vx + w * nx[q];
@@ -439,10 +467,19 @@ void compute_divergence(prg_ctx* c) {
     float* VZ = c->vfz.reserve(MD);
     {
         StageTimer st(c, PRG_STAGE_VECTOR_FIELD);
+        PoolBuf<float> W{&c->pool[75]};
+        W.reserve(size_t(9) * c->N);
+        {
+            TimedScope ts(c->kt, "point_weights", 52.0 * double(c->N));
+            k_point_weights<<<grid_for(c->N, kBlock), kBlock, 0, s>>>(c->key.get(), c->point_leaf.get(), c->px.get(),
+                                                                      c->py.get(), c->pz.get(), c->N, baseD, D, bm,
+                                                                      W.get());
+            PRG_CHECK_LAUNCH();
+        }
         TimedScope ts(c->kt, "vector_field", 12.0 * double(MD) + 116.0 * double(MD) + 24.0 * double(c->N));
-        k_vector_field<<<grid_for(MD, kBlock), kBlock, 0, s>>>(
-            c->key.get(), c->pnt.get() + 27 * c->pnt_off[D], c->pidx.get(), c->pnum.get(), c->px.get(), c->py.get(),
-            c->pz.get(), c->nx.get(), c->ny.get(), c->nz.get(), baseD, MD, D, bm, VX, VY, VZ);
+        k_vector_field<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->pnt.get() + 27 * c->pnt_off[D], c->pidx.get(),
+                                                               c->pnum.get(), W.get(), c->nx.get(), c->ny.get(),
+                                                               c->nz.get(), baseD, MD, VX, VY, VZ);
         PRG_CHECK_LAUNCH();
     }
     StageTimer st(c, PRG_STAGE_DIVERGENCE);
