@@ -31,6 +31,7 @@
 
 #include "../../include/prg_unet.h"
 #include "ctx.h"
+#include "pnt.h"
 
 namespace prg {
 namespace {
@@ -84,29 +85,18 @@ __device__ __forceinline__ int seg_of(const Seg* __restrict__ segs, int nseg, in
     return k;
 }
 
-// neighbour j (compile-time after unrolling) of slot s in the block whose
-// parent-neighbour row is `row`; -1 if absent.
-__device__ __forceinline__ int32_t pnt_neighbor(const int32_t* __restrict__ row, int s, int j) {
-    const int vx = ((s >> 2) & 1) + (j / 9 - 1);
-    const int vy = ((s >> 1) & 1) + ((j / 3) % 3 - 1);
-    const int vz = (s & 1) + (j % 3 - 1);
-    const int pj = 9 * ((vx >> 1) + 1) + 3 * ((vy >> 1) + 1) + ((vz >> 1) + 1);
-    const int cs = ((vx & 1) << 2) | ((vy & 1) << 1) | (vz & 1);
-    const int32_t cb = row[pj];
-    return cb < 0 ? -1 : cb + cs;
-}
-
-// y_t = sum_j L[j] * v[neighbour_j(t)]  (gpuSpMV order, float)
+// y_t = sum_j L[j] * v[neighbour_j(t)] over existing neighbours with a
+// nonzero coefficient (gpuSpMV order, float); the neighbours come from the
+// row of t's sibling block (8 entries serve the 27)
 __device__ __forceinline__ float spmv_pnt(const int32_t* __restrict__ pnt, int64_t t, const float* __restrict__ v,
                                           const Stencil& S) {
-    const int32_t* row = pnt + 27 * (t >> 3);
-    const int s = int(t & 7);
+    int32_t nb[27];
+    pnt_neighbours(pnt + 27 * (t >> 3), int(t & 7), nb);
     float out = 0.0f;
 #pragma unroll
     for (int j = 0; j < 27; j++) {
-        const int32_t nb = pnt_neighbor(row, s, j);
-        if (nb < 0 || S.L[j] == 0.0f) continue;
-        const float prod = S.L[j] * v[nb];
+        if (nb[j] < 0 || S.L[j] == 0.0f) continue;
+        const float prod = S.L[j] * v[nb[j]];
         out = out + prod;
     }
     return out;
@@ -527,14 +517,13 @@ struct SmallJobs {
 __device__ __forceinline__ float spmv_small(const int32_t* __restrict__ pnt, int64_t t, const float* __restrict__ v,
                                             const float* __restrict__ L) {
     if (!pnt) return (L[13] == 0.0f) ? 0.0f : 0.0f + L[13] * v[0];
-    const int32_t* row = pnt + 27 * (t >> 3);
-    const int s = int(t & 7);
+    int32_t nb[27];
+    pnt_neighbours(pnt + 27 * (t >> 3), int(t & 7), nb);
     float out = 0.0f;
 #pragma unroll
     for (int j = 0; j < 27; j++) {
-        const int32_t nb = pnt_neighbor(row, s, j);
-        if (nb < 0 || L[j] == 0.0f) continue;
-        const float prod = L[j] * v[nb];
+        if (nb[j] < 0 || L[j] == 0.0f) continue;
+        const float prod = L[j] * v[nb[j]];
         out = out + prod;
     }
     return out;
