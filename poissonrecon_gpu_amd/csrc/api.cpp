@@ -119,6 +119,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         PRG_HIP(hipStreamCreateWithFlags(&c->stream4, hipStreamNonBlocking));
         PRG_HIP(hipEventCreate(&c->ev_a));
         PRG_HIP(hipEventCreate(&c->ev_b));
+        PRG_HIP(hipEventCreateWithFlags(&c->plan_ev, hipEventDisableTiming));
         PRG_HIP(hipHostMalloc(&c->transform_host, 16, hipHostMallocDefault));
         PRG_HIP(hipHostMalloc(&c->cg_host, 4096, hipHostMallocDefault));
         PRG_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, cf.device));
@@ -142,6 +143,7 @@ void prg_destroy(prg_ctx* c) {
         if (x) (void)hipStreamSynchronize(x);
     if (c->ev_a) (void)hipEventDestroy(c->ev_a);
     if (c->ev_b) (void)hipEventDestroy(c->ev_b);
+    if (c->plan_ev) (void)hipEventDestroy(c->plan_ev);
     if (c->transform_host) (void)hipHostFree(c->transform_host);
     if (c->cg_host) (void)hipHostFree(c->cg_host);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -172,6 +174,8 @@ prg_status prg_build_octree(prg_ctx* c) {
     return guarded(c, [&]() {
         if (!c->have_points) return fail(c, PRG_E_STATE, "prg_set_points first");
         PRG_HIP(hipSetDevice(c->cfg.device));
+        // a plan build of the previous octree may still read its tables
+        if (c->plan_pending) PRG_HIP(hipStreamWaitEvent(c->stream, c->plan_ev, 0));
         {
             prg::StageTimer st(c, PRG_STAGE_OCTREE);
             prg::build_octree(c);
@@ -182,6 +186,7 @@ prg_status prg_build_octree(prg_ctx* c) {
         }
         c->have_tree = true;
         c->tree_gen++;
+        prg::start_plans(c);
         c->have_div = c->have_x = c->have_iso = false;
         PRG_HIP(hipMemsetAsync(c->x.reserve(c->M), 0, size_t(c->M) * 4, c->stream));
         PRG_HIP(hipMemsetAsync(c->div.reserve(c->M), 0, size_t(c->M) * 4, c->stream));

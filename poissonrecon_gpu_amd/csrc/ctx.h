@@ -70,6 +70,9 @@ struct prg_ctx {
     int64_t plan_tile_off[prg::kMaxDepth + 2] = {};
     int32_t plan_maxcount[prg::kMaxDepth + 1] = {};
     uint64_t tree_gen = 0, plan_gen = 0;
+    prg::DBuf<int32_t> plan_mx;       // per-depth largest tile count (device)
+    hipEvent_t plan_ev = nullptr;     // completion of the plans started after an octree build
+    bool plan_pending = false;
     bool neigh_leaf_ready = false;  // the leaf level's 27 neighbour planes are built on demand
     prg_unet* warm_net = nullptr;  // learned warm start (not owned)
     // normalisation transform (centre x, y, z, scale) computed on the device;
@@ -112,6 +115,10 @@ void build_octree(prg_ctx* c);
 // the leaf level's 27-neighbour planes (kernels use the parent-neighbour rows;
 // these serve level views, buffer reads and the subdivision)
 void ensure_leaf_neighbours(prg_ctx* c);
+// SpMV tile plans of the new octree, started on the second stream right after
+// its build (they overlap the vector field and divergence); the solve waits
+// for them
+void start_plans(prg_ctx* c);
 void ensure_tables(prg_ctx* c);
 void compute_divergence(prg_ctx* c);
 prg_status solve_all(prg_ctx* c, prg_solve_stats* stats);

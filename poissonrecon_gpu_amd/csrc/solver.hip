@@ -630,6 +630,11 @@ struct Chain {
 
 constexpr int kMaxSegs = 12;         // depths per solve above kSmallRows rows (D <= 10 has at most 10)
 constexpr size_t kHostStates = 1024;  // pinned read-back area: states first, then the segment table
+constexpr size_t kHostPlanMax = 3840;  // then the plans' per-depth largest tile counts (kMaxDepth + 1 int32)
+
+int32_t* plan_maxcount_host(prg_ctx* c) {
+    return reinterpret_cast<int32_t*>(static_cast<uint8_t*>(c->cg_host) + kHostPlanMax);
+}
 
 // SpMV instance: cells per thread covering the largest tile of the chain's
 // depths.  A chain's launch is one round of workgroups, two per CU, shared
@@ -654,10 +659,13 @@ void launch_spmv(int inst, int32_t nwg, size_t lds, hipStream_t s, const Seg* se
 
 int64_t tiles_of(int64_t n) { return (n / 8 + kTile - 1) / kTile; }
 
-// Tile plans of every depth above kSmallRows rows, once per octree.
-void ensure_plans(prg_ctx* c) {
-    if (c->plan_gen == c->tree_gen) return;
-    hipStream_t s = c->stream;
+}  // namespace
+
+// Tile plans of every depth above kSmallRows rows, once per octree: on the
+// second stream after the build (on the context stream when kernels are
+// timed), with the per-depth largest tile count copied to pinned memory.
+void start_plans(prg_ctx* c) {
+    hipStream_t s = c->kt.enabled ? c->stream : c->stream2;
     const int D = c->D;
     int64_t off = 0;
     for (int d = 0; d <= D; d++) {
@@ -668,9 +676,12 @@ void ensure_plans(prg_ctx* c) {
     int32_t* cnt = c->plan_count.reserve(off + 1);
     int32_t* ub = c->plan_ubase.reserve(size_t(off) * kTileEntries + 1);
     int16_t* rw = c->plan_rowid.reserve(size_t(off) * kTileEntries + 1);
-    DBuf<int32_t> mx;
-    mx.reserve(D + 1);
-    PRG_HIP(hipMemsetAsync(mx.get(), 0, size_t(D + 1) * 4, s));
+    int32_t* mx = c->plan_mx.reserve(kMaxDepth + 1);
+    if (s != c->stream) {
+        PRG_HIP(hipEventRecord(c->plan_ev, c->stream));
+        PRG_HIP(hipStreamWaitEvent(s, c->plan_ev, 0));
+    }
+    PRG_HIP(hipMemsetAsync(mx, 0, size_t(D + 1) * 4, s));
     for (int d = 0; d <= D; d++) {
         const int64_t nt = c->plan_tile_off[d + 1] - c->plan_tile_off[d];
         if (c->counts[d] <= kSmallRows || nt == 0) continue;
@@ -678,12 +689,25 @@ void ensure_plans(prg_ctx* c) {
         TimedScope ts(c->kt, "spmv_tile_plan", 13.5 * double(c->counts[d]));
         k_tile_plan<<<unsigned(nt), kSpmvThreads, 0, s>>>(c->pnt.get() + 27 * c->pnt_off[d], c->counts[d] / 8,
                                                            cnt + t0, ub + t0 * kTileEntries, rw + t0 * kTileEntries,
-                                                           mx.get() + d);
+                                                           mx + d);
         PRG_CHECK_LAUNCH();
     }
-    PRG_HIP(hipMemcpyAsync(c->plan_maxcount, mx.get(), size_t(D + 1) * 4, hipMemcpyDeviceToHost, s));
-    PRG_HIP(hipStreamSynchronize(s));
+    PRG_HIP(hipMemcpyAsync(plan_maxcount_host(c), mx, size_t(D + 1) * 4, hipMemcpyDeviceToHost, s));
+    PRG_HIP(hipEventRecord(c->plan_ev, s));
+    c->plan_pending = true;
     c->plan_gen = c->tree_gen;
+}
+
+namespace {
+
+// the solve's plans: started after the octree build, finished here
+void ensure_plans(prg_ctx* c) {
+    if (c->plan_gen != c->tree_gen) start_plans(c);
+    if (c->plan_pending) {
+        PRG_HIP(hipEventSynchronize(c->plan_ev));
+        for (int d = 0; d <= c->D; d++) c->plan_maxcount[d] = plan_maxcount_host(c)[d];
+        c->plan_pending = false;
+    }
 }
 
 }  // namespace
@@ -767,7 +791,8 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     for (auto& h : ch) nsegs += int(h.depths.size());
     if (nsegs > kMaxSegs) throw Error{PRG_E_ARG, "too many large depths"};
     static_assert(kMaxSegs * sizeof(CgState) <= kHostStates, "chain states exceed their pinned area");
-    static_assert(kHostStates + kMaxSegs * sizeof(Seg) <= 4096, "segment table exceeds the pinned staging area");
+    static_assert(kHostStates + kMaxSegs * sizeof(Seg) <= kHostPlanMax, "segment table exceeds the pinned staging area");
+    static_assert(kHostPlanMax + (kMaxDepth + 1) * 4 <= 4096, "plan counts exceed the pinned area");
     CgState* hst = static_cast<CgState*>(c->cg_host);
     Seg* hseg = reinterpret_cast<Seg*>(static_cast<uint8_t*>(c->cg_host) + kHostStates);
     Seg* dseg = reinterpret_cast<Seg*>(c->cg_seg.reserve(std::max<size_t>(1, size_t(nsegs)) * sizeof(Seg)));
