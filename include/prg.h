@@ -154,7 +154,8 @@ const char* prg_last_error(const prg_ctx* ctx);
 /* Uploads n oriented points and applies the reference normalisation A1
  * (pipelineBuildNodeArray, main.cu:530-571): bbox, scale = 1.25 * max extent,
  * p' = (p - center) / scale, n' = n * 2^(D+1) / |n|.  `on_device` = 1 means the
- * pointers are device memory.  n must be in [1, 2^31). */
+ * pointers are device memory.  n must be in [1, 2^31).  The arrays have been
+ * read when the call returns. */
 prg_status prg_set_points(prg_ctx* ctx, const float* xyz, const float* nxyz, int64_t n, int32_t on_device);
 
 /* Morton keys, sort, run-length leaves, per-depth sibling-complete arrays and

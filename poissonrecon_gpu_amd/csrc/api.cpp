@@ -117,9 +117,11 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         PRG_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
         PRG_HIP(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
         PRG_HIP(hipStreamCreateWithFlags(&c->stream4, hipStreamNonBlocking));
-        PRG_HIP(hipEventCreate(&c->ev_a));
-        PRG_HIP(hipEventCreate(&c->ev_b));
         PRG_HIP(hipEventCreateWithFlags(&c->plan_ev, hipEventDisableTiming));
+        for (auto& e : c->stage_ev) {
+            PRG_HIP(hipEventCreate(&e[0]));
+            PRG_HIP(hipEventCreate(&e[1]));
+        }
         PRG_HIP(hipHostMalloc(&c->transform_host, 16, hipHostMallocDefault));
         PRG_HIP(hipHostMalloc(&c->cg_host, 4096, hipHostMallocDefault));
         PRG_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, cf.device));
@@ -141,9 +143,10 @@ void prg_destroy(prg_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (hipStream_t x : {c->stream2, c->stream3, c->stream4})
         if (x) (void)hipStreamSynchronize(x);
-    if (c->ev_a) (void)hipEventDestroy(c->ev_a);
-    if (c->ev_b) (void)hipEventDestroy(c->ev_b);
     if (c->plan_ev) (void)hipEventDestroy(c->plan_ev);
+    for (auto& e : c->stage_ev)
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
     if (c->transform_host) (void)hipHostFree(c->transform_host);
     if (c->cg_host) (void)hipHostFree(c->cg_host);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -163,8 +166,11 @@ prg_status prg_set_points(prg_ctx* c, const float* xyz, const float* nxyz, int64
         c->have_points = c->have_tree = c->have_div = c->have_x = c->have_iso = c->iso_ready = false;
         c->mesh_nv = c->mesh_nt = 0;
         c->stage_ms.fill(0.0);
+        for (bool& p : c->stage_pending) p = false;
         prg::StageTimer st(c, PRG_STAGE_NORMALIZE);
         prg::normalize_points(c, xyz, nxyz, n, on_device != 0);
+        // the caller's arrays are consumed when this call returns
+        if (on_device) PRG_HIP(hipStreamSynchronize(c->stream));
         c->have_points = true;
         return PRG_OK;
     });
@@ -380,7 +386,10 @@ prg_status prg_reconstruct(prg_ctx* c, int32_t subdivide, prg_mesh* out, prg_sol
 prg_status prg_stage_times(prg_ctx* c, prg_timings* out) {
     return guarded(c, [&]() {
         if (!out) return fail(c, PRG_E_ARG, "null output");
-        for (int i = 0; i < PRG_STAGE_COUNT; i++) out->ms[i] = c->stage_ms[i];
+        for (int i = 0; i < PRG_STAGE_COUNT; i++) {
+            prg::resolve_stage(c, i);
+            out->ms[i] = c->stage_ms[i];
+        }
         return PRG_OK;
     });
 }

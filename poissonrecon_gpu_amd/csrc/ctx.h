@@ -96,9 +96,12 @@ struct prg_ctx {
     int64_t sub_nv = 0, sub_nt = 0, sub_roots = 0, sub_leaves = 0;
     bool have_sub = false;
 
-    // stage timing
+    // stage timing: start/end events per stage, resolved lazily (when the
+    // stage runs again or the times are read), so no stage waits for its own
+    // measurement
     std::array<double, PRG_STAGE_COUNT> stage_ms{};
-    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    hipEvent_t stage_ev[PRG_STAGE_COUNT][2] = {};
+    bool stage_pending[PRG_STAGE_COUNT] = {};
     std::vector<hipEvent_t> ev_pool;  // the solver's events, reused across solves
     int num_cu = 0;                   // compute units of the device
     void* cg_host = nullptr;          // pinned, 4 KiB: CG state read-backs and the segment table's staging
@@ -129,17 +132,27 @@ void extract_subdivision(prg_ctx* c);
 // `depth`, written into the solution vector x (device)
 void unet_warm_start(prg_ctx* c, int depth, hipStream_t s);
 
+// adds a recorded measurement of stage st to stage_ms
+inline void resolve_stage(prg_ctx* c, int st) {
+    if (!c->stage_pending[st]) return;
+    PRG_HIP(hipEventSynchronize(c->stage_ev[st][1]));
+    float ms = 0.0f;
+    PRG_HIP(hipEventElapsedTime(&ms, c->stage_ev[st][0], c->stage_ev[st][1]));
+    c->stage_ms[st] += ms;
+    c->stage_pending[st] = false;
+}
+
 // Measures one stage with hipEvents on the context stream.
 struct StageTimer {
     prg_ctx* c;
     int stage;
-    StageTimer(prg_ctx* ctx, int st) : c(ctx), stage(st) { PRG_HIP(hipEventRecord(c->ev_a, c->stream)); }
+    StageTimer(prg_ctx* ctx, int st) : c(ctx), stage(st) {
+        resolve_stage(c, st);
+        PRG_HIP(hipEventRecord(c->stage_ev[st][0], c->stream));
+    }
     ~StageTimer() noexcept(false) {
-        PRG_HIP(hipEventRecord(c->ev_b, c->stream));
-        PRG_HIP(hipEventSynchronize(c->ev_b));
-        float ms = 0.0f;
-        PRG_HIP(hipEventElapsedTime(&ms, c->ev_a, c->ev_b));
-        c->stage_ms[stage] += ms;
+        PRG_HIP(hipEventRecord(c->stage_ev[stage][1], c->stream));
+        c->stage_pending[stage] = true;
     }
 };
 

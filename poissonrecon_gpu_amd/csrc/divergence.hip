@@ -413,25 +413,27 @@ __global__ void __launch_bounds__(kBlock) k_g_parent_sums(const uint32_t* key, c
 // b_i = sum_j G[neighbour_j(i)][j] for the nodes above the leaf level; the
 // neighbours come from the parent-neighbour rows (depth >= 1), the root's
 // only neighbour is itself.
+// G holds every position of a node with leaf-level slots below it (dnum > 0);
+// a node without any contributes no terms and its G is neither written nor
+// read.
 __global__ void __launch_bounds__(kBlock) k_div_from_g(const int32_t* pnt, const int64_t* pnt_off, Bases B, int D,
-                                                       const double* G, int64_t gstride, int64_t n, float* div) {
+                                                       const int32_t* dnum, const double* G, int64_t gstride,
+                                                       int64_t n, float* div) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     double val = 0.0;
     if (i == 0) {
-        val += G[13 * gstride + 0];
+        if (dnum[0] > 0) val += G[13 * gstride + 0];
     } else {
         const int d = depth_of(B, D, i);
         const int64_t t = i - B.b[d];
-        const int32_t* row = pnt + 27 * (pnt_off[d] + (t >> 3));
-        const int s = int(t & 7);
+        int32_t nbr[27];
+        pnt_neighbours(pnt + 27 * (pnt_off[d] + (t >> 3)), int(t & 7), nbr);
 #pragma unroll
         for (int j = 0; j < 27; j++) {
-            const int vx = ((s >> 2) & 1) + (j / 9 - 1), vy = ((s >> 1) & 1) + ((j / 3) % 3 - 1),
-                      vz = (s & 1) + (j % 3 - 1);
-            const int32_t cb = row[9 * ((vx >> 1) + 1) + 3 * ((vy >> 1) + 1) + ((vz >> 1) + 1)];
-            if (cb < 0) continue;
-            const int64_t nb = B.b[d] + cb + (((vx & 1) << 2) | ((vy & 1) << 1) | (vz & 1));
+            if (nbr[j] < 0) continue;
+            const int64_t nb = B.b[d] + nbr[j];
+            if (dnum[nb] == 0) continue;
             val += G[j * gstride + nb];
         }
     }
@@ -504,8 +506,7 @@ void compute_divergence(prg_ctx* c) {
         moff.reserve(n);
         ctot.reserve(1);
         mtot.reserve(1);
-        G.reserve(size_t(27) * n);
-        PRG_HIP(hipMemsetAsync(G.get(), 0, size_t(27) * n * 8, s));
+        G.reserve(size_t(27) * n);  // written for every node with dnum > 0; only those are read
         const int64_t first_skipped = D >= 1 ? c->bases[D - 1] : n;
         if (D >= 1) {
             TimedScope ts(c->kt, "div_parent_sums", 16.0 * double(MD) + 216.0 * double(c->counts[D - 1]));
@@ -562,7 +563,8 @@ void compute_divergence(prg_ctx* c) {
             PoolBuf<int64_t> poff{&c->pool[74]};
             PRG_HIP(hipMemcpyAsync(poff.reserve(D + 2), c->pnt_off.data(), size_t(D + 2) * 8, hipMemcpyHostToDevice, s));
             TimedScope ts(c->kt, "div_from_subtree_sums", 13.5 * double(n) + 216.0 * double(n) + 4.0 * double(n));
-            k_div_from_g<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->pnt.get(), poff.get(), B, D, G.get(), n, n, DV);
+            k_div_from_g<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->pnt.get(), poff.get(), B, D, c->dnum.get(), G.get(),
+                                                               n, n, DV);
             PRG_CHECK_LAUNCH();
         }
         PRG_HIP(hipStreamSynchronize(s));
