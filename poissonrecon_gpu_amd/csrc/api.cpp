@@ -118,6 +118,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         PRG_HIP(hipStreamCreateWithFlags(&c->stream3, hipStreamNonBlocking));
         PRG_HIP(hipStreamCreateWithFlags(&c->stream4, hipStreamNonBlocking));
         PRG_HIP(hipEventCreateWithFlags(&c->plan_ev, hipEventDisableTiming));
+        PRG_HIP(hipEventCreate(&c->topo_ev));
         for (auto& e : c->stage_ev) {
             PRG_HIP(hipEventCreate(&e[0]));
             PRG_HIP(hipEventCreate(&e[1]));
@@ -144,6 +145,7 @@ void prg_destroy(prg_ctx* c) {
     for (hipStream_t x : {c->stream2, c->stream3, c->stream4})
         if (x) (void)hipStreamSynchronize(x);
     if (c->plan_ev) (void)hipEventDestroy(c->plan_ev);
+    if (c->topo_ev) (void)hipEventDestroy(c->topo_ev);
     for (auto& e : c->stage_ev)
         for (hipEvent_t x : e)
             if (x) (void)hipEventDestroy(x);
@@ -180,8 +182,9 @@ prg_status prg_build_octree(prg_ctx* c) {
     return guarded(c, [&]() {
         if (!c->have_points) return fail(c, PRG_E_STATE, "prg_set_points first");
         PRG_HIP(hipSetDevice(c->cfg.device));
-        // a plan build of the previous octree may still read its tables
+        // plans or topology of the previous octree may still read its tables
         if (c->plan_pending) PRG_HIP(hipStreamWaitEvent(c->stream, c->plan_ev, 0));
+        if (c->topo_pending) PRG_HIP(hipStreamWaitEvent(c->stream, c->topo_ev, 0));
         {
             prg::StageTimer st(c, PRG_STAGE_OCTREE);
             prg::build_octree(c);
@@ -193,6 +196,7 @@ prg_status prg_build_octree(prg_ctx* c) {
         c->have_tree = true;
         c->tree_gen++;
         prg::start_plans(c);
+        prg::start_topology(c);
         c->have_div = c->have_x = c->have_iso = false;
         PRG_HIP(hipMemsetAsync(c->x.reserve(c->M), 0, size_t(c->M) * 4, c->stream));
         PRG_HIP(hipMemsetAsync(c->div.reserve(c->M), 0, size_t(c->M) * 4, c->stream));

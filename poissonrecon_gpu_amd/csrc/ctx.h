@@ -31,6 +31,7 @@ struct prg_ctx {
     std::string err;
     prg::KernelTimer kt;
     prg::Scratch scratch;
+    prg::Scratch scratch2;  // scans of the work started on the second stream
 
     // points
     int64_t N = 0;
@@ -73,6 +74,12 @@ struct prg_ctx {
     prg::DBuf<int32_t> plan_mx;       // per-depth largest tile count (device)
     hipEvent_t plan_ev = nullptr;     // completion of the plans started after an octree build
     bool plan_pending = false;
+    // marching-cubes topology of the octree `topo_gen` (corner owners and
+    // their numbering, the corner basis table), started after the build
+    uint64_t topo_gen = 0;
+    hipEvent_t topo_ev = nullptr;
+    bool topo_pending = false;
+    int32_t topo_corners = 0;  // owned corners
     bool neigh_leaf_ready = false;  // the leaf level's 27 neighbour planes are built on demand
     prg_unet* warm_net = nullptr;  // learned warm start (not owned)
     // normalisation transform (centre x, y, z, scale) computed on the device;
@@ -122,6 +129,9 @@ void ensure_leaf_neighbours(prg_ctx* c);
 // its build (they overlap the vector field and divergence); the solve waits
 // for them
 void start_plans(prg_ctx* c);
+// marching-cubes topology of the new octree, on the second stream after the
+// plans (it depends on the tree only)
+void start_topology(prg_ctx* c);
 void ensure_tables(prg_ctx* c);
 void compute_divergence(prg_ctx* c);
 prg_status solve_all(prg_ctx* c, prg_solve_stats* stats);

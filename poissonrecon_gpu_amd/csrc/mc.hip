@@ -385,48 +385,75 @@ void compute_isovalue(prg_ctx* c) {
     c->iso = float(sum) / float(n);
 }
 
+// pinned area of the owned-corner count (the last bytes of the context's
+// pinned staging page, after the solver's areas)
+int32_t* topo_count_host(prg_ctx* c) { return reinterpret_cast<int32_t*>(static_cast<uint8_t*>(c->cg_host) + 4092); }
+
+void start_topology(prg_ctx* c) {
+    ensure_tables(c);
+    const bool timed = c->kt.enabled;
+    hipStream_t s = timed ? c->stream : c->stream2;  // the plans have made stream2 wait for the tree
+    Scratch& scr = timed ? c->scratch : c->scratch2;
+    const int D = c->D;
+    const int64_t MD = c->MD(), baseD = c->bases[D];
+    KernelTimer& kt = c->kt;
+    PoolBuf<int32_t> cmask{&c->pool[78]}, oref{&c->pool[79]}, cbase{&c->pool[80]}, vtot{&c->pool[81]},
+        ccount{&c->pool[92]};
+    PoolBuf<float> btab{&c->pool[91]};
+    cmask.reserve(MD);
+    ccount.reserve(MD);
+    oref.reserve(8 * MD);
+    cbase.reserve(MD);
+    vtot.reserve(1);
+    // stage time: events on the stream the topology runs on
+    resolve_stage(c, PRG_STAGE_MC_TOPOLOGY);
+    PRG_HIP(hipEventRecord(c->stage_ev[PRG_STAGE_MC_TOPOLOGY][0], s));
+    {
+        TimedScope ts(kt, "corner_owner", 8.0 * 64.0 * double(MD));
+        k_corner_owner<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->pnt.get() + 27 * c->pnt_off[D], baseD,
+                                                               MD, cmask.get(), ccount.get(), oref.get());
+        PRG_CHECK_LAUNCH();
+    }
+    scan_exclusive(ccount.get(), cbase.get(), MD, vtot.get(), s, scr, kt);
+    PRG_HIP(hipMemcpyAsync(topo_count_host(c), vtot.get(), 4, hipMemcpyDeviceToHost, s));
+    const int64_t nbt = int64_t(D + 1) * (int64_t(1) << D) * 2;
+    btab.reserve(4 * nbt);
+    k_corner_basis<<<grid_for(nbt, kBlock), kBlock, 0, s>>>(c->d_base_fn.get(), D, btab.get());
+    PRG_CHECK_LAUNCH();
+    PRG_HIP(hipEventRecord(c->stage_ev[PRG_STAGE_MC_TOPOLOGY][1], s));
+    c->stage_pending[PRG_STAGE_MC_TOPOLOGY] = true;
+    PRG_HIP(hipEventRecord(c->topo_ev, s));
+    c->topo_pending = true;
+    c->topo_gen = c->tree_gen;
+}
+
 void extract_leaf_mesh(prg_ctx* c) {
     ensure_tables(c);
     hipStream_t s = c->stream;
     const int D = c->D;
-    const int64_t M = c->M, MD = c->MD(), baseD = c->bases[D];
+    const int64_t MD = c->MD(), baseD = c->bases[D];
     KernelTimer& kt = c->kt;
     // per leaf slot: owned-corner mask and count, their scan (corner ids);
     // owned sign-change edge mask and count, their scan (vertex ids)
-    PoolBuf<int32_t> cmask{&c->pool[78]}, oref{&c->pool[79]}, cbase{&c->pool[80]}, vtot{&c->pool[81]},
-        emask{&c->pool[82]}, eref{&c->pool[83]}, ebase{&c->pool[84]}, etot{&c->pool[85]}, ntri{&c->pool[86]},
-        toff{&c->pool[87]}, ttot{&c->pool[88]}, ccount{&c->pool[92]}, ecount{&c->pool[93]};
-    PoolBuf<float> vval{&c->pool[89]};
+    PoolBuf<int32_t> cmask{&c->pool[78]}, oref{&c->pool[79]}, cbase{&c->pool[80]}, emask{&c->pool[82]},
+        eref{&c->pool[83]}, ebase{&c->pool[84]}, etot{&c->pool[85]}, ntri{&c->pool[86]}, toff{&c->pool[87]},
+        ttot{&c->pool[88]}, ecount{&c->pool[93]};
+    PoolBuf<float> vval{&c->pool[89]}, btab{&c->pool[91]};
     float* CV = c->corner_vals.reserve(8 * MD);
-    int32_t V = 0, NV = 0, NT = 0;
-    {
-        StageTimer st(c, PRG_STAGE_MC_TOPOLOGY);
-        cmask.reserve(MD);
-        ccount.reserve(MD);
-        oref.reserve(8 * MD);
-        cbase.reserve(MD);
-        vtot.reserve(1);
-        {
-            TimedScope ts(kt, "corner_owner", 8.0 * 64.0 * double(MD));
-            k_corner_owner<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->pnt.get() + 27 * c->pnt_off[D],
-                                                                   baseD, MD, cmask.get(), ccount.get(), oref.get());
-            PRG_CHECK_LAUNCH();
-        }
-        scan_exclusive(ccount.get(), cbase.get(), MD, vtot.get(), s, c->scratch, kt);
-        PRG_HIP(hipMemcpyAsync(&V, vtot.get(), 4, hipMemcpyDeviceToHost, s));
-        PRG_HIP(hipStreamSynchronize(s));
+    int32_t NV = 0, NT = 0;
+    if (c->topo_gen != c->tree_gen) start_topology(c);
+    if (c->topo_pending) {
+        PRG_HIP(hipEventSynchronize(c->topo_ev));
+        c->topo_corners = *topo_count_host(c);
+        c->topo_pending = false;
     }
+    const int32_t V = c->topo_corners;
     {
         StageTimer st(c, PRG_STAGE_MC_EVAL);
         vval.reserve(V);
         PoolBuf<int32_t> vlist{&c->pool[90]};
         vlist.reserve(V);
         k_corner_list<<<grid_for(MD, kBlock), kBlock, 0, s>>>(cmask.get(), cbase.get(), MD, vlist.get());
-        PRG_CHECK_LAUNCH();
-        PoolBuf<float> btab{&c->pool[91]};
-        const int64_t nbt = int64_t(D + 1) * (int64_t(1) << D) * 2;
-        btab.reserve(4 * nbt);
-        k_corner_basis<<<grid_for(nbt, kBlock), kBlock, 0, s>>>(c->d_base_fn.get(), D, btab.get());
         PRG_CHECK_LAUNCH();
         {
             TimedScope ts(kt, "corner_value", 24.0 * double(V));

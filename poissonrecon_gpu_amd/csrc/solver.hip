@@ -792,7 +792,7 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     if (nsegs > kMaxSegs) throw Error{PRG_E_ARG, "too many large depths"};
     static_assert(kMaxSegs * sizeof(CgState) <= kHostStates, "chain states exceed their pinned area");
     static_assert(kHostStates + kMaxSegs * sizeof(Seg) <= kHostPlanMax, "segment table exceeds the pinned staging area");
-    static_assert(kHostPlanMax + (kMaxDepth + 1) * 4 <= 4096, "plan counts exceed the pinned area");
+    static_assert(kHostPlanMax + (kMaxDepth + 1) * 4 <= 4092, "plan counts exceed the pinned area (its last 4 bytes hold the owned-corner count)");
     CgState* hst = static_cast<CgState*>(c->cg_host);
     Seg* hseg = reinterpret_cast<Seg*>(static_cast<uint8_t*>(c->cg_host) + kHostStates);
     Seg* dseg = reinterpret_cast<Seg*>(c->cg_seg.reserve(std::max<size_t>(1, size_t(nsegs)) * sizeof(Seg)));
