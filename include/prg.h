@@ -147,6 +147,12 @@ prg_status prg_config_default(prg_config* cfg);
  * implicit global CUDA state of main() (main.cu:3247). */
 prg_status prg_create(prg_ctx** out, const prg_config* cfg);
 void prg_destroy(prg_ctx* ctx);
+/* Changes the context's maximum depth (2..10), keeping its device buffers for
+ * reuse; points, octree and results are dropped (call prg_set_points next).
+ * No counterpart in the reference, whose depth is a compile-time constant
+ * (main.cu:69); it serves callers that reconstruct clouds of different depths
+ * in one context, such as prg_reconstruct_batch's workers. */
+prg_status prg_set_max_depth(prg_ctx* ctx, int32_t max_depth);
 const char* prg_last_error(const prg_ctx* ctx);
 
 /* ---- stages ------------------------------------------------------------ */

@@ -82,6 +82,7 @@ _F32P = C.POINTER(C.c_float)
 SIGNATURES = {
     "prg_config_default": (C.c_int, [C.POINTER(PrgConfig)]),
     "prg_create": (C.c_int, [C.POINTER(_P), C.POINTER(PrgConfig)]),
+    "prg_set_max_depth": (C.c_int, [_P, C.c_int32]),
     "prg_destroy": (None, [_P]),
     "prg_last_error": (C.c_char_p, [_P]),
     "prg_set_points": (C.c_int, [_P, _P, _P, C.c_int64, C.c_int32]),
@@ -239,6 +240,11 @@ class Reconstructor:
         if st not in (OK, E_NOT_CONVERGED):
             raise PrgError(st, lib().prg_last_error(self._h).decode())
         return st
+
+    def set_max_depth(self, max_depth: int) -> None:
+        """prg_set_max_depth: a new depth for this context, keeping its buffers."""
+        self._call("prg_set_max_depth", int(max_depth))
+        self.max_depth = int(max_depth)
 
     # ---- stages ----
     def set_points(self, xyz: np.ndarray, nxyz: np.ndarray) -> None:

@@ -160,6 +160,23 @@ void prg_destroy(prg_ctx* c) {
 
 const char* prg_last_error(const prg_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+prg_status prg_set_max_depth(prg_ctx* c, int32_t max_depth) {
+    return guarded(c, [&]() {
+        if (max_depth < 2 || max_depth > prg::kMaxDepth) return fail(c, PRG_E_ARG, "max_depth must be in [2, 10]");
+        PRG_HIP(hipSetDevice(c->cfg.device));
+        // work of the previous octree still queued on the other streams ends first
+        for (hipStream_t x : {c->stream, c->stream2, c->stream3, c->stream4}) PRG_HIP(hipStreamSynchronize(x));
+        c->plan_pending = c->topo_pending = false;
+        c->cfg.max_depth = max_depth;
+        c->D = max_depth;
+        c->have_points = c->have_tree = c->have_div = c->have_x = c->have_iso = c->iso_ready = false;
+        c->have_sub = false;
+        c->mesh_nv = c->mesh_nt = 0;
+        c->tree_gen++;
+        return PRG_OK;
+    });
+}
+
 prg_status prg_set_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n, int32_t on_device) {
     return guarded(c, [&]() {
         if (!xyz || !nxyz) return fail(c, PRG_E_ARG, "null point pointer");

@@ -51,7 +51,7 @@ struct prg_ctx {
     std::vector<int64_t> pnt_off;      // first row of depth d (rows = blocks)
 
     // function space
-    std::unique_ptr<prg::FsTables> fs;
+    std::shared_ptr<const prg::FsTables> fs;  // shared by every context of the same depth
     prg::DBuf<float> d_base_fn, d_stencil, d_div_table;
 
     // fields

@@ -17,6 +17,8 @@
 //     added in order), then b_i gathers 27 of them.  No global atomics; the
 //     result is deterministic.
 #include <cmath>
+#include <map>
+#include <mutex>
 
 #include "ctx.h"
 #include "pnt.h"
@@ -442,10 +444,23 @@ __global__ void __launch_bounds__(kBlock) k_div_from_g(const int32_t* pnt, const
 
 }  // namespace
 
+// Function tables depend on the depth only: built once per depth per process
+// and shared (read-only) by every context.
+std::shared_ptr<const FsTables> fs_tables_of(int D) {
+    static std::mutex m;
+    static std::map<int, std::shared_ptr<const FsTables>> cache;
+    std::lock_guard<std::mutex> lock(m);
+    auto it = cache.find(D);
+    if (it != cache.end()) return it->second;
+    auto t = std::make_shared<FsTables>();
+    build_fs_tables(D, t.get());
+    cache[D] = t;
+    return t;
+}
+
 void ensure_tables(prg_ctx* c) {
     if (c->fs && c->fs->D == c->D) return;
-    std::unique_ptr<FsTables> t(new FsTables());
-    build_fs_tables(c->D, t.get());
+    std::shared_ptr<const FsTables> t = fs_tables_of(c->D);
     hipStream_t s = c->stream;
     PRG_HIP(hipMemcpyAsync(c->d_base_fn.reserve(t->base_fn.size()), t->base_fn.data(), t->base_fn.size() * 4,
                            hipMemcpyHostToDevice, s));
@@ -453,8 +468,7 @@ void ensure_tables(prg_ctx* c) {
                            hipMemcpyHostToDevice, s));
     PRG_HIP(hipMemcpyAsync(c->d_div_table.reserve(t->div_table.size()), t->div_table.data(),
                            t->div_table.size() * 4, hipMemcpyHostToDevice, s));
-    PRG_HIP(hipStreamSynchronize(s));
-    c->fs = std::move(t);
+    c->fs = std::move(t);  // the cached host arrays outlive the copies
 }
 
 void compute_divergence(prg_ctx* c) {

@@ -34,3 +34,30 @@ def test_batch_reports_bad_items(prg):
     st, res = prg.reconstruct_batch([(p, n), (p, n)], [6, 12], concurrency=2)
     assert st != prg.OK
     assert res[0]["status"] == prg.OK and res[1]["status"] == prg.E_ARG
+
+
+def _mesh(g, p, n):
+    g.set_points(p, n)
+    g.build_octree()
+    g.compute_divergence()
+    g.solve()
+    g.isovalue()
+    return g.extract_mesh()
+
+
+def test_depth_switch_in_one_context(prg):
+    """prg_set_max_depth: a context that changes depth (deeper, then shallower,
+    then back) gives each cloud the mesh a fresh context gives it."""
+    print("\n[one context: depths 9, 7, 10, 8]", flush=True)
+    cases = [(prg.synth_torus(60000, 2), 9), (prg.synth_sphere(12000, 4), 7),
+             (prg.synth_superquadrics(150000, 3, 0.001, 21), 10), (prg.synth_superquadrics(40000, 2, 0.001, 22), 8)]
+    with prg.Reconstructor(max_depth=6) as shared:
+        for (p, n), d in cases:
+            shared.set_max_depth(d)
+            ms = _mesh(shared, p, n)
+            with prg.Reconstructor(max_depth=d) as fresh:
+                mf = _mesh(fresh, p, n)
+            assert np.array_equal(ms.triangles, mf.triangles)
+            assert np.array_equal(ms.vertices.view(np.uint32), mf.vertices.view(np.uint32))
+        with pytest.raises(prg.PrgError):
+            shared.set_max_depth(11)
