@@ -238,7 +238,7 @@ prg_status prg_level_view(prg_ctx* c, int32_t depth, prg_level_desc* v) {
     return guarded(c, [&]() {
         if (!c->have_tree) return fail(c, PRG_E_STATE, "prg_build_octree first");
         if (!v || depth < 0 || depth > c->D) return fail(c, PRG_E_ARG, "bad depth");
-        if (depth == c->D) prg::ensure_leaf_neighbours(c);
+        prg::ensure_neighbours(c);
         const int64_t b = c->bases[depth];
         v->depth = depth;
         v->count = c->counts[depth];
@@ -452,7 +452,7 @@ prg_status prg_read_buffer(prg_ctx* c, int32_t which, void* host, int64_t bytes)
                 break;
             }
             case PRG_BUF_NEIGHS: {
-                prg::ensure_leaf_neighbours(c);
+                prg::ensure_neighbours(c);
                 std::vector<int32_t> t(static_cast<size_t>(27 * M));
                 d2h(c, t.data(), c->neigh.get(), size_t(27 * M) * 4);
                 PRG_HIP(hipStreamSynchronize(c->stream));

@@ -80,7 +80,7 @@ struct prg_ctx {
     hipEvent_t topo_ev = nullptr;
     bool topo_pending = false;
     int32_t topo_corners = 0;  // owned corners
-    bool neigh_leaf_ready = false;  // the leaf level's 27 neighbour planes are built on demand
+    bool neigh_ready = false;  // the 27 neighbour planes (depths 1..D) are built on demand
     prg_unet* warm_net = nullptr;  // learned warm start (not owned)
     // normalisation transform (centre x, y, z, scale) computed on the device;
     // the host copy is read at the next synchronisation
@@ -122,9 +122,9 @@ namespace prg {
 // Stage entry points implemented in the .hip files.
 void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n, bool on_device);
 void build_octree(prg_ctx* c);
-// the leaf level's 27-neighbour planes (kernels use the parent-neighbour rows;
-// these serve level views, buffer reads and the subdivision)
-void ensure_leaf_neighbours(prg_ctx* c);
+// the 27-neighbour planes of every depth (kernels use the parent-neighbour
+// rows; these serve level views, buffer reads and the subdivision)
+void ensure_neighbours(prg_ctx* c);
 // SpMV tile plans of the new octree, started on the second stream right after
 // its build (they overlap the vector field and divergence); the solve waits
 // for them

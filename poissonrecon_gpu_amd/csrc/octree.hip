@@ -20,6 +20,7 @@
 #include <cmath>
 
 #include "ctx.h"
+#include "pnt.h"
 
 namespace prg {
 namespace {
@@ -347,22 +348,29 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(const uint32_t* key, const
 // does not exist or is empty).  Neighbour j of slot s in the block is
 // pnt[b][LUTparent[s][j]] + LUTchild[s][j] - the same lookup computeNeighbor
 // performs (main.cu:492-509), kept per block so the solver reads 13.5 bytes
-// per node instead of 108.
+// per node instead of 108.  Built top-down: the parent's neighbours come from
+// the parent depth's own table (the root's only neighbour is itself,
+// main.cu:803-808), so no 27-plane neighbour table is needed.  One thread per
+// block.
 __global__ void __launch_bounds__(kBlock) k_parent_neighbor_table(const int32_t* parent, const int32_t* child_base,
-                                                                  const int32_t* neigh, int64_t M, int64_t base,
-                                                                  int64_t nblocks, int32_t* pnt) {
-    const int64_t g = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (g >= nblocks * 27) return;
-    const int64_t b = g / 27;
-    const int j = int(g % 27);
-    const int64_t P = parent[base + 8 * b];
-    const int32_t np = neigh[int64_t(j) * M + P];
-    int32_t v = -1;
-    if (np != -1) {
-        const int32_t cb = child_base[np];
-        if (cb != -1) v = int32_t(cb - base);
+                                                                  const int32_t* pnt_prev, int64_t base_prev,
+                                                                  int64_t base, int64_t nblocks, int32_t* pnt) {
+    const int64_t b = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (b >= nblocks) return;
+    const int64_t tp = parent[base + 8 * b] - base_prev;
+    int32_t nbp[27];
+    if (!pnt_prev) {
+#pragma unroll
+        for (int j = 0; j < 27; j++) nbp[j] = (j == 13) ? 0 : -1;
+    } else {
+        pnt_neighbours(pnt_prev + 27 * (tp >> 3), int(tp & 7), nbp);
     }
-    pnt[g] = v;
+    int32_t cb[27];
+#pragma unroll
+    for (int j = 0; j < 27; j++) cb[j] = nbp[j] < 0 ? -1 : child_base[base_prev + nbp[j]];
+    int32_t* row = pnt + 27 * b;
+#pragma unroll
+    for (int j = 0; j < 27; j++) row[j] = cb[j] == -1 ? -1 : int32_t(cb[j] - base);
 }
 
 }  // namespace
@@ -404,14 +412,15 @@ void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n
     if (!on_device) PRG_HIP(hipStreamSynchronize(s));  // the staging copies must finish before the caller reuses its arrays
 }
 
-void ensure_leaf_neighbours(prg_ctx* c) {
-    if (c->neigh_leaf_ready) return;
-    const int D = c->D;
-    TimedScope ts(c->kt, "neighbors", 128.0 * double(c->counts[D]));
-    k_neighbors<<<grid_for(c->counts[D], kBlock), kBlock, 0, c->stream>>>(
-        c->key.get(), c->parent.get(), c->child_base.get(), c->neigh.get(), c->M, c->bases[D], c->counts[D], D, D);
-    PRG_CHECK_LAUNCH();
-    c->neigh_leaf_ready = true;
+void ensure_neighbours(prg_ctx* c) {
+    if (c->neigh_ready) return;
+    for (int d = 1; d <= c->D; d++) {
+        TimedScope ts(c->kt, "neighbors", 128.0 * double(c->counts[d]));
+        k_neighbors<<<grid_for(c->counts[d], kBlock), kBlock, 0, c->stream>>>(
+            c->key.get(), c->parent.get(), c->child_base.get(), c->neigh.get(), c->M, c->bases[d], c->counts[d], c->D, d);
+        PRG_CHECK_LAUNCH();
+    }
+    c->neigh_ready = true;
 }
 
 void build_octree(prg_ctx* c) {
@@ -594,23 +603,17 @@ void build_octree(prg_ctx* c) {
         k_point_leaf<<<grid_for(n, kBlock), kBlock, 0, s>>>(prank.get(), POS[D]->get(), n, c->point_leaf.reserve(n));
         PRG_CHECK_LAUNCH();
     }
-    // 4. neighbours, top-down, down to the level above the leaves: the leaf
-    // level's planes (27 x 4 B x M_D, the largest) are built on demand
-    c->neigh_leaf_ready = (D == 0);
-    for (int d = 1; d < D; d++) {
-        TimedScope ts(kt, "neighbors", 128.0 * double(counts[d]));
-        k_neighbors<<<grid_for(counts[d], kBlock), kBlock, 0, s>>>(K, PA, CB, NB, M, bases[d], counts[d], D, d);
-        PRG_CHECK_LAUNCH();
-    }
-    // 5. parent-neighbour tables (depths 1..D)
+    // 4. parent-neighbour tables (depths 1..D), top-down; the 27-plane
+    // neighbour table (depths 1..D) is built on demand
+    c->neigh_ready = (D == 0);
     c->pnt_off.assign(D + 2, 0);
     for (int d = 1; d <= D; d++) c->pnt_off[d + 1] = c->pnt_off[d] + counts[d] / 8;
     int32_t* PT = c->pnt.reserve(size_t(27) * std::max<int64_t>(1, c->pnt_off[D + 1]));
     for (int d = 1; d <= D; d++) {
         const int64_t nb = counts[d] / 8;
         TimedScope ts(kt, "parent_neighbor_table", 120.0 * double(nb));
-        k_parent_neighbor_table<<<grid_for(27 * nb, kBlock), kBlock, 0, s>>>(PA, CB, NB, M, bases[d], nb,
-                                                                           PT + 27 * c->pnt_off[d]);
+        k_parent_neighbor_table<<<grid_for(nb, kBlock), kBlock, 0, s>>>(
+            PA, CB, d == 1 ? nullptr : PT + 27 * c->pnt_off[d - 1], bases[d - 1], bases[d], nb, PT + 27 * c->pnt_off[d]);
         PRG_CHECK_LAUNCH();
     }
     PRG_HIP(hipStreamSynchronize(s));

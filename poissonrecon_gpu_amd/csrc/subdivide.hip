@@ -497,7 +497,7 @@ int32_t read_total(const DBuf<int32_t>& tot, hipStream_t s) {
 }  // namespace
 
 void extract_subdivision(prg_ctx* c) {
-    ensure_leaf_neighbours(c);
+    ensure_neighbours(c);
     ensure_tables(c);
     StageTimer st(c, PRG_STAGE_SUBDIVIDE);
     hipStream_t s = c->stream;
