@@ -217,8 +217,7 @@ prg_status prg_build_octree(prg_ctx* c) {
         c->have_div = c->have_x = c->have_iso = false;
         PRG_HIP(hipMemsetAsync(c->x.reserve(c->M), 0, size_t(c->M) * 4, c->stream));
         PRG_HIP(hipMemsetAsync(c->div.reserve(c->M), 0, size_t(c->M) * 4, c->stream));
-        PRG_HIP(hipStreamSynchronize(c->stream));
-        return PRG_OK;
+        return PRG_OK;  // the later stages are ordered on the context's stream
     });
 }
 
@@ -239,6 +238,7 @@ prg_status prg_level_view(prg_ctx* c, int32_t depth, prg_level_desc* v) {
         if (!c->have_tree) return fail(c, PRG_E_STATE, "prg_build_octree first");
         if (!v || depth < 0 || depth > c->D) return fail(c, PRG_E_ARG, "bad depth");
         prg::ensure_neighbours(c);
+        PRG_HIP(hipStreamSynchronize(c->stream));  // the viewed arrays are complete
         const int64_t b = c->bases[depth];
         v->depth = depth;
         v->count = c->counts[depth];

@@ -351,33 +351,37 @@ __global__ void __launch_bounds__(kBlock) k_g_list_multi(const int32_t* flag, co
     if (a < n && flag[a]) list[pos[a]] = int32_t(a);
 }
 
-__global__ void __launch_bounds__(kBlock) k_g_finish(const int32_t* list, const int32_t* chunks,
-                                                     const int32_t* multi_off, const double* partial, double* G,
-                                                     int64_t gstride) {
+__global__ void __launch_bounds__(kBlock) k_g_finish(const int32_t* list, const int32_t* d_count,
+                                                     const int32_t* chunks, const int32_t* multi_off,
+                                                     const double* partial, double* G, int64_t gstride) {
     __shared__ double red[kBlock / 64][27];
-    const int64_t a = list[blockIdx.x];
-    const int32_t c = chunks[a];
-    const double* P = partial + 27 * int64_t(multi_off[a]);
-    double acc[27];
-#pragma unroll
-    for (int j = 0; j < 27; j++) acc[j] = 0.0;
-    for (int32_t i = threadIdx.x; i < c; i += kBlock)
-#pragma unroll
-        for (int j = 0; j < 27; j++) acc[j] += P[27 * int64_t(i) + j];
+    const int64_t count = *d_count;  // the grid is sized by an upper bound and strides over the list
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int64_t l = blockIdx.x; l < count; l += gridDim.x) {
+        const int64_t a = list[l];
+        const int32_t c = chunks[a];
+        const double* P = partial + 27 * int64_t(multi_off[a]);
+        double acc[27];
 #pragma unroll
-    for (int j = 0; j < 27; j++) {
-        double v = acc[j];
+        for (int j = 0; j < 27; j++) acc[j] = 0.0;
+        for (int32_t i = threadIdx.x; i < c; i += kBlock)
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-        if (lane == 0) red[wave][j] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 27) {
-        double v = 0.0;
+            for (int j = 0; j < 27; j++) acc[j] += P[27 * int64_t(i) + j];
 #pragma unroll
-        for (int w = 0; w < kBlock / 64; w++) v += red[w][threadIdx.x];
-        G[threadIdx.x * gstride + a] = v;
+        for (int j = 0; j < 27; j++) {
+            double v = acc[j];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+            if (lane == 0) red[wave][j] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < 27) {
+            double v = 0.0;
+#pragma unroll
+            for (int w = 0; w < kBlock / 64; w++) v += red[w][threadIdx.x];
+            G[threadIdx.x * gstride + a] = v;
+        }
+        __syncthreads();  // red is reused by the next node
     }
 }
 
@@ -533,18 +537,24 @@ void compute_divergence(prg_ctx* c) {
         PRG_CHECK_LAUNCH();
         scan_exclusive(chunks.get(), coff.get(), n, ctot.get(), s, c->scratch, c->kt);
         scan_exclusive(multi.get(), moff.get(), n, mtot.get(), s, c->scratch, c->kt);
-        int32_t nchunks = 0, nmulti = 0;
-        PRG_HIP(hipMemcpyAsync(&nchunks, ctot.get(), 4, hipMemcpyDeviceToHost, s));
-        PRG_HIP(hipMemcpyAsync(&nmulti, mtot.get(), 4, hipMemcpyDeviceToHost, s));
-        PRG_HIP(hipStreamSynchronize(s));
+        // Sizes from upper bounds, the counts stay on the device (no host
+        // round trip): a node's chunks number at most 1 + dnum / kChunk and
+        // every depth's dnum add up to the leaf-level slot count, so
+        // chunks <= n + (D - 1) (MD / kChunk + 1); a node with several chunks
+        // has dnum > kChunk, so those chunks number at most
+        // 2 (D - 1) (MD / kChunk + 1), and such nodes half that.
+        const int64_t per_depth = MD / kChunk + 1;
+        const int64_t nchunks = n + std::max(1, D - 1) * per_depth;
+        const int64_t nmulti = 2 * std::max(1, D - 1) * per_depth;
+        const int64_t nm = std::max(1, D - 1) * per_depth;
         cnode.reserve(nchunks);
-        partial.reserve(size_t(27) * std::max(1, nmulti));
-        k_g_chunk_map<<<grid_for(std::max(1, nchunks), kBlock), kBlock, 0, s>>>(coff.get(), n, ctot.get(), cnode.get());
+        partial.reserve(size_t(27) * nmulti);
+        k_g_chunk_map<<<grid_for(nchunks, kBlock), kBlock, 0, s>>>(coff.get(), n, ctot.get(), cnode.get());
         PRG_CHECK_LAUNCH();
         Bases B{};
         for (int d = 0; d <= D; d++) B.b[d] = c->bases[d];
         B.b[D + 1] = M;
-        const int64_t waves = std::max<int64_t>(1, std::min<int64_t>(int64_t(nchunks), 16384));
+        const int64_t waves = std::max<int64_t>(1, std::min<int64_t>(nchunks, 16384));
         {
             // every leaf slot read once per coarser depth (key + vector field), G written once
             TimedScope ts(c->kt, "div_subtree_sums", 16.0 * double(MD) * double(D) + 216.0 * double(n));
@@ -553,7 +563,7 @@ void compute_divergence(prg_ctx* c) {
                 coff.get(), chunks.get(), moff.get(), ctot.get(), B, D, G.get(), n, partial.get());
             PRG_CHECK_LAUNCH();
         }
-        if (nmulti > 0) {
+        {
             PoolBuf<int32_t> mflag{&c->pool[70]}, mpos{&c->pool[71]}, mcnt{&c->pool[72]}, mlist{&c->pool[73]};
             mflag.reserve(n);
             mpos.reserve(n);
@@ -561,17 +571,14 @@ void compute_divergence(prg_ctx* c) {
             k_g_flag_multi<<<grid_for(n, kBlock), kBlock, 0, s>>>(chunks.get(), n, mflag.get());
             PRG_CHECK_LAUNCH();
             scan_exclusive(mflag.get(), mpos.get(), n, mcnt.get(), s, c->scratch, c->kt);
-            int32_t nm = 0;
-            PRG_HIP(hipMemcpyAsync(&nm, mcnt.get(), 4, hipMemcpyDeviceToHost, s));
-            PRG_HIP(hipStreamSynchronize(s));
             mlist.reserve(nm);
             k_g_list_multi<<<grid_for(n, kBlock), kBlock, 0, s>>>(mflag.get(), mpos.get(), n, mlist.get());
             PRG_CHECK_LAUNCH();
-            TimedScope ts(c->kt, "div_subtree_finish", 216.0 * double(nmulti));
-            k_g_finish<<<unsigned(nm), kBlock, 0, s>>>(mlist.get(), chunks.get(), moff.get(), partial.get(), G.get(),
-                                                       n);
+            TimedScope ts(c->kt, "div_subtree_finish", 216.0 * double(nm));
+            const unsigned fgrid = unsigned(std::min<int64_t>(nm, 2048));
+            k_g_finish<<<fgrid, kBlock, 0, s>>>(mlist.get(), mcnt.get(), chunks.get(), moff.get(), partial.get(),
+                                                G.get(), n);
             PRG_CHECK_LAUNCH();
-            PRG_HIP(hipStreamSynchronize(s));
         }
         {
             PoolBuf<int64_t> poff{&c->pool[74]};
@@ -581,7 +588,6 @@ void compute_divergence(prg_ctx* c) {
                                                                n, n, DV);
             PRG_CHECK_LAUNCH();
         }
-        PRG_HIP(hipStreamSynchronize(s));
     }
 }
 

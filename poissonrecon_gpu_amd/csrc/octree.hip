@@ -616,7 +616,6 @@ void build_octree(prg_ctx* c) {
             PA, CB, d == 1 ? nullptr : PT + 27 * c->pnt_off[d - 1], bases[d - 1], bases[d], nb, PT + 27 * c->pnt_off[d]);
         PRG_CHECK_LAUNCH();
     }
-    PRG_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace prg
