@@ -26,16 +26,21 @@ namespace {
 
 constexpr int kBlock = 256;
 
-__global__ void __launch_bounds__(kBlock) k_point_values(const float* px, const float* py, const float* pz,
-                                                         const int32_t* point_leaf, int64_t n, int64_t baseD,
-                                                         const uint32_t* key, const int32_t* pnt, Levels L,
-                                                         const int32_t* parent, const float* x, const float* bf,
-                                                         int D, float* pvals, double* partial) {
+__global__ void __launch_bounds__(kBlock) k_point_values(const float* __restrict__ px, const float* __restrict__ py,
+                                                         const float* __restrict__ pz,
+                                                         const int32_t* __restrict__ point_leaf, int64_t n,
+                                                         int64_t baseD, const uint32_t* __restrict__ key,
+                                                         const int32_t* __restrict__ pnt, Levels L,
+                                                         const int32_t* __restrict__ parent,
+                                                         const float* __restrict__ x, const float* __restrict__ bf,
+                                                         int D, float* __restrict__ pvals,
+                                                         double* __restrict__ partial) {
     __shared__ double lds[kBlock / 64];
     const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     double v = 0.0;
     if (r < n) {
-        const float f = implicit_value(baseD + point_leaf[r], D, px[r], py[r], pz[r], key, pnt, L, parent, x, bf, D);
+        // points are in leaf order, so a wave's lanes share their coarse ancestors
+        const float f = leaf_chain_value(baseD + point_leaf[r], px[r], py[r], pz[r], key, pnt, L, parent, x, bf, D);
         pvals[r] = f;
         v = double(f);
     }
@@ -159,10 +164,12 @@ __global__ void __launch_bounds__(kBlock) k_corner_basis(const float* bf, int D,
 // f(corner) - iso for the owned corners (computeVertexImplicitFunctionValue,
 // main.cu:2259-2326): the chain sum of implicit_value at the corner's position,
 // with the basis values read from the corner basis table
-__global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* vlist, int64_t V, const uint32_t* key,
-                                                         const int32_t* pnt, Levels L, const int32_t* parent,
-                                                         const float* x, const float4* tab, int64_t baseD, int D,
-                                                         float iso, float* vval) {
+__global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* __restrict__ vlist, int64_t V,
+                                                         const uint32_t* __restrict__ key,
+                                                         const int32_t* __restrict__ pnt, Levels L,
+                                                         const int32_t* __restrict__ parent,
+                                                         const float* __restrict__ x, const float4* __restrict__ tab,
+                                                         int64_t baseD, int D, float iso, float* __restrict__ vval) {
     const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (v >= V) return;
     const int64_t g = vlist[v];
@@ -174,13 +181,28 @@ __global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* vlist, i
     const int64_t rx = 2 * int64_t(compact3(k >> 2)) + (j & 1);
     const int64_t ry = 2 * int64_t(compact3(k >> 1)) + ((j >> 1) & 1);
     const int64_t rz = 2 * int64_t(compact3(k)) + ((j >> 2) & 1);
+    // corners are listed in leaf order: once a wave's lanes share their
+    // ancestor, the rest of the chain runs on one wave-uniform node (rows and
+    // coefficients through the scalar cache); values and order are unchanged
     float val = 0.0f;
-    for (int d = D; d > 0; d--) {
+    int d = D;
+    for (; d > 0; d--) {
+        if (wave_uniform(int32_t(nd))) break;
         const int64_t o = 2 * G * d;
         const float4 tx = tab[o + rx], ty = tab[o + ry], tz = tab[o + rz];
         const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
         val = level_sum(nd - L.base[d], d, bx, by, bz, pnt, L, x, val);
         nd = parent[nd];
+    }
+    if (d > 0) {
+        int32_t u = __builtin_amdgcn_readfirstlane(int32_t(nd));
+        for (; d > 0; d--) {
+            const int64_t o = 2 * G * d;
+            const float4 tx = tab[o + rx], ty = tab[o + ry], tz = tab[o + rz];
+            const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
+            val = level_sum(int64_t(u) - L.base[d], d, bx, by, bz, pnt, L, x, val);
+            u = __builtin_amdgcn_readfirstlane(parent[u]);
+        }
     }
     float tt = x[0] * tab[rx].y;
     tt = tt * tab[ry].y;

@@ -30,17 +30,23 @@ __device__ __forceinline__ uint32_t compact3(uint32_t x) {
 // value() of baseFunctions[idx] (ConfirmedPPolynomial<3,4>, ConfirmedPPolynomial.cuh:78-91)
 __device__ __forceinline__ float basis_value(const float* __restrict__ bf, int64_t idx, float x) {
     const float* f = bf + idx * 20;
+    // the pieces up to the first start not below x are added, in order; all
+    // 20 coefficients load together and a select replaces the early exit
+    float c[20];
+#pragma unroll
+    for (int i = 0; i < 20; i++) c[i] = f[i];
     float res = 0.0f;
+    bool on = true;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        if (!(x > f[5 * i])) break;
+        on = on && (x > c[5 * i]);
         float temp = 1.0f, v = 0.0f;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            v = v + temp * f[5 * i + 1 + j];
+            v = v + temp * c[5 * i + 1 + j];
             temp = temp * x;
         }
-        res = res + v;
+        res = on ? res + v : res;
     }
     return res;
 }
@@ -77,13 +83,17 @@ __device__ __forceinline__ float level_sum(int64_t t, int d, const float bx[3], 
     const float* xd = x + L.base[d];
     int32_t nb[27];
     pnt_neighbours(row, int(t & 7), nb);
+    // all 27 loads issued before the sums (a missing neighbour loads the
+    // level's first value and its term is not added)
+    float xv[27];
+#pragma unroll
+    for (int j = 0; j < 27; j++) xv[j] = xd[nb[j] < 0 ? 0 : nb[j]];
 #pragma unroll
     for (int j = 0; j < 27; j++) {
-        if (nb[j] < 0) continue;
-        float tt = xd[nb[j]] * bx[j / 9];
+        float tt = xv[j] * bx[j / 9];
         tt = tt * by[(j / 3) % 3];
         tt = tt * bz[j % 3];
-        val = val + tt;
+        val = nb[j] < 0 ? val : val + tt;
     }
     return val;
 }
@@ -151,6 +161,40 @@ __device__ __forceinline__ float chain_value(int64_t nd, int d, float px, float 
         val = level_contrib(nd, d, px, py, pz, key, pnt, L, x, bf, D, val);
         nd = parent[nd];
         d--;
+    }
+    float tt = x[0] * basis_value(bf, 0, px);
+    tt = tt * basis_value(bf, 0, py);
+    tt = tt * basis_value(bf, 0, pz);
+    return val + tt;
+}
+
+// True when every active lane of the wave holds the same v.
+__device__ __forceinline__ bool wave_uniform(int32_t v) {
+    return __builtin_amdgcn_ballot_w64(v != __builtin_amdgcn_readfirstlane(v)) == 0;
+}
+
+// chain_value from a leaf-level node, for waves whose lanes all start at depth
+// D on nodes in Morton order.  Such lanes soon share their ancestor; from that
+// depth on the chain runs on one wave-uniform node index, so its neighbour
+// rows, coefficients and parents come through the scalar cache once per wave
+// instead of once per lane.  Values and summation order are chain_value's.
+__device__ __forceinline__ float leaf_chain_value(int64_t nd, float px, float py, float pz,
+                                                  const uint32_t* __restrict__ key, const int32_t* __restrict__ pnt,
+                                                  const Levels& L, const int32_t* __restrict__ parent,
+                                                  const float* __restrict__ x, const float* __restrict__ bf, int D) {
+    float val = 0.0f;
+    int d = D;
+    for (; d > 0; d--) {
+        if (wave_uniform(int32_t(nd))) break;
+        val = level_contrib(nd, d, px, py, pz, key, pnt, L, x, bf, D, val);
+        nd = parent[nd];
+    }
+    if (d > 0) {
+        int32_t u = __builtin_amdgcn_readfirstlane(int32_t(nd));
+        for (; d > 0; d--) {
+            val = level_contrib(u, d, px, py, pz, key, pnt, L, x, bf, D, val);
+            u = __builtin_amdgcn_readfirstlane(parent[u]);
+        }
     }
     float tt = x[0] * basis_value(bf, 0, px);
     tt = tt * basis_value(bf, 0, py);
