@@ -283,7 +283,8 @@ def main():
     for name in ["stencil_spmv_dot_finest", "stencil_spmv_dot", "cg_update", "cg_idle", "cg_true_residual", "cg_refresh", "cg_small", "cg_init", "vector_field",
                  "div_subtree_sums", "div_subtree_finish", "div_from_subtree_sums", "div_leaf", "parent_neighbor_table",
                  "radix_downsweep", "radix_upsweep", "point_values", "corner_value", "neighbors", "levels_fill",
-                 "edge_owner", "corner_owner", "scan_exclusive", "gather_points", "morton"]:
+                 "edge_owner", "corner_owner", "scan_exclusive", "gather_points", "morton", "point_weights",
+                 "spmv_tile_plan", "div_parent_sums"]:
         try:
             cnt, ms, bpl = gt.kernel_stats(name)
             kern[name] = {"launches": cnt, "total_ms": round(ms, 4), "avg_us": round(1000 * ms / cnt, 2),

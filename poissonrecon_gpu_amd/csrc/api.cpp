@@ -214,9 +214,10 @@ prg_status prg_build_octree(prg_ctx* c) {
         c->tree_gen++;
         prg::start_plans(c);
         prg::start_topology(c);
+        // divergence and solution read as zeros until computed (zeroed when
+        // read; the divergence stage writes every node and the solve starts
+        // from zero itself)
         c->have_div = c->have_x = c->have_iso = false;
-        PRG_HIP(hipMemsetAsync(c->x.reserve(c->M), 0, size_t(c->M) * 4, c->stream));
-        PRG_HIP(hipMemsetAsync(c->div.reserve(c->M), 0, size_t(c->M) * 4, c->stream));
         return PRG_OK;  // the later stages are ordered on the context's stream
     });
 }
@@ -462,8 +463,14 @@ prg_status prg_read_buffer(prg_ctx* c, int32_t which, void* host, int64_t bytes)
                 break;
             }
             case PRG_BUF_VECTOR_FIELD: read_xyz(c, c->vfx.get(), c->vfy.get(), c->vfz.get(), c->MD(), (float*)host); break;
-            case PRG_BUF_DIVERGENCE: d2h(c, host, c->div.get(), size_t(b)); break;
-            case PRG_BUF_SOLUTION: d2h(c, host, c->x.get(), size_t(b)); break;
+            case PRG_BUF_DIVERGENCE:
+                if (!c->have_div) PRG_HIP(hipMemsetAsync(c->div.reserve(M), 0, size_t(b), c->stream));
+                d2h(c, host, c->div.get(), size_t(b));
+                break;
+            case PRG_BUF_SOLUTION:
+                if (!c->have_x) PRG_HIP(hipMemsetAsync(c->x.reserve(M), 0, size_t(b), c->stream));
+                d2h(c, host, c->x.get(), size_t(b));
+                break;
             case PRG_BUF_POINT_VALUES: d2h(c, host, c->pvals.get(), size_t(b)); break;
             case PRG_BUF_LEAF_CORNER_VALUES: d2h(c, host, c->corner_vals.get(), size_t(b)); break;
             default: return fail(c, PRG_E_ARG, "unknown buffer");

@@ -129,6 +129,12 @@ void ensure_neighbours(prg_ctx* c);
 // its build (they overlap the vector field and divergence); the solve waits
 // for them
 void start_plans(prg_ctx* c);
+// waits (on the host) for the plans of the current octree
+void ensure_plans(prg_ctx* c);
+// the plan of depth d for work on stream s (which waits for the plans on the
+// device); false where the depth has none (kSmallRows rows or fewer)
+struct TilePlan;
+bool tile_plan_on(prg_ctx* c, int d, TilePlan* P, hipStream_t s);
 // marching-cubes topology of the new octree, on the second stream after the
 // plans (it depends on the tree only)
 void start_topology(prg_ctx* c);

@@ -22,12 +22,16 @@
 
 #include "ctx.h"
 #include "pnt.h"
+#include "tile.h"
 
 namespace prg {
 namespace {
 
 constexpr int kBlock = 256;
 constexpr int kChunk = 2048;  // terms per wave-chunk
+// listed blocks per tile the LDS-staged leaf kernels hold (a larger tile takes
+// their per-row path)
+constexpr int kTileBlocks = 320;
 
 struct BaseMD {
     float v[16];  // 4 pieces x (start, c0, c1, c2)
@@ -122,31 +126,185 @@ __global__ void __launch_bounds__(kBlock) k_point_weights(const uint32_t* key, c
 // w * n_q, w = wx * wy * wz (computeVectorField, main.cu:913-964).  Slot t
 // sits at offset -delta_j from neighbour j's leaf, so its weights are the
 // point's W entries k = 1 - delta_j.
+__device__ __forceinline__ void vector_field_row(const int32_t* __restrict__ pntD, const int32_t* __restrict__ pidx,
+                                                 const int32_t* __restrict__ pnum, const float* __restrict__ W,
+                                                 const float* __restrict__ nx, const float* __restrict__ ny,
+                                                 const float* __restrict__ nz, int64_t baseD, int64_t t,
+                                                 float* __restrict__ vfx, float* __restrict__ vfy,
+                                                 float* __restrict__ vfz) {
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    int32_t nbr[27];
+    pnt_neighbours(pntD + 27 * (t >> 3), int(t & 7), nbr);
+    // the 27 point ranges load together (a missing neighbour has none)
+    int32_t p0[27], pe[27];
+#pragma unroll
+    for (int j = 0; j < 27; j++) {
+        const int64_t nb = baseD + (nbr[j] < 0 ? 0 : nbr[j]);
+        p0[j] = pidx[nb];
+        pe[j] = p0[j] + (nbr[j] < 0 ? 0 : pnum[nb]);
+    }
+    auto add = [&](float fx, float fy, float fz, float ux, float uy, float uz) {
+        float w = fx * fy;
+        w = w * fz;
+        vx = vx + w * ux;
+        vy = vy + w * uy;
+        vz = vz + w * uz;
+    };
+#pragma unroll
+    for (int j = 0; j < 27; j++) {
+        const int kx = 2 - j / 9, ky = 2 - (j / 3) % 3, kz = 2 - j % 3;  // 1 - delta
+        int32_t q = p0[j];
+        // two points per round trip, added in order
+        for (; q + 1 < pe[j]; q += 2) {
+            const float* wa = W + 9 * int64_t(q);
+            const float* wb = wa + 9;
+            const float ax = wa[kx], ay = wa[3 + ky], az = wa[6 + kz];
+            const float bx = wb[kx], by = wb[3 + ky], bz = wb[6 + kz];
+            const float nax = nx[q], nay = ny[q], naz = nz[q];
+            const float nbx = nx[q + 1], nby = ny[q + 1], nbz = nz[q + 1];
+            add(ax, ay, az, nax, nay, naz);
+            add(bx, by, bz, nbx, nby, nbz);
+        }
+        if (q < pe[j]) {
+            const float* wa = W + 9 * int64_t(q);
+            add(wa[kx], wa[3 + ky], wa[6 + kz], nx[q], ny[q], nz[q]);
+        }
+    }
+    vfx[t] = 0.0f + vx;
+    vfy[t] = 0.0f + vy;
+    vfz[t] = 0.0f + vz;
+}
+
 __global__ void __launch_bounds__(kBlock) k_vector_field(const int32_t* pntD, const int32_t* pidx,
                                                          const int32_t* pnum, const float* W, const float* nx,
                                                          const float* ny, const float* nz, int64_t baseD, int64_t MD,
                                                          float* vfx, float* vfy, float* vfz) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
-    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-    int32_t nbr[27];
-    pnt_neighbours(pntD + 27 * (t >> 3), int(t & 7), nbr);
+    vector_field_row(pntD, pidx, pnum, W, nx, ny, nz, baseD, t, vfx, vfy, vfz);
+}
+
+// k_vector_field through the tile plan.  Per tile: the point ranges of the
+// listed blocks' cells (first point, and their offsets in LDS by a scan), then
+// the points themselves (9 weights and the normal, one array per component)
+// staged in LDS once; every node then walks its 27 neighbours and their
+// points in the same order as k_vector_field.  A tile with more than
+// kVfPoints points reads them from global memory instead, and a tile with
+// more than kTileBlocks listed blocks takes the per-row path (the choices are
+// per workgroup).
+constexpr int kVfPoints = 768;
+
+__global__ void __launch_bounds__(kTileThreads) k_vector_field_tiles(TilePlan P, int64_t nblk,
+                                                                     const int32_t* __restrict__ pntD,
+                                                                     const int32_t* __restrict__ pidx,
+                                                                     const int32_t* __restrict__ pnum,
+                                                                     const float* __restrict__ W,
+                                                                     const float* __restrict__ nx,
+                                                                     const float* __restrict__ ny,
+                                                                     const float* __restrict__ nz, int64_t baseD,
+                                                                     float* __restrict__ vfx, float* __restrict__ vfy,
+                                                                     float* __restrict__ vfz) {
+    __shared__ int32_t cp0[8 * kTileBlocks];       // first point of each listed cell
+    __shared__ int32_t coff[8 * kTileBlocks + 1];  // LDS offsets of the cells' points
+    __shared__ float pw[12 * kVfPoints];           // 9 weights and the normal, one array per component
+    __shared__ int16_t rid[kTileEntries];
+    __shared__ int32_t wtot[kTileThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t tile = blockIdx.x;
+    const int32_t ncell = 8 * P.cnt[tile];
+    if (ncell > 8 * kTileBlocks) {
+        const int64_t t = tile * kTileThreads + tid;
+        if (t < 8 * nblk) vector_field_row(pntD, pidx, pnum, W, nx, ny, nz, baseD, t, vfx, vfy, vfz);
+        return;
+    }
+    const int16_t* rw = P.rw + tile * kTileEntries;
+    const int32_t* ub = P.ub + tile * kTileEntries;
+    for (int e = tid; e < kTileEntries; e += kTileThreads) rid[e] = rw[e];
+    // cells [c0, c1) of this thread: first points and counts, then an
+    // exclusive scan of the counts over the tile
+    const int per = (ncell + kTileThreads - 1) / kTileThreads;
+    const int c0 = min(ncell, tid * per), c1 = min(ncell, c0 + per);
+    int32_t own = 0;
+    for (int c = c0; c < c1; c++) {
+        const int64_t gi = baseD + ub[c >> 3] + (c & 7);
+        cp0[c] = pidx[gi];
+        own += pnum[gi];
+    }
+    int32_t inc = own;
 #pragma unroll
-    for (int j = 0; j < 27; j++) {
-        if (nbr[j] < 0) continue;
-        const int64_t nb = baseD + nbr[j];
-        const int32_t p0 = pidx[nb], pn = pnum[nb];
-        const int kx = 2 - j / 9, ky = 2 - (j / 3) % 3, kz = 2 - j % 3;  // 1 - delta
-        for (int32_t q = p0; q < p0 + pn; q++) {
-            const float* wq = W + 9 * int64_t(q);
-            const float fx = wq[kx], fy = wq[3 + ky], fz = wq[6 + kz];
-            float w = fx * fy;
-            w = w * fz;
-            vx = vx + w * nx[q];
-            vy = vy + w * ny[q];
-            vz = vz + w * nz[q];
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+    }
+    if (lane == 63) wtot[wave] = inc;
+    __syncthreads();
+    int32_t base = 0;
+    for (int w = 0; w < wave; w++) base += wtot[w];
+    int32_t run = base + inc - own;
+    for (int c = c0; c < c1; c++) {
+        coff[c] = run;
+        run += pnum[baseD + ub[c >> 3] + (c & 7)];
+    }
+    if (tid == kTileThreads - 1) coff[ncell] = run;
+    __syncthreads();
+    const int32_t total = coff[ncell];
+    const bool staged = total <= kVfPoints;
+    if (staged) {
+        for (int32_t i = tid; i < total; i += kTileThreads) {
+            int lo = 0, hi = ncell;  // the last cell whose offset is <= i
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (coff[mid] <= i)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const int64_t q = int64_t(cp0[lo]) + (i - coff[lo]);
+#pragma unroll
+            for (int k = 0; k < 9; k++) pw[k * kVfPoints + i] = W[9 * q + k];
+            pw[9 * kVfPoints + i] = nx[q];
+            pw[10 * kVfPoints + i] = ny[q];
+            pw[11 * kVfPoints + i] = nz[q];
         }
     }
+    __syncthreads();
+    const int g = tid >> 3, s = tid & 7;
+    const int64_t blk = tile * kTile + g;
+    if (blk >= nblk) return;
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 27; j++) {
+        int entry, child;
+        tile_neighbour(s, j, entry, child);
+        const int id = rid[27 * g + entry];
+        if (id < 0) continue;
+        const int cell = 8 * id + child;
+        const int32_t o = coff[cell], e = coff[cell + 1];
+        const int kx = 2 - j / 9, ky = 2 - (j / 3) % 3, kz = 2 - j % 3;  // 1 - delta
+        if (staged) {
+            for (int32_t q = o; q < e; q++) {
+                const float fx = pw[kx * kVfPoints + q], fy = pw[(3 + ky) * kVfPoints + q],
+                            fz = pw[(6 + kz) * kVfPoints + q];
+                float w = fx * fy;
+                w = w * fz;
+                vx = vx + w * pw[9 * kVfPoints + q];
+                vy = vy + w * pw[10 * kVfPoints + q];
+                vz = vz + w * pw[11 * kVfPoints + q];
+            }
+        } else {
+            const int64_t p0 = cp0[cell];
+            for (int64_t q = p0; q < p0 + (e - o); q++) {
+                const float* wq = W + 9 * q;
+                const float fx = wq[kx], fy = wq[3 + ky], fz = wq[6 + kz];
+                float w = fx * fy;
+                w = w * fz;
+                vx = vx + w * nx[q];
+                vy = vy + w * ny[q];
+                vz = vz + w * nz[q];
+            }
+        }
+    }
+    const int64_t t = 8 * blk + s;
     vfx[t] = 0.0f + vx;
     vfy[t] = 0.0f + vy;
     vfz[t] = 0.0f + vz;
@@ -163,22 +321,86 @@ __device__ __forceinline__ float fdot(float vx, float vy, float vz, float ux, fl
 
 // leaf level: 27 terms per node, the neighbours from the leaf level's
 // parent-neighbour rows
+__device__ __forceinline__ void div_leaf_row(const int32_t* __restrict__ pntD, const float* __restrict__ vfx,
+                                             const float* __restrict__ vfy, const float* __restrict__ vfz,
+                                             const float* __restrict__ T, int64_t baseD, int64_t t,
+                                             float* __restrict__ div) {
+    int32_t nb[27];
+    pnt_neighbours(pntD + 27 * (t >> 3), int(t & 7), nb);
+    // all loads issued before the sum; a missing neighbour reads slot 0 and
+    // its term is not added
+    float ux[27], uy[27], uz[27];
+#pragma unroll
+    for (int j = 0; j < 27; j++) {
+        const int64_t l = nb[j] < 0 ? 0 : nb[j];
+        ux[j] = vfx[l];
+        uy[j] = vfy[l];
+        uz[j] = vfz[l];
+    }
+    double val = 0.0;
+#pragma unroll
+    for (int j = 0; j < 27; j++) {
+        const float r = fdot(ux[j], uy[j], uz[j], T[j / 9], T[(j / 3) % 3], T[j % 3]);
+        val = nb[j] < 0 ? val : val + double(r);
+    }
+    div[baseD + t] = float(val);
+}
+
 __global__ void __launch_bounds__(kBlock) k_div_leaf(const int32_t* pntD, const float* vfx, const float* vfy,
                                                      const float* vfz, const float* T, int64_t baseD, int64_t MD,
                                                      float* div) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
-    int32_t nb[27];
-    pnt_neighbours(pntD + 27 * (t >> 3), int(t & 7), nb);
+    div_leaf_row(pntD, vfx, vfy, vfz, T, baseD, t, div);
+}
+
+// The leaf level through the tile plan: a tile's distinct neighbour blocks
+// are loaded once into LDS (three components), then each node's 27 terms are
+// summed from LDS in the order j = 0..26 (the terms and their order are
+// k_div_leaf's).  A tile with more than kTileBlocks listed blocks takes the
+// per-row path.
+__global__ void __launch_bounds__(kTileThreads) k_div_leaf_tiles(TilePlan P, int64_t nblk,
+                                                                 const int32_t* __restrict__ pntD,
+                                                                 const float* __restrict__ vfx,
+                                                                 const float* __restrict__ vfy,
+                                                                 const float* __restrict__ vfz,
+                                                                 const float* __restrict__ T, int64_t baseD,
+                                                                 float* __restrict__ div) {
+    constexpr int stride = 8 * kTileBlocks;
+    __shared__ float cells[3 * stride];
+    __shared__ int16_t rid[kTileEntries];
+    const int tid = threadIdx.x;
+    const int64_t tile = blockIdx.x;
+    const int32_t cnt = P.cnt[tile];
+    if (cnt > kTileBlocks) {
+        const int64_t t = tile * kTileThreads + tid;
+        if (t < 8 * nblk) div_leaf_row(pntD, vfx, vfy, vfz, T, baseD, t, div);
+        return;
+    }
+    const int16_t* rw = P.rw + tile * kTileEntries;
+    const int32_t* ub = P.ub + tile * kTileEntries;
+    for (int e = tid; e < kTileEntries; e += kTileThreads) rid[e] = rw[e];
+    for (int u = tid; u < 8 * cnt; u += kTileThreads) {
+        const int64_t idx = int64_t(ub[u >> 3]) + (u & 7);
+        cells[u] = vfx[idx];
+        cells[stride + u] = vfy[idx];
+        cells[2 * stride + u] = vfz[idx];
+    }
+    __syncthreads();
+    const int g = tid >> 3, s = tid & 7;
+    const int64_t blk = tile * kTile + g;
+    if (blk >= nblk) return;
     double val = 0.0;
 #pragma unroll
     for (int j = 0; j < 27; j++) {
-        if (nb[j] < 0) continue;
-        const int64_t l = nb[j];
-        const float r = fdot(vfx[l], vfy[l], vfz[l], T[j / 9], T[(j / 3) % 3], T[j % 3]);
-        val += double(r);
+        int entry, child;
+        tile_neighbour(s, j, entry, child);
+        const int id = rid[27 * g + entry];
+        const int l = id < 0 ? 0 : 8 * id + child;
+        const float r = fdot(cells[l], cells[stride + l], cells[2 * stride + l], T[j / 9], T[(j / 3) % 3], T[j % 3]);
+        val = id < 0 ? val : val + double(r);
     }
-    div[baseD + t] = float(val);
+    div[baseD + 8 * blk + s] = float(val);
 }
 
 // The 27 terms fdot(v, (wx[j / 9], wy[(j / 3) % 3], wz[j % 3])) from the 9
@@ -435,13 +657,18 @@ __global__ void __launch_bounds__(kBlock) k_div_from_g(const int32_t* pnt, const
         const int64_t t = i - B.b[d];
         int32_t nbr[27];
         pnt_neighbours(pnt + 27 * (pnt_off[d] + (t >> 3)), int(t & 7), nbr);
+        // loads issued together; G of a node without leaf-level slots is
+        // never written, and its value is read but not added
+        int32_t dn[27];
+        double g[27];
 #pragma unroll
         for (int j = 0; j < 27; j++) {
-            if (nbr[j] < 0) continue;
-            const int64_t nb = B.b[d] + nbr[j];
-            if (dnum[nb] == 0) continue;
-            val += G[j * gstride + nb];
+            const int64_t nb = B.b[d] + (nbr[j] < 0 ? 0 : nbr[j]);
+            dn[j] = dnum[nb];
+            g[j] = G[j * gstride + nb];
         }
+#pragma unroll
+        for (int j = 0; j < 27; j++) val = (nbr[j] < 0 || dn[j] == 0) ? val : val + g[j];
     }
     div[i] = float(val);
 }
@@ -497,9 +724,17 @@ void compute_divergence(prg_ctx* c) {
             PRG_CHECK_LAUNCH();
         }
         TimedScope ts(c->kt, "vector_field", 12.0 * double(MD) + 116.0 * double(MD) + 24.0 * double(c->N));
-        k_vector_field<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->pnt.get() + 27 * c->pnt_off[D], c->pidx.get(),
-                                                               c->pnum.get(), W.get(), c->nx.get(), c->ny.get(),
-                                                               c->nz.get(), baseD, MD, VX, VY, VZ);
+        TilePlan P{};
+        if (tile_plan_on(c, D, &P, s)) {
+            const int64_t nblk = MD / 8, ntiles = (nblk + kTile - 1) / kTile;
+            k_vector_field_tiles<<<unsigned(ntiles), kTileThreads, 0, s>>>(
+                P, nblk, c->pnt.get() + 27 * c->pnt_off[D], c->pidx.get(), c->pnum.get(), W.get(), c->nx.get(),
+                c->ny.get(), c->nz.get(), baseD, VX, VY, VZ);
+        } else {
+            k_vector_field<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->pnt.get() + 27 * c->pnt_off[D], c->pidx.get(),
+                                                                   c->pnum.get(), W.get(), c->nx.get(), c->ny.get(),
+                                                                   c->nz.get(), baseD, MD, VX, VY, VZ);
+        }
         PRG_CHECK_LAUNCH();
     }
     StageTimer st(c, PRG_STAGE_DIVERGENCE);
@@ -509,8 +744,16 @@ void compute_divergence(prg_ctx* c) {
                            hipMemcpyHostToDevice, s));
     {
         TimedScope ts(c->kt, "div_leaf", 124.0 * double(MD));
-        k_div_leaf<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->pnt.get() + 27 * c->pnt_off[D], VX, VY, VZ,
-                                                           c->d_div_table.get() + c->fs->div_off[D], baseD, MD, DV);
+        TilePlan P{};
+        const float* T = c->d_div_table.get() + c->fs->div_off[D];
+        if (tile_plan_on(c, D, &P, s)) {
+            const int64_t nblk = MD / 8, ntiles = (nblk + kTile - 1) / kTile;
+            k_div_leaf_tiles<<<unsigned(ntiles), kTileThreads, 0, s>>>(P, nblk, c->pnt.get() + 27 * c->pnt_off[D], VX,
+                                                                       VY, VZ, T, baseD, DV);
+        } else {
+            k_div_leaf<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->pnt.get() + 27 * c->pnt_off[D], VX, VY, VZ, T,
+                                                               baseD, MD, DV);
+        }
         PRG_CHECK_LAUNCH();
     }
     const int64_t n = baseD;  // nodes of depths 0 .. D-1

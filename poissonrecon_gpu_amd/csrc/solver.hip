@@ -32,6 +32,7 @@
 #include "../../include/prg_unet.h"
 #include "ctx.h"
 #include "pnt.h"
+#include "tile.h"
 
 namespace prg {
 namespace {
@@ -92,12 +93,16 @@ __device__ __forceinline__ float spmv_pnt(const int32_t* __restrict__ pnt, int64
                                           const Stencil& S) {
     int32_t nb[27];
     pnt_neighbours(pnt + 27 * (t >> 3), int(t & 7), nb);
+    // the 27 loads go out together (a missing neighbour reads row 0); terms
+    // of missing neighbours or zero coefficients are not added
+    float vv[27];
+#pragma unroll
+    for (int j = 0; j < 27; j++) vv[j] = v[nb[j] < 0 ? 0 : nb[j]];
     float out = 0.0f;
 #pragma unroll
     for (int j = 0; j < 27; j++) {
-        if (nb[j] < 0 || S.L[j] == 0.0f) continue;
-        const float prod = S.L[j] * v[nb[j]];
-        out = out + prod;
+        const float prod = S.L[j] * vv[j];
+        out = (nb[j] < 0 || S.L[j] == 0.0f) ? out : out + prod;
     }
     return out;
 }
@@ -153,9 +158,7 @@ __device__ __forceinline__ bool publish(double blk, double* partial, CgState* st
 constexpr int kVecThreads = 512;  // init / update / refresh / true residual
 constexpr int kMaxWG = 1024;       // workgroups of a depth in the vector kernels (its partial count)
 constexpr int kMaxSpmvWG = 2048;   // workgroups of a depth in the SpMV (its partial count)
-constexpr int kSpmvThreads = 512;
-constexpr int kTile = kSpmvThreads / 8;   // sibling blocks per SpMV tile, one row per lane
-constexpr int kTileEntries = 27 * kTile;  // parent-neighbour entries of a tile
+constexpr int kSpmvThreads = kTileThreads;  // one row per lane, kTile sibling blocks per tile
 constexpr int kHashSlots = 2048;          // > kTileEntries: linear probing always finds a slot
 constexpr int kGridRow = 72;              // floats per block grid row in the SpMV (64 cells + bank padding)
 static_assert(kHashSlots > kTileEntries, "hash must hold every entry of a tile");
@@ -519,12 +522,14 @@ __device__ __forceinline__ float spmv_small(const int32_t* __restrict__ pnt, int
     if (!pnt) return (L[13] == 0.0f) ? 0.0f : 0.0f + L[13] * v[0];
     int32_t nb[27];
     pnt_neighbours(pnt + 27 * (t >> 3), int(t & 7), nb);
+    float vv[27];
+#pragma unroll
+    for (int j = 0; j < 27; j++) vv[j] = v[nb[j] < 0 ? 0 : nb[j]];
     float out = 0.0f;
 #pragma unroll
     for (int j = 0; j < 27; j++) {
-        if (nb[j] < 0 || L[j] == 0.0f) continue;
-        const float prod = L[j] * v[nb[j]];
-        out = out + prod;
+        const float prod = L[j] * vv[j];
+        out = (nb[j] < 0 || L[j] == 0.0f) ? out : out + prod;
     }
     return out;
 }
@@ -628,6 +633,7 @@ struct Chain {
     std::vector<std::pair<int, int>> launched;
 };
 
+constexpr int kFirstBatch = 24;      // iterations queued before a chain's first state read-back
 constexpr int kMaxSegs = 12;         // depths per solve above kSmallRows rows (D <= 10 has at most 10)
 constexpr size_t kHostStates = 1024;  // pinned read-back area: states first, then the segment table
 constexpr size_t kHostPlanMax = 3840;  // then the plans' per-depth largest tile counts (kMaxDepth + 1 int32)
@@ -698,9 +704,7 @@ void start_plans(prg_ctx* c) {
     c->plan_gen = c->tree_gen;
 }
 
-namespace {
-
-// the solve's plans: started after the octree build, finished here
+// plans started after the octree build, finished here (host wait)
 void ensure_plans(prg_ctx* c) {
     if (c->plan_gen != c->tree_gen) start_plans(c);
     if (c->plan_pending) {
@@ -710,7 +714,17 @@ void ensure_plans(prg_ctx* c) {
     }
 }
 
-}  // namespace
+bool tile_plan_on(prg_ctx* c, int d, TilePlan* P, hipStream_t s) {
+    if (d < 0 || d > c->D || c->counts[d] <= kSmallRows) return false;
+    if (c->plan_gen != c->tree_gen) start_plans(c);
+    const int64_t t0 = c->plan_tile_off[d];
+    if (c->plan_tile_off[d + 1] == t0) return false;
+    if (c->plan_pending) PRG_HIP(hipStreamWaitEvent(s, c->plan_ev, 0));
+    P->cnt = c->plan_count.get() + t0;
+    P->ub = c->plan_ubase.get() + t0 * kTileEntries;
+    P->rw = c->plan_rowid.get() + t0 * kTileEntries;
+    return true;
+}
 
 prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     ensure_tables(c);
@@ -911,46 +925,52 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                 PRG_CHECK_LAUNCH();
             }
             h.issued = 0;
-            h.batch = 24;
+            h.batch = kFirstBatch;
         };
-        auto issue_batch = [&](Chain& h) {
+        // iteration `it` (1-based) of a chain: SpMV, update, and the residual
+        // refresh when due
+        auto launch_iter = [&](Chain& h, int it) {
             const int32_t cnt = int32_t(h.depths.size());
             int64_t rows = 0;
             for (int d : h.depths) rows += c->counts[d];
             const char* spmv_name = h.depths.back() == D ? "stencil_spmv_dot_finest" : "stencil_spmv_dot";
-            h.launched.clear();
-            for (int k = 0; k < h.batch; k++) {
-                const int it = h.issued + k + 1;  // iteration number, 1-based
-                {
-                    // algorithmic bytes of one SpMV row in the reference's representation:
-                    // 27 int32 neighbour indices, p_i read, (Ap)_i written (116 B)
-                    TimedScope ts(c->kt, spmv_name, kSpmvRowBytes * double(rows));
+            {
+                // algorithmic bytes of one SpMV row in the reference's representation:
+                // 27 int32 neighbour indices, p_i read, (Ap)_i written (116 B)
+                TimedScope ts(c->kt, spmv_name, kSpmvRowBytes * double(rows));
+                h.launched.emplace_back(ts.idx, it);
+                launch_spmv(h.inst, h.nwg, h.lds, h.s, dseg + h.seg0, cnt, it, tol, fixed, max_iters);
+            }
+            {
+                TimedScope ts(c->kt, "cg_update", 24.0 * double(rows));
+                h.launched.emplace_back(ts.idx, it);
+                k_cg_update<<<h.nvwg, kVecThreads, 0, h.s>>>(dseg + h.seg0, cnt, it, refresh);
+                PRG_CHECK_LAUNCH();
+            }
+            if ((it % refresh) == 0) {
+                for (int i = 0; i < cnt; i++) {
+                    const int d = h.depths[i];
+                    const Seg& g = hseg[h.seg0 + i];
+                    TimedScope ts(c->kt, "cg_refresh", kSpmvRowBytes * double(g.n));
                     h.launched.emplace_back(ts.idx, it);
-                    launch_spmv(h.inst, h.nwg, h.lds, h.s, dseg + h.seg0, cnt, it, tol, fixed, max_iters);
-                }
-                {
-                    TimedScope ts(c->kt, "cg_update", 24.0 * double(rows));
-                    h.launched.emplace_back(ts.idx, it);
-                    k_cg_update<<<h.nvwg, kVecThreads, 0, h.s>>>(dseg + h.seg0, cnt, it, refresh);
+                    k_cg_refresh<<<g.nvwg, kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], g.n,
+                                                                 stencil_of(d), c->div.get() + c->bases[d], g.XS,
+                                                                 g.R, g.rr + (it & 1) * kMaxWG, g.st);
                     PRG_CHECK_LAUNCH();
                 }
-                if ((it % refresh) == 0) {
-                    for (int i = 0; i < cnt; i++) {
-                        const int d = h.depths[i];
-                        const Seg& g = hseg[h.seg0 + i];
-                        TimedScope ts(c->kt, "cg_refresh", kSpmvRowBytes * double(g.n));
-                        h.launched.emplace_back(ts.idx, it);
-                        k_cg_refresh<<<g.nvwg, kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], g.n,
-                                                                     stencil_of(d), c->div.get() + c->bases[d], g.XS,
-                                                                     g.R, g.rr + (it & 1) * kMaxWG, g.st);
-                        PRG_CHECK_LAUNCH();
-                    }
-                }
             }
+        };
+        // the chain's states read back after its queued iterations
+        auto end_batch = [&](Chain& h) {
             h.issued += h.batch;
-            PRG_HIP(hipMemcpyAsync(h.hst, h.st, size_t(cnt) * sizeof(CgState), hipMemcpyDeviceToHost, h.s));
+            PRG_HIP(hipMemcpyAsync(h.hst, h.st, h.depths.size() * sizeof(CgState), hipMemcpyDeviceToHost, h.s));
             PRG_HIP(hipEventRecord(h.ev, h.s));
             h.waiting = true;
+        };
+        auto issue_batch = [&](Chain& h) {
+            h.launched.clear();
+            for (int k = 0; k < h.batch; k++) launch_iter(h, h.issued + k + 1);
+            end_batch(h);
         };
         auto end_chain = [&](Chain& h) {
             for (size_t i = 0; i < h.depths.size(); i++) {
@@ -983,8 +1003,18 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
         if (serial) {
             if (!serial_order.empty()) start_chain(serial_order[serial_next++]);
         } else {
+            // every chain's init, then the first batches one iteration at a
+            // time across the chains, so no chain waits for another's launches
             for (int k = 0; k < kChains; k++)
-                if (!ch[k].depths.empty()) start_chain(k);
+                if (!ch[k].depths.empty()) {
+                    begin_chain(ch[k]);
+                    ch[k].launched.clear();
+                }
+            for (int i = 0; i < kFirstBatch; i++)
+                for (int k = 0; k < kChains; k++)
+                    if (!ch[k].depths.empty()) launch_iter(ch[k], i + 1);
+            for (int k = 0; k < kChains; k++)
+                if (!ch[k].depths.empty()) end_batch(ch[k]);
         }
         while (true) {
             bool active = false;
