@@ -274,6 +274,7 @@ prg_status prg_solve(prg_ctx* c, prg_solve_stats* per_depth) {
         PRG_HIP(hipSetDevice(c->cfg.device));
         prg_status st = prg::solve_all(c, per_depth);
         c->have_x = true;
+        c->x_gen++;
         c->have_iso = c->iso_ready = false;
         if (st == PRG_E_NOT_CONVERGED) c->err = "CG reached its iteration cap";
         return st;
@@ -496,6 +497,7 @@ prg_status prg_write_buffer(prg_ctx* c, int32_t which, const void* host, int64_t
             c->have_div = true;
         } else {
             c->have_x = true;
+            c->x_gen++;
         }
         return PRG_OK;
     });

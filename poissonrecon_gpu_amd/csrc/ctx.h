@@ -71,6 +71,11 @@ struct prg_ctx {
     int64_t plan_tile_off[prg::kMaxDepth + 2] = {};
     int32_t plan_maxcount[prg::kMaxDepth + 1] = {};
     uint64_t tree_gen = 0, plan_gen = 0;
+    // per-node rows of the coarse depths (27 neighbour solution values and the
+    // parent), for the solution `x_gen` of octree `rows_tree`
+    prg::DBuf<float> coarse_rows;
+    int32_t rows_depth = 0;
+    uint64_t x_gen = 0, rows_x = ~uint64_t(0), rows_tree = ~uint64_t(0);
     prg::DBuf<int32_t> plan_mx;       // per-depth largest tile count (device)
     hipEvent_t plan_ev = nullptr;     // completion of the plans started after an octree build
     bool plan_pending = false;

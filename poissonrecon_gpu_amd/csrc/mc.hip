@@ -26,6 +26,31 @@ namespace {
 
 constexpr int kBlock = 256;
 
+// One coarse row per node of depths 1 .. Dr (the layout kRowFloats describes)
+__global__ void __launch_bounds__(kBlock) k_coarse_rows(const int32_t* __restrict__ pnt, Levels L,
+                                                        const int32_t* __restrict__ parent,
+                                                        const float* __restrict__ x, int Dr, int64_t n,
+                                                        float* __restrict__ rows) {
+    const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (r >= n) return;
+    const int64_t i = L.base[1] + r;
+    int d = 1;
+    while (d < Dr && i >= L.base[d + 1]) d++;
+    const int64_t t = i - L.base[d];
+    int32_t nb[27];
+    pnt_neighbours(pnt + 27 * (L.pnt_off[d] + (t >> 3)), int(t & 7), nb);
+    const float* xd = x + L.base[d];
+    float v[kRowFloats];
+#pragma unroll
+    for (int j = 0; j < 27; j++) v[j] = nb[j] < 0 ? 0.0f : xd[nb[j] < 0 ? 0 : nb[j]];
+    v[27] = __int_as_float(parent[i]);
+#pragma unroll
+    for (int j = 28; j < kRowFloats; j++) v[j] = 0.0f;
+    float4* o = reinterpret_cast<float4*>(rows + kRowFloats * r);
+#pragma unroll
+    for (int q = 0; q < kRowFloats / 4; q++) o[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+}
+
 __global__ void __launch_bounds__(kBlock) k_point_values(const float* __restrict__ px, const float* __restrict__ py,
                                                          const float* __restrict__ pz,
                                                          const int32_t* __restrict__ point_leaf, int64_t n,
@@ -33,14 +58,15 @@ __global__ void __launch_bounds__(kBlock) k_point_values(const float* __restrict
                                                          const int32_t* __restrict__ pnt, Levels L,
                                                          const int32_t* __restrict__ parent,
                                                          const float* __restrict__ x, const float* __restrict__ bf,
-                                                         int D, float* __restrict__ pvals,
-                                                         double* __restrict__ partial) {
+                                                         int D, const float* __restrict__ rows, int rows_depth,
+                                                         float* __restrict__ pvals, double* __restrict__ partial) {
     __shared__ double lds[kBlock / 64];
     const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     double v = 0.0;
     if (r < n) {
         // points are in leaf order, so a wave's lanes share their coarse ancestors
-        const float f = leaf_chain_value(baseD + point_leaf[r], px[r], py[r], pz[r], key, pnt, L, parent, x, bf, D);
+        const float f = leaf_chain_value(baseD + point_leaf[r], px[r], py[r], pz[r], key, pnt, L, parent, x, bf, D,
+                                         rows, rows_depth);
         pvals[r] = f;
         v = double(f);
     }
@@ -169,6 +195,7 @@ __global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* __restri
                                                          const int32_t* __restrict__ pnt, Levels L,
                                                          const int32_t* __restrict__ parent,
                                                          const float* __restrict__ x, const float4* __restrict__ tab,
+                                                         const float* __restrict__ rows, int rows_depth,
                                                          int64_t baseD, int D, float iso, float* __restrict__ vval) {
     const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (v >= V) return;
@@ -200,8 +227,15 @@ __global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* __restri
             const int64_t o = 2 * G * d;
             const float4 tx = tab[o + rx], ty = tab[o + ry], tz = tab[o + rz];
             const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
-            val = level_sum(int64_t(u) - L.base[d], d, bx, by, bz, pnt, L, x, val);
-            u = __builtin_amdgcn_readfirstlane(parent[u]);
+            if (d <= rows_depth) {
+                // one row per depth: neighbour values and the parent in one load
+                const float* R = rows + kRowFloats * (int64_t(u) - L.base[1]);
+                val = row_sum(R, bx, by, bz, val);
+                u = __builtin_amdgcn_readfirstlane(__float_as_int(R[27]));
+            } else {
+                val = level_sum(int64_t(u) - L.base[d], d, bx, by, bz, pnt, L, x, val);
+                u = __builtin_amdgcn_readfirstlane(parent[u]);
+            }
         }
     }
     float tt = x[0] * tab[rx].y;
@@ -381,6 +415,25 @@ __global__ void __launch_bounds__(kBlock) k_tri_emit(const float* cval, const in
 
 }  // namespace
 
+// Coarse rows of depths 1 .. D - 2 for the current solution (rebuilt when the
+// solution or the octree changes); returns the deepest depth they cover (0:
+// none).
+int ensure_coarse_rows(prg_ctx* c, hipStream_t s) {
+    const int D = c->D, Dr = D - 2;
+    if (Dr < 1) return 0;
+    if (c->rows_x == c->x_gen && c->rows_tree == c->tree_gen) return c->rows_depth;
+    const int64_t n = c->bases[Dr + 1] - c->bases[1];
+    float* rows = c->coarse_rows.reserve(size_t(kRowFloats) * std::max<int64_t>(1, n));
+    TimedScope ts(c->kt, "coarse_rows", 128.0 * double(n));
+    k_coarse_rows<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->pnt.get(), levels_of(c), c->parent.get(), c->x.get(),
+                                                         Dr, n, rows);
+    PRG_CHECK_LAUNCH();
+    c->rows_depth = Dr;
+    c->rows_x = c->x_gen;
+    c->rows_tree = c->tree_gen;
+    return Dr;
+}
+
 void compute_isovalue(prg_ctx* c) {
     ensure_tables(c);
     StageTimer st(c, PRG_STAGE_ISOVALUE);
@@ -392,10 +445,11 @@ void compute_isovalue(prg_ctx* c) {
     tot.reserve(1);
     {
         TimedScope ts(c->kt, "point_values", 24.0 * double(n));
+        const int rd = ensure_coarse_rows(c, s);
         k_point_values<<<nb, kBlock, 0, s>>>(c->px.get(), c->py.get(), c->pz.get(), c->point_leaf.get(), n,
                                              c->bases[c->D], c->key.get(), c->pnt.get(), levels_of(c),
                                              c->parent.get(), c->x.get(), c->d_base_fn.get(), c->D,
-                                             c->pvals.reserve(n), part.get());
+                                             c->coarse_rows.get(), rd, c->pvals.reserve(n), part.get());
         PRG_CHECK_LAUNCH();
     }
     k_sum_partials<<<1, 1024, 0, s>>>(part.get(), nb, tot.get());
@@ -479,9 +533,10 @@ void extract_leaf_mesh(prg_ctx* c) {
         PRG_CHECK_LAUNCH();
         {
             TimedScope ts(kt, "corner_value", 24.0 * double(V));
+            const int rd = ensure_coarse_rows(c, s);
             k_corner_value<<<grid_for(V, kBlock), kBlock, 0, s>>>(
                 vlist.get(), V, c->key.get(), c->pnt.get(), levels_of(c), c->parent.get(), c->x.get(),
-                reinterpret_cast<const float4*>(btab.get()), baseD, D, c->iso, vval.get());
+                reinterpret_cast<const float4*>(btab.get()), c->coarse_rows.get(), rd, baseD, D, c->iso, vval.get());
             PRG_CHECK_LAUNCH();
         }
         k_corner_square<<<grid_for(MD, kBlock), kBlock, 0, s>>>(oref.get(), cmask.get(), cbase.get(), vval.get(), MD,

@@ -98,6 +98,22 @@ __device__ __forceinline__ float level_sum(int64_t t, int d, const float bx[3], 
     return val;
 }
 
+// the basis values level_contrib evaluates for node nd of depth d at (px, py, pz)
+__device__ __forceinline__ void node_basis(int64_t nd, int d, float px, float py, float pz,
+                                           const uint32_t* __restrict__ key, const float* __restrict__ bf, int D,
+                                           float bx[3], float by[3], float bz[3]) {
+    const uint32_t k = key[nd];
+    const int64_t fb = (int64_t(1) << d) - 1, width = int64_t(1) << d;
+    const int64_t ox = compact3(k >> 2) >> (D - d), oy = compact3(k >> 1) >> (D - d), oz = compact3(k) >> (D - d);
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int64_t u = ox + a - 1, v = oy + a - 1, w = oz + a - 1;
+        bx[a] = (u >= 0 && u < width) ? basis_value(bf, fb + u, px) : 0.0f;
+        by[a] = (v >= 0 && v < width) ? basis_value(bf, fb + v, py) : 0.0f;
+        bz[a] = (w >= 0 && w < width) ? basis_value(bf, fb + w, pz) : 0.0f;
+    }
+}
+
 // Contribution of node nd (depth d >= 1) and its 27 neighbours to f at
 // (px, py, pz), added to val in the order j = 0..26.  Only three basis
 // functions per axis are involved (offsets -1, 0, +1 of the node's cell), so 9
@@ -106,17 +122,8 @@ __device__ __forceinline__ float level_contrib(int64_t nd, int d, float px, floa
                                                const uint32_t* __restrict__ key, const int32_t* __restrict__ pnt,
                                                const Levels& L, const float* __restrict__ x,
                                                const float* __restrict__ bf, int D, float val) {
-    const uint32_t k = key[nd];
-    const int64_t fb = (int64_t(1) << d) - 1, width = int64_t(1) << d;
-    const int64_t ox = compact3(k >> 2) >> (D - d), oy = compact3(k >> 1) >> (D - d), oz = compact3(k) >> (D - d);
     float bx[3], by[3], bz[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-        const int64_t u = ox + a - 1, v = oy + a - 1, w = oz + a - 1;
-        bx[a] = (u >= 0 && u < width) ? basis_value(bf, fb + u, px) : 0.0f;
-        by[a] = (v >= 0 && v < width) ? basis_value(bf, fb + v, py) : 0.0f;
-        bz[a] = (w >= 0 && w < width) ? basis_value(bf, fb + w, pz) : 0.0f;
-    }
+    node_basis(nd, d, px, py, pz, key, bf, D, bx, by, bz);
     return level_sum(nd - L.base[d], d, bx, by, bz, pnt, L, x, val);
 }
 
@@ -168,6 +175,29 @@ __device__ __forceinline__ float chain_value(int64_t nd, int d, float px, float 
     return val + tt;
 }
 
+// Coarse rows: for every node of depths 1 .. rows_depth, its 27 neighbours'
+// solution values in the order j (0 where a neighbour does not exist: a +-0
+// term added to a sum that started at +0 leaves it unchanged, exactly as
+// skipping the term) and its parent index (float bits of the int32), 32
+// floats per row from the first node of depth 1.
+constexpr int kRowFloats = 32;
+
+// level_sum from a coarse row
+__device__ __forceinline__ float row_sum(const float* __restrict__ R, const float bx[3], const float by[3],
+                                         const float bz[3], float val) {
+    float xv[27];
+#pragma unroll
+    for (int j = 0; j < 27; j++) xv[j] = R[j];
+#pragma unroll
+    for (int j = 0; j < 27; j++) {
+        float tt = xv[j] * bx[j / 9];
+        tt = tt * by[(j / 3) % 3];
+        tt = tt * bz[j % 3];
+        val = val + tt;
+    }
+    return val;
+}
+
 // True when every active lane of the wave holds the same v.
 __device__ __forceinline__ bool wave_uniform(int32_t v) {
     return __builtin_amdgcn_ballot_w64(v != __builtin_amdgcn_readfirstlane(v)) == 0;
@@ -181,7 +211,8 @@ __device__ __forceinline__ bool wave_uniform(int32_t v) {
 __device__ __forceinline__ float leaf_chain_value(int64_t nd, float px, float py, float pz,
                                                   const uint32_t* __restrict__ key, const int32_t* __restrict__ pnt,
                                                   const Levels& L, const int32_t* __restrict__ parent,
-                                                  const float* __restrict__ x, const float* __restrict__ bf, int D) {
+                                                  const float* __restrict__ x, const float* __restrict__ bf, int D,
+                                                  const float* __restrict__ rows, int rows_depth) {
     float val = 0.0f;
     int d = D;
     for (; d > 0; d--) {
@@ -192,8 +223,17 @@ __device__ __forceinline__ float leaf_chain_value(int64_t nd, float px, float py
     if (d > 0) {
         int32_t u = __builtin_amdgcn_readfirstlane(int32_t(nd));
         for (; d > 0; d--) {
-            val = level_contrib(u, d, px, py, pz, key, pnt, L, x, bf, D, val);
-            u = __builtin_amdgcn_readfirstlane(parent[u]);
+            if (d <= rows_depth) {
+                // one row per depth: neighbour values and the parent in one load
+                const float* R = rows + kRowFloats * (int64_t(u) - L.base[1]);
+                float bx[3], by[3], bz[3];
+                node_basis(u, d, px, py, pz, key, bf, D, bx, by, bz);
+                val = row_sum(R, bx, by, bz, val);
+                u = __builtin_amdgcn_readfirstlane(__float_as_int(R[27]));
+            } else {
+                val = level_contrib(u, d, px, py, pz, key, pnt, L, x, bf, D, val);
+                u = __builtin_amdgcn_readfirstlane(parent[u]);
+            }
         }
     }
     float tt = x[0] * basis_value(bf, 0, px);
