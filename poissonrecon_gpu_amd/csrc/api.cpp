@@ -435,8 +435,14 @@ prg_status prg_read_buffer(prg_ctx* c, int32_t which, void* host, int64_t bytes)
         PRG_HIP(hipSetDevice(c->cfg.device));
         const int64_t M = c->M;
         switch (which) {
-            case PRG_BUF_POINTS: read_xyz(c, c->px.get(), c->py.get(), c->pz.get(), c->N, (float*)host); break;
-            case PRG_BUF_NORMALS: read_xyz(c, c->nx.get(), c->ny.get(), c->nz.get(), c->N, (float*)host); break;
+            case PRG_BUF_POINTS:
+                prg::unpack_points(c);
+                read_xyz(c, c->px.get(), c->py.get(), c->pz.get(), c->N, (float*)host);
+                break;
+            case PRG_BUF_NORMALS:
+                prg::unpack_points(c);
+                read_xyz(c, c->nx.get(), c->ny.get(), c->nz.get(), c->N, (float*)host);
+                break;
             case PRG_BUF_POINT_LEAF: d2h(c, host, c->point_leaf.get(), size_t(b)); break;
             case PRG_BUF_KEY: d2h(c, host, c->key.get(), size_t(b)); break;
             case PRG_BUF_PIDX: d2h(c, host, c->pidx.get(), size_t(b)); break;

@@ -38,7 +38,10 @@ struct prg_ctx {
     float center[3] = {0, 0, 0};
     float scale = 1.0f;
     prg::DBuf<float> px, py, pz, nx, ny, nz;       // normalised input, sorted in place by the octree build
-    prg::DBuf<float> tx, ty, tz, tnx, tny, tnz;    // sort scratch
+    // normalised points in input order, 8 floats each (x, y, z, nx, ny, nz, 0, 0);
+    // px .. nz hold them in key order once the octree is built
+    prg::DBuf<float> pstage;
+    bool points_sorted = false;
     prg::DBuf<int32_t> point_leaf;                 // local leaf-level slot of each sorted point
 
     // octree
@@ -127,6 +130,8 @@ namespace prg {
 // Stage entry points implemented in the .hip files.
 void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n, bool on_device);
 void build_octree(prg_ctx* c);
+// px .. nz in input order (before an octree is built)
+void unpack_points(prg_ctx* c);
 // the 27-neighbour planes of every depth (kernels use the parent-neighbour
 // rows; these serve level views, buffer reads and the subdivision)
 void ensure_neighbours(prg_ctx* c);

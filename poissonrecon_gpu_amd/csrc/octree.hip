@@ -122,15 +122,16 @@ __global__ void __launch_bounds__(1024) k_bbox_final(const float* part, int nb, 
     tr[3] = scale;
 }
 
+// one 32-byte record per point (x, y, z, nx, ny, nz, 0, 0), so the sort's
+// permutation later gathers each point with two 16-byte loads
 __global__ void __launch_bounds__(kBlock) k_normalize(const float* xyz, const float* nxyz, int64_t n,
-                                                      const float* tr, float nscale, float* px, float* py, float* pz,
-                                                      float* nx, float* ny, float* nz) {
+                                                      const float* tr, float nscale, float4* stage) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
     const float cx = tr[0], cy = tr[1], cz = tr[2], scale = tr[3];
-    px[i] = (xyz[3 * i + 0] - cx) / scale;
-    py[i] = (xyz[3 * i + 1] - cy) / scale;
-    pz[i] = (xyz[3 * i + 2] - cz) / scale;
+    const float px = (xyz[3 * i + 0] - cx) / scale;
+    const float py = (xyz[3 * i + 1] - cy) / scale;
+    const float pz = (xyz[3 * i + 2] - cz) / scale;
     const float a = nxyz[3 * i + 0], b = nxyz[3 * i + 1], c = nxyz[3 * i + 2];
     float sq = a * a;
     sq = sq + b * b;
@@ -138,9 +139,8 @@ __global__ void __launch_bounds__(kBlock) k_normalize(const float* xyz, const fl
     float len = float(sqrt(double(sq)));  // Length(): double sqrt of a float sum (Geometry.inl:29-33)
     if (len > 1e-6f) len = 1.0f / len;
     len = len * nscale;
-    nx[i] = a * len;
-    ny[i] = b * len;
-    nz[i] = c * len;
+    stage[2 * i] = make_float4(px, py, pz, a * len);
+    stage[2 * i + 1] = make_float4(b * len, c * len, 0.0f, 0.0f);
 }
 
 // ---- keys and sort ----------------------------------------------------------------
@@ -171,27 +171,29 @@ __device__ inline uint32_t encode_point(float x, float y, float z, int D) {
     return key;
 }
 
-__global__ void __launch_bounds__(kBlock) k_morton(const float* px, const float* py, const float* pz, int64_t n,
-                                                   int D, uint32_t* key, uint32_t* val) {
+__global__ void __launch_bounds__(kBlock) k_morton(const float4* stage, int64_t n, int D, uint32_t* key,
+                                                   uint32_t* val) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i >= n) return;
-    key[i] = encode_point(px[i], py[i], pz[i], D);
+    const float4 p = stage[2 * i];
+    key[i] = encode_point(p.x, p.y, p.z, D);
     val[i] = uint32_t(i);
 }
 
-__global__ void __launch_bounds__(kBlock) k_gather6(const uint32_t* perm, int64_t n, const float* a0,
-                                                    const float* a1, const float* a2, const float* a3,
-                                                    const float* a4, const float* a5, float* b0, float* b1,
-                                                    float* b2, float* b3, float* b4, float* b5) {
+// point r of the output is staged record perm[r] (record r without a permutation)
+__global__ void __launch_bounds__(kBlock) k_gather_points(const uint32_t* perm, int64_t n, const float4* stage,
+                                                          float* px, float* py, float* pz, float* nx, float* ny,
+                                                          float* nz) {
     const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (r >= n) return;
-    const uint32_t i = perm[r];
-    b0[r] = a0[i];
-    b1[r] = a1[i];
-    b2[r] = a2[i];
-    b3[r] = a3[i];
-    b4[r] = a4[i];
-    b5[r] = a5[i];
+    const int64_t i = perm ? int64_t(perm[r]) : r;
+    const float4 a = stage[2 * i], b = stage[2 * i + 1];
+    px[r] = a.x;
+    py[r] = a.y;
+    pz[r] = a.z;
+    nx[r] = a.w;
+    ny[r] = b.x;
+    nz[r] = b.y;
 }
 
 // run heads of the sorted keys (generateMark, main.cu:195-211)
@@ -243,24 +245,27 @@ __global__ void __launch_bounds__(kBlock) k_level_up(const uint32_t* nk, const i
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_level_init(int64_t base, int64_t count, bool leaf, int32_t* pnum,
-                                                       int32_t* dnum, int32_t* child_base) {
-    const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (s >= count) return;
+__device__ __forceinline__ void level_init_slot(int64_t base, int64_t s, bool leaf, int32_t* pnum, int32_t* dnum,
+                                                int32_t* child_base) {
     pnum[base + s] = 0;
     dnum[base + s] = leaf ? 1 : 0;
     child_base[base + s] = -1;
 }
 
+__global__ void __launch_bounds__(kBlock) k_level_init(int64_t base, int64_t count, bool leaf, int32_t* pnum,
+                                                       int32_t* dnum, int32_t* child_base) {
+    const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (s >= count) return;
+    level_init_slot(base, s, leaf, pnum, dnum, child_base);
+}
+
 // Scatter the non-empty nodes of depth d into their slots.
-__global__ void __launch_bounds__(kBlock) k_level_scatter(const uint32_t* nk, const int32_t* np, const int32_t* ndi,
-                                                          const int32_t* pos, int64_t ne, int64_t npoints,
-                                                          int64_t nleaf_slots, bool leaf, int64_t base,
-                                                          int64_t child_level_base, uint32_t* key, int32_t* pidx,
-                                                          int32_t* pnum, int32_t* didx, int32_t* dnum,
-                                                          int32_t* child_base) {
-    const int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (q >= ne) return;
+__device__ __forceinline__ void level_scatter_node(int64_t q, const uint32_t* nk, const int32_t* np,
+                                                   const int32_t* ndi, const int32_t* pos, int64_t ne,
+                                                   int64_t npoints, int64_t nleaf_slots, bool leaf, int64_t base,
+                                                   int64_t child_level_base, uint32_t* key, int32_t* pidx,
+                                                   int32_t* pnum, int32_t* didx, int32_t* dnum,
+                                                   int32_t* child_base) {
     const int64_t g = base + pos[q];
     key[g] = nk[q];
     pidx[g] = np[q];
@@ -275,15 +280,25 @@ __global__ void __launch_bounds__(kBlock) k_level_scatter(const uint32_t* nk, co
     }
 }
 
+__global__ void __launch_bounds__(kBlock) k_level_scatter(const uint32_t* nk, const int32_t* np, const int32_t* ndi,
+                                                          const int32_t* pos, int64_t ne, int64_t npoints,
+                                                          int64_t nleaf_slots, bool leaf, int64_t base,
+                                                          int64_t child_level_base, uint32_t* key, int32_t* pidx,
+                                                          int32_t* pnum, int32_t* didx, int32_t* dnum,
+                                                          int32_t* child_base) {
+    const int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (q >= ne) return;
+    level_scatter_node(q, nk, np, ndi, pos, ne, npoints, nleaf_slots, leaf, base, child_level_base, key, pidx, pnum,
+                       didx, dnum, child_base);
+}
+
 // Every slot of depth d: key, running pidx / didx inside the sibling block and
 // the parent index (updateEmptyNodeInfo's intent, main.cu:433-490).
-__global__ void __launch_bounds__(kBlock) k_level_fill(const uint32_t* pk, const int32_t* pp, const int32_t* pdi,
-                                                       const int32_t* ppos, int64_t count, int D, int d, int64_t base,
-                                                       int64_t parent_base, uint32_t* key, int32_t* pidx,
-                                                       const int32_t* pnum, int32_t* didx, const int32_t* dnum,
-                                                       int32_t* parent) {
-    const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (s >= count) return;
+__device__ __forceinline__ void level_fill_slot(int64_t s, const uint32_t* pk, const int32_t* pp,
+                                                const int32_t* pdi, const int32_t* ppos, int D, int d, int64_t base,
+                                                int64_t parent_base, uint32_t* key, int32_t* pidx,
+                                                const int32_t* pnum, int32_t* didx, const int32_t* dnum,
+                                                int32_t* parent) {
     const int64_t b = s >> 3;
     const int j = int(s & 7);
     const int64_t g = base + s;
@@ -298,27 +313,21 @@ __global__ void __launch_bounds__(kBlock) k_level_fill(const uint32_t* pk, const
     parent[g] = int32_t(parent_base + ppos[b]);
 }
 
+__global__ void __launch_bounds__(kBlock) k_level_fill(const uint32_t* pk, const int32_t* pp, const int32_t* pdi,
+                                                       const int32_t* ppos, int64_t count, int D, int d, int64_t base,
+                                                       int64_t parent_base, uint32_t* key, int32_t* pidx,
+                                                       const int32_t* pnum, int32_t* didx, const int32_t* dnum,
+                                                       int32_t* parent) {
+    const int64_t s = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (s >= count) return;
+    level_fill_slot(s, pk, pp, pdi, ppos, D, d, base, parent_base, key, pidx, pnum, didx, dnum, parent);
+}
+
 __global__ void __launch_bounds__(kBlock) k_point_leaf(const int32_t* rank, const int32_t* pos_leaf, int64_t n,
                                                        int32_t* point_leaf) {
     const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (r >= n) return;
     point_leaf[r] = pos_leaf[rank[r]];
-}
-
-__global__ void k_root(uint32_t* key, int32_t* pidx, int32_t* pnum, int32_t* parent, int32_t* child_base,
-                       int32_t* didx, int32_t* dnum, int32_t* neigh, int64_t M, int32_t npoints, int32_t nleaf,
-                       int32_t base1) {
-    const int j = threadIdx.x;
-    if (j < 27) neigh[int64_t(j) * M] = (j == 13) ? 0 : -1;
-    if (j == 0) {
-        key[0] = 0;
-        pidx[0] = 0;
-        pnum[0] = npoints;
-        parent[0] = -1;
-        child_base[0] = base1;
-        didx[0] = 0;
-        dnum[0] = nleaf;
-    }
 }
 
 // computeNeighbor (main.cu:492-509): neighbour j of a depth-d node is child
@@ -352,11 +361,8 @@ __global__ void __launch_bounds__(kBlock) k_neighbors(const uint32_t* key, const
 // the parent depth's own table (the root's only neighbour is itself,
 // main.cu:803-808), so no 27-plane neighbour table is needed.  One thread per
 // block.
-__global__ void __launch_bounds__(kBlock) k_parent_neighbor_table(const int32_t* parent, const int32_t* child_base,
-                                                                  const int32_t* pnt_prev, int64_t base_prev,
-                                                                  int64_t base, int64_t nblocks, int32_t* pnt) {
-    const int64_t b = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (b >= nblocks) return;
+__device__ __forceinline__ void pnt_block(int64_t b, const int32_t* parent, const int32_t* child_base,
+                                          const int32_t* pnt_prev, int64_t base_prev, int64_t base, int32_t* pnt) {
     const int64_t tp = parent[base + 8 * b] - base_prev;
     int32_t nbp[27];
     if (!pnt_prev) {
@@ -371,6 +377,119 @@ __global__ void __launch_bounds__(kBlock) k_parent_neighbor_table(const int32_t*
     int32_t* row = pnt + 27 * b;
 #pragma unroll
     for (int j = 0; j < 27; j++) row[j] = cb[j] == -1 ? -1 : int32_t(cb[j] - base);
+}
+
+__global__ void __launch_bounds__(kBlock) k_parent_neighbor_table(const int32_t* parent, const int32_t* child_base,
+                                                                  const int32_t* pnt_prev, int64_t base_prev,
+                                                                  int64_t base, int64_t nblocks, int32_t* pnt) {
+    const int64_t b = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (b >= nblocks) return;
+    pnt_block(b, parent, child_base, pnt_prev, base_prev, base, pnt);
+}
+
+// ---- small depths in one workgroup -------------------------------------------------
+// The coarse depths hold few nodes; one 1024-thread workgroup runs their whole
+// per-depth sequence in a single launch instead of several launches per depth.
+constexpr int kSmallThreads = 1024;
+constexpr int64_t kSmallLevelSlots = 16384;  // slots of a depth built by k_levels_small
+constexpr int64_t kSmallLevelUp = 32768;     // bound on the non-empty nodes of a depth k_levels_up_small takes
+
+struct LevelArrays {
+    uint32_t* nk[kMaxDepth + 1];  // compact non-empty nodes of each depth: keys,
+    int32_t* np[kMaxDepth + 1];   // first sorted point,
+    int32_t* ndi[kMaxDepth + 1];  // first leaf-level slot,
+    int32_t* pos[kMaxDepth + 1];  // and slot in the depth's level array
+};
+
+// The bottom-up step (k_heads_dev, scan, k_excl_to_incl, k_level_up) for
+// depths ds .. 1: per depth, chunks of the compact nodes with a workgroup
+// scan of the parent-run heads carried across chunks.
+__global__ void __launch_bounds__(kSmallThreads) k_levels_up_small(LevelArrays A, int32_t* ne, int ds, int D) {
+    __shared__ int32_t sc[kSmallThreads];
+    __shared__ int32_t carry;
+    const int tid = threadIdx.x;
+    for (int d = ds; d >= 1; --d) {
+        const int64_t n = ne[d];
+        const int hs = 3 * (D - d + 1), sh = 3 * (D - d);
+        const uint32_t* nk = A.nk[d];
+        if (tid == 0) carry = 0;
+        __syncthreads();
+        for (int64_t c0 = 0; c0 < n; c0 += kSmallThreads) {
+            const int64_t i = c0 + tid;
+            int32_t f = 0;
+            uint32_t k = 0;
+            if (i < n) {
+                k = nk[i];
+                f = (i == 0 || (k >> hs) != (nk[i - 1] >> hs)) ? 1 : 0;
+            }
+            sc[tid] = f;
+            __syncthreads();
+            for (int o = 1; o < kSmallThreads; o <<= 1) {
+                const int32_t v = tid >= o ? sc[tid - o] : 0;
+                __syncthreads();
+                sc[tid] += v;
+                __syncthreads();
+            }
+            const int32_t b = carry + sc[tid] - 1;  // parent block of node i
+            if (i < n) {
+                A.pos[d][i] = 8 * b + int32_t((k >> sh) & 7u);
+                if (f) {
+                    A.nk[d - 1][b] = k & ~(7u << sh);
+                    A.np[d - 1][b] = A.np[d][i];
+                    A.ndi[d - 1][b] = (d == D) ? 8 * b : A.ndi[d][i];
+                }
+            }
+            __syncthreads();
+            if (tid == kSmallThreads - 1) carry = b + 1;
+            __syncthreads();
+        }
+        if (tid == 0) ne[d - 1] = carry;
+        __syncthreads();
+    }
+}
+
+struct SmallLevels {
+    int64_t base[kMaxDepth + 1], count[kMaxDepth + 1], ne[kMaxDepth + 1], pnt_off[kMaxDepth + 2];
+};
+
+// Root, then for depths 1 .. df: slot defaults and non-empty nodes (k_level_init,
+// k_level_scatter), every slot's fields (k_level_fill) and the parent-neighbour
+// rows (k_parent_neighbor_table, top-down).
+__global__ void __launch_bounds__(kSmallThreads) k_levels_small(LevelArrays A, SmallLevels S, int df, int D,
+                                                                int64_t M, int64_t npoints, int64_t MD,
+                                                                uint32_t* key, int32_t* pidx, int32_t* pnum,
+                                                                int32_t* parent, int32_t* child_base, int32_t* didx,
+                                                                int32_t* dnum, int32_t* neigh, int32_t* pnt) {
+    const int tid = threadIdx.x;
+    if (tid < 27) neigh[int64_t(tid) * M] = (tid == 13) ? 0 : -1;
+    if (tid == 0) {
+        key[0] = 0;
+        pidx[0] = 0;
+        pnum[0] = int32_t(npoints);
+        parent[0] = -1;
+        child_base[0] = int32_t(S.base[1]);
+        didx[0] = 0;
+        dnum[0] = int32_t(MD);
+    }
+    for (int d = 1; d <= df; d++)
+        for (int64_t s = tid; s < S.count[d]; s += kSmallThreads) level_init_slot(S.base[d], s, d == D, pnum, dnum, child_base);
+    __syncthreads();
+    for (int d = 1; d <= df; d++)
+        for (int64_t q = tid; q < S.ne[d]; q += kSmallThreads)
+            level_scatter_node(q, A.nk[d], A.np[d], A.ndi[d], A.pos[d], S.ne[d], npoints, MD, d == D, S.base[d],
+                               d == D ? 0 : S.base[d + 1], key, pidx, pnum, didx, dnum, child_base);
+    __syncthreads();
+    for (int d = 1; d <= df; d++)
+        for (int64_t s = tid; s < S.count[d]; s += kSmallThreads)
+            level_fill_slot(s, A.nk[d - 1], A.np[d - 1], A.ndi[d - 1], A.pos[d - 1], D, d, S.base[d], S.base[d - 1],
+                            key, pidx, pnum, didx, dnum, parent);
+    __syncthreads();
+    for (int d = 1; d <= df; d++) {
+        for (int64_t b = tid; b < S.count[d] / 8; b += kSmallThreads)
+            pnt_block(b, parent, child_base, d == 1 ? nullptr : pnt + 27 * S.pnt_off[d - 1], S.base[d - 1],
+                      S.base[d], pnt + 27 * S.pnt_off[d]);
+        __syncthreads();
+    }
 }
 
 }  // namespace
@@ -401,15 +520,24 @@ void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n
     c->N = n;
     {
         TimedScope ts(c->kt, "normalize", 48.0 * double(n));
-        k_normalize<<<grid_for(n, kBlock), kBlock, 0, s>>>(
-            dxyz, dn, n, c->transform_dev.get(), float(2 << c->D), c->px.reserve(n), c->py.reserve(n),
-            c->pz.reserve(n), c->nx.reserve(n), c->ny.reserve(n), c->nz.reserve(n));
+        k_normalize<<<grid_for(n, kBlock), kBlock, 0, s>>>(dxyz, dn, n, c->transform_dev.get(), float(2 << c->D),
+                                                           reinterpret_cast<float4*>(c->pstage.reserve(8 * n)));
         PRG_CHECK_LAUNCH();
     }
+    c->points_sorted = false;
     // the host copy of (centre, scale) arrives with the next synchronisation
     PRG_HIP(hipMemcpyAsync(c->transform_host, c->transform_dev.get(), 16, hipMemcpyDeviceToHost, s));
     c->transform_pending = true;
     if (!on_device) PRG_HIP(hipStreamSynchronize(s));  // the staging copies must finish before the caller reuses its arrays
+}
+
+void unpack_points(prg_ctx* c) {
+    if (c->points_sorted) return;
+    const int64_t n = c->N;
+    k_gather_points<<<grid_for(n, kBlock), kBlock, 0, c->stream>>>(
+        nullptr, n, reinterpret_cast<const float4*>(c->pstage.get()), c->px.reserve(n), c->py.reserve(n),
+        c->pz.reserve(n), c->nx.reserve(n), c->ny.reserve(n), c->nz.reserve(n));
+    PRG_CHECK_LAUNCH();
 }
 
 void ensure_neighbours(prg_ctx* c) {
@@ -443,31 +571,19 @@ void build_octree(prg_ctx* c) {
     val2.reserve(n);
     {
         TimedScope ts(kt, "morton", 20.0 * double(n));
-        k_morton<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->px.get(), c->py.get(), c->pz.get(), n, D, key.get(),
-                                                         val.get());
+        k_morton<<<grid_for(n, kBlock), kBlock, 0, s>>>(reinterpret_cast<const float4*>(c->pstage.get()), n, D,
+                                                         key.get(), val.get());
         PRG_CHECK_LAUNCH();
     }
     radix_sort_pairs(key.get(), val.get(), key2.get(), val2.get(), n, 3 * D, s, c->scratch, kt);
     {
         TimedScope ts(kt, "gather_points", 52.0 * double(n));
-        k_gather6<<<grid_for(n, kBlock), kBlock, 0, s>>>(
-            val.get(), n, c->px.get(), c->py.get(), c->pz.get(), c->nx.get(), c->ny.get(), c->nz.get(),
-            c->tx.reserve(n), c->ty.reserve(n), c->tz.reserve(n), c->tnx.reserve(n), c->tny.reserve(n),
-            c->tnz.reserve(n));
+        k_gather_points<<<grid_for(n, kBlock), kBlock, 0, s>>>(
+            val.get(), n, reinterpret_cast<const float4*>(c->pstage.get()), c->px.reserve(n), c->py.reserve(n),
+            c->pz.reserve(n), c->nx.reserve(n), c->ny.reserve(n), c->nz.reserve(n));
         PRG_CHECK_LAUNCH();
     }
-    std::swap(c->px.p, c->tx.p);
-    std::swap(c->px.n, c->tx.n);
-    std::swap(c->py.p, c->ty.p);
-    std::swap(c->py.n, c->ty.n);
-    std::swap(c->pz.p, c->tz.p);
-    std::swap(c->pz.n, c->tz.n);
-    std::swap(c->nx.p, c->tnx.p);
-    std::swap(c->nx.n, c->tnx.n);
-    std::swap(c->ny.p, c->tny.p);
-    std::swap(c->ny.n, c->tny.n);
-    std::swap(c->nz.p, c->tnz.p);
-    std::swap(c->nz.n, c->tnz.n);
+    c->points_sorted = true;
 
     // 2. unique leaves
     PoolBuf<int32_t> flag{&c->pool[7]}, rank{&c->pool[8]}, tot{&c->pool[9]};
@@ -523,7 +639,10 @@ void build_octree(prg_ctx* c) {
     PoolBuf<int32_t> bflag{&c->pool[11]}, brank{&c->pool[12]};
     bflag.reserve(ub[D]);
     brank.reserve(ub[D]);
-    for (int d = D; d >= 1; --d) {
+    // depths ds .. 1 (at most kSmallLevelUp non-empty nodes each) in one launch
+    int ds = 0;
+    while (ds < D && ub[ds + 1] <= kSmallLevelUp) ds++;
+    for (int d = D; d > ds; --d) {
         {
             TimedScope ts(kt, "level_heads", 8.0 * double(ub[d]));
             k_heads_dev<<<grid_for(ub[d], kBlock), kBlock, 0, s>>>(NK[d]->get(), ne_dev.get() + d, ub[d],
@@ -547,6 +666,25 @@ void build_octree(prg_ctx* c) {
             PRG_CHECK_LAUNCH();
         }
     }
+    LevelArrays LA{};
+    for (int d = ds; d >= 1; --d) {
+        NK[d - 1]->reserve(ub[d - 1]);
+        NP[d - 1]->reserve(ub[d - 1]);
+        NDI[d - 1]->reserve(ub[d - 1]);
+        POS[d]->reserve(ub[d]);
+    }
+    POS[0]->reserve(1);
+    for (int d = 0; d <= D; d++) {
+        LA.nk[d] = NK[d]->get();
+        LA.np[d] = NP[d]->get();
+        LA.ndi[d] = NDI[d]->get();
+        LA.pos[d] = POS[d]->get();
+    }
+    if (ds >= 1) {
+        TimedScope ts(kt, "level_up_small", 28.0 * double(ub[ds]));
+        k_levels_up_small<<<1, kSmallThreads, 0, s>>>(LA, ne_dev.get(), ds, D);
+        PRG_CHECK_LAUNCH();
+    }
     {
         std::vector<int32_t> hne(D + 1);
         PRG_HIP(hipMemcpyAsync(hne.data(), ne_dev.get(), size_t(D + 1) * 4, hipMemcpyDeviceToHost, s));
@@ -556,7 +694,6 @@ void build_octree(prg_ctx* c) {
     }
     counts[0] = 1;
     // root compact arrays: key 0, pidx 0, didx 0, slot 0
-    POS[0]->reserve(1);
     PRG_HIP(hipMemsetAsync(POS[0]->get(), 0, 4, s));
 
     std::vector<int64_t> bases(D + 1, 0);
@@ -575,11 +712,28 @@ void build_octree(prg_ctx* c) {
     int32_t* DN = c->dnum.reserve(M);
     int32_t* NB = c->neigh.reserve(size_t(27) * M);
     const int64_t MD = counts[D];
+    c->pnt_off.assign(D + 2, 0);
+    for (int d = 1; d <= D; d++) c->pnt_off[d + 1] = c->pnt_off[d] + counts[d] / 8;
+    int32_t* PT = c->pnt.reserve(size_t(27) * std::max<int64_t>(1, c->pnt_off[D + 1]));
+    // depths 1 .. df (at most kSmallLevelSlots slots each): root, levels and
+    // parent-neighbour rows in one launch
+    int df = 0;
+    while (df < D && counts[df + 1] <= kSmallLevelSlots) df++;
+    {
+        SmallLevels SL{};
+        for (int d = 0; d <= D; d++) {
+            SL.base[d] = bases[d];
+            SL.count[d] = counts[d];
+            SL.ne[d] = NE[d];
+        }
+        for (int d = 0; d <= D + 1; d++) SL.pnt_off[d] = c->pnt_off[d];
+        TimedScope ts(kt, "levels_small", 44.0 * double(bases[df] + counts[df]));
+        k_levels_small<<<1, kSmallThreads, 0, s>>>(LA, SL, df, D, M, n, MD, K, PI, PN, PA, CB, DI, DN, NB, PT);
+        PRG_CHECK_LAUNCH();
+    }
     {
         TimedScope ts(kt, "levels_fill", 44.0 * double(M));
-        k_root<<<1, 64, 0, s>>>(K, PI, PN, PA, CB, DI, DN, NB, M, int32_t(n), int32_t(MD), int32_t(bases[1]));
-        PRG_CHECK_LAUNCH();
-        for (int d = 1; d <= D; d++) {
+        for (int d = df + 1; d <= D; d++) {
             const bool leaf = (d == D);
             k_level_init<<<grid_for(counts[d], kBlock), kBlock, 0, s>>>(bases[d], counts[d], leaf, PN, DN, CB);
             PRG_CHECK_LAUNCH();
@@ -588,7 +742,7 @@ void build_octree(prg_ctx* c) {
                 leaf ? 0 : bases[d + 1], K, PI, PN, DI, DN, CB);
             PRG_CHECK_LAUNCH();
         }
-        for (int d = 1; d <= D; d++) {
+        for (int d = df + 1; d <= D; d++) {
             k_level_fill<<<grid_for(counts[d], kBlock), kBlock, 0, s>>>(
                 NK[d - 1]->get(), NP[d - 1]->get(), NDI[d - 1]->get(), POS[d - 1]->get(), counts[d], D, d,
                 bases[d], bases[d - 1], K, PI, PN, DI, DN, PA);
@@ -606,10 +760,7 @@ void build_octree(prg_ctx* c) {
     // 4. parent-neighbour tables (depths 1..D), top-down; the 27-plane
     // neighbour table (depths 1..D) is built on demand
     c->neigh_ready = (D == 0);
-    c->pnt_off.assign(D + 2, 0);
-    for (int d = 1; d <= D; d++) c->pnt_off[d + 1] = c->pnt_off[d] + counts[d] / 8;
-    int32_t* PT = c->pnt.reserve(size_t(27) * std::max<int64_t>(1, c->pnt_off[D + 1]));
-    for (int d = 1; d <= D; d++) {
+    for (int d = df + 1; d <= D; d++) {
         const int64_t nb = counts[d] / 8;
         TimedScope ts(kt, "parent_neighbor_table", 120.0 * double(nb));
         k_parent_neighbor_table<<<grid_for(nb, kBlock), kBlock, 0, s>>>(
