@@ -391,7 +391,7 @@ __global__ void __launch_bounds__(kBlock) k_parent_neighbor_table(const int32_t*
 // The coarse depths hold few nodes; one 1024-thread workgroup runs their whole
 // per-depth sequence in a single launch instead of several launches per depth.
 constexpr int kSmallThreads = 1024;
-constexpr int64_t kSmallLevelSlots = 16384;  // slots of a depth built by k_levels_small
+constexpr int64_t kSmallLevelSlots = 4096;  // slots of a depth built by k_levels_small
 constexpr int64_t kSmallLevelUp = 32768;     // bound on the non-empty nodes of a depth k_levels_up_small takes
 
 struct LevelArrays {
@@ -598,15 +598,10 @@ void build_octree(prg_ctx* c) {
     PoolBuf<int32_t> ne_dev{&c->pool[14]};  // non-empty nodes per depth
     ne_dev.reserve(D + 1);
     scan_exclusive(flag.get(), rank.get(), n, ne_dev.get() + D, s, c->scratch, kt);
-    int32_t U = 0;
-    PRG_HIP(hipMemcpyAsync(&U, ne_dev.get() + D, 4, hipMemcpyDeviceToHost, s));
-    PRG_HIP(hipStreamSynchronize(s));
-    if (c->transform_pending) {
-        for (int a = 0; a < 3; a++) c->center[a] = c->transform_host[a];
-        c->scale = c->transform_host[3];
-        c->transform_pending = false;
-    }
-    // upper bounds of the non-empty node counts: min(U, 8^d)
+    // upper bounds of the non-empty node counts, min(n, 8^d): the unique
+    // leaves number at most n, and the counts stay on the device until the
+    // levels are built
+    const int64_t U = std::max<int64_t>(1, n);
     std::vector<int64_t> ub(D + 1);
     for (int d = 0; d <= D; d++) ub[d] = std::min<int64_t>(U, int64_t(1) << (3 * d));
 
@@ -690,6 +685,11 @@ void build_octree(prg_ctx* c) {
         PRG_HIP(hipMemcpyAsync(hne.data(), ne_dev.get(), size_t(D + 1) * 4, hipMemcpyDeviceToHost, s));
         PRG_HIP(hipStreamSynchronize(s));
         for (int d = 0; d <= D; d++) NE[d] = hne[d];
+        if (c->transform_pending) {
+            for (int a = 0; a < 3; a++) c->center[a] = c->transform_host[a];
+            c->scale = c->transform_host[3];
+            c->transform_pending = false;
+        }
         for (int d = 1; d <= D; d++) counts[d] = 8 * NE[d - 1];
     }
     counts[0] = 1;
