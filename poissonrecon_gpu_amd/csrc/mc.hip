@@ -244,28 +244,6 @@ __global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* __restri
     vval[v] = (val + tt) - iso;
 }
 
-// values of each leaf node's corners in square numbering (vertices[], main.cu:1659-1668)
-// values of each leaf node's corners in square numbering (vertices[],
-// main.cu:1659-1668): one thread per leaf slot
-__global__ void __launch_bounds__(kBlock) k_corner_square(const int32_t* oref, const int32_t* cmask,
-                                                          const int32_t* cbase, const float* vval, int64_t MD,
-                                                          float* cval) {
-    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (t >= MD) return;
-    const int4* o4 = reinterpret_cast<const int4*>(oref + 8 * t);
-    const int4 oa = o4[0], ob = o4[1];
-    const int32_t o[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
-    float v[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const int y = (q >> 1) & 1, x = (q & 1) ^ y, z = q >> 2;
-        v[q] = vval[corner_id(cbase, cmask, o[x | (y << 1) | (z << 2)])];
-    }
-    float4* c4 = reinterpret_cast<float4*>(cval + 8 * t);
-    c4[0] = make_float4(v[0], v[1], v[2], v[3]);
-    c4[1] = make_float4(v[4], v[5], v[6], v[7]);
-}
-
 // edge owners (initEdgeArray / maintainEdgeNodePointer, main.cu:1795-2082) and
 // the sign-change test of the owner's corners (generateVexNums, :2457-2479).
 // One thread per leaf slot.  Corner values are per corner, so an edge's
@@ -273,15 +251,10 @@ __global__ void __launch_bounds__(kBlock) k_corner_square(const int32_t* oref, c
 // none of whose 12 edges changes sign flags nothing and skips the ownership
 // search (its eref entries are never read: only edges of cubes with
 // triangles are, and those change sign).
-__global__ void __launch_bounds__(kBlock) k_edge_owner(const uint32_t* key, const int32_t* pntD, const float* cval,
-                                                       int64_t baseD, int64_t MD, int32_t* emask, int32_t* ecount,
-                                                       int32_t* eref) {
-    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (t >= MD) return;
+__device__ __forceinline__ void edge_owner_slot(int64_t t, const float cv[8], const uint32_t* __restrict__ key,
+                                                const int32_t* __restrict__ pntD, int64_t baseD, int32_t* emask,
+                                                int32_t* ecount, int32_t* eref) {
     const int64_t i = baseD + t;
-    const float4* c4 = reinterpret_cast<const float4*>(cval + 8 * t);
-    const float4 ca = c4[0], cb = c4[1];
-    const float cv[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
     bool sc[12];
     bool any = false;
 #pragma unroll
@@ -343,6 +316,42 @@ __global__ void __launch_bounds__(kBlock) k_edge_owner(const uint32_t* key, cons
     r4[2] = make_int4(er[8], er[9], er[10], er[11]);
 }
 
+// Per leaf slot: its 8 corner values in square numbering (vertices[],
+// main.cu:1659-1668, written for the vertex positions and the buffer read),
+// the cube index and triangle count, and the edge owners (edge_owner_slot),
+// all from the corner values in registers.
+__global__ void __launch_bounds__(kBlock) k_corner_edges(const int32_t* __restrict__ oref,
+                                                         const int32_t* __restrict__ cmask,
+                                                         const int32_t* __restrict__ cbase,
+                                                         const float* __restrict__ vval,
+                                                         const uint32_t* __restrict__ key,
+                                                         const int32_t* __restrict__ pntD, int64_t baseD, int64_t MD,
+                                                         float* __restrict__ cval, uint8_t* __restrict__ cubes,
+                                                         int32_t* __restrict__ ntri, int32_t* __restrict__ emask,
+                                                         int32_t* __restrict__ ecount, int32_t* __restrict__ eref) {
+    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (t >= MD) return;
+    const int4* o4 = reinterpret_cast<const int4*>(oref + 8 * t);
+    const int4 oa = o4[0], ob = o4[1];
+    const int32_t o[8] = {oa.x, oa.y, oa.z, oa.w, ob.x, ob.y, ob.z, ob.w};
+    float v[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int y = (q >> 1) & 1, x = (q & 1) ^ y, z = q >> 2;
+        v[q] = vval[corner_id(cbase, cmask, o[x | (y << 1) | (z << 2)])];
+    }
+    float4* c4 = reinterpret_cast<float4*>(cval + 8 * t);
+    c4[0] = make_float4(v[0], v[1], v[2], v[3]);
+    c4[1] = make_float4(v[4], v[5], v[6], v[7]);
+    int cube = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        if (v[q] < 0) cube |= 1 << q;
+    cubes[t] = uint8_t(cube);
+    ntri[t] = tri_count(cube);
+    edge_owner_slot(t, v, key, pntD, baseD, emask, ecount, eref);
+}
+
 // interpolatePoint (main.cu:2584-2597) for every owned edge with a sign
 // change; one thread per leaf slot
 // Mesh vertices are numbered in (slot, edge kind) order: ebase is the
@@ -395,18 +404,12 @@ __global__ void __launch_bounds__(kBlock) k_edge_vertex(const int32_t* emask, co
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_tri_count(const float* cval, int64_t MD, int32_t* ntri) {
-    const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (t >= MD) return;
-    ntri[t] = tri_count(cube_of(cval, t));
-}
-
-__global__ void __launch_bounds__(kBlock) k_tri_emit(const float* cval, const int32_t* toff, const int32_t* eref,
+__global__ void __launch_bounds__(kBlock) k_tri_emit(const uint8_t* cubes, const int32_t* toff, const int32_t* eref,
                                                      const int32_t* emask, const int32_t* ebase, int64_t MD,
                                                      int32_t* mesh_t) {
     const int64_t t = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (t >= MD) return;
-    const int cube = cube_of(cval, t);
+    const int cube = cubes[t];
     const int n = 3 * tri_count(cube);
     const int64_t o = 3 * int64_t(toff[t]);
     for (int k = 0; k < n; k++) mesh_t[o + k] = edge_id(ebase, emask, eref[12 * t + c_tri[cube][k]]);
@@ -539,26 +542,26 @@ void extract_leaf_mesh(prg_ctx* c) {
                 reinterpret_cast<const float4*>(btab.get()), c->coarse_rows.get(), rd, baseD, D, c->iso, vval.get());
             PRG_CHECK_LAUNCH();
         }
-        k_corner_square<<<grid_for(MD, kBlock), kBlock, 0, s>>>(oref.get(), cmask.get(), cbase.get(), vval.get(), MD,
-                                                                CV);
-        PRG_CHECK_LAUNCH();
         emask.reserve(MD);
         ecount.reserve(MD);
         eref.reserve(12 * MD);
         ebase.reserve(MD);
         etot.reserve(1);
-        {
-            TimedScope ts(kt, "edge_owner", 12.0 * 40.0 * double(MD));
-            k_edge_owner<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->pnt.get() + 27 * c->pnt_off[D], CV,
-                                                                 baseD, MD, emask.get(), ecount.get(), eref.get());
-            PRG_CHECK_LAUNCH();
-        }
-        scan_exclusive(ecount.get(), ebase.get(), MD, etot.get(), s, c->scratch, kt);
         ntri.reserve(MD);
         toff.reserve(MD);
         ttot.reserve(1);
-        k_tri_count<<<grid_for(MD, kBlock), kBlock, 0, s>>>(CV, MD, ntri.get());
-        PRG_CHECK_LAUNCH();
+        PoolBuf<uint8_t> cubes{&c->pool[94]};
+        cubes.reserve(MD);
+        {
+            // corner values gathered (8 floats), cube index, triangle count and edge owners per slot
+            TimedScope ts(kt, "edge_owner", (32.0 + 32.0 + 32.0 + 12.0 * 40.0) * double(MD));
+            k_corner_edges<<<grid_for(MD, kBlock), kBlock, 0, s>>>(oref.get(), cmask.get(), cbase.get(), vval.get(),
+                                                                   c->key.get(), c->pnt.get() + 27 * c->pnt_off[D],
+                                                                   baseD, MD, CV, cubes.get(), ntri.get(), emask.get(),
+                                                                   ecount.get(), eref.get());
+            PRG_CHECK_LAUNCH();
+        }
+        scan_exclusive(ecount.get(), ebase.get(), MD, etot.get(), s, c->scratch, kt);
         scan_exclusive(ntri.get(), toff.get(), MD, ttot.get(), s, c->scratch, kt);
         PRG_HIP(hipMemcpyAsync(&NV, etot.get(), 4, hipMemcpyDeviceToHost, s));
         PRG_HIP(hipMemcpyAsync(&NT, ttot.get(), 4, hipMemcpyDeviceToHost, s));
@@ -568,8 +571,8 @@ void extract_leaf_mesh(prg_ctx* c) {
         k_edge_vertex<<<grid_for(MD, kBlock), kBlock, 0, s>>>(emask.get(), ebase.get(), c->key.get(), CV, baseD, MD,
                                                               D, MV);
         PRG_CHECK_LAUNCH();
-        k_tri_emit<<<grid_for(MD, kBlock), kBlock, 0, s>>>(CV, toff.get(), eref.get(), emask.get(), ebase.get(), MD,
-                                                           MT);
+        k_tri_emit<<<grid_for(MD, kBlock), kBlock, 0, s>>>(cubes.get(), toff.get(), eref.get(), emask.get(),
+                                                           ebase.get(), MD, MT);
         PRG_CHECK_LAUNCH();
         PRG_HIP(hipStreamSynchronize(s));
     }
