@@ -119,6 +119,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         PRG_HIP(hipStreamCreateWithFlags(&c->stream4, hipStreamNonBlocking));
         PRG_HIP(hipEventCreateWithFlags(&c->plan_ev, hipEventDisableTiming));
         PRG_HIP(hipEventCreate(&c->topo_ev));
+        PRG_HIP(hipEventCreate(&c->mesh_ev));
         for (auto& e : c->stage_ev) {
             PRG_HIP(hipEventCreate(&e[0]));
             PRG_HIP(hipEventCreate(&e[1]));
@@ -146,6 +147,7 @@ void prg_destroy(prg_ctx* c) {
         if (x) (void)hipStreamSynchronize(x);
     if (c->plan_ev) (void)hipEventDestroy(c->plan_ev);
     if (c->topo_ev) (void)hipEventDestroy(c->topo_ev);
+    if (c->mesh_ev) (void)hipEventDestroy(c->mesh_ev);
     for (auto& e : c->stage_ev)
         for (hipEvent_t x : e)
             if (x) (void)hipEventDestroy(x);
@@ -332,8 +334,10 @@ void build_mesh(prg_ctx* c, int32_t subdivide, prg_mesh* out) {
 // Copies the leaf-level batch and the subdivision batches, in that order.
 void copy_mesh(prg_ctx* c, float* vertices, int32_t* triangles) {
     prg::StageTimer st(c, PRG_STAGE_MESH_COPY);
-    if (c->mesh_nv) d2h(c, vertices, c->mesh_v.get(), size_t(3 * c->mesh_nv) * 4);
-    if (c->mesh_nt) d2h(c, triangles, c->mesh_t.get(), size_t(3 * c->mesh_nt) * 4);
+    if (!c->out_copied) {
+        if (c->mesh_nv) d2h(c, vertices, c->mesh_v.get(), size_t(3 * c->mesh_nv) * 4);
+        if (c->mesh_nt) d2h(c, triangles, c->mesh_t.get(), size_t(3 * c->mesh_nt) * 4);
+    }
     if (c->sub_nv) d2h(c, vertices + 3 * c->mesh_nv, c->sub_v.get(), size_t(3 * c->sub_nv) * 4);
     if (c->sub_nt) d2h(c, triangles + 3 * c->mesh_nt, c->sub_t.get(), size_t(3 * c->sub_nt) * 4);
     PRG_HIP(hipStreamSynchronize(c->stream));
@@ -364,6 +368,21 @@ prg_status prg_extract_mesh_into(prg_ctx* c, int32_t subdivide, float* vertices,
         if (!c->have_x) return fail(c, PRG_E_STATE, "prg_solve first");
         if (!c->iso_ready) return fail(c, PRG_E_STATE, "prg_isovalue or prg_set_isovalue first");
         if (!out) return fail(c, PRG_E_ARG, "null mesh");
+        // a leaf-level mesh goes straight into the caller's buffers while it is produced
+        if (!subdivide) {
+            c->out_v = vertices;
+            c->out_t = triangles;
+            c->out_vcap = vcap;
+            c->out_tcap = tcap;
+        }
+        struct Reset {
+            prg_ctx* c;
+            ~Reset() {
+                c->out_v = nullptr;
+                c->out_t = nullptr;
+                c->out_vcap = c->out_tcap = 0;
+            }
+        } reset{c};
         build_mesh(c, subdivide, out);
         const int64_t nv = out->n_vertices, nt = out->n_triangles;
         if (3 * nv > vcap || 3 * nt > tcap || (!vertices && nv) || (!triangles && nt))

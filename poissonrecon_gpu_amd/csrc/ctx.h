@@ -86,6 +86,13 @@ struct prg_ctx {
     // their numbering, the corner basis table), started after the build
     uint64_t topo_gen = 0;
     hipEvent_t topo_ev = nullptr;
+    // caller buffers a leaf-level mesh is copied into while it is produced
+    // (prg_extract_mesh_into), and whether the last extraction did so
+    float* out_v = nullptr;
+    int32_t* out_t = nullptr;
+    int64_t out_vcap = 0, out_tcap = 0;
+    bool out_copied = false;
+    hipEvent_t mesh_ev = nullptr;
     bool topo_pending = false;
     int32_t topo_corners = 0;  // owned corners
     bool neigh_ready = false;  // the 27 neighbour planes (depths 1..D) are built on demand

@@ -527,6 +527,7 @@ void extract_leaf_mesh(prg_ctx* c) {
         c->topo_pending = false;
     }
     const int32_t V = c->topo_corners;
+    bool direct = false;
     {
         StageTimer st(c, PRG_STAGE_MC_EVAL);
         vval.reserve(V);
@@ -568,16 +569,32 @@ void extract_leaf_mesh(prg_ctx* c) {
         PRG_HIP(hipStreamSynchronize(s));
         float* MV = c->mesh_v.reserve(size_t(3) * NV);
         int32_t* MT = c->mesh_t.reserve(size_t(3) * NT);
+        // caller buffers that hold the mesh: the vertex copy (second stream)
+        // runs while the triangles are emitted
+        direct = c->out_v && c->out_t && 3 * int64_t(NV) <= c->out_vcap && 3 * int64_t(NT) <= c->out_tcap;
         k_edge_vertex<<<grid_for(MD, kBlock), kBlock, 0, s>>>(emask.get(), ebase.get(), c->key.get(), CV, baseD, MD,
                                                               D, MV);
         PRG_CHECK_LAUNCH();
+        if (direct && NV) {
+            PRG_HIP(hipEventRecord(c->mesh_ev, s));
+            PRG_HIP(hipStreamWaitEvent(c->stream2, c->mesh_ev, 0));
+            PRG_HIP(hipMemcpyAsync(c->out_v, MV, size_t(3) * NV * 4, hipMemcpyDeviceToHost, c->stream2));
+            PRG_HIP(hipEventRecord(c->mesh_ev, c->stream2));
+        }
         k_tri_emit<<<grid_for(MD, kBlock), kBlock, 0, s>>>(cubes.get(), toff.get(), eref.get(), emask.get(),
                                                            ebase.get(), MD, MT);
         PRG_CHECK_LAUNCH();
-        PRG_HIP(hipStreamSynchronize(s));
+        if (!direct) PRG_HIP(hipStreamSynchronize(s));
     }
     c->mesh_nv = NV;
     c->mesh_nt = NT;
+    c->out_copied = direct;
+    if (direct) {
+        StageTimer st(c, PRG_STAGE_MESH_COPY);
+        if (NT) PRG_HIP(hipMemcpyAsync(c->out_t, c->mesh_t.get(), size_t(3) * NT * 4, hipMemcpyDeviceToHost, s));
+        if (NV) PRG_HIP(hipStreamWaitEvent(s, c->mesh_ev, 0));
+        PRG_HIP(hipStreamSynchronize(s));
+    }
 }
 
 }  // namespace prg
