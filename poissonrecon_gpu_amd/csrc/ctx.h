@@ -66,7 +66,6 @@ struct prg_ctx {
     prg::DBuf<double> cg_part[3][4];  // r.r (two slots), p.q, b.b / true-residual partials
     prg::DBuf<uint8_t> cg_state[3];
     prg::DBuf<uint8_t> cg_seg;  // the solver's per-depth segment table
-    prg::DBuf<float> cg_small[3];
     prg::DBuf<double> cg_out;
     // SpMV tile plans of the large depths, valid for octree `plan_gen`
     prg::DBuf<int32_t> plan_count, plan_ubase;

@@ -40,7 +40,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxBlocks = 2048;
 constexpr int kSmallBlock = 1024;
-constexpr int64_t kSmallRows = 4096;
+constexpr int64_t kSmallRows = 2048;
 // Algorithmic bytes per SpMV row under the reference's CSR-free representation
 // (int32 neighbour indices, FP32 values): 108 B of neighbour indices, p_i read,
 // (Ap)_i written.  The kernels here read 27 parent-neighbour entries per sibling
@@ -515,92 +515,167 @@ struct SmallJobs {
     int64_t pnt_row[kMaxDepth + 1];  // first parent-neighbour row of the depth (-1 at the root)
 };
 
-// row t of a small depth: neighbours from the parent-neighbour table (depth
-// >= 1); the root's only neighbour is itself.
-__device__ __forceinline__ float spmv_small(const int32_t* __restrict__ pnt, int64_t t, const float* __restrict__ v,
-                                            const float* __restrict__ L) {
-    if (!pnt) return (L[13] == 0.0f) ? 0.0f : 0.0f + L[13] * v[0];
-    int32_t nb[27];
-    pnt_neighbours(pnt + 27 * (t >> 3), int(t & 7), nb);
-    float vv[27];
+// Row t of a small depth from v in LDS: neighbours from the row's 8
+// parent-neighbour entries (depth >= 1; packed two int16 per word, each in
+// -1 .. kSmallRows - 1; the root's only neighbour is itself), the terms and
+// order of spmv_pnt.
+__device__ __forceinline__ float spmv_small(const uint32_t ep[4], int s, bool root, const float* v, const float* L) {
+    if (root) return (L[13] == 0.0f) ? 0.0f : 0.0f + L[13] * v[0];
+    int32_t e[8];
 #pragma unroll
-    for (int j = 0; j < 27; j++) vv[j] = v[nb[j] < 0 ? 0 : nb[j]];
+    for (int m = 0; m < 4; m++) {
+        e[2 * m] = int32_t(int16_t(ep[m] & 0xffffu));
+        e[2 * m + 1] = int32_t(int16_t(ep[m] >> 16));
+    }
+    int32_t nb[27];
+    pnt_expand(e, s, nb);
     float out = 0.0f;
 #pragma unroll
     for (int j = 0; j < 27; j++) {
-        const float prod = L[j] * vv[j];
+        const float prod = L[j] * v[nb[j] < 0 ? 0 : nb[j]];
         out = (nb[j] < 0 || L[j] == 0.0f) ? out : out + prod;
     }
     return out;
 }
 
+// One workgroup runs a small depth's whole CG.  Its rows (at most
+// kSmallRows) sit kSmallRows / kSmallBlock per thread: x, r, b, q in
+// registers with each row's 8 parent-neighbour entries, and the vector the
+// SpMV reads (p, or x for a residual) in LDS.  The arithmetic, the stop test
+// and the reduction order (each thread's rows in order, then the workgroup
+// sum) are the large depths'.
 __global__ void __launch_bounds__(kSmallBlock) k_cg_small(SmallJobs jobs, const float* stencils, const int32_t* pnt_all,
-                                                          const float* div, float* xall, float* rall,
-                                                          float* pall, float* qall, float tol, int32_t fixed,
+                                                          const float* div, float* xall, float tol, int32_t fixed,
                                                           int32_t max_iters, int32_t refresh, double* out) {
+    constexpr int R = int(kSmallRows / kSmallBlock);
+    static_assert(kSmallRows <= 32768, "parent-neighbour entries are packed as int16");
     __shared__ double lds[kSmallBlock / 64];
     __shared__ float L[27];
+    __shared__ float sv[kSmallRows];  // p, or x while a residual is formed
+    __shared__ float ps[kSmallRows];  // p of this thread's rows while sv holds x
+    const int tid = threadIdx.x;
     const int job = blockIdx.x;
     const int d = jobs.depth[job];
     const int64_t base = jobs.base[job], n = jobs.n[job];
-    const int32_t* pnt = jobs.pnt_row[job] < 0 ? nullptr : pnt_all + 27 * jobs.pnt_row[job];
-    if (threadIdx.x < 27) L[threadIdx.x] = stencils[27 * d + threadIdx.x];
-    __syncthreads();
-    const float* b = div + base;
-    float* x = xall + base;
-    float* r = rall + base;
-    float* p = pall + base;
-    float* q = qall + base;
-    double bb = 0.0, rr = 0.0;
-    for (int64_t t = threadIdx.x; t < n; t += kSmallBlock) {
-        const float rv = b[t] - spmv_small(pnt, t, x, L);
-        r[t] = rv;
-        p[t] = rv;
-        bb += double(b[t]) * double(b[t]);
-        rr += double(rv) * double(rv);
+    const bool root = jobs.pnt_row[job] < 0;
+    const int32_t* pnt = root ? nullptr : pnt_all + 27 * jobs.pnt_row[job];
+    if (tid < 27) L[tid] = stencils[27 * d + tid];
+    const float* bg = div + base;
+    float* xg = xall + base;
+    float x[R], r[R], b[R], q[R];
+    uint32_t ep[R][4];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int64_t t = tid + int64_t(k) * kSmallBlock;
+        const bool ok = t < n;
+        b[k] = ok ? bg[t] : 0.0f;
+        x[k] = ok ? xg[t] : 0.0f;
+        int32_t e[8];
+        if (ok && !root) {
+            pnt_entries(pnt + 27 * (t >> 3), int(t & 7), e);
+        } else {
+#pragma unroll
+            for (int m = 0; m < 8; m++) e[m] = -1;
+        }
+#pragma unroll
+        for (int m = 0; m < 4; m++) ep[k][m] = (uint32_t(e[2 * m]) & 0xffffu) | (uint32_t(e[2 * m + 1]) << 16);
+        if (ok) sv[t] = x[k];
     }
-    bb = block_allsum<kSmallBlock>(bb, lds);
+    __syncthreads();  // x and L in LDS
+    double bb = 0.0, rr = 0.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int64_t t = tid + int64_t(k) * kSmallBlock;
+        if (t < n) {
+            r[k] = b[k] - spmv_small(ep[k], int(t & 7), root, sv, L);
+            bb += double(b[k]) * double(b[k]);
+            rr += double(r[k]) * double(r[k]);
+        }
+    }
+    bb = block_allsum<kSmallBlock>(bb, lds);  // every read of sv above is complete
     rr = block_allsum<kSmallBlock>(rr, lds);
     const double tol2 = double(tol) * double(tol) * bb;
     int it = 0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int64_t t = tid + int64_t(k) * kSmallBlock;
+        if (t < n) sv[t] = r[k];
+    }
     while (bb > 0.0 && rr != 0.0 && it < max_iters && (fixed > 0 ? it < fixed : rr > tol2)) {
-        __syncthreads();  // p complete
+        __syncthreads();  // p complete in LDS
         double pq = 0.0;
-        for (int64_t t = threadIdx.x; t < n; t += kSmallBlock) {
-            const float qv = spmv_small(pnt, t, p, L);
-            q[t] = qv;
-            pq += double(p[t]) * double(qv);
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int64_t t = tid + int64_t(k) * kSmallBlock;
+            if (t < n) {
+                q[k] = spmv_small(ep[k], int(t & 7), root, sv, L);
+                pq += double(sv[t]) * double(q[k]);
+            }
         }
         pq = block_allsum<kSmallBlock>(pq, lds);
         if (!(pq > 0.0)) break;
         const float af = float(rr / pq);
-        for (int64_t t = threadIdx.x; t < n; t += kSmallBlock) {
-            const float t1 = af * p[t];
-            x[t] = x[t] + t1;
-            const float t2 = af * q[t];
-            r[t] = r[t] - t2;
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int64_t t = tid + int64_t(k) * kSmallBlock;
+            if (t < n) {
+                const float pk = sv[t];
+                const float t1 = af * pk;
+                x[k] = x[k] + t1;
+                const float t2 = af * q[k];
+                r[k] = r[k] - t2;
+                ps[t] = pk;
+            }
         }
         it++;
         if (refresh > 0 && it % refresh == 0) {
-            __syncthreads();  // x complete
-            for (int64_t t = threadIdx.x; t < n; t += kSmallBlock)
-                r[t] = b[t] - spmv_small(pnt, t, x, L);
+            __syncthreads();  // every row's p has been read from sv
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const int64_t t = tid + int64_t(k) * kSmallBlock;
+                if (t < n) sv[t] = x[k];
+            }
+            __syncthreads();  // x complete in LDS
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const int64_t t = tid + int64_t(k) * kSmallBlock;
+                if (t < n) r[k] = b[k] - spmv_small(ep[k], int(t & 7), root, sv, L);
+            }
         }
         double rr2 = 0.0;
-        for (int64_t t = threadIdx.x; t < n; t += kSmallBlock) rr2 += double(r[t]) * double(r[t]);
-        rr2 = block_allsum<kSmallBlock>(rr2, lds);
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int64_t t = tid + int64_t(k) * kSmallBlock;
+            if (t < n) rr2 += double(r[k]) * double(r[k]);
+        }
+        rr2 = block_allsum<kSmallBlock>(rr2, lds);  // every read of sv above is complete
         const float bf = float(rr2 / rr);
         rr = rr2;
-        for (int64_t t = threadIdx.x; t < n; t += kSmallBlock) {
-            const float tp = bf * p[t];
-            p[t] = r[t] + tp;
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const int64_t t = tid + int64_t(k) * kSmallBlock;
+            if (t < n) {
+                const float tp = bf * ps[t];
+                sv[t] = r[k] + tp;
+            }
         }
+    }
+    __syncthreads();  // the last SpMV's reads of sv are complete
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int64_t t = tid + int64_t(k) * kSmallBlock;
+        if (t < n) sv[t] = x[k];
     }
     __syncthreads();
     double tr = 0.0;
-    for (int64_t t = threadIdx.x; t < n; t += kSmallBlock) {
-        const double rv = double(b[t]) - double(spmv_small(pnt, t, x, L));
-        tr += rv * rv;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const int64_t t = tid + int64_t(k) * kSmallBlock;
+        if (t < n) {
+            const double rv = double(b[k]) - double(spmv_small(ep[k], int(t & 7), root, sv, L));
+            tr += rv * rv;
+            xg[t] = x[k];
+        }
     }
     tr = block_allsum<kSmallBlock>(tr, lds);
     if (threadIdx.x == 0) {
@@ -895,15 +970,11 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             PRG_HIP(hipEventRecord(ew, ws));
         }
         if (jobs.count) {
-            const int64_t small_end = jobs.base[jobs.count - 1] + jobs.n[jobs.count - 1];
-            float* Rs = c->cg_small[0].reserve(small_end);
-            float* Ps = c->cg_small[1].reserve(small_end);
-            float* Qs = c->cg_small[2].reserve(small_end);
             es0 = new_event();
             PRG_HIP(hipEventRecord(es0, small_s));
             k_cg_small<<<jobs.count, kSmallBlock, 0, small_s>>>(jobs, c->d_stencil.get(), c->pnt.get(),
-                                                                c->div.get(), X, Rs, Ps, Qs, tol, fixed, max_iters,
-                                                                refresh, outv);
+                                                                c->div.get(), X, tol, fixed, max_iters, refresh,
+                                                                outv);
             PRG_CHECK_LAUNCH();
             PRG_HIP(hipEventRecord(es1, small_s));
         }
