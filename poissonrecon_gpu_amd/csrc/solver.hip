@@ -545,8 +545,9 @@ __device__ __forceinline__ float spmv_small(const uint32_t ep[4], int s, bool ro
 // and the reduction order (each thread's rows in order, then the workgroup
 // sum) are the large depths'.
 __global__ void __launch_bounds__(kSmallBlock) k_cg_small(SmallJobs jobs, const float* stencils, const int32_t* pnt_all,
-                                                          const float* div, float* xall, float tol, int32_t fixed,
-                                                          int32_t max_iters, int32_t refresh, double* out) {
+                                                          const float* div, float* xall, int32_t warm_depth, float tol,
+                                                          int32_t fixed, int32_t max_iters, int32_t refresh,
+                                                          double* out) {
     constexpr int R = int(kSmallRows / kSmallBlock);
     static_assert(kSmallRows <= 32768, "parent-neighbour entries are packed as int16");
     __shared__ double lds[kSmallBlock / 64];
@@ -569,7 +570,7 @@ __global__ void __launch_bounds__(kSmallBlock) k_cg_small(SmallJobs jobs, const 
         const int64_t t = tid + int64_t(k) * kSmallBlock;
         const bool ok = t < n;
         b[k] = ok ? bg[t] : 0.0f;
-        x[k] = ok ? xg[t] : 0.0f;
+        x[k] = (ok && d == warm_depth) ? xg[t] : 0.0f;  // zero start unless warm-started
         int32_t e[8];
         if (ok && !root) {
             pnt_entries(pnt + 27 * (t >> 3), int(t & 7), e);
@@ -763,7 +764,9 @@ void start_plans(prg_ctx* c) {
         PRG_HIP(hipStreamWaitEvent(s, c->plan_ev, 0));
     }
     PRG_HIP(hipMemsetAsync(mx, 0, size_t(D + 1) * 4, s));
-    for (int d = 0; d <= D; d++) {
+    // the leaf depth first: the vector field and divergence wait for its plan
+    for (int k = 0; k <= D; k++) {
+        const int d = k == 0 ? D : k - 1;
         const int64_t nt = c->plan_tile_off[d + 1] - c->plan_tile_off[d];
         if (c->counts[d] <= kSmallRows || nt == 0) continue;
         const int64_t t0 = c->plan_tile_off[d];
@@ -808,8 +811,9 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     const int D = c->D;
     const int64_t M = c->M;
     ensure_plans(c);
+    // every depth's solution is written by its solve (the large depths'
+    // copies, the small depths' kernel); only a warm-started depth reads it first
     float* X = c->x.reserve(M);
-    PRG_HIP(hipMemsetAsync(X, 0, size_t(M) * 4, s));
     // the learned warm start predicts the depth of its network's grid; the
     // other depths start from zero
     int warm_depth = -1;
@@ -952,9 +956,8 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     try {
         // the segment table and zeroed states, then the other streams fork
         // from the context stream
+        // (k_cg_init resets each depth's state)
         if (nsegs) PRG_HIP(hipMemcpyAsync(dseg, hseg, size_t(nsegs) * sizeof(Seg), hipMemcpyHostToDevice, s));
-        for (auto& h : ch)
-            if (!h.depths.empty()) PRG_HIP(hipMemsetAsync(h.st, 0, h.depths.size() * sizeof(CgState), s));
         hipEvent_t es0 = new_event(), es1 = new_event();
         PRG_HIP(hipEventRecord(es0, s));
         for (hipStream_t x : {c->stream2, c->stream3, c->stream4}) PRG_HIP(hipStreamWaitEvent(x, es0, 0));
@@ -973,8 +976,8 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             es0 = new_event();
             PRG_HIP(hipEventRecord(es0, small_s));
             k_cg_small<<<jobs.count, kSmallBlock, 0, small_s>>>(jobs, c->d_stencil.get(), c->pnt.get(),
-                                                                c->div.get(), X, tol, fixed, max_iters, refresh,
-                                                                outv);
+                                                                c->div.get(), X, warm_depth, tol, fixed, max_iters,
+                                                                refresh, outv);
             PRG_CHECK_LAUNCH();
             PRG_HIP(hipEventRecord(es1, small_s));
         }
