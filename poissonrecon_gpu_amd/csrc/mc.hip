@@ -411,8 +411,23 @@ __global__ void __launch_bounds__(kBlock) k_tri_emit(const uint8_t* cubes, const
     if (t >= MD) return;
     const int cube = cubes[t];
     const int n = 3 * tri_count(cube);
+    if (n == 0) return;
     const int64_t o = 3 * int64_t(toff[t]);
-    for (int k = 0; k < n; k++) mesh_t[o + k] = edge_id(ebase, emask, eref[12 * t + c_tri[cube][k]]);
+    // the 12 edge references and their vertex ids at once (a slot with
+    // triangles has all 12 references), then the triangles' picks
+    const int4* r4 = reinterpret_cast<const int4*>(eref + 12 * t);
+    const int4 ra = r4[0], rb = r4[1], rc = r4[2];
+    const int32_t r[12] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w, rc.x, rc.y, rc.z, rc.w};
+    int32_t id[12];
+#pragma unroll
+    for (int e = 0; e < 12; e++) id[e] = edge_id(ebase, emask, r[e]);
+    for (int k = 0; k < n; k++) {
+        const int e = c_tri[cube][k];
+        int32_t pick = id[0];
+#pragma unroll
+        for (int q = 1; q < 12; q++) pick = e == q ? id[q] : pick;
+        mesh_t[o + k] = pick;
+    }
 }
 
 
