@@ -130,8 +130,8 @@ typedef enum prg_buffer {
     PRG_BUF_DIDX = 9,         /* int32[M]                                            */
     PRG_BUF_DNUM = 10,        /* int32[M]                                            */
     PRG_BUF_VECTOR_FIELD = 11,/* float[3 M_D] (main.cu:932-964)                      */
-    PRG_BUF_DIVERGENCE = 12,  /* float[M] (main.cu:1007-1141); writable              */
-    PRG_BUF_SOLUTION = 13,    /* float[M] (d_x, main.cu:1223-1332); writable         */
+    PRG_BUF_DIVERGENCE = 12,  /* float[M] (main.cu:1007-1141); writable; 0 until computed */
+    PRG_BUF_SOLUTION = 13,    /* float[M] (d_x, main.cu:1223-1332); writable; 0 until solved */
     PRG_BUF_POINT_VALUES = 14,/* float[N] implicit function at sorted samples (main.cu:1334-1381) */
     PRG_BUF_LEAF_CORNER_VALUES = 15, /* float[8 M_D] f - iso at each leaf-level node corner,
                                         corner j in square numbering (main.cu:1659-1668) */
@@ -198,8 +198,9 @@ void prg_mesh_release(prg_mesh* mesh);
  * example pinned memory reused across calls): vertices holds vcap floats,
  * triangles tcap ints.  out receives the counts and, on success, the given
  * pointers; when a capacity is too small the call returns PRG_E_ARG with the
- * required counts in out and copies nothing.  prg_mesh_release must not be
- * called on such a mesh. */
+ * required counts in out and copies nothing.  With subdivide == 0 and
+ * buffers that fit, the vertices are copied while the triangles are still
+ * being emitted.  prg_mesh_release must not be called on such a mesh. */
 prg_status prg_extract_mesh_into(prg_ctx* ctx, int32_t subdivide, float* vertices, int64_t vcap, int32_t* triangles,
                                  int64_t tcap, prg_mesh* out);
 
