@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(kBlock) k_vector_field(const int32_t* pntD, co
 // per workgroup).
 constexpr int kVfPoints = 768;
 
-__global__ void __launch_bounds__(kTileThreads) k_vector_field_tiles(TilePlan P, int64_t nblk,
+__global__ void __launch_bounds__(kTileThreads, 6) k_vector_field_tiles(TilePlan P, int64_t nblk,
                                                                      const int32_t* __restrict__ pntD,
                                                                      const int32_t* __restrict__ pidx,
                                                                      const int32_t* __restrict__ pnum,
@@ -204,7 +204,10 @@ __global__ void __launch_bounds__(kTileThreads) k_vector_field_tiles(TilePlan P,
                                                                      const float* __restrict__ nz, int64_t baseD,
                                                                      float* __restrict__ vfx, float* __restrict__ vfy,
                                                                      float* __restrict__ vfz) {
-    __shared__ int32_t cp0[8 * kTileBlocks];       // first point of each listed cell
+    // A sibling block's points are contiguous in key order, cell after cell,
+    // from the block's first point (the first cell's pidx, also when empty):
+    // a cell's first point is bfirst[block] + coff[cell] - coff[8 block].
+    __shared__ int32_t bfirst[kTileBlocks];        // first point of each listed block
     __shared__ int32_t coff[8 * kTileBlocks + 1];  // LDS offsets of the cells' points
     __shared__ float pw[12 * kVfPoints];           // 9 weights and the normal, one array per component
     __shared__ int16_t rid[kTileEntries];
@@ -227,7 +230,7 @@ __global__ void __launch_bounds__(kTileThreads) k_vector_field_tiles(TilePlan P,
     int32_t own = 0;
     for (int c = c0; c < c1; c++) {
         const int64_t gi = baseD + ub[c >> 3] + (c & 7);
-        cp0[c] = pidx[gi];
+        if ((c & 7) == 0) bfirst[c >> 3] = pidx[gi];
         own += pnum[gi];
     }
     int32_t inc = own;
@@ -259,7 +262,7 @@ __global__ void __launch_bounds__(kTileThreads) k_vector_field_tiles(TilePlan P,
                 else
                     hi = mid;
             }
-            const int64_t q = int64_t(cp0[lo]) + (i - coff[lo]);
+            const int64_t q = int64_t(bfirst[lo >> 3]) + (i - coff[lo & ~7]);
 #pragma unroll
             for (int k = 0; k < 9; k++) pw[k * kVfPoints + i] = W[9 * q + k];
             pw[9 * kVfPoints + i] = nx[q];
@@ -292,7 +295,7 @@ __global__ void __launch_bounds__(kTileThreads) k_vector_field_tiles(TilePlan P,
                 vz = vz + w * pw[11 * kVfPoints + q];
             }
         } else {
-            const int64_t p0 = cp0[cell];
+            const int64_t p0 = int64_t(bfirst[cell >> 3]) + (o - coff[cell & ~7]);
             for (int64_t q = p0; q < p0 + (e - o); q++) {
                 const float* wq = W + 9 * q;
                 const float fx = wq[kx], fy = wq[3 + ky], fz = wq[6 + kz];
