@@ -256,8 +256,13 @@ __global__ void __launch_bounds__(kSpmvThreads) k_tile_plan(const int32_t* pnt, 
 // the cell values of tile t + 1 and the plan (count, row ids, block bases) of
 // tile t + 2 are in flight in registers.  K = cells per thread covers the
 // depth's largest tile (8 * count <= K * kSpmvThreads).
+// Instances with at most kSpmvSmallK cells per thread fit 80 VGPRs: six waves
+// per SIMD, three workgroups per CU; the larger ones run two.
+constexpr int kSpmvSmallK = 5;
+
 template <int K>
-__global__ void __launch_bounds__(kSpmvThreads) k_cg_spmv(const Seg* __restrict__ segs, int32_t nseg, int32_t it,
+__global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 6 : (K <= 8 ? 4 : 2)) k_cg_spmv(const Seg* __restrict__ segs,
+                                                                                 int32_t nseg, int32_t it,
                                                           float tol, int32_t fixed, int32_t max_iters) {
     static_assert(K * kSpmvThreads <= 8 * kTileEntries, "cells per thread beyond a tile's plan");
     constexpr int kRidWords = kTileEntries / 2;  // row ids as 32-bit words
@@ -722,9 +727,10 @@ int32_t* plan_maxcount_host(prg_ctx* c) {
 // depths.  A chain's launch is one round of workgroups, two per CU, shared
 // among its depths by tile count, each pipelining a run of tiles (four per CU
 // measured the same at depth 10).
-int spmv_instance(int kcells) { return kcells <= 2 ? 0 : kcells <= 4 ? 1 : kcells <= 8 ? 2 : 3; }
+int spmv_instance(int kcells) { return kcells <= 2 ? 0 : kcells <= 4 ? 1 : kcells <= 5 ? 4 : kcells <= 8 ? 2 : 3; }
 
-constexpr int kSpmvWGPerCU = 2;
+// workgroups per CU of an instance (its waves per SIMD / 2)
+int spmv_wg_per_cu(int inst) { return (inst == 0 || inst == 1 || inst == 4) ? 3 : 2; }
 
 void launch_spmv(int inst, int32_t nwg, size_t lds, hipStream_t s, const Seg* segs, int32_t nseg, int32_t it,
                  float tol, int32_t fixed, int32_t max_iters) {
@@ -733,6 +739,7 @@ void launch_spmv(int inst, int32_t nwg, size_t lds, hipStream_t s, const Seg* se
         case 0: PRG_SPMV(2); break;
         case 1: PRG_SPMV(4); break;
         case 2: PRG_SPMV(8); break;
+        case 4: PRG_SPMV(5); break;
         default: PRG_SPMV(27); break;
     }
 #undef PRG_SPMV
@@ -911,9 +918,14 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             h.st = reinterpret_cast<CgState*>(c->cg_state[k].reserve(size_t(cnt) * sizeof(CgState)));
             h.hst = hst + si;
             h.ev = new_event();
-            // the chain's SpMV workgroups: one round (two per CU), shared by tile count
-            const int64_t budget = int64_t(kSpmvWGPerCU) * std::max(1, c->num_cu);
-            int32_t wg = 0, vwg = 0, kc = 1;
+            // the chain's SpMV instance (cells per thread for its largest tile)
+            // and workgroups: one round (two or three per CU), shared by tile count
+            int32_t kc = 1;
+            for (int i = 0; i < cnt; i++)
+                kc = std::max(kc, (8 * std::max(1, c->plan_maxcount[h.depths[i]]) + kSpmvThreads - 1) / kSpmvThreads);
+            h.inst = spmv_instance(kc);
+            const int64_t budget = int64_t(spmv_wg_per_cu(h.inst)) * std::max(1, c->num_cu);
+            int32_t wg = 0, vwg = 0;
             size_t lds = 0;
             for (int i = 0; i < cnt; i++) {
                 const int d = h.depths[i];
@@ -943,12 +955,10 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                 const Stencil S = stencil_of(d);
                 for (int j = 0; j < 27; j++) g.L[j] = S.L[j];
                 const int mc = std::max(1, c->plan_maxcount[d]);
-                kc = std::max(kc, (8 * mc + kSpmvThreads - 1) / kSpmvThreads);
                 lds = std::max(lds, size_t(8) * 4 * mc);
             }
             h.nwg = wg;
             h.nvwg = vwg;
-            h.inst = spmv_instance(kc);
             h.lds = lds;
             si += cnt;
         }
