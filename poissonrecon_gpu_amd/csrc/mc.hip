@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(kBlock) k_coarse_rows(const int32_t* __restric
     for (int q = 0; q < kRowFloats / 4; q++) o[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
 }
 
-__global__ void __launch_bounds__(kBlock) k_point_values(const float* __restrict__ px, const float* __restrict__ py,
+__global__ void __launch_bounds__(kBlock, 6) k_point_values(const float* __restrict__ px, const float* __restrict__ py,
                                                          const float* __restrict__ pz,
                                                          const int32_t* __restrict__ point_leaf, int64_t n,
                                                          int64_t baseD, const uint32_t* __restrict__ key,
@@ -190,7 +190,7 @@ __global__ void __launch_bounds__(kBlock) k_corner_basis(const float* bf, int D,
 // f(corner) - iso for the owned corners (computeVertexImplicitFunctionValue,
 // main.cu:2259-2326): the chain sum of implicit_value at the corner's position,
 // with the basis values read from the corner basis table
-__global__ void __launch_bounds__(kBlock) k_corner_value(const int32_t* __restrict__ vlist, int64_t V,
+__global__ void __launch_bounds__(kBlock, 6) k_corner_value(const int32_t* __restrict__ vlist, int64_t V,
                                                          const uint32_t* __restrict__ key,
                                                          const int32_t* __restrict__ pnt, Levels L,
                                                          const int32_t* __restrict__ parent,
