@@ -612,8 +612,8 @@ __global__ void __launch_bounds__(kBlock) k_g_finish(const int32_t* list, const 
 
 // Depth D-1: every node's leaf range is its own sibling block of 8 leaf-level
 // slots (or empty), so one lane per slot computes its 27 terms and the 8 lanes
-// of a block add them with a fixed butterfly (xor 1, 2, 4: every lane ends
-// with the same sums); lane s stores positions j = s, s + 8, s + 16, s + 24.
+// of a block add them in a fixed order (a reduce-scatter: 28 double
+// exchanges instead of the 81 of a full butterfly).
 __global__ void __launch_bounds__(kBlock) k_g_parent_sums(const uint32_t* key, const int32_t* parent,
                                                           const float* vfx, const float* vfy, const float* vfz,
                                                           const float* T, int64_t baseD, int64_t MD, double* G,
@@ -627,18 +627,41 @@ __global__ void __launch_bounds__(kBlock) k_g_parent_sums(const uint32_t* key, c
     const float wz[3] = {T[rz], T[2 + rz], T[4 + rz]};
     float r[27];
     terms27(vfx[k], vfy[k], vfz[k], wx, wy, wz, r);
-    double acc[27];
+    // reduce-scatter over the block's 8 lanes (xor 4, 2, 1): each level
+    // exchanges half of the values still held; lane s ends with positions
+    // j0 .. j0 + 3, j0 = 16 b2 + 8 b1 + 4 b0 (b_i = bit i of s), its own part
+    // first in every sum
+    double v[32];
 #pragma unroll
-    for (int j = 0; j < 27; j++) acc[j] = double(r[j]);
+    for (int j = 0; j < 32; j++) v[j] = j < 27 ? double(r[j]) : 0.0;
+    const int s = int(k & 7);
+    const bool h2 = (s >> 2) & 1, h1 = (s >> 1) & 1, h0 = s & 1;
+    double k16[16];
 #pragma unroll
-    for (int m = 1; m < 8; m <<= 1)
+    for (int i = 0; i < 16; i++) {
+        const double give = h2 ? v[i] : v[16 + i];
+        const double keep = h2 ? v[16 + i] : v[i];
+        k16[i] = keep + __shfl_xor(give, 4, 64);
+    }
+    double k8[8];
 #pragma unroll
-        for (int j = 0; j < 27; j++) acc[j] += __shfl_xor(acc[j], m, 64);
+    for (int i = 0; i < 8; i++) {
+        const double give = h1 ? k16[i] : k16[8 + i];
+        const double keep = h1 ? k16[8 + i] : k16[i];
+        k8[i] = keep + __shfl_xor(give, 2, 64);
+    }
+    double k4[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double give = h0 ? k8[i] : k8[4 + i];
+        const double keep = h0 ? k8[4 + i] : k8[i];
+        k4[i] = keep + __shfl_xor(give, 1, 64);
+    }
     const int64_t a = parent[baseD + k];
-    const int sl = int(k & 7);
+    const int j0 = 16 * h2 + 8 * h1 + 4 * h0;
 #pragma unroll
-    for (int j = 0; j < 27; j++)
-        if ((j & 7) == sl) G[j * gstride + a] = acc[j];
+    for (int i = 0; i < 4; i++)
+        if (j0 + i < 27) G[(j0 + i) * gstride + a] = k4[i];
 }
 
 // b_i = sum_j G[neighbour_j(i)][j] for the nodes above the leaf level; the
@@ -647,7 +670,7 @@ __global__ void __launch_bounds__(kBlock) k_g_parent_sums(const uint32_t* key, c
 // G holds every position of a node with leaf-level slots below it (dnum > 0);
 // a node without any contributes no terms and its G is neither written nor
 // read.
-__global__ void __launch_bounds__(kBlock) k_div_from_g(const int32_t* pnt, const int64_t* pnt_off, Bases B, int D,
+__global__ void __launch_bounds__(kBlock, 6) k_div_from_g(const int32_t* pnt, const int64_t* pnt_off, Bases B, int D,
                                                        const int32_t* dnum, const double* G, int64_t gstride,
                                                        int64_t n, float* div) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -660,18 +683,22 @@ __global__ void __launch_bounds__(kBlock) k_div_from_g(const int32_t* pnt, const
         const int64_t t = i - B.b[d];
         int32_t nbr[27];
         pnt_neighbours(pnt + 27 * (pnt_off[d] + (t >> 3)), int(t & 7), nbr);
-        // loads issued together; G of a node without leaf-level slots is
-        // never written, and its value is read but not added
-        int32_t dn[27];
-        double g[27];
+        // loads issued nine at a time; G of a node without leaf-level slots
+        // is never written, and its value is read but not added
 #pragma unroll
-        for (int j = 0; j < 27; j++) {
-            const int64_t nb = B.b[d] + (nbr[j] < 0 ? 0 : nbr[j]);
-            dn[j] = dnum[nb];
-            g[j] = G[j * gstride + nb];
+        for (int j0 = 0; j0 < 27; j0 += 9) {
+            int32_t dn[9];
+            double g[9];
+#pragma unroll
+            for (int q = 0; q < 9; q++) {
+                const int j = j0 + q;
+                const int64_t nb = B.b[d] + (nbr[j] < 0 ? 0 : nbr[j]);
+                dn[q] = dnum[nb];
+                g[q] = G[j * gstride + nb];
+            }
+#pragma unroll
+            for (int q = 0; q < 9; q++) val = (nbr[j0 + q] < 0 || dn[q] == 0) ? val : val + g[q];
         }
-#pragma unroll
-        for (int j = 0; j < 27; j++) val = (nbr[j] < 0 || dn[j] == 0) ? val : val + g[j];
     }
     div[i] = float(val);
 }
