@@ -201,7 +201,7 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_init(const int32_t* pnt, int
 // neighbouring rows repeat (about 2 distinct blocks per block), so the SpMV
 // loads each distinct block once per tile as full 32-byte segments.
 __global__ void __launch_bounds__(kSpmvThreads) k_tile_plan(const int32_t* pnt, int64_t nblk, int32_t* count,
-                                                            int32_t* ubase, int16_t* rowid, int32_t* maxcount) {
+                                                            int32_t* ubase, int16_t* rowid) {
     __shared__ int32_t hkey[kHashSlots];
     __shared__ int16_t hslot[kHashSlots];
     __shared__ int16_t code[kTileEntries];
@@ -238,9 +238,27 @@ __global__ void __launch_bounds__(kSpmvThreads) k_tile_plan(const int32_t* pnt, 
         const int c = code[e];
         rowid[tile * kTileEntries + e] = c >= 0 ? hslot[c] : int16_t(-1);
     }
-    if (tid == 0) {
-        count[tile] = ucount;
-        atomicMax(maxcount, ucount);
+    if (tid == 0) count[tile] = ucount;
+}
+
+// Largest tile count of every depth (one workgroup per depth; the per-tile
+// counts of depth d are count[off[d] .. off[d + 1]))
+struct PlanOffsets {
+    int64_t off[kMaxDepth + 2];
+};
+__global__ void __launch_bounds__(1024) k_plan_max(const int32_t* count, PlanOffsets P, int32_t* maxcount) {
+    __shared__ int32_t red[1024 / 64];
+    const int d = blockIdx.x;
+    int32_t m = 0;
+    for (int64_t t = P.off[d] + threadIdx.x; t < P.off[d + 1]; t += 1024) m = max(m, count[t]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_down(m, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t r = 0;
+        for (int w = 0; w < 1024 / 64; w++) r = max(r, red[w]);
+        maxcount[d] = r;
     }
 }
 
@@ -770,7 +788,6 @@ void start_plans(prg_ctx* c) {
         PRG_HIP(hipEventRecord(c->plan_ev, c->stream));
         PRG_HIP(hipStreamWaitEvent(s, c->plan_ev, 0));
     }
-    PRG_HIP(hipMemsetAsync(mx, 0, size_t(D + 1) * 4, s));
     // the leaf depth first: the vector field and divergence wait for its plan
     for (int k = 0; k <= D; k++) {
         const int d = k == 0 ? D : k - 1;
@@ -779,8 +796,13 @@ void start_plans(prg_ctx* c) {
         const int64_t t0 = c->plan_tile_off[d];
         TimedScope ts(c->kt, "spmv_tile_plan", 13.5 * double(c->counts[d]));
         k_tile_plan<<<unsigned(nt), kSpmvThreads, 0, s>>>(c->pnt.get() + 27 * c->pnt_off[d], c->counts[d] / 8,
-                                                           cnt + t0, ub + t0 * kTileEntries, rw + t0 * kTileEntries,
-                                                           mx + d);
+                                                           cnt + t0, ub + t0 * kTileEntries, rw + t0 * kTileEntries);
+        PRG_CHECK_LAUNCH();
+    }
+    {
+        PlanOffsets P{};
+        for (int d = 0; d <= D + 1; d++) P.off[d] = c->plan_tile_off[d];
+        k_plan_max<<<unsigned(D + 1), 1024, 0, s>>>(cnt, P, mx);
         PRG_CHECK_LAUNCH();
     }
     PRG_HIP(hipMemcpyAsync(plan_maxcount_host(c), mx, size_t(D + 1) * 4, hipMemcpyDeviceToHost, s));
