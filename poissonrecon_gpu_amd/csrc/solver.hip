@@ -14,10 +14,11 @@
 //   k_cg_spmv    forms p_k = r + beta p_{k-1} on the fly for every row and
 //                neighbour it touches (p double-buffered, so the direction
 //                update needs no pass of its own), q = A p_k, FP64 partials of
-//                p.q; the last workgroup (atomic ticket) reduces the partials
-//                in a fixed order and sets alpha;
-//   k_cg_update  x += alpha p, r -= alpha q, FP64 partials of r.r; the last
-//                workgroup sets beta, rr and the stop flag.
+//                p.q;
+//   k_cg_update  every workgroup sums the p.q partials in a fixed order and
+//                forms alpha; x += alpha p, r -= alpha q, FP64 partials of r.r
+//                (summed the same way by the next SpMV for beta and the stop
+//                test).
 // Neighbours come from the parent-neighbour table (13.5 bytes per row instead
 // of 27 neighbour indices); the SpMV stages each sibling block's 4x4x4
 // neighbourhood in LDS.  Kernels return at once when the stop flag is
@@ -58,8 +59,6 @@ struct CgState {
     double rr;          // r.r at the last stop test
     double rr_hist[2];  // rr_k in slot k & 1
     int32_t done, iters;
-    uint32_t ticket;    // the final true-residual kernel's last-workgroup ticket
-    int32_t pad;
 };
 
 // One depth of a chain: its rows, its workgroup ranges in the chain's SpMV and
@@ -138,23 +137,6 @@ __device__ __forceinline__ double sum_partials(const double* __restrict__ part, 
     return block_allsum<BS>(s, lds);
 }
 
-// Publishes this workgroup's partial; returns true in every thread of the
-// last workgroup to finish, after which `partial` is complete (used once per
-// solve, by the final true-residual kernel).
-template <int BS>
-__device__ __forceinline__ bool publish(double blk, double* partial, CgState* st, unsigned* flag) {
-    if (threadIdx.x == 0) {
-        partial[blockIdx.x] = blk;
-        __threadfence();
-        const unsigned t = atomicAdd(&st->ticket, 1u);
-        *flag = (t == gridDim.x - 1) ? 1u : 0u;
-    }
-    __syncthreads();
-    const bool last = *flag != 0u;
-    if (last) __threadfence();
-    return last;
-}
-
 constexpr int kVecThreads = 512;  // init / update / refresh / true residual
 constexpr int kMaxWG = 1024;       // workgroups of a depth in the vector kernels (its partial count)
 constexpr int kMaxSpmvWG = 2048;   // workgroups of a depth in the SpMV (its partial count)
@@ -190,7 +172,6 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_init(const int32_t* pnt, int
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->done = 0;
         st->iters = 0;
-        st->ticket = 0u;
     }
 }
 
@@ -497,15 +478,11 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_refresh(const int32_t* pnt, 
     if (threadIdx.x == 0) rr_part[blockIdx.x] = sum;
 }
 
-// ||b - A x|| / ||b|| of the final iterate (FP32 SpMV, FP64 norm), the
-// recursive residual of the last iteration (its r.r partials, np of them, in
-// slot iters & 1) and the iteration count.
+// ||b - A x||^2 of the final iterate, one FP64 partial per workgroup (FP32
+// SpMV, FP64 sum of squares).
 __global__ void __launch_bounds__(kVecThreads) k_true_residual(const int32_t* pnt, int64_t n, Stencil S, const float* b,
-                                                               const float* x, double* partial,
-                                                               const double* rr_slots, int32_t np, int32_t slot_stride,
-                                                               CgState* st, double* out) {
+                                                               const float* x, double* partial) {
     __shared__ double lds[kVecThreads / 64];
-    __shared__ unsigned flag;
     double rr = 0.0;
     for (int64_t t = int64_t(blockIdx.x) * kVecThreads + threadIdx.x; t < n; t += int64_t(gridDim.x) * kVecThreads) {
         const float ax = spmv_pnt(pnt, t, x, S);
@@ -513,19 +490,25 @@ __global__ void __launch_bounds__(kVecThreads) k_true_residual(const int32_t* pn
         rr += rv * rv;
     }
     const double sum = block_allsum<kVecThreads>(rr, lds);
-    if (publish<kVecThreads>(sum, partial, st, &flag)) {
-        double s = 0.0;
-        for (int w = threadIdx.x; w < int(gridDim.x); w += kVecThreads) s += ((volatile const double*)partial)[w];
-        const double RR = block_allsum<kVecThreads>(s, lds);
-        const int32_t iters = st->iters;
-        const double rec = sum_partials<kVecThreads>(rr_slots + int64_t(iters & 1) * slot_stride, np, lds);
-        if (threadIdx.x == 0) {
-            const double bb = st->bb;
-            out[0] = (bb > 0.0) ? sqrt(RR / bb) : 0.0;
-            out[1] = (bb > 0.0) ? sqrt(rec / bb) : 0.0;
-            out[2] = double(iters);
-            st->ticket = 0u;
-        }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sum;
+}
+
+// One workgroup: ||b - A x|| / ||b|| from the true-residual partials (nt of
+// them), the recursive residual of the last iteration (its r.r partials, np
+// of them, in slot iters & 1) and the iteration count.
+__global__ void __launch_bounds__(kVecThreads) k_true_residual_finish(const double* partial, int32_t nt,
+                                                                      const double* rr_slots, int32_t np,
+                                                                      int32_t slot_stride, const CgState* st,
+                                                                      double* out) {
+    __shared__ double lds[kVecThreads / 64];
+    const double RR = sum_partials<kVecThreads>(partial, nt, lds);
+    const int32_t iters = st->iters;
+    const double rec = sum_partials<kVecThreads>(rr_slots + int64_t(iters & 1) * slot_stride, np, lds);
+    if (threadIdx.x == 0) {
+        const double bb = st->bb;
+        out[0] = (bb > 0.0) ? sqrt(RR / bb) : 0.0;
+        out[1] = (bb > 0.0) ? sqrt(rec / bb) : 0.0;
+        out[2] = double(iters);
     }
 }
 
@@ -1086,10 +1069,12 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                 {
                     TimedScope ts(c->kt, "cg_true_residual", kSpmvRowBytes * double(n));
                     k_true_residual<<<g.nvwg, kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], n,
-                                                                    stencil_of(d), c->div.get() + base, g.XS, g.bb,
-                                                                    g.rr, g.nvwg, kMaxWG, g.st, outv + 4 * d);
+                                                                    stencil_of(d), c->div.get() + base, g.XS, g.bb);
                     PRG_CHECK_LAUNCH();
                 }
+                k_true_residual_finish<<<1, kVecThreads, 0, h.s>>>(g.bb, g.nvwg, g.rr, g.nvwg, kMaxWG, g.st,
+                                                                  outv + 4 * d);
+                PRG_CHECK_LAUNCH();
                 PRG_HIP(hipMemcpyAsync(X + base, g.XS, size_t(n) * 4, hipMemcpyDeviceToDevice, h.s));
                 PRG_HIP(hipEventRecord(dend[d], h.s));
             }
