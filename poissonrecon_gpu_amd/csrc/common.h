@@ -169,6 +169,9 @@ void build_fs_tables(int D, FsTables* out);
 // ---- primitives (primitives.hip) ------------------------------------------------------
 struct Scratch;  // grows on demand
 // Exclusive scan of n int32 values; writes the total to *d_total (device) if non-null.
+// two arrays of the same length (exclusive scans and totals), in one pass each
+void scan_exclusive2(const int32_t* a_in, int32_t* a_out, int32_t* a_total, const int32_t* b_in, int32_t* b_out,
+                     int32_t* b_total, int64_t n, hipStream_t s, Scratch& scratch, KernelTimer& kt);
 void scan_exclusive(const int32_t* d_in, int32_t* d_out, int64_t n, int32_t* d_total, hipStream_t s,
                     Scratch& scratch, KernelTimer& kt);
 // Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the key.

@@ -808,8 +808,8 @@ void compute_divergence(prg_ctx* c) {
         }
         k_g_chunks<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->dnum.get(), n, first_skipped, chunks.get(), multi.get());
         PRG_CHECK_LAUNCH();
-        scan_exclusive(chunks.get(), coff.get(), n, ctot.get(), s, c->scratch, c->kt);
-        scan_exclusive(multi.get(), moff.get(), n, mtot.get(), s, c->scratch, c->kt);
+        scan_exclusive2(chunks.get(), coff.get(), ctot.get(), multi.get(), moff.get(), mtot.get(), n, s, c->scratch,
+                        c->kt);
         // Sizes from upper bounds, the counts stay on the device (no host
         // round trip): a node's chunks number at most 1 + dnum / kChunk and
         // every depth's dnum add up to the leaf-level slot count, so

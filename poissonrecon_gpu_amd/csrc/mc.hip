@@ -577,8 +577,8 @@ void extract_leaf_mesh(prg_ctx* c) {
                                                                    ecount.get(), eref.get());
             PRG_CHECK_LAUNCH();
         }
-        scan_exclusive(ecount.get(), ebase.get(), MD, etot.get(), s, c->scratch, kt);
-        scan_exclusive(ntri.get(), toff.get(), MD, ttot.get(), s, c->scratch, kt);
+        scan_exclusive2(ecount.get(), ebase.get(), etot.get(), ntri.get(), toff.get(), ttot.get(), MD, s, c->scratch,
+                        kt);
         PRG_HIP(hipMemcpyAsync(&NV, etot.get(), 4, hipMemcpyDeviceToHost, s));
         PRG_HIP(hipMemcpyAsync(&NT, ttot.get(), 4, hipMemcpyDeviceToHost, s));
         PRG_HIP(hipStreamSynchronize(s));

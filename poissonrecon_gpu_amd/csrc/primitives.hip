@@ -97,6 +97,77 @@ __global__ void __launch_bounds__(kScanBlock) k_scan_apply(const int32_t* in, in
     }
 }
 
+// Two arrays of the same length scanned together: one reduce, one
+// two-workgroup sums pass and one apply (sums_a, sums_b: the tile sums).
+__global__ void __launch_bounds__(kScanBlock) k_scan_reduce2(const int32_t* in_a, const int32_t* in_b, int64_t n,
+                                                               int32_t* sums_a, int32_t* sums_b) {
+    __shared__ int32_t w[kScanBlock / 64];
+    const int64_t base = int64_t(blockIdx.x) * kScanTile;
+    int32_t acc_a = 0, acc_b = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int64_t i = base + k * kScanBlock + threadIdx.x;
+        if (i < n) {
+            acc_a += in_a[i];
+            acc_b += in_b[i];
+        }
+    }
+    int32_t tot_a, tot_b;
+    (void)block_exclusive_scan(acc_a, w, tot_a);
+    (void)block_exclusive_scan(acc_b, w, tot_b);
+    if (threadIdx.x == 0) {
+        sums_a[blockIdx.x] = tot_a;
+        sums_b[blockIdx.x] = tot_b;
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_sums2(int32_t* sums_a, int32_t* sums_b, int64_t n, int32_t* total_a,
+                                                     int32_t* total_b) {
+    __shared__ int32_t w[16];
+    int32_t* sums = blockIdx.x == 0 ? sums_a : sums_b;
+    int32_t* d_total = blockIdx.x == 0 ? total_a : total_b;
+    int32_t carry = 0;
+    for (int64_t base = 0; base < n; base += 1024) {
+        int64_t i = base + threadIdx.x;
+        int32_t v = (i < n) ? sums[i] : 0;
+        int32_t tot;
+        int32_t ex = block_exclusive_scan(v, w, tot);
+        if (i < n) sums[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && d_total) *d_total = carry;
+}
+
+__global__ void __launch_bounds__(kScanBlock) k_scan_apply2(const int32_t* in_a, int32_t* out_a, const int32_t* in_b,
+                                                              int32_t* out_b, int64_t n, const int32_t* sums_a,
+                                                              const int32_t* sums_b) {
+    __shared__ int32_t w[kScanBlock / 64];
+    const int64_t base = int64_t(blockIdx.x) * kScanTile;
+    int32_t va[kScanItems], vb[kScanItems];
+    int32_t acc_a = 0, acc_b = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int64_t i = base + int64_t(threadIdx.x) * kScanItems + k;
+        va[k] = (i < n) ? in_a[i] : 0;
+        vb[k] = (i < n) ? in_b[i] : 0;
+        acc_a += va[k];
+        acc_b += vb[k];
+    }
+    int32_t tot;
+    int32_t ea = block_exclusive_scan(acc_a, w, tot) + sums_a[blockIdx.x];
+    int32_t eb = block_exclusive_scan(acc_b, w, tot) + sums_b[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        int64_t i = base + int64_t(threadIdx.x) * kScanItems + k;
+        if (i < n) {
+            out_a[i] = ea;
+            out_b[i] = eb;
+        }
+        ea += va[k];
+        eb += vb[k];
+    }
+}
+
 // ---- radix sort ------------------------------------------------------------------
 constexpr int kSortBlock = 256;           // 4 waves
 constexpr int kSortItems = 16;            // keys per thread
@@ -214,6 +285,24 @@ void scan_exclusive(const int32_t* d_in, int32_t* d_out, int64_t n, int32_t* d_t
     k_scan_sums<<<1, 1024, 0, s>>>(sums, tiles, d_total);
     PRG_CHECK_LAUNCH();
     k_scan_apply<<<unsigned(tiles), kScanBlock, 0, s>>>(d_in, d_out, n, sums);
+    PRG_CHECK_LAUNCH();
+}
+
+void scan_exclusive2(const int32_t* a_in, int32_t* a_out, int32_t* a_total, const int32_t* b_in, int32_t* b_out,
+                     int32_t* b_total, int64_t n, hipStream_t s, Scratch& scratch, KernelTimer& kt) {
+    if (n <= 0) {
+        scan_exclusive(a_in, a_out, n, a_total, s, scratch, kt);
+        scan_exclusive(b_in, b_out, n, b_total, s, scratch, kt);
+        return;
+    }
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    int32_t* sums = scratch.a.reserve(size_t(2 * tiles));
+    TimedScope ts(kt, "scan_exclusive", 16.0 * double(n));
+    k_scan_reduce2<<<unsigned(tiles), kScanBlock, 0, s>>>(a_in, b_in, n, sums, sums + tiles);
+    PRG_CHECK_LAUNCH();
+    k_scan_sums2<<<2, 1024, 0, s>>>(sums, sums + tiles, tiles, a_total, b_total);
+    PRG_CHECK_LAUNCH();
+    k_scan_apply2<<<unsigned(tiles), kScanBlock, 0, s>>>(a_in, a_out, b_in, b_out, n, sums, sums + tiles);
     PRG_CHECK_LAUNCH();
 }
 
