@@ -204,14 +204,6 @@ __global__ void __launch_bounds__(kBlock) k_heads(const uint32_t* key, int64_t n
 }
 
 // the same over the first *count of ub entries (0 beyond)
-__global__ void __launch_bounds__(kBlock) k_heads_dev(const uint32_t* key, const int32_t* count, int64_t ub, int shift,
-                                                      int32_t* flag) {
-    const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (i >= ub) return;
-    const int64_t n = *count;
-    flag[i] = (i < n && (i == 0 || (key[i] >> shift) != (key[i - 1] >> shift))) ? 1 : 0;
-}
-
 __global__ void __launch_bounds__(kBlock) k_unique(const uint32_t* key, const int32_t* flag, const int32_t* rank,
                                                    int64_t n, uint32_t* uk, int32_t* upidx) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -225,24 +217,6 @@ __global__ void __launch_bounds__(kBlock) k_excl_to_incl(int32_t* out, const int
                                                          int64_t n) {
     const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (i < n) out[i] = excl[i] + flag[i] - 1;
-}
-
-// Non-empty nodes of depth d (compact, NE_d entries, key order) -> their slot
-// in the depth-d level array and the compact arrays of depth d-1.
-__global__ void __launch_bounds__(kBlock) k_level_up(const uint32_t* nk, const int32_t* np, const int32_t* ndi,
-                                                     const int32_t* flag, const int32_t* brank, const int32_t* ne_dev,
-                                                     int64_t ub, int D, int d, int32_t* pos, uint32_t* pk,
-                                                     int32_t* pp, int32_t* pdi) {
-    const int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (q >= ub || q >= *ne_dev) return;
-    const int sh = 3 * (D - d);
-    const int32_t b = brank[q];
-    pos[q] = 8 * b + int32_t((nk[q] >> sh) & 7u);
-    if (flag[q]) {
-        pk[b] = nk[q] & ~(7u << sh);
-        pp[b] = np[q];
-        pdi[b] = (d == D) ? 8 * b : ndi[q];
-    }
 }
 
 __device__ __forceinline__ void level_init_slot(int64_t base, int64_t s, bool leaf, int32_t* pnum, int32_t* dnum,
@@ -387,6 +361,102 @@ __global__ void __launch_bounds__(kBlock) k_parent_neighbor_table(const int32_t*
     pnt_block(b, parent, child_base, pnt_prev, base_prev, base, pnt);
 }
 
+// ---- the bottom-up step of a large depth in three launches ---------------------------
+// The non-empty nodes of depth d (compact, key order) give their slot in the
+// depth-d level array and the compact arrays of depth d - 1: the parent-run
+// heads are counted per tile, the tile counts scanned (their total is the
+// parent depth's non-empty count), and the apply pass recomputes the heads,
+// scans them inside the tile and writes each node's slot and, per head, its
+// parent's key, first point and first leaf-level slot.
+constexpr int kLvlBlock = 256;
+constexpr int kLvlItems = 8;
+constexpr int kLvlTile = kLvlBlock * kLvlItems;
+
+__device__ __forceinline__ int32_t head_flag(const uint32_t* nk, int64_t i, int64_t n, int hs) {
+    return (i < n && (i == 0 || (nk[i] >> hs) != (nk[i - 1] >> hs))) ? 1 : 0;
+}
+
+// exclusive scan of one value per thread over the workgroup, and its total
+__device__ __forceinline__ int32_t lvl_block_scan(int32_t v, int32_t* lds, int32_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    int32_t before = 0, tot = 0;
+    for (int w = 0; w < int(blockDim.x >> 6); w++) {
+        const int32_t t = lds[w];
+        if (w < wave) before += t;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return before + x - v;
+}
+
+__global__ void __launch_bounds__(kLvlBlock) k_lvl_reduce(const uint32_t* nk, const int32_t* ne, int hs,
+                                                          int32_t* sums) {
+    __shared__ int32_t lds[kLvlBlock / 64];
+    const int64_t n = *ne, base = int64_t(blockIdx.x) * kLvlTile;
+    int32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < kLvlItems; k++) acc += head_flag(nk, base + int64_t(k) * kLvlBlock + threadIdx.x, n, hs);
+    int32_t tot;
+    (void)lvl_block_scan(acc, lds, tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// single workgroup: exclusive scan of the tile counts in place, total to *ne_parent
+__global__ void __launch_bounds__(1024) k_lvl_sums(int32_t* sums, int64_t tiles, int32_t* ne_parent) {
+    __shared__ int32_t lds[16];
+    int32_t carry = 0;
+    for (int64_t base = 0; base < tiles; base += 1024) {
+        const int64_t i = base + threadIdx.x;
+        const int32_t v = i < tiles ? sums[i] : 0;
+        int32_t tot;
+        const int32_t ex = lvl_block_scan(v, lds, tot);
+        if (i < tiles) sums[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *ne_parent = carry;
+}
+
+__global__ void __launch_bounds__(kLvlBlock) k_lvl_apply(const uint32_t* nk, const int32_t* np, const int32_t* ndi,
+                                                         const int32_t* ne, int hs, int D, int d, const int32_t* sums,
+                                                         int32_t* pos, uint32_t* pk, int32_t* pp, int32_t* pdi) {
+    __shared__ int32_t lds[kLvlBlock / 64];
+    const int64_t n = *ne, base = int64_t(blockIdx.x) * kLvlTile + int64_t(threadIdx.x) * kLvlItems;
+    const int sh = 3 * (D - d);
+    int32_t f[kLvlItems];
+    int32_t acc = 0;
+#pragma unroll
+    for (int k = 0; k < kLvlItems; k++) {
+        f[k] = head_flag(nk, base + k, n, hs);
+        acc += f[k];
+    }
+    int32_t tot;
+    int32_t ex = lvl_block_scan(acc, lds, tot) + sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kLvlItems; k++) {
+        const int64_t i = base + k;
+        if (i < n) {
+            const int32_t b = ex + f[k] - 1;  // parent block: heads up to i, inclusive, minus one
+            const uint32_t key = nk[i];
+            pos[i] = 8 * b + int32_t((key >> sh) & 7u);
+            if (f[k]) {
+                pk[b] = key & ~(7u << sh);
+                pp[b] = np[i];
+                pdi[b] = (d == D) ? 8 * b : ndi[i];
+            }
+        }
+        ex += f[k];
+    }
+}
+
 // ---- small depths in one workgroup -------------------------------------------------
 // The coarse depths hold few nodes; one 1024-thread workgroup runs their whole
 // per-depth sequence in a single launch instead of several launches per depth.
@@ -401,9 +471,9 @@ struct LevelArrays {
     int32_t* pos[kMaxDepth + 1];  // and slot in the depth's level array
 };
 
-// The bottom-up step (k_heads_dev, scan, k_excl_to_incl, k_level_up) for
-// depths ds .. 1: per depth, chunks of the compact nodes with a workgroup
-// scan of the parent-run heads carried across chunks.
+// The bottom-up step (k_lvl_reduce, k_lvl_sums, k_lvl_apply) for depths
+// ds .. 1: per depth, chunks of the compact nodes with a workgroup scan of
+// the parent-run heads carried across chunks.
 __global__ void __launch_bounds__(kSmallThreads) k_levels_up_small(LevelArrays A, int32_t* ne, int ds, int D) {
     __shared__ int32_t sc[kSmallThreads];
     __shared__ int32_t carry;
@@ -631,35 +701,27 @@ void build_octree(prg_ctx* c) {
     // 3. levels bottom-up: sizes stay on the device (upper bounds size the
     // launches); one read-back after the loop
     std::vector<int64_t> counts(D + 1, 0);
-    PoolBuf<int32_t> bflag{&c->pool[11]}, brank{&c->pool[12]};
-    bflag.reserve(ub[D]);
-    brank.reserve(ub[D]);
     // depths ds .. 1 (at most kSmallLevelUp non-empty nodes each) in one launch
     int ds = 0;
     while (ds < D && ub[ds + 1] <= kSmallLevelUp) ds++;
+    PoolBuf<int32_t> lsums{&c->pool[95]};
     for (int d = D; d > ds; --d) {
-        {
-            TimedScope ts(kt, "level_heads", 8.0 * double(ub[d]));
-            k_heads_dev<<<grid_for(ub[d], kBlock), kBlock, 0, s>>>(NK[d]->get(), ne_dev.get() + d, ub[d],
-                                                                   3 * (D - d + 1), bflag.get());
-            PRG_CHECK_LAUNCH();
-        }
-        scan_exclusive(bflag.get(), brank.get(), ub[d], ne_dev.get() + d - 1, s, c->scratch, kt);
         NK[d - 1]->reserve(ub[d - 1]);
         NP[d - 1]->reserve(ub[d - 1]);
         NDI[d - 1]->reserve(ub[d - 1]);
         POS[d]->reserve(ub[d]);
-        {
-            // block index of every non-empty node: heads before it, inclusive, minus one
-            TimedScope ts(kt, "level_up", 28.0 * double(ub[d]));
-            k_excl_to_incl<<<grid_for(ub[d], kBlock), kBlock, 0, s>>>(brank.get(), brank.get(), bflag.get(), ub[d]);
-            PRG_CHECK_LAUNCH();
-            k_level_up<<<grid_for(ub[d], kBlock), kBlock, 0, s>>>(NK[d]->get(), NP[d]->get(), NDI[d]->get(),
-                                                                   bflag.get(), brank.get(), ne_dev.get() + d, ub[d],
-                                                                   D, d, POS[d]->get(), NK[d - 1]->get(),
-                                                                   NP[d - 1]->get(), NDI[d - 1]->get());
-            PRG_CHECK_LAUNCH();
-        }
+        const int64_t tiles = (ub[d] + kLvlTile - 1) / kLvlTile;
+        int32_t* sums = lsums.reserve(tiles);
+        const int hs = 3 * (D - d + 1);
+        TimedScope ts(kt, "level_up", 28.0 * double(ub[d]));
+        k_lvl_reduce<<<unsigned(tiles), kLvlBlock, 0, s>>>(NK[d]->get(), ne_dev.get() + d, hs, sums);
+        PRG_CHECK_LAUNCH();
+        k_lvl_sums<<<1, 1024, 0, s>>>(sums, tiles, ne_dev.get() + d - 1);
+        PRG_CHECK_LAUNCH();
+        k_lvl_apply<<<unsigned(tiles), kLvlBlock, 0, s>>>(NK[d]->get(), NP[d]->get(), NDI[d]->get(),
+                                                           ne_dev.get() + d, hs, D, d, sums, POS[d]->get(),
+                                                           NK[d - 1]->get(), NP[d - 1]->get(), NDI[d - 1]->get());
+        PRG_CHECK_LAUNCH();
     }
     LevelArrays LA{};
     for (int d = ds; d >= 1; --d) {
