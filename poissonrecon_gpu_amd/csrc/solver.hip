@@ -709,6 +709,7 @@ struct Chain {
     int issued = 0;              // iterations queued
     int batch = 24;
     bool waiting = false;
+    bool ended = false;          // true residuals queued after the latest batch
     CgState* st = nullptr;       // device states, one per depth
     CgState* hst = nullptr;      // pinned host copies
     hipEvent_t ev = nullptr;     // marks the state copy
@@ -1104,8 +1105,16 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             for (int i = 0; i < kFirstBatch; i++)
                 for (int k = 0; k < kChains; k++)
                     if (!ch[k].depths.empty()) launch_iter(ch[k], i + 1);
+            // The first batch usually covers the whole solve, so each chain's
+            // true residuals and solution copy go right behind it (they are
+            // queued again, and supersede these, if more iterations follow);
+            // the end of a chain then waits for no host poll.
             for (int k = 0; k < kChains; k++)
-                if (!ch[k].depths.empty()) end_batch(ch[k]);
+                if (!ch[k].depths.empty()) {
+                    end_batch(ch[k]);
+                    end_chain(ch[k]);
+                    ch[k].ended = true;
+                }
         }
         while (true) {
             bool active = false;
@@ -1126,11 +1135,13 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                 for (const auto& l : h.launched)
                     if (l.second > last && all_done) c->kt.relabel(l.first, "cg_idle");
                 if (all_done) {
-                    end_chain(h);
+                    if (!h.ended) end_chain(h);
                     if (serial && serial_next < serial_order.size()) start_chain(serial_order[serial_next++]);
                 } else {
                     h.batch = 6;
                     issue_batch(h);
+                    h.ended = !serial;
+                    if (h.ended) end_chain(h);
                 }
             }
             if (!active) break;
