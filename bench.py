@@ -79,7 +79,7 @@ def run_step(g, dp, dn, n, pinned=None):
     g.build_octree()
     g.compute_divergence()
     stats = g.solve()
-    g.isovalue()
+    g._call("prg_isovalue", None)  # the value stays on the device for the mesh
     if pinned is None:
         mesh = g.extract_mesh()
         return stats, (len(mesh.vertices), len(mesh.triangles))

@@ -183,7 +183,9 @@ prg_status prg_compute_divergence(prg_ctx* ctx);
  * the cap (stats are still filled). */
 prg_status prg_solve(prg_ctx* ctx, prg_solve_stats* per_depth);
 
-/* Mean implicit-function value at the samples (main.cu:3480-3499). */
+/* Mean implicit-function value at the samples (main.cu:3480-3499).  The
+ * value stays in device memory for prg_extract_mesh; with out == NULL the
+ * call does not wait for it. */
 prg_status prg_isovalue(prg_ctx* ctx, float* out);
 /* Overrides the isovalue used by prg_extract_mesh (stage-wise parity). */
 prg_status prg_set_isovalue(prg_ctx* ctx, float iso);

@@ -290,14 +290,20 @@ prg_status prg_isovalue(prg_ctx* c, float* out) {
         prg::compute_isovalue(c);
         c->have_iso = true;
         c->iso_ready = true;
-        if (out) *out = c->iso;
+        if (out) {
+            PRG_HIP(hipMemcpyAsync(&c->iso, c->iso_dev.get(), 4, hipMemcpyDeviceToHost, c->stream));
+            PRG_HIP(hipStreamSynchronize(c->stream));
+            *out = c->iso;
+        }
         return PRG_OK;
     });
 }
 
 prg_status prg_set_isovalue(prg_ctx* c, float iso) {
     return guarded(c, [&]() {
+        PRG_HIP(hipSetDevice(c->cfg.device));
         c->iso = iso;
+        PRG_HIP(hipMemcpyAsync(c->iso_dev.reserve(1), &c->iso, 4, hipMemcpyHostToDevice, c->stream));
         c->iso_ready = true;
         return PRG_OK;
     });

@@ -105,6 +105,9 @@ struct prg_ctx {
     prg::DBuf<uint8_t> pool[128];
     float iso = 0.0f;
     bool iso_ready = false;  // iso computed or set by the caller
+    // the isovalue in device memory (read by the marching-cubes kernels);
+    // `iso` holds the last value read back or set by the caller
+    prg::DBuf<float> iso_dev;
 
     // leaf-level marching cubes results (device)
     prg::DBuf<float> corner_vals;  // 8 * M_D, square corner numbering

@@ -83,7 +83,9 @@ __global__ void __launch_bounds__(kBlock, 6) k_point_values(const float* __restr
 }
 
 // sum of n partials in a fixed order: contiguous runs per thread, then a tree
-__global__ void __launch_bounds__(1024) k_sum_partials(const double* partial, int64_t n, double* out) {
+// ... and the isovalue sum / count (main.cu:3494-3496) in float
+__global__ void __launch_bounds__(1024) k_sum_partials(const double* partial, int64_t n, int64_t count,
+                                                       float* iso) {
     __shared__ double red[1024];
     const int tid = threadIdx.x;
     const int64_t per = (n + 1023) / 1024;
@@ -96,7 +98,7 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const double* partial, in
         if (tid < o) red[tid] += red[tid + o];
         __syncthreads();
     }
-    if (tid == 0) *out = red[0];
+    if (tid == 0) *iso = float(red[0]) / float(count);
 }
 
 // corner owners (initVertexOwner, main.cu:1438-1498): corner j = x | y<<1 | z<<2.
@@ -196,7 +198,8 @@ __global__ void __launch_bounds__(kBlock, 6) k_corner_value(const int32_t* __res
                                                          const int32_t* __restrict__ parent,
                                                          const float* __restrict__ x, const float4* __restrict__ tab,
                                                          const float* __restrict__ rows, int rows_depth,
-                                                         int64_t baseD, int D, float iso, float* __restrict__ vval) {
+                                                         int64_t baseD, int D, const float* __restrict__ iso,
+                                                         float* __restrict__ vval) {
     const int64_t v = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (v >= V) return;
     const int64_t g = vlist[v];
@@ -241,7 +244,7 @@ __global__ void __launch_bounds__(kBlock, 6) k_corner_value(const int32_t* __res
     float tt = x[0] * tab[rx].y;
     tt = tt * tab[ry].y;
     tt = tt * tab[rz].y;
-    vval[v] = (val + tt) - iso;
+    vval[v] = (val + tt) - *iso;
 }
 
 // edge owners (initEdgeArray / maintainEdgeNodePointer, main.cu:1795-2082) and
@@ -458,9 +461,8 @@ void compute_isovalue(prg_ctx* c) {
     hipStream_t s = c->stream;
     const int64_t n = c->N;
     const unsigned nb = grid_for(n, kBlock);
-    PoolBuf<double> part{&c->pool[76]}, tot{&c->pool[77]};
+    PoolBuf<double> part{&c->pool[76]};
     part.reserve(nb);
-    tot.reserve(1);
     {
         TimedScope ts(c->kt, "point_values", 24.0 * double(n));
         const int rd = ensure_coarse_rows(c, s);
@@ -470,13 +472,9 @@ void compute_isovalue(prg_ctx* c) {
                                              c->coarse_rows.get(), rd, c->pvals.reserve(n), part.get());
         PRG_CHECK_LAUNCH();
     }
-    k_sum_partials<<<1, 1024, 0, s>>>(part.get(), nb, tot.get());
+    // the isovalue stays on the device (no host wait); prg_isovalue reads it back when asked
+    k_sum_partials<<<1, 1024, 0, s>>>(part.get(), nb, n, c->iso_dev.reserve(1));
     PRG_CHECK_LAUNCH();
-    double sum = 0.0;
-    PRG_HIP(hipMemcpyAsync(&sum, tot.get(), 8, hipMemcpyDeviceToHost, s));
-    PRG_HIP(hipStreamSynchronize(s));
-    // isoValue = sum / count (main.cu:3494-3496)
-    c->iso = float(sum) / float(n);
 }
 
 // pinned area of the owned-corner count (the last bytes of the context's
@@ -555,7 +553,8 @@ void extract_leaf_mesh(prg_ctx* c) {
             const int rd = ensure_coarse_rows(c, s);
             k_corner_value<<<grid_for(V, kBlock), kBlock, 0, s>>>(
                 vlist.get(), V, c->key.get(), c->pnt.get(), levels_of(c), c->parent.get(), c->x.get(),
-                reinterpret_cast<const float4*>(btab.get()), c->coarse_rows.get(), rd, baseD, D, c->iso, vval.get());
+                reinterpret_cast<const float4*>(btab.get()), c->coarse_rows.get(), rd, baseD, D, c->iso_dev.get(),
+                vval.get());
             PRG_CHECK_LAUNCH();
         }
         emask.reserve(MD);

@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(kBlockS) k_coarse_corner_owner(const int32_t* 
 __global__ void __launch_bounds__(kBlockS) k_coarse_corner_value(const int32_t* list, int64_t n, const uint32_t* key,
                                                                  const int32_t* pnt, Levels L, const int32_t* parent,
                                                                  const int32_t* child_base, const float* x,
-                                                                 const float* bf, int D, float iso, float* out) {
+                                                                 const float* bf, int D, const float* iso, float* out) {
     const int64_t i = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
     if (i >= n) return;
     const int64_t O = list[i] >> 3;
@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(kBlockS) k_coarse_corner_value(const int32_t* 
         nd = cb + ex;
         f = level_contrib(nd, dd, px, py, pz, key, pnt, L, x, bf, D, f);
     }
-    out[i] = f - iso;
+    out[i] = f - *iso;
 }
 
 // hasTriangle || hasIntersection for each coarse leaf (ProcessLeafNodesAtOtherDepth
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(kBlockS) k_vcorner_value(const int64_t* clist,
                                                            const uint32_t* key, const int32_t* neigh, int64_t M,
                                                            const int32_t* child_base, const int32_t* pnt, Levels Lv,
                                                            const int32_t* parent, const float* x, const float* bf,
-                                                           int D, float iso, float* vval) {
+                                                           int D, const float* iso, float* vval) {
     const int64_t i = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
     if (i >= n) return;
     const int64_t v = clist[i] >> 3;
@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(kBlockS) k_vcorner_value(const int64_t* clist,
         }
     }
     f = chain_value(L.rho, L.r, px, py, pz, key, pnt, Lv, parent, x, bf, D, f);
-    vval[i] = f - iso;
+    vval[i] = f - *iso;
 }
 
 // Corner values of each virtual leaf in square numbering (vertices[],
@@ -553,7 +553,7 @@ void extract_subdivision(prg_ctx* c) {
         TimedScope ts(kt, "coarse_corner_value", 24.0 * double(ncv));
         k_coarse_corner_value<<<grid_for(ncv, kBlockS), kBlockS, 0, s>>>(
             clist.get(), ncv, c->key.get(), c->pnt.get(), Lv, c->parent.get(), c->child_base.get(), c->x.get(),
-            c->d_base_fn.get(), D, c->iso, cv.get());
+            c->d_base_fn.get(), D, c->iso_dev.get(), cv.get());
         PRG_CHECK_LAUNCH();
     }
     // roots
@@ -625,7 +625,7 @@ void extract_subdivision(prg_ctx* c) {
         TimedScope ts(kt, "subdivide_corner_value", 24.0 * double(nvc));
         k_vcorner_value<<<grid_for(nvc, kBlockS), kBlockS, 0, s>>>(
             vlist.get(), nvc, G, c->key.get(), c->neigh.get(), M, c->child_base.get(), c->pnt.get(), Lv,
-            c->parent.get(), c->x.get(), c->d_base_fn.get(), D, c->iso, vval.get());
+            c->parent.get(), c->x.get(), c->d_base_fn.get(), D, c->iso_dev.get(), vval.get());
         PRG_CHECK_LAUNCH();
     }
     k_vcorner_square<<<gv, kBlockS, 0, s>>>(nv, G, c->key.get(), c->neigh.get(), M, D, cmask.get(), coff.get(),
