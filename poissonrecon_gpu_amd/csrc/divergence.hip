@@ -515,14 +515,33 @@ __global__ void __launch_bounds__(kBlock) k_g_chunk_map(const int32_t* chunk_off
     chunk_node[c] = int32_t(lo);
 }
 
-// one wave per chunk: 27 double accumulators per lane, cross-lane reduction
+// the 27 terms of one leaf slot for a node whose cell spans sc leaf cells
+// (offsets inside it from the slot's key, weights from the depth's table T)
+__device__ __forceinline__ void leaf_terms(uint32_t kl, float vx, float vy, float vz, const float* T, int sc,
+                                           uint32_t rmask, float r[27]) {
+    const uint32_t rx = compact3(kl >> 2) & rmask, ry = compact3(kl >> 1) & rmask, rz = compact3(kl) & rmask;
+    const float wx[3] = {T[rx], T[sc + rx], T[2 * sc + rx]};
+    const float wy[3] = {T[ry], T[sc + ry], T[2 * sc + ry]};
+    const float wz[3] = {T[rz], T[sc + rz], T[2 * sc + rz]};
+    terms27(vx, vy, vz, wx, wy, wz, r);
+}
+
+// the coarse depths' tables (depths 0 .. kMaxDepth - 2: 3 * 2^(D - d) floats
+// each) held in LDS by every k_g_sums block
+constexpr int kCoarseTableFloats = 3 * ((2 << kMaxDepth) - 4);
+
+// one wave per chunk: 27 double accumulators per lane, cross-lane reduction.
+// The tables of depths 0 .. D-2 (table[0 .. tcount)) are staged in LDS.
 __global__ void __launch_bounds__(kBlock) k_g_sums(const uint32_t* key, const int32_t* didx, const int32_t* dnum,
                                                    const float* vfx, const float* vfy, const float* vfz,
-                                                   const float* table, const int64_t* table_off,
+                                                   const float* table, int32_t tcount, const int64_t* table_off,
                                                    const int32_t* chunk_node, const int32_t* chunk_off,
                                                    const int32_t* chunks, const int32_t* multi_off,
                                                    const int32_t* d_nchunks, Bases B, int D, double* G,
                                                    int64_t gstride, double* partial) {
+    __shared__ float tab[kCoarseTableFloats];
+    for (int i = threadIdx.x; i < tcount; i += kBlock) tab[i] = table[i];
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int64_t nchunks = *d_nchunks;
     const int64_t wave0 = (int64_t(blockIdx.x) * kBlock + threadIdx.x) >> 6;
@@ -534,20 +553,32 @@ __global__ void __launch_bounds__(kBlock) k_g_sums(const uint32_t* key, const in
         const int64_t k0 = int64_t(didx[a]) + ci * kChunk;
         const int64_t k1 = min(k0 + kChunk, int64_t(didx[a]) + dnum[a]);
         const int d = depth_of(B, D, a);
-        const int64_t sc = int64_t(1) << (D - d);
+        const int sc = 1 << (D - d);
         const uint32_t rmask = uint32_t(sc - 1);
-        const float* T = table + table_off[d];
+        const float* T = tab + table_off[d];
         double acc[32];
 #pragma unroll
         for (int j = 0; j < 32; j++) acc[j] = 0.0;
-        for (int64_t k = k0 + lane; k < k1; k += 64) {
-            const uint32_t kl = key[baseD + k];
-            const int64_t rx = compact3(kl >> 2) & rmask, ry = compact3(kl >> 1) & rmask, rz = compact3(kl) & rmask;
-            const float wx[3] = {T[rx], T[sc + rx], T[2 * sc + rx]};
-            const float wy[3] = {T[ry], T[sc + ry], T[2 * sc + ry]};
-            const float wz[3] = {T[rz], T[sc + rz], T[2 * sc + rz]};
+        // full steps of 128 slots: two per lane, k and k + 64, both loaded
+        // before either's terms are computed; then the remaining slots one per
+        // lane.  Each lane adds its slots in increasing order.
+        int64_t kb = k0;
+        for (; kb + 128 <= k1; kb += 128) {
+            const int64_t k = kb + lane;
+            const uint32_t kla = key[baseD + k], klb = key[baseD + k + 64];
+            const float vxa = vfx[k], vya = vfy[k], vza = vfz[k];
+            const float vxb = vfx[k + 64], vyb = vfy[k + 64], vzb = vfz[k + 64];
             float r[27];
-            terms27(vfx[k], vfy[k], vfz[k], wx, wy, wz, r);
+            leaf_terms(kla, vxa, vya, vza, T, sc, rmask, r);
+#pragma unroll
+            for (int j = 0; j < 27; j++) acc[j] += double(r[j]);
+            leaf_terms(klb, vxb, vyb, vzb, T, sc, rmask, r);
+#pragma unroll
+            for (int j = 0; j < 27; j++) acc[j] += double(r[j]);
+        }
+        for (int64_t k = kb + lane; k < k1; k += 64) {
+            float r[27];
+            leaf_terms(key[baseD + k], vfx[k], vfy[k], vfz[k], T, sc, rmask, r);
 #pragma unroll
             for (int j = 0; j < 27; j++) acc[j] += double(r[j]);
         }
@@ -828,11 +859,14 @@ void compute_divergence(prg_ctx* c) {
         for (int d = 0; d <= D; d++) B.b[d] = c->bases[d];
         B.b[D + 1] = M;
         const int64_t waves = std::max<int64_t>(1, std::min<int64_t>(nchunks, 16384));
+        // depths 0 .. D-2 come first in the table, 3 * 2^(D - d) floats each
+        const int32_t tcount = D >= 2 ? int32_t(c->fs->div_off[D - 1]) : 0;
+        if (tcount > kCoarseTableFloats) throw Error{PRG_E_ARG, "divergence: coarse tables exceed their LDS area"};
         {
             // every leaf slot read once per coarser depth (key + vector field), G written once
             TimedScope ts(c->kt, "div_subtree_sums", 16.0 * double(MD) * double(D) + 216.0 * double(n));
             k_g_sums<<<grid_for(waves * 64, kBlock), kBlock, 0, s>>>(
-                c->key.get(), c->didx.get(), c->dnum.get(), VX, VY, VZ, c->d_div_table.get(), toff.get(), cnode.get(),
+                c->key.get(), c->didx.get(), c->dnum.get(), VX, VY, VZ, c->d_div_table.get(), tcount, toff.get(), cnode.get(),
                 coff.get(), chunks.get(), moff.get(), ctot.get(), B, D, G.get(), n, partial.get());
             PRG_CHECK_LAUNCH();
         }
