@@ -544,7 +544,8 @@ __global__ void __launch_bounds__(kBlock) k_g_sums(const uint32_t* key, const in
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int64_t nchunks = *d_nchunks;
-    const int64_t wave0 = (int64_t(blockIdx.x) * kBlock + threadIdx.x) >> 6;
+    // the chunk, its node and their ranges are wave-uniform (scalar loads)
+    const int64_t wave0 = __builtin_amdgcn_readfirstlane(int32_t((blockIdx.x * kBlock + threadIdx.x) >> 6));
     const int64_t nwaves = (int64_t(gridDim.x) * kBlock) >> 6;
     const int64_t baseD = B.b[D];
     for (int64_t c = wave0; c < nchunks; c += nwaves) {
