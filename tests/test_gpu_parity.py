@@ -9,6 +9,7 @@ order could differ), so every comparison isolates one stage:
   CG solution (same b, fixed iterations): rel L2 <= 1e-5
   implicit values at samples / corners (same x): bit-exact
   leaf-level mesh (same x and iso): bit-exact vertices and triangles
+  mesh from the device-resident isovalue: bit-exact with the host-set one
 """
 import numpy as np
 import pytest
@@ -169,3 +170,19 @@ def test_leaf_mesh_bit_exact(pair, prg):
     assert mg.vertices.shape == vo.shape and mg.triangles.shape == to.shape
     assert np.array_equal(mg.triangles, to), "triangle connectivity differs"
     assert np.array_equal(np.nan_to_num(mg.vertices).view(np.uint32), np.nan_to_num(vo).view(np.uint32))
+
+
+def test_isovalue_kept_on_device(pair, prg):
+    """prg_isovalue with out == NULL leaves the value in device memory for the
+    mesh: the mesh equals the one made after reading the value back and
+    setting it again from the host."""
+    name, D, g, o = pair
+    g.write_buffer(prg.BUF_SOLUTION, o.read_buffer(prg.BUF_SOLUTION))
+    g._call("prg_isovalue", None)
+    m_dev = g.extract_mesh(subdivide=False)
+    g.set_isovalue(g.isovalue())
+    m_host = g.extract_mesh(subdivide=False)
+    assert m_dev.vertices.shape == m_host.vertices.shape
+    assert np.array_equal(m_dev.triangles, m_host.triangles)
+    assert np.array_equal(np.nan_to_num(m_dev.vertices).view(np.uint32),
+                          np.nan_to_num(m_host.vertices).view(np.uint32))
