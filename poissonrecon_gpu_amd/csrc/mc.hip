@@ -26,14 +26,18 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// One coarse row per node of depths 1 .. Dr (the layout kRowFloats describes)
+// One row per node of depths 1 .. Dr (the layout kRowFloats describes).  The
+// evaluation chains visit only ancestors of leaf-level slots, so a node
+// without any below it (dnum 0) gets no row.
 __global__ void __launch_bounds__(kBlock) k_coarse_rows(const int32_t* __restrict__ pnt, Levels L,
                                                         const int32_t* __restrict__ parent,
+                                                        const int32_t* __restrict__ dnum,
                                                         const float* __restrict__ x, int Dr, int64_t n,
                                                         float* __restrict__ rows) {
     const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     if (r >= n) return;
     const int64_t i = L.base[1] + r;
+    if (dnum[i] == 0) return;
     int d = 1;
     while (d < Dr && i >= L.base[d + 1]) d++;
     const int64_t t = i - L.base[d];
@@ -221,8 +225,15 @@ __global__ void __launch_bounds__(kBlock, 6) k_corner_value(const int32_t* __res
         const int64_t o = 2 * G * d;
         const float4 tx = tab[o + rx], ty = tab[o + ry], tz = tab[o + rz];
         const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
-        val = level_sum(nd - L.base[d], d, bx, by, bz, pnt, L, x, val);
-        nd = parent[nd];
+        if (d <= rows_depth) {
+            // the node's row: its 27 neighbour values and its parent
+            const float* R = rows + kRowFloats * (nd - L.base[1]);
+            val = row_sum(R, bx, by, bz, val);
+            nd = __float_as_int(R[27]);
+        } else {
+            val = level_sum(nd - L.base[d], d, bx, by, bz, pnt, L, x, val);
+            nd = parent[nd];
+        }
     }
     if (d > 0) {
         int32_t u = __builtin_amdgcn_readfirstlane(int32_t(nd));
@@ -436,18 +447,18 @@ __global__ void __launch_bounds__(kBlock) k_tri_emit(const uint8_t* cubes, const
 
 }  // namespace
 
-// Coarse rows of depths 1 .. D - 2 for the current solution (rebuilt when the
+// Rows of depths 1 .. D - 1 (every depth above the leaf level) for the current solution (rebuilt when the
 // solution or the octree changes); returns the deepest depth they cover (0:
 // none).
 int ensure_coarse_rows(prg_ctx* c, hipStream_t s) {
-    const int D = c->D, Dr = D - 2;
+    const int D = c->D, Dr = D - 1;
     if (Dr < 1) return 0;
     if (c->rows_x == c->x_gen && c->rows_tree == c->tree_gen) return c->rows_depth;
     const int64_t n = c->bases[Dr + 1] - c->bases[1];
     float* rows = c->coarse_rows.reserve(size_t(kRowFloats) * std::max<int64_t>(1, n));
     TimedScope ts(c->kt, "coarse_rows", 128.0 * double(n));
-    k_coarse_rows<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->pnt.get(), levels_of(c), c->parent.get(), c->x.get(),
-                                                         Dr, n, rows);
+    k_coarse_rows<<<grid_for(n, kBlock), kBlock, 0, s>>>(c->pnt.get(), levels_of(c), c->parent.get(),
+                                                         c->dnum.get(), c->x.get(), Dr, n, rows);
     PRG_CHECK_LAUNCH();
     c->rows_depth = Dr;
     c->rows_x = c->x_gen;

@@ -217,8 +217,17 @@ __device__ __forceinline__ float leaf_chain_value(int64_t nd, float px, float py
     int d = D;
     for (; d > 0; d--) {
         if (wave_uniform(int32_t(nd))) break;
-        val = level_contrib(nd, d, px, py, pz, key, pnt, L, x, bf, D, val);
-        nd = parent[nd];
+        if (d <= rows_depth) {
+            // the node's row: its 27 neighbour values and its parent
+            const float* R = rows + kRowFloats * (nd - L.base[1]);
+            float bx[3], by[3], bz[3];
+            node_basis(nd, d, px, py, pz, key, bf, D, bx, by, bz);
+            val = row_sum(R, bx, by, bz, val);
+            nd = __float_as_int(R[27]);
+        } else {
+            val = level_contrib(nd, d, px, py, pz, key, pnt, L, x, bf, D, val);
+            nd = parent[nd];
+        }
     }
     if (d > 0) {
         int32_t u = __builtin_amdgcn_readfirstlane(int32_t(nd));
