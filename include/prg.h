@@ -206,6 +206,18 @@ void prg_mesh_release(prg_mesh* mesh);
 prg_status prg_extract_mesh_into(prg_ctx* ctx, int32_t subdivide, float* vertices, int64_t vcap, int32_t* triangles,
                                  int64_t tcap, prg_mesh* out);
 
+/* Optional cross-batch vertex welding (no counterpart in the reference, which
+ * leaves the vertices that the leaf-level and subdivision batches share at
+ * their seams duplicated, main.cu:3793,4183,4543).  Vertices whose three
+ * coordinates are bit-identical are merged into their first occurrence: the
+ * first occurrences keep the original order, vertices are compacted in place
+ * (mesh->vertices is not reallocated), every triangle entry is renumbered,
+ * and n_vertices / n_leaf_vertices are reduced accordingly; triangles are
+ * kept (one whose corners merge stays, degenerate).  Runs on ctx's GPU.
+ * n_merged (may be NULL) receives the number of vertices removed.  Every
+ * triangle entry must lie in [0, n_vertices) (PRG_E_ARG otherwise). */
+prg_status prg_weld_mesh(prg_ctx* ctx, prg_mesh* mesh, int64_t* n_merged);
+
 /* After a prg_extract_mesh* call with subdivide != 0: the number of coarse
  * leaves that were refined (SubdivideNum, main.cu:3826-3834) and of the
  * virtual depth-D leaves marching cubes ran over. */

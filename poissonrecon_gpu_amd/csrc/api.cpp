@@ -409,6 +409,26 @@ prg_status prg_subdivision_info(prg_ctx* c, int64_t* n_roots, int64_t* n_virtual
     });
 }
 
+prg_status prg_weld_mesh(prg_ctx* c, prg_mesh* mesh, int64_t* n_merged) {
+    return guarded(c, [&]() {
+        if (!mesh) return fail(c, PRG_E_ARG, "null mesh");
+        const int64_t n = mesh->n_vertices, m = mesh->n_triangles;
+        if (n < 0 || m < 0 || (n > 0 && !mesh->vertices) || (m > 0 && !mesh->triangles))
+            return fail(c, PRG_E_ARG, "mesh arrays missing");
+        if (n >= (int64_t(1) << 31)) return fail(c, PRG_E_ARG, "more than 2^31 - 1 vertices");
+        for (int64_t i = 0; i < 3 * m; i++)
+            if (mesh->triangles[i] < 0 || mesh->triangles[i] >= n)
+                return fail(c, PRG_E_ARG, "triangle index out of range");
+        PRG_HIP(hipSetDevice(c->cfg.device));
+        int64_t nv = n, nleaf = mesh->n_leaf_vertices;
+        prg::weld_mesh(c, mesh->vertices, n, mesh->triangles, m, mesh->n_leaf_vertices, &nv, &nleaf);
+        if (n_merged) *n_merged = n - nv;
+        mesh->n_vertices = nv;
+        mesh->n_leaf_vertices = nleaf;
+        return PRG_OK;
+    });
+}
+
 void prg_mesh_release(prg_mesh* m) {
     if (!m) return;
     std::free(m->vertices);

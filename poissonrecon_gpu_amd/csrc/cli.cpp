@@ -4,6 +4,11 @@
 //
 //   poisson_recon --in points.ply --out mesh.ply [--depth 8] [--tol 1e-6]
 //                 [--maxIters 2000] [--refresh 20] [--device 0] [--json]
+//                 [--subdivide] [--weld]
+//
+// --subdivide adds the coarse-leaf subdivision batches (main.cu:3799-4561);
+// --weld merges the vertices those batches duplicate (prg_weld_mesh, which
+// the reference does not do).
 //
 // Input formats by extension (.ply, .bnpts, ASCII otherwise, main.cu:517-520);
 // output is an ASCII PLY triangle mesh (PlyWriteTriangles, plyfile.cu:2769).
@@ -20,7 +25,7 @@ namespace {
 void usage() {
     std::fprintf(stderr,
                  "usage: poisson_recon --in <points> --out <mesh.ply> [--depth D] [--tol T] [--maxIters N]\n"
-                 "                     [--refresh K] [--device I] [--json]\n");
+                 "                     [--refresh K] [--device I] [--json] [--subdivide] [--weld]\n");
 }
 
 const char* kStageNames[PRG_STAGE_COUNT] = {
@@ -34,7 +39,7 @@ int main(int argc, char** argv) {
     std::string in, out;
     prg_config cfg;
     prg_config_default(&cfg);
-    bool json = false;
+    bool json = false, subdivide = false, weld = false;
     for (int i = 1; i < argc; i++) {
         std::string a = argv[i];
         auto next = [&]() -> const char* {
@@ -52,6 +57,8 @@ int main(int argc, char** argv) {
         else if (a == "--refresh") cfg.cg_refresh = std::atoi(next());
         else if (a == "--device") cfg.device = std::atoi(next());
         else if (a == "--json") json = true;
+        else if (a == "--subdivide") subdivide = true;
+        else if (a == "--weld") weld = true;
         else {
             usage();
             return 2;
@@ -79,13 +86,20 @@ int main(int argc, char** argv) {
     prg_solve_stats stats[11] = {};
     st = prg_set_points(ctx, xyz, nxyz, n, 0);
     prg_points_release(xyz, nxyz);
-    if (st == PRG_OK) st = prg_reconstruct(ctx, 0, &mesh, stats);
+    if (st == PRG_OK) st = prg_reconstruct(ctx, subdivide ? 1 : 0, &mesh, stats);
     if (st != PRG_OK && st != PRG_E_NOT_CONVERGED) {
         std::fprintf(stderr, "reconstruction failed (status %d): %s\n", int(st), prg_last_error(ctx));
         prg_destroy(ctx);
         return 1;
     }
     if (st == PRG_E_NOT_CONVERGED) std::fprintf(stderr, "warning: %s\n", prg_last_error(ctx));
+    int64_t merged = 0;
+    if (weld && prg_weld_mesh(ctx, &mesh, &merged) != PRG_OK) {
+        std::fprintf(stderr, "welding failed: %s\n", prg_last_error(ctx));
+        prg_mesh_release(&mesh);
+        prg_destroy(ctx);
+        return 1;
+    }
     prg_status ws = prg_write_mesh_ply(out.c_str(), &mesh);
     prg_timings t{};
     prg_stage_times(ctx, &t);
@@ -106,6 +120,7 @@ int main(int argc, char** argv) {
                         (long long)stats[d].nodes, stats[d].iters, stats[d].true_rel_residual);
         std::printf("Vertices:%lld Triangles:%lld Total:%.3fms\n", (long long)mesh.n_vertices,
                     (long long)mesh.n_triangles, total);
+        if (weld) std::printf("Welded vertices:%lld\n", (long long)merged);
     }
     prg_mesh_release(&mesh);
     prg_destroy(ctx);

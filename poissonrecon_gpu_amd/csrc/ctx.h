@@ -141,6 +141,10 @@ void normalize_points(prg_ctx* c, const float* xyz, const float* nxyz, int64_t n
 void build_octree(prg_ctx* c);
 // px .. nz in input order (before an octree is built)
 void unpack_points(prg_ctx* c);
+// merges bit-identical vertices of a host mesh in place (triangle entries
+// already checked to lie in [0, n))
+void weld_mesh(prg_ctx* c, float* vertices, int64_t n, int32_t* triangles, int64_t m, int64_t n_leaf,
+               int64_t* n_out, int64_t* n_leaf_out);
 // the 27-neighbour planes of every depth (kernels use the parent-neighbour
 // rows; these serve level views, buffer reads and the subdivision)
 void ensure_neighbours(prg_ctx* c);
