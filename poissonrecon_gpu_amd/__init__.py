@@ -277,7 +277,7 @@ class Reconstructor:
     def set_isovalue(self, iso: float) -> None:
         self._call("prg_set_isovalue", C.c_float(iso))
 
-    def extract_mesh(self, subdivide: bool = False) -> Mesh:
+    def extract_mesh(self, subdivide: bool = True) -> Mesh:
         m = PrgMesh()
         self._call("prg_extract_mesh", int(subdivide), C.byref(m))
         try:
@@ -449,7 +449,7 @@ class UNet:
         return loss.value
 
 
-def reconstruct_batch(clouds, depths, concurrency: int = 4, subdivide: bool = False, device: int = 0,
+def reconstruct_batch(clouds, depths, concurrency: int = 4, subdivide: bool = True, device: int = 0,
                       cg_tol: float = 1e-6, return_meshes: bool = False):
     """Reconstructs independent clouds [(xyz, nxyz), ...] in one pass
     (prg_reconstruct_batch).  Returns per-cloud dicts: status, mesh sizes, ms,

@@ -543,13 +543,19 @@ prg_status prg_write_buffer(prg_ctx* c, int32_t which, const void* host, int64_t
         float* dst = (which == PRG_BUF_DIVERGENCE) ? c->div.reserve(c->M) : c->x.reserve(c->M);
         PRG_HIP(hipMemcpyAsync(dst, host, size_t(b), hipMemcpyHostToDevice, c->stream));
         PRG_HIP(hipStreamSynchronize(c->stream));
+        // results derived from the overwritten buffer are stale: a new
+        // divergence invalidates the solution, a new solution the isovalue
+        // and the values at the samples (prg_isovalue or prg_set_isovalue
+        // must run again before a mesh is extracted)
         if (which == PRG_BUF_DIVERGENCE) {
             prg::ensure_tables(c);
             c->have_div = true;
+            c->have_x = false;
         } else {
             c->have_x = true;
             c->x_gen++;
         }
+        c->have_iso = c->iso_ready = false;
         return PRG_OK;
     });
 }

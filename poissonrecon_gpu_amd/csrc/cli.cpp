@@ -4,11 +4,14 @@
 //
 //   poisson_recon --in points.ply --out mesh.ply [--depth 8] [--tol 1e-6]
 //                 [--maxIters 2000] [--refresh 20] [--device 0] [--json]
-//                 [--subdivide] [--weld]
+//                 [--no-subdivide] [--weld]
 //
-// --subdivide adds the coarse-leaf subdivision batches (main.cu:3799-4561);
-// --weld merges the vertices those batches duplicate (prg_weld_mesh, which
-// the reference does not do).
+// The mesh holds the leaf-level batch and the coarse-leaf subdivision batches,
+// as the reference's main() always writes (main.cu:3799-4561);
+// --no-subdivide keeps only the leaf-level batch (main.cu:3793), and
+// --subdivide is accepted for older scripts (it is the default).  --weld
+// merges the vertices the batches duplicate (prg_weld_mesh, which the
+// reference does not do).
 //
 // Input formats by extension (.ply, .bnpts, ASCII otherwise, main.cu:517-520);
 // output is an ASCII PLY triangle mesh (PlyWriteTriangles, plyfile.cu:2769).
@@ -25,7 +28,7 @@ namespace {
 void usage() {
     std::fprintf(stderr,
                  "usage: poisson_recon --in <points> --out <mesh.ply> [--depth D] [--tol T] [--maxIters N]\n"
-                 "                     [--refresh K] [--device I] [--json] [--subdivide] [--weld]\n");
+                 "                     [--refresh K] [--device I] [--json] [--no-subdivide] [--weld]\n");
 }
 
 const char* kStageNames[PRG_STAGE_COUNT] = {
@@ -39,7 +42,7 @@ int main(int argc, char** argv) {
     std::string in, out;
     prg_config cfg;
     prg_config_default(&cfg);
-    bool json = false, subdivide = false, weld = false;
+    bool json = false, subdivide = true, weld = false;
     for (int i = 1; i < argc; i++) {
         std::string a = argv[i];
         auto next = [&]() -> const char* {
@@ -58,6 +61,7 @@ int main(int argc, char** argv) {
         else if (a == "--device") cfg.device = std::atoi(next());
         else if (a == "--json") json = true;
         else if (a == "--subdivide") subdivide = true;
+        else if (a == "--no-subdivide") subdivide = false;
         else if (a == "--weld") weld = true;
         else {
             usage();

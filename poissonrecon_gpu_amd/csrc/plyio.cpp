@@ -227,10 +227,12 @@ prg_status prg_write_mesh_ply(const char* path, const prg_mesh* m) {
             const float t = m->vertices[3 * i + a] * m->scale;
             o[a] = t + m->center[a];
         }
-        std::fprintf(f.f, "%g %g %g\n", double(o[0]), double(o[1]), double(o[2]));
+        // every item is followed by one space, every element by a newline
+        // (ply_put_element / write_ascii_item, plyfile.cu:597-626,2117-2147)
+        std::fprintf(f.f, "%g %g %g \n", double(o[0]), double(o[1]), double(o[2]));
     }
     for (int64_t i = 0; i < m->n_triangles; i++)
-        std::fprintf(f.f, "3 %d %d %d\n", m->triangles[3 * i], m->triangles[3 * i + 1], m->triangles[3 * i + 2]);
+        std::fprintf(f.f, "3 %d %d %d \n", m->triangles[3 * i], m->triangles[3 * i + 1], m->triangles[3 * i + 2]);
     return std::ferror(f.f) ? PRG_E_IO : PRG_OK;
 }
 

@@ -66,8 +66,10 @@ def test_weld_rejects_bad_index(prg):
         assert removed == 0 and len(wm.vertices) == 0
 
 
-def test_cli_subdivide_weld(prg, tmp_path):
-    """The CLI with --subdivide --weld writes the welded mesh of the API path."""
+def test_cli_default_subdivides_and_welds(prg, tmp_path):
+    """The CLI writes the reference main()'s mesh by default, leaf-level batch
+    plus subdivision batches (main.cu:3799-4561); --weld writes the welded mesh
+    of the API path, --no-subdivide the leaf-level batch alone."""
     import json
     import subprocess
     from pathlib import Path
@@ -75,21 +77,28 @@ def test_cli_subdivide_weld(prg, tmp_path):
     p, n = prg.synth_sphere(35947, 1)
     src = tmp_path / "sphere.bnpts"
     np.hstack([p, n]).astype(np.float32).tofile(src)
-    out = tmp_path / "mesh.ply"
     exe = Path(prg.__file__).resolve().parent / "poisson_recon"
-    print(f"\n[cli: {exe.name} --depth 7 --subdivide --weld]", flush=True)
-    r = subprocess.run([str(exe), "--in", str(src), "--out", str(out), "--depth", "7", "--subdivide", "--weld",
-                        "--json"], capture_output=True, text=True, timeout=120)
-    assert r.returncode == 0, r.stderr
-    info = json.loads(r.stdout.strip().splitlines()[-1])
     with prg.Reconstructor(max_depth=7) as g:
         g.set_points(p, n)
         g.build_octree()
         g.compute_divergence()
         g.solve()
         g.isovalue()
-        wm, removed = g.weld_mesh(g.extract_mesh(subdivide=True))
-    assert removed > 0
-    assert info["vertices"] == len(wm.vertices) and info["triangles"] == len(wm.triangles)
-    head = out.read_bytes()[:400].decode("ascii", "replace")
-    assert f"element vertex {len(wm.vertices)}" in head and f"element face {len(wm.triangles)}" in head
+        full = g.extract_mesh(subdivide=True)
+        leaf = g.extract_mesh(subdivide=False)
+        wm, removed = g.weld_mesh(full)
+    assert removed > 0 and len(full.triangles) > len(leaf.triangles)
+    for flags, want in (([], full), (["--weld"], wm), (["--no-subdivide"], leaf)):
+        out = tmp_path / "mesh.ply"
+        print(f"\n[cli: {exe.name} --depth 7 {' '.join(flags)}]", flush=True)
+        r = subprocess.run([str(exe), "--in", str(src), "--out", str(out), "--depth", "7", *flags, "--json"],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        info = json.loads(r.stdout.strip().splitlines()[-1])
+        assert info["vertices"] == len(want.vertices) and info["triangles"] == len(want.triangles), flags
+        body = out.read_bytes()
+        head = body[:400].decode("ascii", "replace")
+        assert f"element vertex {len(want.vertices)}" in head and f"element face {len(want.triangles)}" in head
+        faces = body.split(b"end_header\n", 1)[1].splitlines()[len(want.vertices):]
+        expect = [("3 " + "".join("%d " % i for i in tri)).encode() for tri in want.triangles]
+        assert faces == expect, flags

@@ -58,14 +58,24 @@ def test_truncated_bnpts_and_missing_file(prg, tmp_path):
 
 
 def test_ply_writer(prg, tmp_path):
-    v = np.array([[0.1, 0.2, 0.3], [0.4, 0.5, 0.6], [0.7, 0.8, 0.9]], np.float32)
-    mesh = prg.Mesh(v, np.array([[0, 1, 2]], np.int32), np.array([1.0, 2.0, 3.0], np.float32), 2.0, 3, 1)
+    """Byte-for-byte the layout of the reference's ASCII writer: the header of
+    ply_header_complete (plyfile.cu:470-531) for PlyWriteTriangles' two
+    elements (:2769-2793), then per element each item as "%g " (float) or
+    "%u "/"%d " (uchar count, int index) and a newline (ply_put_element,
+    write_ascii_item, :567-626,2102-2147).  Vertices are v * scale + center
+    in float (:2800-2806)."""
+    v = np.array([[0.1, 0.2, 0.3], [0.4, 0.5, 0.6], [0.7, 0.8, 0.9], [-1.5e-7, 12345.678, 0.0]], np.float32)
+    t = np.array([[0, 1, 2], [3, 2, 1]], np.int32)
+    center, scale = np.array([1.0, 2.0, -3.0], np.float32), np.float32(2.0)
+    mesh = prg.Mesh(v, t, center, float(scale), 4, 2)
     path = tmp_path / "m.ply"
     prg.write_mesh_ply(str(path), mesh)
-    lines = path.read_text().splitlines()
-    assert lines[:3] == ["ply", "format ascii 1.0", "element vertex 3"]
-    assert "element face 1" in lines and "property list uchar int vertex_indices" in lines
-    body = lines[lines.index("end_header") + 1:]
-    w = (v * np.float32(2.0)) + np.float32([1.0, 2.0, 3.0])
-    assert body[0] == " ".join("%g" % float(x) for x in w[0])
-    assert body[3] == "3 0 1 2"
+    w = (v * scale) + center  # float32 arithmetic, as the reference's host loop
+    expect = ("ply\nformat ascii 1.0\nelement vertex 4\nproperty float x\nproperty float y\nproperty float z\n"
+              "element face 2\nproperty list uchar int vertex_indices\nend_header\n")
+    for row in w:
+        expect += "".join("%g " % float(x) for x in row) + "\n"
+    for tri in t:
+        expect += "%u " % 3 + "".join("%d " % int(i) for i in tri) + "\n"
+    assert path.read_bytes() == expect.encode()
+    assert path.read_bytes().splitlines()[-1] == b"3 3 2 1 "
