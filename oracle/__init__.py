@@ -46,7 +46,9 @@ def lib() -> C.CDLL:
             "orc_extract_leaf_mesh": (C.c_int32, [P, P, P]),
             "orc_copy_mesh": (C.c_int32, [P, P, P]),
             "orc_extract_subdivision": (C.c_int32, [P, P, P]),
+            "orc_extract_subdivision_from": (C.c_int32, [P, C.c_int32, P, P]),
             "orc_subdivision_info": (None, [P, P, P]),
+            "orc_subdivision_roots": (C.c_int32, [P, P]),
             "orc_buffer_size": (C.c_int64, [P, C.c_int32]),
             "orc_read_buffer": (C.c_int32, [P, C.c_int32, P, C.c_int64]),
             "orc_write_buffer": (C.c_int32, [P, C.c_int32, P, C.c_int64]),
@@ -136,11 +138,12 @@ class Oracle:
         lib().orc_copy_mesh(self._h, _p(v), _p(t))
         return v, t
 
-    def extract_subdivision(self):
-        """Appends the coarse-leaf subdivision batches; returns the whole mesh."""
+    def extract_subdivision(self, min_root_depth=0):
+        """Appends the coarse-leaf subdivision batches (those of the groups whose
+        roots lie at min_root_depth or deeper); returns the whole mesh."""
         nv = np.zeros(1, np.int64)
         nt = np.zeros(1, np.int64)
-        assert lib().orc_extract_subdivision(self._h, _p(nv), _p(nt)) == 0
+        assert lib().orc_extract_subdivision_from(self._h, int(min_root_depth), _p(nv), _p(nt)) == 0
         v = np.zeros((int(nv[0]), 3), np.float32)
         t = np.zeros((int(nt[0]), 3), np.int32)
         lib().orc_copy_mesh(self._h, _p(v), _p(t))
@@ -152,6 +155,12 @@ class Oracle:
         b = np.zeros(1, np.int64)
         lib().orc_subdivision_info(self._h, _p(a), _p(b))
         return int(a[0]), int(b[0])
+
+    def subdivision_roots(self):
+        """Subdivision roots per depth 0..D (after extract_leaf_mesh; nothing is meshed)."""
+        a = np.zeros(self.max_depth + 1, np.int64)
+        assert lib().orc_subdivision_roots(self._h, _p(a)) == 0
+        return a
 
     def read_buffer(self, which: int) -> np.ndarray:
         n = lib().orc_buffer_size(self._h, which)

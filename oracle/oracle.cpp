@@ -1300,8 +1300,10 @@ void process_group(orc_state* s, const std::vector<int64_t>& roots) {
 
 }  // namespace
 
-int32_t orc_extract_subdivision(orc_state* s, int64_t* n_vertices, int64_t* n_triangles) {
-    if (s->node_centers.empty()) return 6;  // needs orc_extract_leaf_mesh first
+namespace {
+// ProcessLeafNodesAtOtherDepth + ifSubdivide (main.cu:2827-2992): the coarse
+// leaves that are refined, in NodeArray order.
+std::vector<int64_t> select_roots(orc_state* s) {
     const int D = s->D;
     const int64_t M = s->M, baseD = s->bases[D], MD = s->MD();
     const std::vector<float>& C = s->node_centers;
@@ -1396,6 +1398,17 @@ int32_t orc_extract_subdivision(orc_state* s, int64_t* n_vertices, int64_t* n_tr
             if (inter[faces[6 * i + j] - 1]) hit = true;
         if (tri || hit) roots.push_back(i);
     }
+    return roots;
+}
+}  // namespace
+
+// Appends the subdivision batches of the groups whose roots lie at depth
+// min_depth or deeper (0: every group).  Groups come in root-depth order, so
+// these batches are the tail of the full mesh.
+int32_t orc_extract_subdivision_from(orc_state* s, int32_t min_depth, int64_t* n_vertices, int64_t* n_triangles) {
+    if (s->node_centers.empty()) return 6;  // needs orc_extract_leaf_mesh first
+    const int D = s->D;
+    const std::vector<int64_t> roots = select_roots(s);
     s->sub_roots = int64_t(roots.size());
     s->sub_leaves = 0;
     for (int64_t q : roots) s->sub_leaves += int64_t(1) << (3 * (D - s->depth_of[q]));
@@ -1403,14 +1416,27 @@ int32_t orc_extract_subdivision(orc_state* s, int64_t* n_vertices, int64_t* n_tr
     // then all roots of each depth 3..D-1 together (main.cu:4212-4561)
     const int kFinerDepth = 3;
     size_t k = 0;
-    for (; k < roots.size() && s->depth_of[roots[k]] < kFinerDepth; k++) process_group(s, {roots[k]});
+    for (; k < roots.size() && s->depth_of[roots[k]] < kFinerDepth; k++)
+        if (s->depth_of[roots[k]] >= min_depth) process_group(s, {roots[k]});
     for (int d = kFinerDepth; d < D; d++) {
         std::vector<int64_t> g;
         for (; k < roots.size() && s->depth_of[roots[k]] == d; k++) g.push_back(roots[k]);
-        if (!g.empty()) process_group(s, g);
+        if (!g.empty() && d >= min_depth) process_group(s, g);
     }
     *n_vertices = int64_t(s->mesh_v.size()) / 3;
     *n_triangles = int64_t(s->mesh_t.size()) / 3;
+    return 0;
+}
+
+int32_t orc_extract_subdivision(orc_state* s, int64_t* n_vertices, int64_t* n_triangles) {
+    return orc_extract_subdivision_from(s, 0, n_vertices, n_triangles);
+}
+
+// Number of subdivision roots at each depth 0..D (no marching cubes run).
+int32_t orc_subdivision_roots(orc_state* s, int64_t* roots_per_depth) {
+    if (s->node_centers.empty()) return 6;
+    for (int d = 0; d <= s->D; d++) roots_per_depth[d] = 0;
+    for (int64_t q : select_roots(s)) roots_per_depth[s->depth_of[q]]++;
     return 0;
 }
 

@@ -44,7 +44,9 @@ int32_t orc_copy_mesh(orc_state* s, float* vertices, int32_t* triangles);
  * orc_extract_leaf_mesh, appends one batch per group of subdivided coarse
  * leaves; the counts returned are the whole mesh's. */
 int32_t orc_extract_subdivision(orc_state* s, int64_t* n_vertices, int64_t* n_triangles);
+int32_t orc_extract_subdivision_from(orc_state* s, int32_t min_depth, int64_t* n_vertices, int64_t* n_triangles);
 void orc_subdivision_info(orc_state* s, int64_t* n_roots, int64_t* n_virtual_leaves);
+int32_t orc_subdivision_roots(orc_state* s, int64_t* roots_per_depth);
 
 /* Buffers: ids and layouts of prg_buffer (include/prg.h). */
 int64_t orc_buffer_size(orc_state* s, int32_t which);

@@ -126,6 +126,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         }
         PRG_HIP(hipHostMalloc(&c->transform_host, 16, hipHostMallocDefault));
         PRG_HIP(hipHostMalloc(&c->cg_host, 4096, hipHostMallocDefault));
+        PRG_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->sub_host), 64, hipHostMallocDefault));
         PRG_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, cf.device));
         c->kt.enabled = cf.timing != 0;
         c->kt.stream = c->stream;
@@ -153,6 +154,7 @@ void prg_destroy(prg_ctx* c) {
             if (x) (void)hipEventDestroy(x);
     if (c->transform_host) (void)hipHostFree(c->transform_host);
     if (c->cg_host) (void)hipHostFree(c->cg_host);
+    if (c->sub_host) (void)hipHostFree(c->sub_host);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     hipStream_t ss[4] = {c->stream, c->stream2, c->stream3, c->stream4};
     delete c;  // releases device buffers

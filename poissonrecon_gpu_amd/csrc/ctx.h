@@ -129,6 +129,7 @@ struct prg_ctx {
     std::vector<hipEvent_t> ev_pool;  // the solver's events, reused across solves
     int num_cu = 0;                   // compute units of the device
     void* cg_host = nullptr;          // pinned, 4 KiB: CG state read-backs and the segment table's staging
+    int32_t* sub_host = nullptr;      // pinned, 16 ints: the subdivision's counters
 
     int64_t MD() const { return counts.empty() ? 0 : counts[D]; }
     int64_t base_of(int d) const { return bases[d]; }

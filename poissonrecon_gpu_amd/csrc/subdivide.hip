@@ -11,10 +11,9 @@
 // redirected to the virtual root, and runs marching cubes over the virtual
 // leaves, one mesh batch per group.
 //
-// Here every virtual leaf of every group is handled in one pass, one thread
-// per leaf, without building the virtual tree.  A virtual leaf is (root,
-// Morton offset inside the root); its key is the root's key with the offset in
-// the low bits.  The combined tree of a group has exactly these facts:
+// Here no virtual tree is built.  A virtual leaf is (root, Morton offset inside
+// the root); its key is the root's key with the offset in the low bits.  The
+// combined tree of a group has exactly these facts:
 //   - a depth-D cell is virtual iff its ancestor at the root depth r is the
 //     root itself or (roots at depth >= 3) another root of depth r;
 //   - a real node below depth r is reached from one of the root's 27 real
@@ -26,6 +25,19 @@
 // then the root's real ancestor chain (main.cu:2346-2380).  Leaves are
 // numbered in root order, Morton order inside a root: the reference's batch
 // order, so one scan numbers every batch's vertices and triangles.
+//
+// Work is split into bricks: Morton-contiguous blocks of up to 8^3 virtual
+// leaves of one root (the whole root when it is smaller), one wavefront each.
+// A wave computes the brick's (S+1)^3 corner values into LDS, each from its
+// owner's point of view (the owner fixes the summation order), then the
+// leaves' cubes and owned crossing edges from LDS.  Three passes:
+//   count     every brick: vertex and triangle counts      (k_brick_count)
+//   vertices  bricks with output: per-leaf edge masks, cube indices and
+//             vertex offsets into small per-brick tables, and the vertices
+//   triangles bricks with output: triangle entries, a vertex owned by a leaf
+//             of another brick found through that brick's table
+// Memory grows with the number of bricks (about 1 M for 506 M virtual
+// leaves) and with the bricks that carry output, never with the virtual leaves.
 #include <algorithm>
 #include <vector>
 
@@ -36,8 +48,7 @@ namespace prg {
 namespace {
 
 constexpr int kBlockS = 256;
-constexpr int kFinerDepth = 3;           // main.cu:3894: shallower roots go one at a time
-constexpr int64_t kMaxVirtualLeaves = int64_t(1) << 28;
+constexpr int kFinerDepth = 3;  // main.cu:3894: shallower roots go one at a time
 
 // parentFaceKind (MarchingCubes.cuh:708-717), indexed [sonKey][face] literally
 __constant__ int8_t c_parent_face[8][6] = {{0, -1, 2, -1, 4, -1}, {-1, 1, 2, -1, 4, -1}, {0, -1, -1, 3, 4, -1},
@@ -49,6 +60,10 @@ __constant__ uint16_t c_face_edges[6] = {(1 << 4) | (1 << 6) | (1 << 8) | (1 << 
                                          (1 << 0) | (1 << 1) | (1 << 4) | (1 << 5),  (1 << 2) | (1 << 3) | (1 << 6) | (1 << 7)};
 // childrenVertexKind (MarchingCubes.cuh:720-723), used literally
 __constant__ int8_t c_children_vertex_kind[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+
+// slots of the device counter array
+enum { kCntCoarse = 0, kCntCorners = 1, kCntRoots = 2, kCntBricks = 3, kCntVerts = 4, kCntTris = 5, kCntActive = 6,
+       kCntLeaves = 7, kCntErr = 8, kCntSlots = 16 };
 
 __device__ __forceinline__ int depth_of(int64_t i, const Levels& L, int D) {
     int d = 0;
@@ -120,11 +135,12 @@ __global__ void __launch_bounds__(kBlockS) k_compact(const int32_t* flag, const 
 
 // Corner owners of the coarse leaves' corners among the same-depth nodes
 // (initVertexOwner, main.cu:1438-1498); marks each (owner, corner) needed.
-__global__ void __launch_bounds__(kBlockS) k_coarse_corner_owner(const int32_t* cl, int64_t ncl, const uint32_t* key,
-                                                                 const int32_t* neigh, int64_t M, int32_t* oref,
-                                                                 int32_t* need) {
+// ncl (device) coarse leaves in cl.
+__global__ void __launch_bounds__(kBlockS) k_coarse_corner_owner(const int32_t* cl, const int32_t* ncl,
+                                                                 const uint32_t* key, const int32_t* neigh, int64_t M,
+                                                                 int32_t* oref, int32_t* need) {
     const int64_t g = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
-    if (g >= 8 * ncl) return;
+    if (g >= 8 * int64_t(*ncl)) return;
     const int64_t X = cl[g >> 3];
     const int j = int(g & 7);
     const int cx = j & 1, cy = (j >> 1) & 1, cz = (j >> 2) & 1;
@@ -150,12 +166,13 @@ __global__ void __launch_bounds__(kBlockS) k_coarse_corner_owner(const int32_t* 
 // computeVertexImplicitFunctionValue (main.cu:2259-2326) for a corner of a
 // node above the leaf level: the owner's ancestor chain, then the descent
 // through children[childrenVertexKind[corner]] down to D.
-__global__ void __launch_bounds__(kBlockS) k_coarse_corner_value(const int32_t* list, int64_t n, const uint32_t* key,
-                                                                 const int32_t* pnt, Levels L, const int32_t* parent,
-                                                                 const int32_t* child_base, const float* x,
-                                                                 const float* bf, int D, const float* iso, float* out) {
+__global__ void __launch_bounds__(kBlockS) k_coarse_corner_value(const int32_t* list, const int32_t* n_dev,
+                                                                 const uint32_t* key, const int32_t* pnt, Levels L,
+                                                                 const int32_t* parent, const int32_t* child_base,
+                                                                 const float* x, const float* bf, int D,
+                                                                 const float* iso, float* out) {
     const int64_t i = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
-    if (i >= n) return;
+    if (i >= *n_dev) return;
     const int64_t O = list[i] >> 3;
     const int jo = list[i] & 7;
     const int d = depth_of(O, L, D);
@@ -178,13 +195,18 @@ __global__ void __launch_bounds__(kBlockS) k_coarse_corner_value(const int32_t* 
 }
 
 // hasTriangle || hasIntersection for each coarse leaf (ProcessLeafNodesAtOtherDepth
-// + ifSubdivide, main.cu:2957-2992)
-__global__ void __launch_bounds__(kBlockS) k_coarse_select(const int32_t* cl, int64_t ncl, const int32_t* oref,
-                                                           const int32_t* nid, const float* cv, const uint32_t* key,
-                                                           const int32_t* neigh, int64_t M, const uint8_t* inter,
-                                                           int32_t* rflag) {
+// + ifSubdivide, main.cu:2957-2992); 0 past the ncl coarse leaves, so a scan
+// over all n slots counts the roots.
+__global__ void __launch_bounds__(kBlockS) k_coarse_select(const int32_t* cl, const int32_t* ncl, int64_t n,
+                                                           const int32_t* oref, const int32_t* nid, const float* cv,
+                                                           const uint32_t* key, const int32_t* neigh, int64_t M,
+                                                           const uint8_t* inter, int32_t* rflag) {
     const int64_t c = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
-    if (c >= ncl) return;
+    if (c >= n) return;
+    if (c >= *ncl) {
+        rflag[c] = 0;
+        return;
+    }
     const int64_t X = cl[c];
     // vertices[0] is the (0,0,0) corner in either numbering
     const float v0 = cv[nid[oref[8 * c]]];
@@ -198,256 +220,450 @@ __global__ void __launch_bounds__(kBlockS) k_coarse_select(const int32_t* cl, in
     rflag[c] = (tri || hit) ? 1 : 0;
 }
 
-__global__ void __launch_bounds__(kBlockS) k_root_of(const int32_t* roots, int64_t nr, int32_t* root_of) {
-    const int64_t i = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
-    if (i < nr) root_of[roots[i]] = int32_t(i);
+// ---- roots and bricks ---------------------------------------------------------------
+constexpr int kWave = 64;
+constexpr int kBrickLog = 3;  // bricks of up to 8^3 virtual leaves
+constexpr int kBrickLeaves = 1 << (3 * kBrickLog);
+constexpr int kBrickPts = ((1 << kBrickLog) + 1) * ((1 << kBrickLog) + 1) * ((1 << kBrickLog) + 1);
+
+__device__ __forceinline__ int brick_log(int D, int r) { return D - r < kBrickLog ? D - r : kBrickLog; }
+
+// Per root (q < nroots): its real index, bricks and virtual leaves; zero past
+// the roots, so scans over all n slots give the brick and leaf offsets.
+__global__ void __launch_bounds__(kBlockS) k_root_info(const int32_t* cl, const int32_t* ridx, const int32_t* nroots,
+                                                       int64_t n, Levels L, int D, int32_t* roots, int32_t* rbricks,
+                                                       int32_t* rleaves) {
+    const int64_t q = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
+    if (q >= n) return;
+    if (q >= *nroots) {
+        rbricks[q] = 0;
+        rleaves[q] = 0;
+        return;
+    }
+    const int32_t rho = cl[ridx[q]];
+    const int r = depth_of(rho, L, D);
+    roots[q] = rho;
+    rbricks[q] = 1 << (3 * (D - r - brick_log(D, r)));
+    rleaves[q] = 1 << (3 * (D - r));
 }
 
-// ---- virtual leaves ------------------------------------------------------------------
-struct Groups {
-    const int64_t* loff;     // first virtual leaf of each root, nr + 1 entries
-    const int32_t* roots;    // real index of each root, in node order
-    const int8_t* rdepth;    // depth of each root
-    const int32_t* root_of;  // per real node: its root number or -1
-    int64_t nr;
+__global__ void __launch_bounds__(kBlockS) k_root_of(const int32_t* roots, const int32_t* nroots, int32_t* root_of) {
+    const int64_t i = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
+    if (i < *nroots) root_of[roots[i]] = int32_t(i);
+}
+
+struct SubArgs {
+    const uint32_t* key;
+    const int32_t* neigh;  // 27 planes
+    int64_t M;
+    const int32_t* child_base;
+    const int32_t* parent;
+    const int32_t* pnt;
+    Levels L;
+    const float* x;
+    const float* bf;
+    const float4* btab;  // corner basis table (corner_basis_entry)
+    const float* rows;   // coarse rows of depths 1 .. D - 1
+    int D;
+    const float* iso;
+    const int32_t* roots;
+    const int32_t* boff;     // first brick of each root (exclusive scan, nroots + 1 entries)
+    const int32_t* root_of;  // per real node: its root number, or -1
+    const int32_t* cnt;      // device counters
 };
 
-struct VLeaf {
-    int64_t ri;
-    int32_t rho;
-    int r;
-    uint32_t key;
-    int32_t cx, cy, cz;  // depth-D cell coordinates
-    int32_t rx, ry, rz;  // the root's depth-r coordinates
+// A brick and its root (wave-uniform).
+struct Brick {
+    int32_t q, rho, r, kb;
+    int32_t R[3];     // root coordinates at depth r
+    int32_t R0[3];    // first depth-D cell of the root
+    int32_t Sr;       // root side in depth-D cells
+    int32_t org[3];   // first depth-D cell of the brick
+    uint32_t bkey;    // key of the brick's first leaf
+    bool deep;        // a neighbour of the root has children (real nodes below depth r)
 };
 
-__device__ __forceinline__ VLeaf decode(int64_t v, const Groups& G, const uint32_t* __restrict__ key, int D) {
-    int64_t lo = 0, hi = G.nr - 1;
+// LDS of one brick's wave
+struct BrickLds {
+    float G[kBrickPts];                // f - iso at the brick's corners (grid points)
+    float X[(kMaxDepth + 1) * 27];     // the root's chain: neighbour values at depths 0..r
+    int32_t NB[27], NBK[27], NBR[27];  // the root's neighbours, their child bases, their root numbers (group)
+};
+
+__device__ __forceinline__ void decode_cell(uint32_t k, int32_t c[3]) {
+    c[0] = int32_t(compact3(k >> 2));
+    c[1] = int32_t(compact3(k >> 1));
+    c[2] = int32_t(compact3(k));
+}
+
+// Finds brick b's root (binary search of the brick offsets) and loads the
+// root's neighbours and ancestor chain into LDS; `chain` = 0 skips the chain.
+__device__ void brick_setup(const SubArgs& A, int32_t b, Brick& B, BrickLds& S, bool chain) {
+    const int D = A.D;
+    int32_t lo = 0, hi = A.cnt[kCntRoots] - 1;
     while (lo < hi) {
-        const int64_t mid = (lo + hi + 1) >> 1;
-        if (G.loff[mid] <= v) lo = mid;
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (A.boff[mid] <= b) lo = mid;
         else hi = mid - 1;
     }
-    VLeaf L;
-    L.ri = lo;
-    L.rho = G.roots[lo];
-    L.r = G.rdepth[lo];
-    L.key = key[L.rho] | uint32_t(v - G.loff[lo]);
-    L.cx = int32_t(compact3(L.key >> 2));
-    L.cy = int32_t(compact3(L.key >> 1));
-    L.cz = int32_t(compact3(L.key));
-    L.rx = L.cx >> (D - L.r);
-    L.ry = L.cy >> (D - L.r);
-    L.rz = L.cz >> (D - L.r);
-    return L;
-}
-
-// Virtual leaf number of the depth-D cell (nx, ny, nz) in the group of leaf L,
-// or -1 if that cell is not a virtual leaf of the group.
-__device__ __forceinline__ int64_t virt_index(const VLeaf& L, int32_t nx, int32_t ny, int32_t nz, const Groups& G,
-                                              const int32_t* __restrict__ neigh, int64_t M, int D) {
-    const int32_t W = 1 << D;
-    if (nx < 0 || ny < 0 || nz < 0 || nx >= W || ny >= W || nz >= W) return -1;
-    const int sh = D - L.r;
-    const int ax = (nx >> sh) - L.rx, ay = (ny >> sh) - L.ry, az = (nz >> sh) - L.rz;
-    const uint32_t low = cell_key(uint32_t(nx), uint32_t(ny), uint32_t(nz)) & ((1u << (3 * sh)) - 1u);
-    if (!(ax | ay | az)) return G.loff[L.ri] + low;
-    if (L.r < kFinerDepth) return -1;
-    const int32_t a = neigh[int64_t(nidx(ax, ay, az)) * M + L.rho];
-    if (a < 0) return -1;
-    const int32_t q = G.root_of[a];
-    if (q < 0) return -1;
-    return G.loff[q] + low;
-}
-
-// initSubdivideVertexOwner (main.cu:1500-1568): corners of leaf v it owns
-__global__ void __launch_bounds__(kBlockS) k_vcorner_own(int64_t nv, Groups G, const uint32_t* key,
-                                                         const int32_t* neigh, int64_t M, int D, uint8_t* cmask,
-                                                         int32_t* ccnt) {
-    const int64_t v = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
-    if (v >= nv) return;
-    const VLeaf L = decode(v, G, key, D);
-    int mask = 0;
-    for (int j = 0; j < 8; j++) {
-        const int sx = (j & 1) ? 1 : -1, sy = (j & 2) ? 1 : -1, sz = (j & 4) ? 1 : -1;
-        bool own = true;
-        for (int q = 1; q < 8 && own; q++) {
-            const int32_t nx = L.cx + ((q & 1) ? sx : 0), ny = L.cy + ((q & 2) ? sy : 0), nz = L.cz + ((q & 4) ? sz : 0);
-            if (virt_index(L, nx, ny, nz, G, neigh, M, D) < 0) continue;
-            if (cell_key(uint32_t(nx), uint32_t(ny), uint32_t(nz)) < L.key) own = false;
-        }
-        if (own) mask |= 1 << j;
+    B.q = lo;
+    B.rho = A.roots[lo];
+    B.r = depth_of(B.rho, A.L, D);
+    B.kb = brick_log(D, B.r);
+    const uint32_t rkey = A.key[B.rho];
+    int32_t rc[3];
+    decode_cell(rkey, rc);
+    B.Sr = 1 << (D - B.r);
+    for (int a = 0; a < 3; a++) {
+        B.R[a] = rc[a] >> (D - B.r);
+        B.R0[a] = B.R[a] << (D - B.r);
     }
-    cmask[v] = uint8_t(mask);
-    ccnt[v] = __popc(mask);
-}
-
-__global__ void __launch_bounds__(kBlockS) k_vcorner_list(int64_t nv, const uint8_t* cmask, const int32_t* coff,
-                                                          int64_t* clist) {
-    const int64_t v = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
-    if (v >= nv) return;
-    const int m = cmask[v];
-    int rank = 0;
-    for (int j = 0; j < 8; j++)
-        if (m & (1 << j)) clist[coff[v] + rank++] = 8 * v + j;
-}
-
-// computeSubdivideVertexImplicitFunctionValue (main.cu:2328-2442) for each
-// owned corner: the real neighbours of the owner's virtual ancestors (depth D
-// up to r + 1), then the root's chain through its real neighbours and real
-// ancestors.
-__global__ void __launch_bounds__(kBlockS) k_vcorner_value(const int64_t* clist, int64_t n, Groups G,
-                                                           const uint32_t* key, const int32_t* neigh, int64_t M,
-                                                           const int32_t* child_base, const int32_t* pnt, Levels Lv,
-                                                           const int32_t* parent, const float* x, const float* bf,
-                                                           int D, const float* iso, float* vval) {
-    const int64_t i = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
-    if (i >= n) return;
-    const int64_t v = clist[i] >> 3;
-    const int j = int(clist[i] & 7);
-    const VLeaf L = decode(v, G, key, D);
-    float c[3];
-    node_center(L.key, D, D, c);
-    const float hw = 1.0f / float(1 << (D + 1));
-    const float px = c[0] + float(2 * (j & 1) - 1) * hw;
-    const float py = c[1] + float(2 * ((j & 2) >> 1) - 1) * hw;
-    const float pz = c[2] + float(2 * ((j & 4) >> 2) - 1) * hw;
-    float f = 0.0f;
-    for (int dd = D; dd > L.r; dd--) {
-        const int32_t axo = L.cx >> (D - dd), ayo = L.cy >> (D - dd), azo = L.cz >> (D - dd);
-        const int64_t fb = (int64_t(1) << dd) - 1;
-        const int32_t width = 1 << dd;
-        float bx[3], by[3], bz[3];
-#pragma unroll
-        for (int a = 0; a < 3; a++) {
-            const int32_t u = axo + a - 1, w = ayo + a - 1, z = azo + a - 1;
-            bx[a] = (u >= 0 && u < width) ? basis_value(bf, fb + u, px) : 0.0f;
-            by[a] = (w >= 0 && w < width) ? basis_value(bf, fb + w, py) : 0.0f;
-            bz[a] = (z >= 0 && z < width) ? basis_value(bf, fb + z, pz) : 0.0f;
+    B.bkey = rkey | (uint32_t(b - A.boff[lo]) << (3 * B.kb));
+    decode_cell(B.bkey, B.org);
+    const int lane = threadIdx.x;
+    if (lane < 27) {
+        const int32_t nb = A.neigh[int64_t(lane) * A.M + B.rho];
+        S.NB[lane] = nb;
+        S.NBK[lane] = nb < 0 ? -1 : A.child_base[nb];
+        S.NBR[lane] = (nb < 0 || B.r < kFinerDepth) ? -1 : A.root_of[nb];
+        if (chain) S.X[27 * B.r + lane] = nb < 0 ? 0.0f : A.x[nb];
+    }
+    if (chain) {
+        // ancestors' rows (their 27 neighbour values, then the parent), depths r-1 .. 1
+        int32_t a = A.parent[B.rho];
+        for (int d = B.r - 1; d >= 1; d--) {
+            const float* R = A.rows + kRowFloats * (int64_t(a) - A.L.base[1]);
+            if (lane < 27) S.X[27 * d + lane] = R[lane];
+            a = __builtin_amdgcn_readfirstlane(__float_as_int(R[27]));
         }
-        const int up = dd - L.r;
+        if (lane == 0) S.X[0] = A.x[0];
+    }
+    __syncthreads();
+    bool deep = false;
+    for (int j = 0; j < 27; j++)
+        if (j != 13 && S.NBK[j] >= 0) deep = true;
+    B.deep = deep;
+}
+
+// Whether the depth-D cell c (inside the root's 3x3x3 neighbourhood) is a
+// virtual leaf of the brick's group; *q receives its root's neighbour slot
+// (13 for the brick's own root).
+__device__ __forceinline__ bool cell_virtual(const Brick& B, const BrickLds& S, int D, const int32_t c[3], int* slot) {
+    const int W = 1 << D;
+    if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= W || c[1] >= W || c[2] >= W) return false;
+    const int sh = D - B.r;
+    const int ax = (c[0] >> sh) - B.R[0], ay = (c[1] >> sh) - B.R[1], az = (c[2] >> sh) - B.R[2];
+    const int j = nidx(ax, ay, az);
+    *slot = j;
+    if (j == 13) return true;
+    return S.NBR[j] >= 0;
+}
+
+// Sum over the real nodes below depth r (virtual coefficients are 0) for the
+// corner of owner leaf o on sides s, in the reference's order: depth D up to
+// r + 1, neighbours j = 0..26 of the owner's virtual ancestor
+// (computeSubdivideVertexImplicitFunctionValue, main.cu:2346-2372).  R is the
+// owner's root at depth r; nb(j) its neighbour j and kids(j) that neighbour's
+// child base.
+template <class NbFn, class KidsFn>
+__device__ float deep_sum(const SubArgs& A, int r, const int32_t R[3], const int32_t o[3], const int s[3], NbFn nb,
+                          KidsFn kids) {
+    const int D = A.D;
+    float f = 0.0f;
+    for (int dd = D; dd > r; dd--) {
+        const int up = dd - r, sh = D - dd;
+        const int32_t ax = o[0] >> sh, ay = o[1] >> sh, az = o[2] >> sh;
+        const int32_t ext = (1 << up) - 1;
+        const int32_t lx = R[0] << up, ly = R[1] << up, lz = R[2] << up;
+        // all 27 neighbours inside the root: virtual, nothing to add
+        if (ax - 1 >= lx && ax + 1 <= lx + ext && ay - 1 >= ly && ay + 1 <= ly + ext && az - 1 >= lz &&
+            az + 1 <= lz + ext)
+            continue;
+        const float4 tx = A.btab[((int64_t(dd) << D) + o[0]) * 2 + s[0]];
+        const float4 ty = A.btab[((int64_t(dd) << D) + o[1]) * 2 + s[1]];
+        const float4 tz = A.btab[((int64_t(dd) << D) + o[2]) * 2 + s[2]];
+        const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
+        const int32_t width = 1 << dd;
         for (int k = 0; k < 27; k++) {
-            const int32_t nx = axo + (k / 9 - 1), ny = ayo + ((k / 3) % 3 - 1), nz = azo + (k % 3 - 1);
+            const int32_t nx = ax + (k / 9 - 1), ny = ay + ((k / 3) % 3 - 1), nz = az + (k % 3 - 1);
             if (nx < 0 || ny < 0 || nz < 0 || nx >= width || ny >= width || nz >= width) continue;
-            const int ax = (nx >> up) - L.rx, ay = (ny >> up) - L.ry, az = (nz >> up) - L.rz;
-            if (!(ax | ay | az)) continue;  // inside the root: virtual, coefficient 0
-            int32_t cur = neigh[int64_t(nidx(ax, ay, az)) * M + L.rho];
-            for (int l = L.r + 1; l <= dd && cur >= 0; l++) {
-                const int32_t cb = child_base[cur];
-                const int sh = dd - l;
-                cur = cb < 0 ? -1 : cb + ((((nx >> sh) & 1) << 2) | (((ny >> sh) & 1) << 1) | ((nz >> sh) & 1));
+            const int cx = (nx >> up) - R[0], cy = (ny >> up) - R[1], cz = (nz >> up) - R[2];
+            if (!(cx | cy | cz)) continue;  // inside the root: virtual
+            const int jn = nidx(cx, cy, cz);
+            if (kids(jn) < 0) continue;  // absent, or a coarse leaf: no real node below
+            int32_t cur = nb(jn);
+            for (int l = r + 1; l <= dd && cur >= 0; l++) {
+                const int32_t cb = A.child_base[cur];
+                const int s2 = dd - l;
+                cur = cb < 0 ? -1 : cb + ((((nx >> s2) & 1) << 2) | (((ny >> s2) & 1) << 1) | ((nz >> s2) & 1));
             }
-            if (cur < 0) continue;  // absent, or below a coarse leaf (virtual or none)
-            float tt = x[cur] * bx[k / 9];
+            if (cur < 0) continue;
+            float tt = A.x[cur] * bx[k / 9];
             tt = tt * by[(k / 3) % 3];
             tt = tt * bz[k % 3];
             f = f + tt;
         }
     }
-    f = chain_value(L.rho, L.r, px, py, pz, key, pnt, Lv, parent, x, bf, D, f);
-    vval[i] = f - *iso;
+    return f;
 }
 
-// Corner values of each virtual leaf in square numbering (vertices[],
-// maintainSubdivideVertexNodePointer main.cu:1683-1740), from the owners' records.
-__global__ void __launch_bounds__(kBlockS) k_vcorner_square(int64_t nv, Groups G, const uint32_t* key,
-                                                            const int32_t* neigh, int64_t M, int D,
-                                                            const uint8_t* cmask, const int32_t* coff,
-                                                            const float* vval, float* cval) {
-    const int64_t v = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
-    if (v >= nv) return;
-    const VLeaf L = decode(v, G, key, D);
-    for (int q = 0; q < 8; q++) {
-        const int y = (q >> 1) & 1, xx = (q & 1) ^ y, z = q >> 2;
-        const int j = xx | (y << 1) | (z << 2);
-        const int sx = xx ? 1 : -1, sy = y ? 1 : -1, sz = z ? 1 : -1;
-        uint32_t best = L.key;
-        int64_t ov = v;
-        int jo = j;
-        for (int m = 1; m < 8; m++) {
-            const int32_t nx = L.cx + ((m & 1) ? sx : 0), ny = L.cy + ((m & 2) ? sy : 0), nz = L.cz + ((m & 4) ? sz : 0);
-            const int64_t vi = virt_index(L, nx, ny, nz, G, neigh, M, D);
-            if (vi < 0) continue;
-            const uint32_t kk = cell_key(uint32_t(nx), uint32_t(ny), uint32_t(nz));
-            if (kk < best) {
-                best = kk;
-                ov = vi;
-                jo = j ^ m;
+// The root's chain from LDS (chain_value's order: depth r down to 1, then the
+// root of the tree), continuing from f.
+__device__ __forceinline__ float chain_lds(const SubArgs& A, const BrickLds& S, int r, const int32_t o[3],
+                                           const int s[3], float f) {
+    const int D = A.D;
+    for (int d = r; d >= 1; d--) {
+        const float4 tx = A.btab[((int64_t(d) << D) + o[0]) * 2 + s[0]];
+        const float4 ty = A.btab[((int64_t(d) << D) + o[1]) * 2 + s[1]];
+        const float4 tz = A.btab[((int64_t(d) << D) + o[2]) * 2 + s[2]];
+        const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
+        const float* X = S.X + 27 * d;
+#pragma unroll
+        for (int j = 0; j < 27; j++) {
+            float tt = X[j] * bx[j / 9];
+            tt = tt * by[(j / 3) % 3];
+            tt = tt * bz[j % 3];
+            f = f + tt;
+        }
+    }
+    const float b0x = A.btab[o[0] * 2 + s[0]].y, b0y = A.btab[o[1] * 2 + s[1]].y, b0z = A.btab[o[2] * 2 + s[2]].y;
+    float tt = S.X[0] * b0x;
+    tt = tt * b0y;
+    tt = tt * b0z;
+    return f + tt;
+}
+
+// f - iso at grid point g (depth-D corner coordinates) of the brick's root cell,
+// computed by its owner: the smallest-key virtual leaf around it.
+__device__ float grid_value(const SubArgs& A, const Brick& B, const BrickLds& S, const int32_t g[3]) {
+    const int D = A.D;
+    int32_t o[3];
+    int side[3];
+    int slot = 13;
+    const bool interior = g[0] > B.R0[0] && g[0] < B.R0[0] + B.Sr && g[1] > B.R0[1] && g[1] < B.R0[1] + B.Sr &&
+                          g[2] > B.R0[2] && g[2] < B.R0[2] + B.Sr;
+    if (interior) {
+        // all 8 cells around are leaves of this root; (g - 1) has the smallest key
+        for (int a = 0; a < 3; a++) {
+            o[a] = g[a] - 1;
+            side[a] = 1;
+        }
+    } else {
+        uint32_t best = 0xffffffffu;
+        for (int t = 0; t < 8; t++) {
+            const int32_t c[3] = {g[0] - 1 + (t & 1), g[1] - 1 + ((t >> 1) & 1), g[2] - 1 + (t >> 2)};
+            int sl;
+            if (!cell_virtual(B, S, D, c, &sl)) continue;
+            const uint32_t k = cell_key(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]));
+            if (k < best) {
+                best = k;
+                slot = sl;
+                for (int a = 0; a < 3; a++) o[a] = c[a];
             }
         }
-        const int idx = coff[ov] + __popc(uint32_t(cmask[ov]) & ((1u << jo) - 1u));
-        cval[8 * v + q] = vval[idx];
+        for (int a = 0; a < 3; a++) side[a] = g[a] - o[a];
     }
+    if (slot == 13) {
+        float f = 0.0f;
+        if (B.deep)
+            f = deep_sum(
+                A, B.r, B.R, o, side, [&](int j) { return S.NB[j]; }, [&](int j) { return S.NBK[j]; });
+        return chain_lds(A, S, B.r, o, side, f) - *A.iso;
+    }
+    // owned by a leaf of another root of the group: that root's neighbours and chain
+    const int32_t rho2 = S.NB[slot];
+    const int32_t R2[3] = {B.R[0] + slot / 9 - 1, B.R[1] + (slot / 3) % 3 - 1, B.R[2] + slot % 3 - 1};
+    const float f = deep_sum(
+        A, B.r, R2, o, side, [&](int j) { return A.neigh[int64_t(j) * A.M + rho2]; },
+        [&](int j) {
+            const int32_t a = A.neigh[int64_t(j) * A.M + rho2];
+            return a < 0 ? -1 : A.child_base[a];
+        });
+    float c[3];
+    node_center(cell_key(uint32_t(o[0]), uint32_t(o[1]), uint32_t(o[2])), D, D, c);
+    const float hw = 1.0f / float(1 << (D + 1));
+    const float px = c[0] + float(2 * side[0] - 1) * hw;
+    const float py = c[1] + float(2 * side[1] - 1) * hw;
+    const float pz = c[2] + float(2 * side[2] - 1) * hw;
+    return chain_value(rho2, B.r, px, py, pz, A.key, A.pnt, A.L, A.parent, A.x, A.bf, D, f) - *A.iso;
 }
 
-// Owner of edge e of leaf L among the virtual leaves sharing it
-// (initSubdivideEdgeArray, main.cu:1867-2019); returns its number and the
-// edge kind seen from it.
-__device__ __forceinline__ int64_t edge_owner(const VLeaf& L, int64_t v, int e, const Groups& G,
-                                              const int32_t* __restrict__ neigh, int64_t M, int D, int* eo) {
+__device__ void brick_values(const SubArgs& A, const Brick& B, BrickLds& S) {
+    const int P = (1 << B.kb) + 1;
+    for (int p = threadIdx.x; p < P * P * P; p += kWave) {
+        const int32_t g[3] = {B.org[0] + p % P, B.org[1] + (p / P) % P, B.org[2] + p / (P * P)};
+        S.G[p] = grid_value(A, B, S, g);
+    }
+    __syncthreads();
+}
+
+// Owner of edge e of leaf v among the virtual leaves sharing it
+// (initSubdivideEdgeArray, main.cu:1867-2019): its cell and the edge kind seen
+// from it.
+__device__ __forceinline__ void edge_owner(const Brick& B, const BrickLds& S, int D, const int32_t v[3], int e,
+                                           int32_t ow[3], int* eo) {
     const int o = e >> 2;
     const int a1 = (o == 0) ? 1 : 0, a2 = (o == 2) ? 1 : 2;
     const int b1 = e & 1, b2 = (e >> 1) & 1;
-    uint32_t best = L.key;
-    int64_t ov = v;
+    uint32_t best = cell_key(uint32_t(v[0]), uint32_t(v[1]), uint32_t(v[2]));
+    for (int a = 0; a < 3; a++) ow[a] = v[a];
     *eo = e;
     for (int u = 0; u < 2; u++)
         for (int w = 0; w < 2; w++) {
             if (!(u | w)) continue;
-            int dd[3] = {0, 0, 0};
-            dd[a1] = u ? (2 * b1 - 1) : 0;
-            dd[a2] = w ? (2 * b2 - 1) : 0;
-            const int32_t nx = L.cx + dd[0], ny = L.cy + dd[1], nz = L.cz + dd[2];
-            const int64_t vi = virt_index(L, nx, ny, nz, G, neigh, M, D);
-            if (vi < 0) continue;
-            const uint32_t kk = cell_key(uint32_t(nx), uint32_t(ny), uint32_t(nz));
+            int32_t n[3] = {v[0], v[1], v[2]};
+            if (u) n[a1] += 2 * b1 - 1;
+            if (w) n[a2] += 2 * b2 - 1;
+            int sl;
+            if (!cell_virtual(B, S, D, n, &sl)) continue;
+            const uint32_t kk = cell_key(uint32_t(n[0]), uint32_t(n[1]), uint32_t(n[2]));
             if (kk < best) {
                 best = kk;
-                ov = vi;
+                for (int a = 0; a < 3; a++) ow[a] = n[a];
                 *eo = (o << 2) | ((b2 ^ w) << 1) | (b1 ^ u);
             }
         }
-    return ov;
 }
 
-// owned edges with a sign change (generateSubdivideVexNums, main.cu:2481-2531)
-// and triangle counts (generateTriNums, main.cu:2562-2582)
-__global__ void __launch_bounds__(kBlockS) k_vedges(int64_t nv, Groups G, const uint32_t* key, const int32_t* neigh,
-                                                    int64_t M, int D, const float* cval, uint16_t* emask,
-                                                    int32_t* ecnt, int32_t* tcnt) {
-    const int64_t v = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
-    if (v >= nv) return;
-    const VLeaf L = decode(v, G, key, D);
-    int mask = 0;
-    for (int e = 0; e < 12; e++) {
-        int eo;
-        if (edge_owner(L, v, e, G, neigh, M, D, &eo) != v) continue;
-        const float f1 = cval[8 * v + c_edge_vertex[e][0]];
-        const float f2 = cval[8 * v + c_edge_vertex[e][1]];
-        if (f1 * f2 <= 0) mask |= 1 << e;
+// Cube index and owned crossing edges of leaf l of the brick
+// (generateSubdivideVexNums, main.cu:2481-2531; generateTriNums, :2562-2582).
+__device__ __forceinline__ void leaf_cube(const Brick& B, const BrickLds& S, int D, int l, int32_t v[3], float cv[8],
+                                          int* cube, int* emask) {
+    const int P = (1 << B.kb) + 1;
+    const int lx = int(compact3(uint32_t(l) >> 2)), ly = int(compact3(uint32_t(l) >> 1)), lz = int(compact3(uint32_t(l)));
+    v[0] = B.org[0] + lx;
+    v[1] = B.org[1] + ly;
+    v[2] = B.org[2] + lz;
+    int cb = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int y = (q >> 1) & 1, xx = (q & 1) ^ y, z = q >> 2;
+        cv[q] = S.G[((lz + z) * P + (ly + y)) * P + (lx + xx)];
+        if (cv[q] < 0) cb |= 1 << q;
     }
-    emask[v] = uint16_t(mask);
-    ecnt[v] = __popc(mask);
-    tcnt[v] = tri_count(cube_of(cval, v));
+    *cube = cb;
+    // a leaf not on its root's boundary owns exactly its edges on the high side
+    // of both other axes (kinds 3, 7, 11): its 26 neighbours are all in the root
+    bool inner = true;
+    for (int a = 0; a < 3; a++) inner = inner && v[a] > B.R0[a] && v[a] < B.R0[a] + B.Sr - 1;
+    int m = 0;
+#pragma unroll
+    for (int e = 0; e < 12; e++) {
+        const float f1 = cv[c_edge_vertex_h[e][0]], f2 = cv[c_edge_vertex_h[e][1]];
+        if (!(f1 * f2 <= 0)) continue;
+        bool own;
+        if (inner) {
+            own = (e & 3) == 3;
+        } else {
+            int32_t ow[3];
+            int eo;
+            edge_owner(B, S, D, v, e, ow, &eo);
+            own = ow[0] == v[0] && ow[1] == v[1] && ow[2] == v[2];
+        }
+        if (own) m |= 1 << e;
+    }
+    *emask = m;
 }
 
-// interpolatePoint (main.cu:2584-2597) per owned crossing edge and the
-// triangles of each leaf (generateSubdivideTrianglePos, main.cu:2759-2825)
-__global__ void __launch_bounds__(kBlockS) k_vemit(int64_t nv, Groups G, const uint32_t* key, const int32_t* neigh,
-                                                   int64_t M, int D, const float* cval, const uint16_t* emask,
-                                                   const int32_t* eoff, const int32_t* toff, int64_t vbase,
-                                                   float* out_v, int32_t* out_t) {
-    const int64_t v = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
-    if (v >= nv) return;
-    const VLeaf L = decode(v, G, key, D);
-    const int mine = emask[v];
-    if (mine) {
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__device__ __forceinline__ int wave_excl_scan(int v) {
+    int inc = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(inc, o, kWave);
+        if (int(threadIdx.x) >= o) inc += t;
+    }
+    return inc - v;
+}
+
+// Pass 1: vertex and triangle counts of every brick.
+__global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, int32_t* bnv, int32_t* bnt) {
+    __shared__ BrickLds S;
+    Brick B;
+    const int32_t b = blockIdx.x;
+    brick_setup(A, b, B, S, true);
+    brick_values(A, B, S);
+    int nv = 0, nt = 0;
+    const int L = 1 << (3 * B.kb);
+    for (int l = threadIdx.x; l < L; l += kWave) {
+        int32_t v[3];
+        float cv[8];
+        int cube, em;
+        leaf_cube(B, S, A.D, l, v, cv, &cube, &em);
+        nv += __popc(uint32_t(em));
+        nt += tri_count(cube);
+    }
+    nv = wave_sum(nv);
+    nt = wave_sum(nt);
+    if (threadIdx.x == 0) {
+        bnv[b] = nv;
+        bnt[b] = nt;
+    }
+}
+
+__global__ void __launch_bounds__(kBlockS) k_brick_active(const int32_t* bnv, const int32_t* bnt, int64_t n,
+                                                          int32_t* flag) {
+    const int64_t i = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
+    if (i < n) flag[i] = (bnv[i] | bnt[i]) ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kBlockS) k_brick_compact(const int32_t* flag, const int32_t* off, int64_t n,
+                                                           int32_t* alist, int32_t* amap) {
+    const int64_t i = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
+    if (i >= n) return;
+    if (flag[i]) {
+        alist[off[i]] = int32_t(i);
+        amap[i] = off[i];
+    } else {
+        amap[i] = -1;
+    }
+}
+
+// Pass 2 (bricks with output): per-leaf edge masks, cube indices and vertex
+// offsets into the brick's tables; the vertices (interpolatePoint and
+// generateSubdivideIntersectionPoint, main.cu:2584-2597, 2629-2697).
+__global__ void __launch_bounds__(kWave) k_brick_vertices(SubArgs A, const int32_t* alist, const int32_t* bvoff,
+                                                          uint16_t* lofs_tab, uint16_t* emask_tab, uint8_t* cube_tab,
+                                                          float* out_v) {
+    __shared__ BrickLds S;
+    Brick B;
+    const int32_t ai = blockIdx.x;
+    const int32_t b = alist[ai];
+    brick_setup(A, b, B, S, true);
+    brick_values(A, B, S);
+    const int D = A.D;
+    const int L = 1 << (3 * B.kb);
+    const int per = (L + kWave - 1) / kWave;
+    const int l0 = min(L, int(threadIdx.x) * per), l1 = min(L, l0 + per);
+    int cnt = 0;
+    for (int l = l0; l < l1; l++) {
+        int32_t v[3];
+        float cv[8];
+        int cube, em;
+        leaf_cube(B, S, D, l, v, cv, &cube, &em);
+        emask_tab[int64_t(ai) * kBrickLeaves + l] = uint16_t(em);
+        cube_tab[int64_t(ai) * kBrickLeaves + l] = uint8_t(cube);
+        cnt += __popc(uint32_t(em));
+    }
+    int run = wave_excl_scan(cnt);
+    const float hw = 1.0f / float(1 << (D + 1));
+    for (int l = l0; l < l1; l++) {
+        lofs_tab[int64_t(ai) * kBrickLeaves + l] = uint16_t(run);
+        const int em = emask_tab[int64_t(ai) * kBrickLeaves + l];
+        if (!em) continue;
+        int32_t v[3];
+        float cv[8];
+        int cube, em2;
+        leaf_cube(B, S, D, l, v, cv, &cube, &em2);
         float c[3];
-        node_center(L.key, D, D, c);
-        const float hw = 1.0f / float(1 << (D + 1));
-        int rank = 0;
+        node_center(B.bkey | uint32_t(l), D, D, c);
         for (int e = 0; e < 12; e++) {
-            if (!(mine & (1 << e))) continue;
+            if (!(em & (1 << e))) continue;
             const int q1 = c_edge_vertex[e][0], q2 = c_edge_vertex[e][1];
             float p1[3], p2[3];
             {
@@ -462,7 +678,7 @@ __global__ void __launch_bounds__(kBlockS) k_vemit(int64_t nv, Groups G, const u
                 p2[1] = c[1] + float(2 * y - 1) * hw;
                 p2[2] = c[2] + float(2 * z - 1) * hw;
             }
-            const float f1 = cval[8 * v + q1], f2 = cval[8 * v + q2];
+            const float f1 = cv[q1], f2 = cv[q2];
             const int dim = e >> 2;
             float o3[3] = {p1[0], p1[1], p1[2]};
             const float pivot = f1 / (f1 - f2);
@@ -470,31 +686,71 @@ __global__ void __launch_bounds__(kBlockS) k_vemit(int64_t nv, Groups G, const u
             const float s1 = p2[dim] * pivot;
             const float s2 = p1[dim] * another;
             o3[dim] = s1 + s2;
-            const int64_t id = eoff[v] + rank++;
+            const int64_t id = int64_t(bvoff[b]) + run++;
             out_v[3 * id + 0] = o3[0];
             out_v[3 * id + 1] = o3[1];
             out_v[3 * id + 2] = o3[2];
         }
     }
-    const int cube = cube_of(cval, v);
-    const int n = 3 * tri_count(cube);
-    const int64_t o = 3 * int64_t(toff[v]);
-    for (int k = 0; k < n; k++) {
-        const int e = c_tri[cube][k];
-        int eo;
-        const int64_t ov = edge_owner(L, v, e, G, neigh, M, D, &eo);
-        out_t[o + k] = int32_t(vbase + eoff[ov] + __popc(uint32_t(emask[ov]) & ((1u << eo) - 1u)));
+}
+
+// Pass 3 (bricks with output): triangle entries (generateSubdivideTrianglePos,
+// main.cu:2759-2825).  A vertex owned by a leaf of another brick (of this
+// root or of another root of the group) is found through that brick's tables.
+__global__ void __launch_bounds__(kWave) k_brick_triangles(SubArgs A, const int32_t* alist, const int32_t* amap,
+                                                           const int32_t* bvoff, const int32_t* btoff,
+                                                           const uint16_t* lofs_tab, const uint16_t* emask_tab,
+                                                           const uint8_t* cube_tab, int64_t vbase, int32_t* out_t,
+                                                           int32_t* err) {
+    __shared__ BrickLds S;
+    Brick B;
+    const int32_t ai = blockIdx.x;
+    const int32_t b = alist[ai];
+    brick_setup(A, b, B, S, false);
+    const int D = A.D;
+    const int L = 1 << (3 * B.kb);
+    const int per = (L + kWave - 1) / kWave;
+    const int l0 = min(L, int(threadIdx.x) * per), l1 = min(L, l0 + per);
+    int cnt = 0;
+    for (int l = l0; l < l1; l++) cnt += tri_count(cube_tab[int64_t(ai) * kBrickLeaves + l]);
+    int64_t o = 3 * (int64_t(btoff[b]) + wave_excl_scan(cnt));
+    const int sh = 3 * (D - B.r);
+    const uint32_t rmask = sh >= 32 ? 0xffffffffu : ((1u << sh) - 1u);
+    for (int l = l0; l < l1; l++) {
+        const int cube = cube_tab[int64_t(ai) * kBrickLeaves + l];
+        const int n = 3 * tri_count(cube);
+        if (!n) continue;
+        const int32_t v[3] = {B.org[0] + int32_t(compact3(uint32_t(l) >> 2)),
+                              B.org[1] + int32_t(compact3(uint32_t(l) >> 1)), B.org[2] + int32_t(compact3(uint32_t(l)))};
+        for (int k = 0; k < n; k++) {
+            const int e = c_tri[cube][k];
+            int32_t ow[3];
+            int eo;
+            edge_owner(B, S, D, v, e, ow, &eo);
+            int sl = 13;
+            cell_virtual(B, S, D, ow, &sl);
+            const int32_t q2 = sl == 13 ? B.q : S.NBR[sl];
+            const uint32_t mort = cell_key(uint32_t(ow[0]), uint32_t(ow[1]), uint32_t(ow[2])) & rmask;
+            const int32_t bi = q2 < 0 ? 0 : A.boff[q2] + int32_t(mort >> (3 * B.kb));
+            const int32_t lu = int32_t(mort & ((1u << (3 * B.kb)) - 1u));
+            const int32_t au = q2 < 0 ? -1 : amap[bi];
+            int32_t id = 0;
+            if (au < 0) {
+                err[0] = 1;  // a crossing edge whose owner brick has no output: inconsistent counts
+            } else {
+                const int64_t t = int64_t(au) * kBrickLeaves + lu;
+                id = int32_t(vbase + bvoff[bi] + lofs_tab[t] + __popc(uint32_t(emask_tab[t]) & ((1u << eo) - 1u)));
+            }
+            out_t[o + k] = id;
+        }
+        o += n;
     }
 }
 
-int32_t read_total(const DBuf<int32_t>& tot, hipStream_t s) {
-    int32_t v = 0;
-    PRG_HIP(hipMemcpyAsync(&v, tot.get(), 4, hipMemcpyDeviceToHost, s));
-    PRG_HIP(hipStreamSynchronize(s));
-    return v;
-}
-
 }  // namespace
+
+// the coarse rows of the current solution (mc.hip)
+int ensure_coarse_rows(prg_ctx* c, hipStream_t s);
 
 void extract_subdivision(prg_ctx* c) {
     ensure_neighbours(c);
@@ -509,9 +765,20 @@ void extract_subdivision(prg_ctx* c) {
     c->sub_roots = c->sub_leaves = 0;
     if (baseD <= 1) return;
     if (8 * baseD >= (int64_t(1) << 31)) throw Error{PRG_E_ARG, "too many nodes above the leaf level for subdivision"};
+    const int rd = ensure_coarse_rows(c, s);
+    if (rd < D - 1) throw Error{PRG_E_STATE, "coarse rows missing"};
+
+    // pool slots 97..127
+    PoolBuf<uint8_t> inter{&c->pool[97]};
+    PoolBuf<int32_t> cnt{&c->pool[98]}, flag{&c->pool[99]}, off{&c->pool[100]}, cl{&c->pool[101]},
+        oref{&c->pool[102]}, need{&c->pool[103]}, nid{&c->pool[104]}, clist{&c->pool[105]}, rflag{&c->pool[107]},
+        roff{&c->pool[108]}, ridx{&c->pool[109]}, roots{&c->pool[110]}, rbricks{&c->pool[111]},
+        boff{&c->pool[112]}, rleaves{&c->pool[113]}, loff{&c->pool[114]}, root_of{&c->pool[115]};
+    PoolBuf<float> cv{&c->pool[106]};
+    int32_t* C = cnt.reserve(kCntSlots);
+    PRG_HIP(hipMemsetAsync(C, 0, kCntSlots * 4, s));
 
     // faces touched by the leaf-level surface
-    DBuf<uint8_t> inter;
     inter.reserve(6 * M);
     PRG_HIP(hipMemsetAsync(inter.get(), 0, size_t(6 * M), s));
     {
@@ -520,139 +787,131 @@ void extract_subdivision(prg_ctx* c) {
                                                                 c->neigh.get(), M, c->parent.get(), Lv, D, inter.get());
         PRG_CHECK_LAUNCH();
     }
-    // coarse leaves
-    DBuf<int32_t> flag, off, tot, cl;
+    // coarse leaves (counts stay on the device; launches are sized by upper bounds)
     flag.reserve(baseD);
     off.reserve(baseD);
-    tot.reserve(1);
+    cl.reserve(baseD);
     k_flag_coarse_leaves<<<grid_for(baseD, kBlockS), kBlockS, 0, s>>>(c->child_base.get(), baseD, flag.get());
     PRG_CHECK_LAUNCH();
-    scan_exclusive(flag.get(), off.get(), baseD, tot.get(), s, c->scratch, kt);
-    const int64_t ncl = read_total(tot, s);
-    if (ncl == 0) return;
-    cl.reserve(ncl);
+    scan_exclusive(flag.get(), off.get(), baseD, C + kCntCoarse, s, c->scratch, kt);
     k_compact<<<grid_for(baseD, kBlockS), kBlockS, 0, s>>>(flag.get(), off.get(), baseD, cl.get());
     PRG_CHECK_LAUNCH();
     // their corner values
-    DBuf<int32_t> oref, need, nid, clist;
-    oref.reserve(8 * ncl);
+    oref.reserve(8 * baseD);
     need.reserve(8 * baseD);
     nid.reserve(8 * baseD);
+    clist.reserve(8 * baseD);
+    cv.reserve(8 * baseD);
     PRG_HIP(hipMemsetAsync(need.get(), 0, size_t(8 * baseD) * 4, s));
-    k_coarse_corner_owner<<<grid_for(8 * ncl, kBlockS), kBlockS, 0, s>>>(cl.get(), ncl, c->key.get(), c->neigh.get(),
-                                                                          M, oref.get(), need.get());
+    k_coarse_corner_owner<<<grid_for(8 * baseD, kBlockS), kBlockS, 0, s>>>(cl.get(), C + kCntCoarse, c->key.get(),
+                                                                            c->neigh.get(), M, oref.get(), need.get());
     PRG_CHECK_LAUNCH();
-    scan_exclusive(need.get(), nid.get(), 8 * baseD, tot.get(), s, c->scratch, kt);
-    const int64_t ncv = read_total(tot, s);
-    clist.reserve(ncv);
+    scan_exclusive(need.get(), nid.get(), 8 * baseD, C + kCntCorners, s, c->scratch, kt);
     k_compact<<<grid_for(8 * baseD, kBlockS), kBlockS, 0, s>>>(need.get(), nid.get(), 8 * baseD, clist.get());
     PRG_CHECK_LAUNCH();
-    DBuf<float> cv;
-    cv.reserve(ncv);
     {
-        TimedScope ts(kt, "coarse_corner_value", 24.0 * double(ncv));
-        k_coarse_corner_value<<<grid_for(ncv, kBlockS), kBlockS, 0, s>>>(
-            clist.get(), ncv, c->key.get(), c->pnt.get(), Lv, c->parent.get(), c->child_base.get(), c->x.get(),
-            c->d_base_fn.get(), D, c->iso_dev.get(), cv.get());
+        TimedScope ts(kt, "coarse_corner_value", 24.0 * double(baseD));
+        k_coarse_corner_value<<<grid_for(8 * baseD, kBlockS), kBlockS, 0, s>>>(
+            clist.get(), C + kCntCorners, c->key.get(), c->pnt.get(), Lv, c->parent.get(), c->child_base.get(),
+            c->x.get(), c->d_base_fn.get(), D, c->iso_dev.get(), cv.get());
         PRG_CHECK_LAUNCH();
     }
-    // roots
-    DBuf<int32_t> rflag, roff, roots;
-    rflag.reserve(ncl);
-    roff.reserve(ncl);
-    k_coarse_select<<<grid_for(ncl, kBlockS), kBlockS, 0, s>>>(cl.get(), ncl, oref.get(), nid.get(), cv.get(),
-                                                                c->key.get(), c->neigh.get(), M, inter.get(),
-                                                                rflag.get());
+    // roots, their bricks and virtual leaves
+    rflag.reserve(baseD);
+    roff.reserve(baseD);
+    ridx.reserve(baseD);
+    roots.reserve(baseD);
+    rbricks.reserve(baseD);
+    boff.reserve(baseD);
+    rleaves.reserve(baseD);
+    loff.reserve(baseD);
+    k_coarse_select<<<grid_for(baseD, kBlockS), kBlockS, 0, s>>>(cl.get(), C + kCntCoarse, baseD, oref.get(),
+                                                                  nid.get(), cv.get(), c->key.get(), c->neigh.get(), M,
+                                                                  inter.get(), rflag.get());
     PRG_CHECK_LAUNCH();
-    scan_exclusive(rflag.get(), roff.get(), ncl, tot.get(), s, c->scratch, kt);
-    const int64_t nr = read_total(tot, s);
-    if (nr == 0) return;
-    DBuf<int32_t> ridx;  // positions in cl
-    ridx.reserve(nr);
-    k_compact<<<grid_for(ncl, kBlockS), kBlockS, 0, s>>>(rflag.get(), roff.get(), ncl, ridx.get());
+    scan_exclusive(rflag.get(), roff.get(), baseD, C + kCntRoots, s, c->scratch, kt);
+    k_compact<<<grid_for(baseD, kBlockS), kBlockS, 0, s>>>(rflag.get(), roff.get(), baseD, ridx.get());
     PRG_CHECK_LAUNCH();
-    std::vector<int32_t> hpos(nr), hcl(ncl);
-    PRG_HIP(hipMemcpyAsync(hpos.data(), ridx.get(), size_t(nr) * 4, hipMemcpyDeviceToHost, s));
-    PRG_HIP(hipMemcpyAsync(hcl.data(), cl.get(), size_t(ncl) * 4, hipMemcpyDeviceToHost, s));
+    k_root_info<<<grid_for(baseD, kBlockS), kBlockS, 0, s>>>(cl.get(), ridx.get(), C + kCntRoots, baseD, Lv, D,
+                                                              roots.get(), rbricks.get(), rleaves.get());
+    PRG_CHECK_LAUNCH();
+    scan_exclusive2(rbricks.get(), boff.get(), C + kCntBricks, rleaves.get(), loff.get(), C + kCntLeaves, baseD, s,
+                    c->scratch, kt);
+    int32_t* H = c->sub_host;
+    PRG_HIP(hipMemcpyAsync(H, C, kCntSlots * 4, hipMemcpyDeviceToHost, s));
     PRG_HIP(hipStreamSynchronize(s));
-    std::vector<int32_t> hroots(nr);
-    std::vector<int8_t> hdepth(nr);
-    std::vector<int64_t> hloff(nr + 1, 0);
-    for (int64_t i = 0; i < nr; i++) {
-        hroots[i] = hcl[hpos[i]];
-        int d = 0;
-        while (d < D && hroots[i] >= c->bases[d + 1]) d++;
-        hdepth[i] = int8_t(d);
-        hloff[i + 1] = hloff[i] + (int64_t(1) << (3 * (D - d)));
-    }
-    const int64_t nv = hloff[nr];
-    if (nv > kMaxVirtualLeaves)
-        throw Error{PRG_E_NOMEM, "coarse-leaf subdivision needs " + std::to_string(nv) +
-                                     " virtual leaves, more than the supported 2^28"};
+    const int64_t nr = H[kCntRoots], nb = H[kCntBricks];
     c->sub_roots = nr;
-    c->sub_leaves = nv;
-    DBuf<int32_t> droots, root_of;
-    DBuf<int8_t> ddepth;
-    DBuf<int64_t> dloff;
-    PRG_HIP(hipMemcpyAsync(droots.reserve(nr), hroots.data(), size_t(nr) * 4, hipMemcpyHostToDevice, s));
-    PRG_HIP(hipMemcpyAsync(ddepth.reserve(nr), hdepth.data(), size_t(nr), hipMemcpyHostToDevice, s));
-    PRG_HIP(hipMemcpyAsync(dloff.reserve(nr + 1), hloff.data(), size_t(nr + 1) * 8, hipMemcpyHostToDevice, s));
+    c->sub_leaves = H[kCntLeaves];
+    if (nr == 0) return;
+    if (nb <= 0 || H[kCntLeaves] <= 0) throw Error{PRG_E_HIP, "subdivision: inconsistent brick counts"};
     root_of.reserve(M);
     PRG_HIP(hipMemsetAsync(root_of.get(), 0xff, size_t(M) * 4, s));
-    k_root_of<<<grid_for(nr, kBlockS), kBlockS, 0, s>>>(droots.get(), nr, root_of.get());
+    k_root_of<<<grid_for(nr, kBlockS), kBlockS, 0, s>>>(roots.get(), C + kCntRoots, root_of.get());
     PRG_CHECK_LAUNCH();
-    Groups G{dloff.get(), droots.get(), ddepth.get(), root_of.get(), nr};
 
-    // marching cubes over the virtual leaves
-    DBuf<uint8_t> cmask;
-    DBuf<int32_t> ccnt, coff;
-    cmask.reserve(nv);
-    ccnt.reserve(nv);
-    coff.reserve(nv);
-    const unsigned gv = grid_for(nv, kBlockS);
-    k_vcorner_own<<<gv, kBlockS, 0, s>>>(nv, G, c->key.get(), c->neigh.get(), M, D, cmask.get(), ccnt.get());
-    PRG_CHECK_LAUNCH();
-    scan_exclusive(ccnt.get(), coff.get(), nv, tot.get(), s, c->scratch, kt);
-    const int64_t nvc = read_total(tot, s);
-    DBuf<int64_t> vlist;
-    DBuf<float> vval, cval;
-    vlist.reserve(nvc);
-    vval.reserve(nvc);
-    cval.reserve(8 * nv);
-    k_vcorner_list<<<gv, kBlockS, 0, s>>>(nv, cmask.get(), coff.get(), vlist.get());
-    PRG_CHECK_LAUNCH();
+    PoolBuf<float> btab{&c->pool[91]};
+    const SubArgs A{c->key.get(),      c->neigh.get(),  M,
+                    c->child_base.get(), c->parent.get(), c->pnt.get(),
+                    Lv,                c->x.get(),      c->d_base_fn.get(),
+                    reinterpret_cast<const float4*>(btab.get()), c->coarse_rows.get(), D,
+                    c->iso_dev.get(),  roots.get(),     boff.get(),
+                    root_of.get(),     C};
+
+    // pass 1: counts per brick
+    PoolBuf<int32_t> bnv{&c->pool[116]}, bnt{&c->pool[117]}, bvoff{&c->pool[118]}, btoff{&c->pool[119]},
+        aflag{&c->pool[120]}, aoff{&c->pool[121]}, alist{&c->pool[122]}, amap{&c->pool[123]};
+    bnv.reserve(nb);
+    bnt.reserve(nb);
+    bvoff.reserve(nb);
+    btoff.reserve(nb);
+    aflag.reserve(nb);
+    aoff.reserve(nb);
+    alist.reserve(nb);
+    amap.reserve(nb);
     {
-        TimedScope ts(kt, "subdivide_corner_value", 24.0 * double(nvc));
-        k_vcorner_value<<<grid_for(nvc, kBlockS), kBlockS, 0, s>>>(
-            vlist.get(), nvc, G, c->key.get(), c->neigh.get(), M, c->child_base.get(), c->pnt.get(), Lv,
-            c->parent.get(), c->x.get(), c->d_base_fn.get(), D, c->iso_dev.get(), vval.get());
+        TimedScope ts(kt, "subdivide_count", 4.0 * double(c->sub_leaves));
+        k_brick_count<<<unsigned(nb), kWave, 0, s>>>(A, bnv.get(), bnt.get());
         PRG_CHECK_LAUNCH();
     }
-    k_vcorner_square<<<gv, kBlockS, 0, s>>>(nv, G, c->key.get(), c->neigh.get(), M, D, cmask.get(), coff.get(),
-                                            vval.get(), cval.get());
+    scan_exclusive2(bnv.get(), bvoff.get(), C + kCntVerts, bnt.get(), btoff.get(), C + kCntTris, nb, s, c->scratch,
+                    kt);
+    k_brick_active<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(bnv.get(), bnt.get(), nb, aflag.get());
     PRG_CHECK_LAUNCH();
-    DBuf<uint16_t> emask;
-    DBuf<int32_t> ecnt, eoff, tcnt, toff, ttot;
-    emask.reserve(nv);
-    ecnt.reserve(nv);
-    eoff.reserve(nv);
-    tcnt.reserve(nv);
-    toff.reserve(nv);
-    ttot.reserve(1);
-    k_vedges<<<gv, kBlockS, 0, s>>>(nv, G, c->key.get(), c->neigh.get(), M, D, cval.get(), emask.get(), ecnt.get(),
-                                    tcnt.get());
+    scan_exclusive(aflag.get(), aoff.get(), nb, C + kCntActive, s, c->scratch, kt);
+    k_brick_compact<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(aflag.get(), aoff.get(), nb, alist.get(), amap.get());
     PRG_CHECK_LAUNCH();
-    scan_exclusive(ecnt.get(), eoff.get(), nv, tot.get(), s, c->scratch, kt);
-    scan_exclusive(tcnt.get(), toff.get(), nv, ttot.get(), s, c->scratch, kt);
-    const int64_t snv = read_total(tot, s);
-    const int64_t snt = read_total(ttot, s);
+    PRG_HIP(hipMemcpyAsync(H, C, kCntSlots * 4, hipMemcpyDeviceToHost, s));
+    PRG_HIP(hipStreamSynchronize(s));
+    const int64_t snv = H[kCntVerts], snt = H[kCntTris], nact = H[kCntActive];
+    if (snv < 0 || snt < 0 || nact < 0) throw Error{PRG_E_NOMEM, "subdivision mesh exceeds int32 counts"};
     if (c->mesh_nv + snv >= (int64_t(1) << 31)) throw Error{PRG_E_NOMEM, "mesh vertex count exceeds int32 indices"};
+    if (nact == 0) return;
+
+    // passes 2 and 3 over the bricks with output
+    PoolBuf<uint16_t> lofs_tab{&c->pool[124]}, emask_tab{&c->pool[125]};
+    PoolBuf<uint8_t> cube_tab{&c->pool[126]};
+    lofs_tab.reserve(size_t(nact) * kBrickLeaves);
+    emask_tab.reserve(size_t(nact) * kBrickLeaves);
+    cube_tab.reserve(size_t(nact) * kBrickLeaves);
     float* SV = c->sub_v.reserve(size_t(3) * snv + 3);
     int32_t* ST = c->sub_t.reserve(size_t(3) * snt + 3);
-    k_vemit<<<gv, kBlockS, 0, s>>>(nv, G, c->key.get(), c->neigh.get(), M, D, cval.get(), emask.get(), eoff.get(),
-                                   toff.get(), c->mesh_nv, SV, ST);
-    PRG_CHECK_LAUNCH();
+    {
+        TimedScope ts(kt, "subdivide_vertices", 12.0 * double(snv));
+        k_brick_vertices<<<unsigned(nact), kWave, 0, s>>>(A, alist.get(), bvoff.get(), lofs_tab.get(),
+                                                          emask_tab.get(), cube_tab.get(), SV);
+        PRG_CHECK_LAUNCH();
+    }
+    {
+        TimedScope ts(kt, "subdivide_triangles", 12.0 * double(snt));
+        k_brick_triangles<<<unsigned(nact), kWave, 0, s>>>(A, alist.get(), amap.get(), bvoff.get(), btoff.get(),
+                                                           lofs_tab.get(), emask_tab.get(), cube_tab.get(),
+                                                           c->mesh_nv, ST, C + kCntErr);
+        PRG_CHECK_LAUNCH();
+    }
+    PRG_HIP(hipMemcpyAsync(H + kCntErr, C + kCntErr, 4, hipMemcpyDeviceToHost, s));
     PRG_HIP(hipStreamSynchronize(s));
+    if (H[kCntErr]) throw Error{PRG_E_HIP, "subdivision: a triangle edge has no owner vertex"};
     c->sub_nv = snv;
     c->sub_nt = snt;
 }
