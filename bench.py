@@ -3,10 +3,13 @@
 
 Workload: 2,000,000 float32 points on the union of 3 seeded superquadrics
 (noise sigma 0.001, analytic normals), octree depth 10, FP32 CG to 1e-6
-relative residual with FP64 dot products.  A step is the whole pipeline
-(normalisation, octree, divergence, per-depth solves, isovalue, leaf-level
-marching cubes, mesh copied to host memory) starting from points already
-resident in device memory.
+relative residual with FP64 dot products.  A step is the reference's whole
+main() minus file I/O: the points are uploaded from (pinned) host memory,
+normalised, the octree, divergence, per-depth solves, isovalue, leaf-level
+marching cubes and the coarse-leaf subdivision batches run, and the mesh lands
+in (pinned) host memory.  Beside it the line reports the same step from
+device-resident points, the leaf-level mesh alone, and one run with file I/O
+(a .bnpts read and the ASCII PLY write).
 
 Prints one JSON line (rank 0).  One GPU only: --gpus N with N > 1 exits.
 """
@@ -72,40 +75,91 @@ def workload(args):
     raise SystemExit(f"unknown workload {args.workload}")
 
 
-def run_step(g, dp, dn, n, pinned=None):
+def run_step(g, p, nrm, n, on_device, pinned=None, subdivide=1):
+    """One pass of the pipeline: points (device or host pointers) to the mesh in host memory."""
     import poissonrecon_gpu_amd as prg
 
-    g._call("prg_set_points", dp, dn, n, 1)
+    g._call("prg_set_points", p, nrm, n, int(on_device))
     g.build_octree()
     g.compute_divergence()
     stats = g.solve()
     g._call("prg_isovalue", None)  # the value stays on the device for the mesh
     if pinned is None:
-        mesh = g.extract_mesh()
+        mesh = g.extract_mesh(subdivide=bool(subdivide))
         return stats, (len(mesh.vertices), len(mesh.triangles))
     m = prg.PrgMesh()
-    g._call("prg_extract_mesh_into", 0, pinned.v, pinned.vcap, pinned.t, pinned.tcap, C.byref(m))
+    g._call("prg_extract_mesh_into", int(subdivide), pinned.v, pinned.vcap, pinned.t, pinned.tcap, C.byref(m))
     return stats, (m.n_vertices, m.n_triangles)
 
 
-def cpu_baseline(p, n, D, sample):
-    """The CPU oracle (restatement of the reference stages) on a bounded sample."""
+def timed_steps(hip, g, p, nrm, n, on_device, pinned, subdivide, steps):
+    """Seconds per step over `steps` steps (device synchronised on both sides)."""
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = run_step(g, p, nrm, n, on_device, pinned, subdivide)
+    hip.hipDeviceSynchronize()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(p, n, D, threads, min_root_depth):
+    """The CPU oracle (restatement of the reference stages) on the whole cloud:
+    normalisation through the leaf-level mesh and the subdivision batches of the
+    roots at depth >= min_root_depth (the shallower groups would need the
+    reference's explicit virtual trees, hundreds of GB at depth 10)."""
     import oracle
 
-    q, m = p[:sample], n[:sample]
+    oracle.set_num_threads(threads)
     o = oracle.Oracle(D)
+    t = {}
     t0 = time.perf_counter()
-    o.set_points(q, m)
+    o.set_points(p, n)
     o.build_octree()
+    t["octree"] = time.perf_counter() - t0
+    t1 = time.perf_counter()
     o.compute_divergence()
+    t["divergence"] = time.perf_counter() - t1
+    t1 = time.perf_counter()
     o.solve(mode=0)
+    t["solve"] = time.perf_counter() - t1
+    t1 = time.perf_counter()
     o.isovalue()
-    o.extract_leaf_mesh()
+    vl, tl = o.extract_leaf_mesh()
+    t["leaf_mesh"] = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    v, tr = o.extract_subdivision(min_root_depth)
+    t["subdivision"] = time.perf_counter() - t1
     dt = time.perf_counter() - t0
-    o.close()
-    return {"value": sample / dt, "unit": "points/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": f"first {sample} points of the same cloud at depth {D}, full pipeline to the leaf mesh "
-                      f"({dt:.1f} s, OpenMP threads={oracle.num_threads()})"}
+    return o, dt, {k: round(x, 2) for k, x in t.items()}, (len(v), len(tr))
+
+
+def fp64_errors(o, gt, counts, bases):
+    """Per-depth error of the GPU solve against the oracle's FP64 solve of the
+    same system (the oracle's divergence injected into the GPU context)."""
+    import poissonrecon_gpu_amd as prg
+
+    b = o.read_buffer(prg.BUF_DIVERGENCE)
+    gt.write_buffer(prg.BUF_DIVERGENCE, b)
+    stats = gt.solve()
+    x = gt.read_buffer(prg.BUF_SOLUTION).astype(np.float64)
+    x64 = o.solve_fp64(tol=1e-12)
+    out = []
+    for d, s in enumerate(stats):
+        sl = slice(int(bases[d]), int(bases[d] + counts[d]))
+        den = np.linalg.norm(x64[sl])
+        out.append({"depth": d, "iters": s["iters"], "true_rel_residual": float(f"{s['true_rel_residual']:.3e}"),
+                    "rel_l2_vs_fp64": float(f"{np.linalg.norm(x[sl] - x64[sl]) / den:.3e}") if den > 0 else 0.0})
+    return out
 
 
 def learned_leg(args):
@@ -230,8 +284,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="superquadrics_2M_d10")
     ap.add_argument("--points", type=int, default=2_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=400_000)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline and the FP64 error leg")
+    ap.add_argument("--cpu-min-root-depth", type=int, default=8,
+                    help="CPU baseline: subdivision batches of roots at this depth or deeper")
+    ap.add_argument("--no-io", action="store_true", help="skip the run with file I/O")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
     ap.add_argument("--learn-train", type=int, default=512)
     ap.add_argument("--learn-test", type=int, default=64)
@@ -254,45 +310,75 @@ def main():
     p, nrm, D = workload(args)
     N = len(p)
     hip = _hip()
-    dp, dn = C.c_void_p(), C.c_void_p()
+    # inputs: pinned host copies (the headline step uploads them) and device copies
+    hp, hn, dp, dn = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    assert hip.hipHostMalloc(C.byref(hp), p.nbytes, 0) == 0 and hip.hipHostMalloc(C.byref(hn), nrm.nbytes, 0) == 0
+    C.memmove(hp, p.ctypes.data, p.nbytes)
+    C.memmove(hn, nrm.ctypes.data, nrm.nbytes)
     assert hip.hipMalloc(C.byref(dp), p.nbytes) == 0 and hip.hipMalloc(C.byref(dn), nrm.nbytes) == 0
     assert hip.hipMemcpy(dp, p.ctypes.data, p.nbytes, 1) == 0
     assert hip.hipMemcpy(dn, nrm.ctypes.data, nrm.nbytes, 1) == 0
 
     g = prg.Reconstructor(max_depth=D)
-    _, (nv, nt) = run_step(g, dp, dn, N)  # sizes the pinned mesh buffers
+    _, (nv, nt) = run_step(g, hp, hn, N, False)  # sizes the pinned mesh buffers
     pinned = PinnedMesh(hip, int(nv * 1.25) + 1024, int(nt * 1.25) + 1024)
     for _ in range(max(0, args.warmup - 1)):
-        run_step(g, dp, dn, N, pinned)
-    hip.hipDeviceSynchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stats, mesh = run_step(g, dp, dn, N, pinned)
-    hip.hipDeviceSynchronize()
-    dt = (time.perf_counter() - t0) / args.steps
+        run_step(g, hp, hn, N, False, pinned)
+    # headline: host points -> full mesh (leaf level + subdivision) in host memory
+    dt, (stats, mesh) = timed_steps(hip, g, hp, hn, N, False, pinned, 1, args.steps)
     stage_ms = {k: round(v, 3) for k, v in g.stage_times().items()}
-    counts, _ = g.level_info()
+    roots, vleaves = C.c_int64(), C.c_int64()
+    g._call("prg_subdivision_info", C.byref(roots), C.byref(vleaves))
+    counts, bases = g.level_info()
+    # the same step from device-resident points, and the leaf-level mesh alone
+    dt_dev, _ = timed_steps(hip, g, dp, dn, N, True, pinned, 1, args.steps)
+    run_step(g, dp, dn, N, True, pinned, 0)
+    dt_leaf, (_, mesh_leaf) = timed_steps(hip, g, dp, dn, N, True, pinned, 0, args.steps)
+    variants = {
+        "device_points_full_mesh": {"ms_per_step": round(1000 * dt_dev, 3), "points_per_s": round(N / dt_dev, 1)},
+        "device_points_leaf_mesh_only": {"ms_per_step": round(1000 * dt_leaf, 3), "points_per_s": round(N / dt_leaf, 1),
+                                         "mesh": {"vertices": int(mesh_leaf[0]), "triangles": int(mesh_leaf[1])}},
+    }
     g.close()
+    if not args.no_io:
+        import tempfile
+
+        with tempfile.TemporaryDirectory() as td:
+            src, dst = Path(td) / "cloud.bnpts", Path(td) / "mesh.ply"
+            np.concatenate([p, nrm], axis=1).astype(np.float32).tofile(src)
+            with prg.Reconstructor(max_depth=D) as gi:
+                t0 = time.perf_counter()
+                q, qn = prg.read_points(str(src))
+                gi.set_points(q, qn)
+                gi.build_octree()
+                gi.compute_divergence()
+                gi.solve()
+                gi.isovalue()
+                m = gi.extract_mesh(subdivide=True)
+                prg.write_mesh_ply(str(dst), m)
+                t_io = time.perf_counter() - t0
+            variants["with_file_io"] = {"s": round(t_io, 3), "points_per_s": round(N / t_io, 1),
+                                        "ply_bytes": dst.stat().st_size,
+                                        "note": "one run: .bnpts read, pipeline, ASCII PLY write"}
 
     # one more pass with per-kernel hipEvents (on the context stream) for the roofline
     gt = prg.Reconstructor(max_depth=D, timing=True)
-    run_step(gt, dp, dn, N)  # sizes this context's buffers; its first-touch costs are not recorded
+    run_step(gt, dp, dn, N, True)  # sizes this context's buffers; its first-touch costs are not recorded
     gt.kernel_stats_reset()
-    run_step(gt, dp, dn, N)
+    run_step(gt, dp, dn, N, True)
     kern = {}
-    for name in ["stencil_spmv_dot_finest", "stencil_spmv_dot", "cg_update", "cg_idle", "cg_true_residual", "cg_refresh", "cg_small", "cg_init", "vector_field",
-                 "div_subtree_sums", "div_subtree_finish", "div_from_subtree_sums", "div_leaf", "parent_neighbor_table",
-                 "radix_downsweep", "radix_upsweep", "point_values", "corner_value", "neighbors", "levels_fill",
-                 "edge_owner", "corner_owner", "scan_exclusive", "gather_points", "morton", "point_weights",
-                 "spmv_tile_plan", "div_parent_sums"]:
+    for name in ["stencil_spmv_dot_finest", "stencil_spmv_dot", "cg_update", "cg_idle", "cg_true_residual", "cg_refresh",
+                 "cg_small", "cg_init", "vector_field", "div_subtree_sums", "div_subtree_finish",
+                 "div_from_subtree_sums", "div_leaf", "parent_neighbor_table", "radix_downsweep", "radix_upsweep",
+                 "point_values", "corner_value", "neighbors", "levels_fill", "edge_owner", "corner_owner",
+                 "scan_exclusive", "gather_points", "morton", "point_weights", "spmv_tile_plan", "div_parent_sums",
+                 "face_marks", "coarse_corner_value", "subdivide_count", "subdivide_vertices", "subdivide_triangles"]:
         try:
             cnt, ms, bpl = gt.kernel_stats(name)
             kern[name] = {"launches": cnt, "total_ms": round(ms, 4), "avg_us": round(1000 * ms / cnt, 2),
                           "alg_bytes_per_launch": bpl}
         except prg.PrgError:
             pass
-    gt.close()
-    pinned.free()
     hip.hipFree(dp)
     hip.hipFree(dn)
     if args.calibrate_copy:
@@ -304,31 +390,59 @@ def main():
         hip.hipFree(a)
         hip.hipFree(b)
 
-    # Dominant kernel of the path: the CG SpMV at the finest depth (k_cg_spmv; one
-    # launch per CG iteration over all M_D rows).  Algorithmic bytes: 116 B per row
-    # (27 int32 neighbour indices, p_i read, (Ap)_i written); launches that found
-    # the solve finished are filed apart and not averaged in.
+    # Dominant kernel of the solve: the CG SpMV at the finest depth (k_cg_spmv;
+    # one launch per CG iteration over all M_D rows).  Its algorithmic bytes in
+    # this build's layout (29.5 B/row: the parent-neighbour entries of the
+    # row's sibling block, r_i and p_{k-1,i} read, p_{k,i} and q_i written) give
+    # `achieved`; the reference's CSR representation (116 B/row) is reported
+    # beside it as an equivalent rate, and `traffic` is the measured HBM bytes
+    # per launch (FETCH_SIZE x2 + WRITE_SIZE, calibrated on a copy kernel).
     dom = "stencil_spmv_dot_finest"
     k = kern[dom]
     rows = k["alg_bytes_per_launch"] / SPMV_ROW_BYTES
-    achieved = k["alg_bytes_per_launch"] / (k["avg_us"] * 1e-6) / 1e9
-    own = rows * SPMV_OWN_ROW_BYTES / (k["avg_us"] * 1e-6) / 1e9
+    sec = k["avg_us"] * 1e-6
+    achieved = rows * SPMV_OWN_ROW_BYTES / sec / 1e9
     roofline = {"kernel": dom, "hip_kernel": "k_cg_spmv", "rows_per_launch": int(rows), "bound": "hbm",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "achieved_compulsory_own_layout": round(own, 1)}
+                "alg_bytes_per_row": SPMV_OWN_ROW_BYTES,
+                "reference_representation_rate_gbs": round(rows * SPMV_ROW_BYTES / sec / 1e9, 1),
+                "avg_us": k["avg_us"]}
     traffic_file = ROOT / "profiles" / f"traffic_k_cg_spmv_{args.workload}.json"
     if traffic_file.exists():
         t = json.loads(traffic_file.read_text())
         roofline["traffic"] = t.get("hbm_bytes_per_launch")
         roofline["traffic_source"] = str(traffic_file.relative_to(ROOT))
+        if roofline["traffic"]:
+            roofline["measured_traffic_rate_gbs"] = round(roofline["traffic"] / sec / 1e9, 1)
+            roofline["measured_traffic_frac"] = round(roofline["traffic"] / sec / 1e9 / HBM_PEAK_GBS, 4)
+
+    base = None
+    solve_fp64 = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        o, t_cpu, cpu_stages, cpu_mesh = cpu_baseline(p, nrm, D, threads, args.cpu_min_root_depth)
+        solve_fp64 = fp64_errors(o, gt, counts, bases)
+        o.close()
+        base = {"value": round(N / t_cpu, 1), "unit": "points/s", "cores": threads, "kind": "port",
+                "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+                "sample": f"the whole {N}-point cloud at depth {D}: octree, divergence, FP32 CG, isovalue, leaf-level "
+                          f"mesh and the subdivision batches of roots at depth >= {args.cpu_min_root_depth} "
+                          f"({t_cpu:.1f} s on {threads} OpenMP threads)",
+                "stage_s": cpu_stages, "mesh": {"vertices": cpu_mesh[0], "triangles": cpu_mesh[1]}}
+        # one-thread run of the 35,947-point sphere at depth 8, whole mesh
+        ps, ns = prg.synth_sphere(35947, 1)
+        o1, t1, st1, m1 = cpu_baseline(ps, ns, 8, 1, 0)
+        o1.close()
+        base["one_thread_sphere_36k_d8"] = {"s": round(t1, 2), "points_per_s": round(35947 / t1, 1),
+                                            "stage_s": st1, "mesh": {"vertices": m1[0], "triangles": m1[1]}}
+    gt.close()
+    pinned.free()
+    hip.hipHostFree(hp)
+    hip.hipHostFree(hn)
 
     learned = None if args.no_learned else learned_leg(args)
     batch = batch_leg(args) if args.batch else None
-
-    base = None
-    if not args.no_cpu:
-        base = cpu_baseline(p, nrm, D, min(args.cpu_sample, N))
 
     out = {
         "metric": "points/s end-to-end at depth 10; solve ms learned vs plain CG at equal residual",
@@ -344,11 +458,15 @@ def main():
         "dtype": "f32 (f64 reductions)",
         "data": "synthetic",
         "config": {"workload": args.workload, "points": N, "max_depth": D, "nodes": int(counts.sum()),
-                   "leaf_nodes": int(counts[-1]), "cg_tol": 1e-6, "parallelism": "single GPU"},
+                   "leaf_nodes": int(counts[-1]), "cg_tol": 1e-6, "parallelism": "single GPU",
+                   "step": "host points uploaded, leaf-level mesh and coarse-leaf subdivision, mesh in host memory"},
         "stage_ms": stage_ms,
         "solve_ms": stage_ms["solve"],
         "cg_iters": [s["iters"] for s in stats],
-        "mesh": {"vertices": int(mesh[0]), "triangles": int(mesh[1])},
+        "solve_vs_fp64_oracle": solve_fp64,
+        "mesh": {"vertices": int(mesh[0]), "triangles": int(mesh[1]), "subdivision_roots": roots.value,
+                 "virtual_leaves": vleaves.value},
+        "variants": variants,
         "kernels": kern,
         "roofline": roofline,
         "cpu_baseline": base,

@@ -218,6 +218,18 @@ prg_status prg_extract_mesh_into(prg_ctx* ctx, int32_t subdivide, float* vertice
  * triangle entry must lie in [0, n_vertices) (PRG_E_ARG otherwise). */
 prg_status prg_weld_mesh(prg_ctx* ctx, prg_mesh* mesh, int64_t* n_merged);
 
+/* Mesh decimation on a host mesh, in place: EdgeCollapse (mode 0) or
+ * TriangleCollapse (mode 1) of Geometry.inl:50-254 (declared
+ * Geometry.cuh:135-137; the reference's main() never calls them).  Triangles
+ * are visited from the last to the first; one whose sum of squared edge
+ * lengths exceeds edge_ratio times that of an equilateral triangle of its
+ * area has its shortest edge (or itself) collapsed into the smallest vertex
+ * index, positions averaging.  Vertices are compacted in order and triangles
+ * renumbered; n_vertices and n_triangles are updated and the leaf-batch
+ * counts set to 0 (the batches no longer separate).  Host code: each
+ * collapse depends on all earlier ones.  edge_ratio must be > 0. */
+prg_status prg_decimate_mesh(prg_mesh* mesh, int32_t mode, float edge_ratio);
+
 /* After a prg_extract_mesh* call with subdivide != 0: the number of coarse
  * leaves that were refined (SubdivideNum, main.cu:3826-3834) and of the
  * virtual depth-D leaves marching cubes ran over. */

@@ -96,6 +96,7 @@ SIGNATURES = {
     "prg_extract_mesh": (C.c_int, [_P, C.c_int32, C.POINTER(PrgMesh)]),
     "prg_mesh_release": (None, [C.POINTER(PrgMesh)]),
     "prg_weld_mesh": (C.c_int, [_P, C.POINTER(PrgMesh), C.POINTER(C.c_int64)]),
+    "prg_decimate_mesh": (C.c_int, [C.POINTER(PrgMesh), C.c_int32, C.c_float]),
     "prg_extract_mesh_into": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, C.c_int64, C.POINTER(PrgMesh)]),
     "prg_subdivision_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "prg_reconstruct": (C.c_int, [_P, C.c_int32, C.POINTER(PrgMesh), C.POINTER(PrgSolveStats)]),
@@ -370,6 +371,19 @@ def read_points(path: str) -> tuple[np.ndarray, np.ndarray]:
     finally:
         lib().prg_points_release(x, n)
     return xyz, nxyz
+
+
+def decimate_mesh(mesh: Mesh, mode: int, edge_ratio: float) -> Mesh:
+    """prg_decimate_mesh on a copy of `mesh`: EdgeCollapse (mode 0) or
+    TriangleCollapse (mode 1) with the given edge ratio."""
+    v = np.ascontiguousarray(mesh.vertices, np.float32).copy()
+    t = np.ascontiguousarray(mesh.triangles, np.int32).copy()
+    m = PrgMesh()
+    m.n_vertices, m.n_triangles = len(v), len(t)
+    m.vertices = v.ctypes.data_as(_F32P)
+    m.triangles = t.ctypes.data_as(C.POINTER(C.c_int32))
+    _check(lib().prg_decimate_mesh(C.byref(m), int(mode), C.c_float(edge_ratio)), "prg_decimate_mesh")
+    return Mesh(v[: m.n_vertices].copy(), t[: m.n_triangles].copy(), mesh.center, mesh.scale, 0, 0)
 
 
 def write_mesh_ply(path: str, mesh: Mesh) -> None:

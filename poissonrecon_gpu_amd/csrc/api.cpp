@@ -1,6 +1,7 @@
 // C ABI of the library (include/prg.h): argument checks, state machine,
 // error capture and host <-> device buffer access.  Every entry point
 // catches prg::Error and returns its status; nothing aborts the process.
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -128,6 +129,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         PRG_HIP(hipHostMalloc(&c->cg_host, 4096, hipHostMallocDefault));
         PRG_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->sub_host), 64, hipHostMallocDefault));
         PRG_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, cf.device));
+        if (const char* lim = std::getenv("PRG_TILE_LDS_LIMIT")) c->tile_limit = std::max(0, std::atoi(lim));
         c->kt.enabled = cf.timing != 0;
         c->kt.stream = c->stream;
         return PRG_OK;
@@ -314,7 +316,7 @@ prg_status prg_set_isovalue(prg_ctx* c, float iso) {
 namespace {
 
 // The host copy of the normalisation transform, once its copy has landed.
-void resolve_transform(prg_ctx* c) {
+static void resolve_transform(prg_ctx* c) {
     if (!c->transform_pending) return;
     PRG_HIP(hipStreamSynchronize(c->stream));
     for (int a = 0; a < 3; a++) c->center[a] = c->transform_host[a];
@@ -324,7 +326,7 @@ void resolve_transform(prg_ctx* c) {
 
 // Leaf-level marching cubes, then (subdivide != 0) the coarse-leaf
 // subdivision batches; sets the counts of `out`.
-void build_mesh(prg_ctx* c, int32_t subdivide, prg_mesh* out) {
+static void build_mesh(prg_ctx* c, int32_t subdivide, prg_mesh* out) {
     PRG_HIP(hipSetDevice(c->cfg.device));
     prg::extract_leaf_mesh(c);
     c->sub_nv = c->sub_nt = 0;
@@ -340,7 +342,7 @@ void build_mesh(prg_ctx* c, int32_t subdivide, prg_mesh* out) {
 }
 
 // Copies the leaf-level batch and the subdivision batches, in that order.
-void copy_mesh(prg_ctx* c, float* vertices, int32_t* triangles) {
+static void copy_mesh(prg_ctx* c, float* vertices, int32_t* triangles) {
     prg::StageTimer st(c, PRG_STAGE_MESH_COPY);
     if (!c->out_copied) {
         if (c->mesh_nv) d2h(c, vertices, c->mesh_v.get(), size_t(3 * c->mesh_nv) * 4);
@@ -400,6 +402,25 @@ prg_status prg_extract_mesh_into(prg_ctx* c, int32_t subdivide, float* vertices,
         out->triangles = triangles;
         return PRG_OK;
     });
+}
+
+prg_status prg_decimate_mesh(prg_mesh* mesh, int32_t mode, float edge_ratio) {
+    if (!mesh || (mode != 0 && mode != 1) || !(edge_ratio > 0.0f)) return PRG_E_ARG;
+    const int64_t n = mesh->n_vertices, m = mesh->n_triangles;
+    if (n < 0 || m < 0 || (n > 0 && !mesh->vertices) || (m > 0 && !mesh->triangles)) return PRG_E_ARG;
+    if (n >= (int64_t(1) << 31)) return PRG_E_ARG;
+    for (int64_t i = 0; i < 3 * m; i++)
+        if (mesh->triangles[i] < 0 || mesh->triangles[i] >= n) return PRG_E_ARG;
+    try {
+        int64_t nv = n, nt = m;
+        prg::decimate(mode, edge_ratio, mesh->vertices, &nv, mesh->triangles, &nt);
+        mesh->n_vertices = nv;
+        mesh->n_triangles = nt;
+        mesh->n_leaf_vertices = mesh->n_leaf_triangles = 0;
+    } catch (const std::bad_alloc&) {
+        return PRG_E_NOMEM;
+    }
+    return PRG_OK;
 }
 
 prg_status prg_subdivision_info(prg_ctx* c, int64_t* n_roots, int64_t* n_virtual_leaves) {

@@ -4,14 +4,16 @@
 //
 //   poisson_recon --in points.ply --out mesh.ply [--depth 8] [--tol 1e-6]
 //                 [--maxIters 2000] [--refresh 20] [--device 0] [--json]
-//                 [--no-subdivide] [--weld]
+//                 [--no-subdivide] [--weld] [--edgeCollapse R | --triangleCollapse R]
 //
 // The mesh holds the leaf-level batch and the coarse-leaf subdivision batches,
 // as the reference's main() always writes (main.cu:3799-4561);
 // --no-subdivide keeps only the leaf-level batch (main.cu:3793), and
 // --subdivide is accepted for older scripts (it is the default).  --weld
 // merges the vertices the batches duplicate (prg_weld_mesh, which the
-// reference does not do).
+// reference does not do).  --edgeCollapse R / --triangleCollapse R decimate
+// the mesh afterwards (prg_decimate_mesh: EdgeCollapse / TriangleCollapse of
+// Geometry.inl:50-254, which the reference's main() does not call).
 //
 // Input formats by extension (.ply, .bnpts, ASCII otherwise, main.cu:517-520);
 // output is an ASCII PLY triangle mesh (PlyWriteTriangles, plyfile.cu:2769).
@@ -28,7 +30,8 @@ namespace {
 void usage() {
     std::fprintf(stderr,
                  "usage: poisson_recon --in <points> --out <mesh.ply> [--depth D] [--tol T] [--maxIters N]\n"
-                 "                     [--refresh K] [--device I] [--json] [--no-subdivide] [--weld]\n");
+                 "                     [--refresh K] [--device I] [--json] [--no-subdivide] [--weld]\n"
+                 "                     [--edgeCollapse R | --triangleCollapse R]\n");
 }
 
 const char* kStageNames[PRG_STAGE_COUNT] = {
@@ -43,6 +46,8 @@ int main(int argc, char** argv) {
     prg_config cfg;
     prg_config_default(&cfg);
     bool json = false, subdivide = true, weld = false;
+    int decimate_mode = -1;
+    float decimate_ratio = 0.0f;
     for (int i = 1; i < argc; i++) {
         std::string a = argv[i];
         auto next = [&]() -> const char* {
@@ -63,6 +68,14 @@ int main(int argc, char** argv) {
         else if (a == "--subdivide") subdivide = true;
         else if (a == "--no-subdivide") subdivide = false;
         else if (a == "--weld") weld = true;
+        else if (a == "--edgeCollapse" || a == "--triangleCollapse") {
+            decimate_mode = a == "--edgeCollapse" ? 0 : 1;
+            decimate_ratio = float(std::atof(next()));
+            if (!(decimate_ratio > 0.0f)) {
+                usage();
+                return 2;
+            }
+        }
         else {
             usage();
             return 2;
@@ -100,6 +113,12 @@ int main(int argc, char** argv) {
     int64_t merged = 0;
     if (weld && prg_weld_mesh(ctx, &mesh, &merged) != PRG_OK) {
         std::fprintf(stderr, "welding failed: %s\n", prg_last_error(ctx));
+        prg_mesh_release(&mesh);
+        prg_destroy(ctx);
+        return 1;
+    }
+    if (decimate_mode >= 0 && prg_decimate_mesh(&mesh, decimate_mode, decimate_ratio) != PRG_OK) {
+        std::fprintf(stderr, "decimation failed\n");
         prg_mesh_release(&mesh);
         prg_destroy(ctx);
         return 1;

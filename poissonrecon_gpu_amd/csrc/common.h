@@ -114,10 +114,13 @@ struct KernelTimer {
         if (idx < 0) return;
         PRG_HIP(hipEventRecord(pending[idx].b, stream));
     }
-    // Files a recorded launch under another name (e.g. one that found its
-    // work already finished), before flush().
-    void relabel(int idx, const char* name) {
-        if (idx >= 0 && idx < int(pending.size())) pending[idx].name = name;
+    // Files a recorded launch under another name with other algorithmic bytes
+    // (e.g. one that found its work already finished and moved nothing), before flush().
+    void relabel(int idx, const char* name, double bytes = 0.0) {
+        if (idx >= 0 && idx < int(pending.size())) {
+            pending[idx].name = name;
+            pending[idx].bytes = bytes;
+        }
     }
     // Resolves recorded events (synchronises on them).
     void flush() {

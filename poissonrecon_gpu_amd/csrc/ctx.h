@@ -130,6 +130,10 @@ struct prg_ctx {
     int num_cu = 0;                   // compute units of the device
     void* cg_host = nullptr;          // pinned, 4 KiB: CG state read-backs and the segment table's staging
     int32_t* sub_host = nullptr;      // pinned, 16 ints: the subdivision's counters
+    // test setting (environment PRG_TILE_LDS_LIMIT at creation): caps the
+    // listed blocks and points the leaf-level tile kernels stage in LDS, so
+    // their per-row and global-memory paths run on ordinary inputs
+    int32_t tile_limit = 0;
 
     int64_t MD() const { return counts.empty() ? 0 : counts[D]; }
     int64_t base_of(int d) const { return bases[d]; }
@@ -146,6 +150,8 @@ void unpack_points(prg_ctx* c);
 // already checked to lie in [0, n))
 void weld_mesh(prg_ctx* c, float* vertices, int64_t n, int32_t* triangles, int64_t m, int64_t n_leaf,
                int64_t* n_out, int64_t* n_leaf_out);
+// EdgeCollapse (mode 0) / TriangleCollapse (mode 1) of a host mesh in place (decimate.cpp)
+void decimate(int mode, float edge_ratio, float* vertices, int64_t* n_io, int32_t* triangles, int64_t* m_io);
 // the 27-neighbour planes of every depth (kernels use the parent-neighbour
 // rows; these serve level views, buffer reads and the subdivision)
 void ensure_neighbours(prg_ctx* c);

@@ -20,6 +20,8 @@
 #include <map>
 #include <mutex>
 
+#include <algorithm>
+
 #include "ctx.h"
 #include "pnt.h"
 #include "tile.h"
@@ -203,7 +205,8 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_vector_field_tiles(TilePlan
                                                                      const float* __restrict__ ny,
                                                                      const float* __restrict__ nz, int64_t baseD,
                                                                      float* __restrict__ vfx, float* __restrict__ vfy,
-                                                                     float* __restrict__ vfz) {
+                                                                     float* __restrict__ vfz, int32_t lim_blocks,
+                                                                     int32_t lim_points) {
     // A sibling block's points are contiguous in key order, cell after cell,
     // from the block's first point (the first cell's pidx, also when empty):
     // a cell's first point is bfirst[block] + coff[cell] - coff[8 block].
@@ -215,7 +218,7 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_vector_field_tiles(TilePlan
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t tile = blockIdx.x;
     const int32_t ncell = 8 * P.cnt[tile];
-    if (ncell > 8 * kTileBlocks) {
+    if (ncell > 8 * lim_blocks) {
         const int64_t t = tile * kTileThreads + tid;
         if (t < 8 * nblk) vector_field_row(pntD, pidx, pnum, W, nx, ny, nz, baseD, t, vfx, vfy, vfz);
         return;
@@ -251,7 +254,7 @@ __global__ void __launch_bounds__(kTileThreads, 6) k_vector_field_tiles(TilePlan
     if (tid == kTileThreads - 1) coff[ncell] = run;
     __syncthreads();
     const int32_t total = coff[ncell];
-    const bool staged = total <= kVfPoints;
+    const bool staged = total <= lim_points;
     if (staged) {
         for (int32_t i = tid; i < total; i += kTileThreads) {
             int lo = 0, hi = ncell;  // the last cell whose offset is <= i
@@ -368,14 +371,14 @@ __global__ void __launch_bounds__(kTileThreads) k_div_leaf_tiles(TilePlan P, int
                                                                  const float* __restrict__ vfy,
                                                                  const float* __restrict__ vfz,
                                                                  const float* __restrict__ T, int64_t baseD,
-                                                                 float* __restrict__ div) {
+                                                                 float* __restrict__ div, int32_t lim_blocks) {
     constexpr int stride = 8 * kTileBlocks;
     __shared__ float cells[3 * stride];
     __shared__ int16_t rid[kTileEntries];
     const int tid = threadIdx.x;
     const int64_t tile = blockIdx.x;
     const int32_t cnt = P.cnt[tile];
-    if (cnt > kTileBlocks) {
+    if (cnt > lim_blocks) {
         const int64_t t = tile * kTileThreads + tid;
         if (t < 8 * nblk) div_leaf_row(pntD, vfx, vfy, vfz, T, baseD, t, div);
         return;
@@ -771,6 +774,10 @@ void compute_divergence(prg_ctx* c) {
     const int64_t M = c->M, MD = c->MD(), baseD = c->bases[D];
     BaseMD bm;
     for (int k = 0; k < 16; k++) bm.v[k] = c->fs->base_maxdepth[k];
+    // LDS capacities of the tile kernels; a smaller limit (a test setting)
+    // sends more tiles down the per-row and global-memory paths
+    const int32_t lim_blocks = c->tile_limit > 0 ? std::min(c->tile_limit, kTileBlocks) : kTileBlocks;
+    const int32_t lim_points = c->tile_limit > 0 ? std::min(c->tile_limit, kVfPoints) : kVfPoints;
     float* VX = c->vfx.reserve(MD);
     float* VY = c->vfy.reserve(MD);
     float* VZ = c->vfz.reserve(MD);
@@ -791,7 +798,7 @@ void compute_divergence(prg_ctx* c) {
             const int64_t nblk = MD / 8, ntiles = (nblk + kTile - 1) / kTile;
             k_vector_field_tiles<<<unsigned(ntiles), kTileThreads, 0, s>>>(
                 P, nblk, c->pnt.get() + 27 * c->pnt_off[D], c->pidx.get(), c->pnum.get(), W.get(), c->nx.get(),
-                c->ny.get(), c->nz.get(), baseD, VX, VY, VZ);
+                c->ny.get(), c->nz.get(), baseD, VX, VY, VZ, lim_blocks, lim_points);
         } else {
             k_vector_field<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->pnt.get() + 27 * c->pnt_off[D], c->pidx.get(),
                                                                    c->pnum.get(), W.get(), c->nx.get(), c->ny.get(),
@@ -811,7 +818,7 @@ void compute_divergence(prg_ctx* c) {
         if (tile_plan_on(c, D, &P, s)) {
             const int64_t nblk = MD / 8, ntiles = (nblk + kTile - 1) / kTile;
             k_div_leaf_tiles<<<unsigned(ntiles), kTileThreads, 0, s>>>(P, nblk, c->pnt.get() + 27 * c->pnt_off[D], VX,
-                                                                       VY, VZ, T, baseD, DV);
+                                                                       VY, VZ, T, baseD, DV, lim_blocks);
         } else {
             k_div_leaf<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->pnt.get() + 27 * c->pnt_off[D], VX, VY, VZ, T,
                                                                baseD, MD, DV);

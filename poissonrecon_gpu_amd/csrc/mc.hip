@@ -512,7 +512,9 @@ void start_topology(prg_ctx* c) {
     resolve_stage(c, PRG_STAGE_MC_TOPOLOGY);
     PRG_HIP(hipEventRecord(c->stage_ev[PRG_STAGE_MC_TOPOLOGY][0], s));
     {
-        TimedScope ts(kt, "corner_owner", 8.0 * 64.0 * double(MD));
+        // per leaf slot: its key, its share of the block's parent-neighbour row
+        // (13.5 B), the mask and count written, 8 owner references written
+        TimedScope ts(kt, "corner_owner", (4.0 + 13.5 + 8.0 + 32.0) * double(MD));
         k_corner_owner<<<grid_for(MD, kBlock), kBlock, 0, s>>>(c->key.get(), c->pnt.get() + 27 * c->pnt_off[D], baseD,
                                                                MD, cmask.get(), ccount.get(), oref.get());
         PRG_CHECK_LAUNCH();
@@ -580,7 +582,10 @@ void extract_leaf_mesh(prg_ctx* c) {
         cubes.reserve(MD);
         {
             // corner values gathered (8 floats), cube index, triangle count and edge owners per slot
-            TimedScope ts(kt, "edge_owner", (32.0 + 32.0 + 32.0 + 12.0 * 40.0) * double(MD));
+            // per leaf slot: 8 owner references and 8 corner values read, its
+            // row share (13.5 B); 8 corner values, cube, triangle count, edge
+            // mask and count, 12 edge references written
+            TimedScope ts(kt, "edge_owner", (32.0 + 32.0 + 13.5 + 32.0 + 1.0 + 4.0 + 8.0 + 48.0) * double(MD));
             k_corner_edges<<<grid_for(MD, kBlock), kBlock, 0, s>>>(oref.get(), cmask.get(), cbase.get(), vval.get(),
                                                                    c->key.get(), c->pnt.get() + 27 * c->pnt_off[D],
                                                                    baseD, MD, CV, cubes.get(), ntri.get(), emask.get(),

@@ -252,6 +252,41 @@ __global__ void __launch_bounds__(kBlockS) k_root_of(const int32_t* roots, const
     if (i < *nroots) root_of[roots[i]] = int32_t(i);
 }
 
+// Largest |x| of each depth (one workgroup per depth).
+__global__ void __launch_bounds__(1024) k_depth_absmax(const float* x, Levels L, int D, int64_t M, float* out) {
+    __shared__ float red[1024];
+    const int d = blockIdx.x;
+    const int64_t b = L.base[d], e = d < D ? L.base[d + 1] : M;
+    float m = 0.0f;
+    for (int64_t i = b + threadIdx.x; i < e; i += 1024) m = fmaxf(m, fabsf(x[i]));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[d] = red[0];
+}
+
+// Largest |value| of each depth's entries of the corner basis table.
+__global__ void __launch_bounds__(1024) k_table_absmax(const float4* tab, int D, float* out) {
+    __shared__ float red[1024];
+    const int d = blockIdx.x;
+    const int64_t n = int64_t(2) << D;  // (ix, side) entries per depth
+    float m = 0.0f;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) {
+        const float4 t = tab[int64_t(d) * n + i];
+        m = fmaxf(m, fmaxf(fabsf(t.x), fmaxf(fabsf(t.y), fabsf(t.z))));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (int(threadIdx.x) < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[d] = red[0];
+}
+
 struct SubArgs {
     const uint32_t* key;
     const int32_t* neigh;  // 27 planes
@@ -270,6 +305,8 @@ struct SubArgs {
     const int32_t* boff;     // first brick of each root (exclusive scan, nroots + 1 entries)
     const int32_t* root_of;  // per real node: its root number, or -1
     const int32_t* cnt;      // device counters
+    const float* xmax;       // per depth: largest |x|
+    const float* bmax;       // per depth: largest |basis value| in the corner basis table
 };
 
 // A brick and its root (wave-uniform).
@@ -283,10 +320,31 @@ struct Brick {
     bool deep;        // a neighbour of the root has children (real nodes below depth r)
 };
 
+constexpr int kP = (1 << kBrickLog) + 1;  // grid points per brick side (at most)
+
+// Scratch of the brick certificate (brick_certify): per depth, the
+// coefficients of a 4x4x4 block of nodes and the separable partial sums.
+struct CertLds {
+    float w[3][kP][4], wa[3][kP][4];  // per axis and grid coordinate: basis weights over the block (and |.|)
+    float X4[64], X4a[64];
+    float Z[16][kP], Za[16][kP];
+    float Y[4][kP][kP], Ya[4][kP][kP];
+};
+
+// Blocks of the real nodes below the root that a brick's corner owners see:
+// at most (D - 3 - r) blocks of 4^3 cells and blocks of 5^3, 7^3 and 11^3
+// cells at the three finest depths.
+constexpr int kXBlk = 2304;
+
 // LDS of one brick's wave
 struct BrickLds {
-    float G[kBrickPts];                // f - iso at the brick's corners (grid points)
+    union {
+        float G[kBrickPts];  // f - iso at the brick's corners (grid points)
+        CertLds cert;
+    };
     float X[(kMaxDepth + 1) * 27];     // the root's chain: neighbour values at depths 0..r
+    float XB[kXBlk];                   // per depth below the root: its block (xb_off)
+    int32_t xb_off[kMaxDepth + 1];     // start of depth dd's block in XB, or -1 (no real node in it)
     int32_t NB[27], NBK[27], NBR[27];  // the root's neighbours, their child bases, their root numbers (group)
 };
 
@@ -357,6 +415,106 @@ __device__ __forceinline__ bool cell_virtual(const Brick& B, const BrickLds& S, 
     *slot = j;
     if (j == 13) return true;
     return S.NBR[j] >= 0;
+}
+
+// Width in cells of the block at depth dd: the owners' ancestors (the cells
+// the brick spans, and one more below for owners in the lower brick) +-1.
+__device__ __forceinline__ int block_width(int D, int kb, int dd) {
+    const int e = dd - (D - kb);
+    return (e > 0 ? (1 << e) : 1) + 3;
+}
+
+// Coefficient of the depth-dd cell c for corners of the brick's root: the
+// real node reached from the root's neighbour by c's key bits, or 0 (inside
+// the root: virtual; absent; below a coarse leaf).
+__device__ __forceinline__ float deep_coef(const SubArgs& A, const Brick& B, const BrickLds& S, int dd,
+                                           const int32_t c[3]) {
+    const int32_t W = 1 << dd;
+    if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= W || c[1] >= W || c[2] >= W) return 0.0f;
+    const int up = dd - B.r;
+    const int ox = (c[0] >> up) - B.R[0], oy = (c[1] >> up) - B.R[1], oz = (c[2] >> up) - B.R[2];
+    if (!(ox | oy | oz)) return 0.0f;
+    const int jn = nidx(ox, oy, oz);
+    int32_t cur = S.NBK[jn] < 0 ? -1 : S.NB[jn];
+    for (int l = B.r + 1; l <= dd && cur >= 0; l++) {
+        const int32_t cb = A.child_base[cur];
+        const int s2 = dd - l;
+        cur = cb < 0 ? -1 : cb + ((((c[0] >> s2) & 1) << 2) | (((c[1] >> s2) & 1) << 1) | ((c[2] >> s2) & 1));
+    }
+    return cur >= 0 ? A.x[cur] : 0.0f;
+}
+
+// Fills the brick's blocks (depths r+1..D): each cell's coefficient, one
+// walk down the tree per cell instead of one per corner and neighbour.  A
+// depth whose block lies inside the root, or holds no real node, gets none.
+__device__ void brick_blocks(const SubArgs& A, const Brick& B, BrickLds& S) {
+    const int D = A.D, lane = threadIdx.x;
+    int off = 0;
+    for (int dd = B.r + 1; dd <= D; dd++) {
+        const int W = block_width(D, B.kb, dd), sh = D - dd, up = dd - B.r;
+        int32_t lo[3];
+        bool inside = true;
+        for (int a = 0; a < 3; a++) {
+            lo[a] = (B.org[a] >> sh) - 2;
+            const int32_t rlo = B.R[a] << up, rhi = rlo + (1 << up) - 1;
+            inside = inside && lo[a] >= rlo && lo[a] + W - 1 <= rhi;
+        }
+        int slot = -1;
+        if (!inside && B.deep) {
+            const int n = W * W * W;
+            bool any = false;
+            for (int e = lane; e < n; e += kWave) {
+                const int32_t c[3] = {lo[0] + e / (W * W), lo[1] + (e / W) % W, lo[2] + e % W};
+                const float xv = deep_coef(A, B, S, dd, c);
+                S.XB[off + e] = xv;
+                any = any || xv != 0.0f;
+            }
+            if (__builtin_amdgcn_ballot_w64(any)) {
+                slot = off;
+                off += n;
+            }
+        }
+        if (lane == 0) S.xb_off[dd] = slot;
+    }
+    __syncthreads();
+}
+
+// deep_sum for an owner leaf o of the brick's own root, reading the blocks.
+// Every one of the 27 terms is added: a neighbour that is virtual, absent or
+// outside the domain contributes +-0 (its coefficient or its basis value is
+// 0), which leaves a float sum that started at +0 unchanged, so the result
+// is deep_sum's bit for bit.
+__device__ float deep_sum_lds(const SubArgs& A, const Brick& B, const BrickLds& S, const int32_t o[3], const int s[3]) {
+    const int D = A.D;
+    float f = 0.0f;
+    for (int dd = D; dd > B.r; dd--) {
+        const int slot = S.xb_off[dd];
+        if (slot < 0) continue;
+        const int up = dd - B.r, sh = D - dd;
+        const int32_t ax = o[0] >> sh, ay = o[1] >> sh, az = o[2] >> sh;
+        const int32_t ext = (1 << up) - 1;
+        const int32_t lx = B.R[0] << up, ly = B.R[1] << up, lz = B.R[2] << up;
+        if (ax - 1 >= lx && ax + 1 <= lx + ext && ay - 1 >= ly && ay + 1 <= ly + ext && az - 1 >= lz &&
+            az + 1 <= lz + ext)
+            continue;
+        const int W = block_width(D, B.kb, dd);
+        const int bx0 = ax - 1 - ((B.org[0] >> sh) - 2), by0 = ay - 1 - ((B.org[1] >> sh) - 2),
+                  bz0 = az - 1 - ((B.org[2] >> sh) - 2);
+        const float4 tx = A.btab[((int64_t(dd) << D) + o[0]) * 2 + s[0]];
+        const float4 ty = A.btab[((int64_t(dd) << D) + o[1]) * 2 + s[1]];
+        const float4 tz = A.btab[((int64_t(dd) << D) + o[2]) * 2 + s[2]];
+        const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
+        const float* Xb = S.XB + slot;
+#pragma unroll
+        for (int k = 0; k < 27; k++) {
+            const float xv = Xb[((bx0 + k / 9) * W + (by0 + (k / 3) % 3)) * W + (bz0 + k % 3)];
+            float tt = xv * bx[k / 9];
+            tt = tt * by[(k / 3) % 3];
+            tt = tt * bz[k % 3];
+            f = f + tt;
+        }
+    }
+    return f;
 }
 
 // Sum over the real nodes below depth r (virtual coefficients are 0) for the
@@ -464,10 +622,7 @@ __device__ float grid_value(const SubArgs& A, const Brick& B, const BrickLds& S,
         for (int a = 0; a < 3; a++) side[a] = g[a] - o[a];
     }
     if (slot == 13) {
-        float f = 0.0f;
-        if (B.deep)
-            f = deep_sum(
-                A, B.r, B.R, o, side, [&](int j) { return S.NB[j]; }, [&](int j) { return S.NBK[j]; });
+        const float f = B.deep ? deep_sum_lds(A, B, S, o, side) : 0.0f;
         return chain_lds(A, S, B.r, o, side, f) - *A.iso;
     }
     // owned by a leaf of another root of the group: that root's neighbours and chain
@@ -581,12 +736,175 @@ __device__ __forceinline__ int wave_excl_scan(int v) {
     return inc - v;
 }
 
+// Owner cell along one axis of grid coordinate g for a root whose owners are
+// all its own leaves: the smaller of the two cells around g that lies in the
+// root (Morton keys grow with each coordinate).
+__device__ __forceinline__ int32_t own_cell(int32_t g, int32_t R0) { return g == R0 ? g : g - 1; }
+
+// Proves, when it can, that every corner value of the brick has one strict
+// sign, so the brick has no crossing edge and no triangle.  For each depth
+// whose nodes are fixed across the brick (the chain depths 0..r and the
+// depths r+1..D-kb, where the whole brick lies in one cell) the sum
+// S(g) = sum_k x_k b_x b_y b_z over the corner owner's 27 neighbours is
+// evaluated exactly as a separable contraction of a 4x4x4 block of
+// coefficients with per-axis weights taken from the same float basis table
+// the evaluation reads (the fourth layer holds the neighbours of owners in the
+// lower brick), together with A(g) = sum |x_k b_x b_y b_z|.  The float value
+// the evaluation computes, a sum of n <= 27 (D+1) + 1 products of three
+// roundings each, differs from S(g) by at most (n+3) 2^-24 A(g) (recursive
+// summation bound) plus the rounding of this contraction; each finer depth
+// contributes at most U(g): the sum of |x| over the 27 neighbours the owner's
+// ancestor sees there (from the brick's blocks) times the depth's largest
+// basis value cubed.  The brick is certified when
+// |S(g) - iso| exceeds 420 2^-24 (A + U) + U with one sign at every corner.
+// Bricks whose corners may be owned by another root of the group are not.
+__device__ bool brick_certify(const SubArgs& A, const Brick& B, BrickLds& S) {
+    const int D = A.D, kb = B.kb, Sb = 1 << kb, P = Sb + 1;
+    const int lane = threadIdx.x;
+    CertLds& L = S.cert;
+    bool lo[3], hi[3];
+    for (int a = 0; a < 3; a++) {
+        lo[a] = B.org[a] == B.R0[a];
+        hi[a] = B.org[a] + Sb == B.R0[a] + B.Sr;
+    }
+    for (int j = 0; j < 27; j++) {
+        if (j == 13 || S.NBR[j] < 0) continue;
+        const int dv[3] = {j / 9 - 1, (j / 3) % 3 - 1, j % 3 - 1};
+        bool touches = true;
+        for (int a = 0; a < 3; a++)
+            if (dv[a] != 0) touches = touches && (dv[a] < 0 ? lo[a] : hi[a]);
+        if (touches) return false;
+    }
+    constexpr int kPer = (kBrickPts + kWave - 1) / kWave;
+    float sacc[kPer], aacc[kPer];
+#pragma unroll
+    for (int t = 0; t < kPer; t++) sacc[t] = aacc[t] = 0.0f;
+    const int dlast = D - kb;  // deepest depth whose cell holds the whole brick
+    for (int d = 0; d <= dlast; d++) {
+        // the block's coefficients: cells A - 2 .. A + 1 per axis around the
+        // brick's ancestor A at depth d
+        float xv = 0.0f;
+        {
+            const int mx = lane >> 4, my = (lane >> 2) & 3, mz = lane & 3;
+            if (d <= B.r) {
+                if (mx >= 1 && my >= 1 && mz >= 1) {
+                    if (d == 0) xv = (mx == 2 && my == 2 && mz == 2) ? S.X[0] : 0.0f;
+                    else xv = S.X[27 * d + 9 * (mx - 1) + 3 * (my - 1) + (mz - 1)];
+                }
+            } else if (S.xb_off[d] >= 0) {
+                xv = S.XB[S.xb_off[d] + lane];  // a 4^3 block, cells A - 2 .. A + 1
+            }
+        }
+        if (__builtin_amdgcn_ballot_w64(xv != 0.0f) == 0) continue;  // nothing at this depth
+        L.X4[lane] = xv;
+        L.X4a[lane] = fabsf(xv);
+        // per-axis weights: the owner's three basis values, placed at its
+        // ancestor's offset inside the block
+        if (lane < 3 * P) {
+            const int ax = lane / P, i = lane % P;
+            const int32_t g = B.org[ax] + i;
+            const int32_t o = own_cell(g, B.R0[ax]);
+            const int side = g - o;
+            const int delta = (o >> (D - d)) - (B.org[ax] >> (D - d));  // -1 or 0
+            const float4 t = A.btab[((int64_t(d) << D) + o) * 2 + side];
+            const float tv[3] = {t.x, t.y, t.z};
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const int a = m - delta - 1;
+                const float v = (a >= 0 && a <= 2) ? tv[a] : 0.0f;
+                L.w[ax][i][m] = v;
+                L.wa[ax][i][m] = fabsf(v);
+            }
+        }
+        __syncthreads();
+        for (int e = lane; e < 16 * P; e += kWave) {
+            const int mn = e / P, k = e % P;
+            float z = 0.0f, za = 0.0f;
+#pragma unroll
+            for (int l = 0; l < 4; l++) {
+                z = __builtin_fmaf(L.X4[4 * mn + l], L.w[2][k][l], z);
+                za = __builtin_fmaf(L.X4a[4 * mn + l], L.wa[2][k][l], za);
+            }
+            L.Z[mn][k] = z;
+            L.Za[mn][k] = za;
+        }
+        __syncthreads();
+        for (int e = lane; e < 4 * P * P; e += kWave) {
+            const int m = e / (P * P), j = (e / P) % P, k = e % P;
+            float y = 0.0f, ya = 0.0f;
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                y = __builtin_fmaf(L.w[1][j][n], L.Z[4 * m + n][k], y);
+                ya = __builtin_fmaf(L.wa[1][j][n], L.Za[4 * m + n][k], ya);
+            }
+            L.Y[m][j][k] = y;
+            L.Ya[m][j][k] = ya;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < kPer; t++) {
+            const int p = lane + kWave * t;
+            if (p < P * P * P) {
+                const int i = p % P, j = (p / P) % P, k = p / (P * P);
+                float v = 0.0f, va = 0.0f;
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    v = __builtin_fmaf(L.w[0][i][m], L.Y[m][j][k], v);
+                    va = __builtin_fmaf(L.wa[0][i][m], L.Ya[m][j][k], va);
+                }
+                sacc[t] += v;
+                aacc[t] += va;
+            }
+        }
+        __syncthreads();
+    }
+    const float iso = *A.iso;
+    bool ok = true, pos = false, neg = false;
+#pragma unroll
+    for (int t = 0; t < kPer; t++) {
+        const int p = lane + kWave * t;
+        if (p >= P * P * P) continue;
+        const int32_t g[3] = {B.org[0] + p % P, B.org[1] + (p / P) % P, B.org[2] + p / (P * P)};
+        // the finer depths, where the owner's ancestor touches a root face
+        // whose neighbours have children
+        float U = 0.0f;
+        for (int d = max(dlast + 1, B.r + 1); d <= D; d++) {
+            const int slot = S.xb_off[d];
+            if (slot < 0) continue;
+            // the 27 coefficients the owner's ancestor sees at this depth
+            const int sh = D - d, W = block_width(D, kb, d);
+            int b0[3];
+            for (int a = 0; a < 3; a++) b0[a] = (own_cell(g[a], B.R0[a]) >> sh) - 1 - ((B.org[a] >> sh) - 2);
+            float sx = 0.0f;
+            for (int k = 0; k < 27; k++)
+                sx += fabsf(S.XB[slot + ((b0[0] + k / 9) * W + (b0[1] + (k / 3) % 3)) * W + (b0[2] + k % 3)]);
+            const float bm = A.bmax[d];
+            U += 1.0001f * sx * (bm * bm * bm);
+        }
+        const float v = sacc[t] - iso;
+        const float margin = 420.0f * 5.9604645e-8f * (aacc[t] + U) + U + 1.2e-7f * (fabsf(sacc[t]) + fabsf(iso)) + 1e-15f;
+        ok = ok && fabsf(v) > margin;
+        pos = pos || v > 0;
+        neg = neg || v < 0;
+    }
+    const bool lane_good = ok && !(pos && neg);
+    const uint64_t good = __builtin_amdgcn_ballot_w64(lane_good);
+    const uint64_t anypos = __builtin_amdgcn_ballot_w64(pos), anyneg = __builtin_amdgcn_ballot_w64(neg);
+    return good == ~uint64_t(0) && !(anypos && anyneg);
+}
+
 // Pass 1: vertex and triangle counts of every brick.
 __global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, int32_t* bnv, int32_t* bnt) {
     __shared__ BrickLds S;
     Brick B;
     const int32_t b = blockIdx.x;
     brick_setup(A, b, B, S, true);
+    brick_blocks(A, B, S);
+    if (brick_certify(A, B, S)) {
+        if (threadIdx.x == 0) bnv[b] = bnt[b] = 0;
+        return;
+    }
+    __syncthreads();
     brick_values(A, B, S);
     int nv = 0, nt = 0;
     const int L = 1 << (3 * B.kb);
@@ -635,6 +953,7 @@ __global__ void __launch_bounds__(kWave) k_brick_vertices(SubArgs A, const int32
     const int32_t ai = blockIdx.x;
     const int32_t b = alist[ai];
     brick_setup(A, b, B, S, true);
+    brick_blocks(A, B, S);
     brick_values(A, B, S);
     const int D = A.D;
     const int L = 1 << (3 * B.kb);
@@ -850,13 +1169,19 @@ void extract_subdivision(prg_ctx* c) {
     k_root_of<<<grid_for(nr, kBlockS), kBlockS, 0, s>>>(roots.get(), C + kCntRoots, root_of.get());
     PRG_CHECK_LAUNCH();
 
-    PoolBuf<float> btab{&c->pool[91]};
+    PoolBuf<float> btab{&c->pool[91]}, bounds{&c->pool[127]};
+    float* XB = bounds.reserve(2 * (D + 1));
+    k_depth_absmax<<<D + 1, 1024, 0, s>>>(c->x.get(), Lv, D, M, XB);
+    PRG_CHECK_LAUNCH();
+    k_table_absmax<<<D + 1, 1024, 0, s>>>(reinterpret_cast<const float4*>(btab.get()), D, XB + D + 1);
+    PRG_CHECK_LAUNCH();
     const SubArgs A{c->key.get(),      c->neigh.get(),  M,
                     c->child_base.get(), c->parent.get(), c->pnt.get(),
                     Lv,                c->x.get(),      c->d_base_fn.get(),
                     reinterpret_cast<const float4*>(btab.get()), c->coarse_rows.get(), D,
                     c->iso_dev.get(),  roots.get(),     boff.get(),
-                    root_of.get(),     C};
+                    root_of.get(),     C,               XB,
+                    XB + D + 1};
 
     // pass 1: counts per brick
     PoolBuf<int32_t> bnv{&c->pool[116]}, bnt{&c->pool[117]}, bvoff{&c->pool[118]}, btoff{&c->pool[119]},

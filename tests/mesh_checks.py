@@ -303,3 +303,61 @@ def check_mc_mesh(V, T, D, fval, iso, sample=None, rng=None, value_tol=1.0, pos_
         else:
             raise AssertionError(f"triangle {t} ({T[t].tolist()}) does not match the cube of its cell")
     return checked, skipped
+
+
+def check_mc_structure(V, T, D, sample=None, rng=None):
+    """Value-free check of a marching-cubes mesh, for depths where the float
+    evaluation under test errs by more than its distance to the isovalue
+    (the reference's depth-d basis values carry errors of about 4^d 2^-22):
+    every vertex lies on a depth-D grid edge, and every triangle's three
+    vertices lie on edges of one depth-D cell, forming a triangle the
+    triangle table holds for some cube index.  Returns the triangles checked."""
+    tab = mc_table()
+    tris = set()
+    for cube in range(256):
+        row = tab[cube]
+        n = int(np.argmax(row < 0)) if (row < 0).any() else 16
+        tris.update(tuple(row[i:i + 3]) for i in range(0, n, 3))
+    V = np.asarray(V, np.float64)
+    T = np.asarray(T)
+    if sample is not None and len(T) > sample:
+        T = T[(rng or np.random.default_rng(0)).choice(len(T), sample, replace=False)]
+    g = V * float(1 << D)
+    isint = g == np.round(g)
+    assert np.all(isint.sum(axis=1) >= 2), "a vertex is off the depth-D grid lines"
+    for t in T:
+        lo = np.full(3, -(1 << 40))
+        hi = np.full(3, 1 << 40)
+        for k in range(3):
+            fl = np.floor(g[t[k]]).astype(np.int64)
+            lo = np.maximum(lo, np.where(isint[t[k]], fl - 1, fl))
+            hi = np.minimum(hi, fl)
+        assert np.all(lo <= hi), f"triangle {t.tolist()}: vertices share no cell"
+        found = False
+        cells = {tuple(lo), tuple(hi)} | {(a, b, c) for a in (lo[0], hi[0]) for b in (lo[1], hi[1])
+                                          for c in (lo[2], hi[2])}
+        for cell in cells:
+            cand = []  # per vertex: the cell edges it can lie on
+            for k in range(3):
+                rel = g[t[k]] - np.array(cell)
+                if np.any(rel < 0) or np.any(rel > 1):
+                    cand = None
+                    break
+                axes = [a for a in range(3) if not isint[t[k], a]]
+                # a vertex interpolated to within rounding of a corner sits on the
+                # corner: any of the corner's three cell edges
+                axes = axes if axes else [0, 1, 2]
+                e = [j for j, (qa, qb) in enumerate(EDGE_VERTEX) for ax in axes
+                     if (j >> 2) == ax and all(square_xyz(qa)[a] == int(round(rel[a])) for a in range(3) if a != ax)]
+                if not e:
+                    cand = None
+                    break
+                cand.append(e)
+            if cand is None:
+                continue
+            if any((e0, e1, e2) in tris for e0 in cand[0] for e1 in cand[1] for e2 in cand[2]):
+                found = True
+                break
+        assert found, (f"triangle {t.tolist()} is no triangle-table triangle of a depth-{D} cell: "
+                       f"{[g[t[k]].tolist() for k in range(3)]}")
+    return len(T)

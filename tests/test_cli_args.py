@@ -11,11 +11,15 @@ pytestmark = pytest.mark.skipif(not EXE.exists(), reason="poisson_recon not buil
 
 
 @pytest.mark.parametrize("args", [[], ["--in", "x.ply"], ["--out", "m.ply"], ["--bogus"], ["--in"],
-                                  ["--in", "x.ply", "--out", "m.ply", "--weldd"], ["--in", "x.ply", "--out", "m.ply", "--nosubdivide"]])
+                                  ["--in", "x.ply", "--out", "m.ply", "--weldd"], ["--in", "x.ply", "--out", "m.ply", "--nosubdivide"],
+                                  ["--in", "x.ply", "--out", "m.ply", "--edgeCollapse", "0"],
+                                  ["--in", "x.ply", "--out", "m.ply", "--triangleCollapse", "-1"],
+                                  ["--in", "x.ply", "--out", "m.ply", "--edgeCollapse"]])
 def test_usage_errors(args):
     r = subprocess.run([str(EXE), *args], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2
     assert "usage: poisson_recon" in r.stderr and "--no-subdivide" in r.stderr and "--weld" in r.stderr
+    assert "--edgeCollapse" in r.stderr
 
 
 def test_missing_input_file(tmp_path):
