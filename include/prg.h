@@ -218,6 +218,22 @@ prg_status prg_extract_mesh_into(prg_ctx* ctx, int32_t subdivide, float* vertice
  * triangle entry must lie in [0, n_vertices) (PRG_E_ARG otherwise). */
 prg_status prg_weld_mesh(prg_ctx* ctx, prg_mesh* mesh, int64_t* n_merged);
 
+/* prg_weld_mesh without allocation.  No counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The device copy of
+ * the mesh and every temporary of the weld live in one fixed array of the
+ * library per GPU (PRG_WELD_NOALLOC_BYTES, held until the process ends), so
+ * the mesh may hold at most PRG_WELD_NOALLOC_MAX_VERTICES vertices and
+ * PRG_WELD_NOALLOC_MAX_TRIANGLES triangles (PRG_E_ARG beyond; nothing is
+ * changed then).  The call allocates nothing (no hipMalloc, hipFree or
+ * allocating library routine) and returns after its work is done, with the
+ * result of prg_weld_mesh.  Calls that overlap on one GPU (from several
+ * threads or contexts) race on that array: serialise them. */
+#define PRG_WELD_NOALLOC_MAX_VERTICES (1 << 19)
+#define PRG_WELD_NOALLOC_MAX_TRIANGLES (1 << 20)
+#define PRG_WELD_NOALLOC_BYTES 48643328
+prg_status prg_weld_mesh_noalloc(prg_ctx* ctx, prg_mesh* mesh, int64_t* n_merged);
+
 /* Mesh decimation on a host mesh, in place: EdgeCollapse (mode 0) or
  * TriangleCollapse (mode 1) of Geometry.inl:50-254 (declared
  * Geometry.cuh:135-137; the reference's main() never calls them).  Triangles

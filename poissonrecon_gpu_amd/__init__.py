@@ -97,6 +97,7 @@ SIGNATURES = {
     "prg_mesh_release": (None, [C.POINTER(PrgMesh)]),
     "prg_weld_mesh": (C.c_int, [_P, C.POINTER(PrgMesh), C.POINTER(C.c_int64)]),
     "prg_decimate_mesh": (C.c_int, [C.POINTER(PrgMesh), C.c_int32, C.c_float]),
+    "prg_weld_mesh_noalloc": (C.c_int, [_P, C.POINTER(PrgMesh), C.POINTER(C.c_int64)]),
     "prg_extract_mesh_into": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, C.c_int64, C.POINTER(PrgMesh)]),
     "prg_subdivision_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "prg_reconstruct": (C.c_int, [_P, C.c_int32, C.POINTER(PrgMesh), C.POINTER(PrgSolveStats)]),
@@ -289,9 +290,10 @@ class Reconstructor:
         finally:
             lib().prg_mesh_release(C.byref(m))
 
-    def weld_mesh(self, mesh: Mesh) -> tuple[Mesh, int]:
-        """prg_weld_mesh on a copy of `mesh`: bit-identical vertices merged into
-        their first occurrence; returns (welded mesh, vertices removed)."""
+    def weld_mesh(self, mesh: Mesh, noalloc: bool = False) -> tuple[Mesh, int]:
+        """prg_weld_mesh (or, with noalloc, prg_weld_mesh_noalloc) on a copy of
+        `mesh`: bit-identical vertices merged into their first occurrence;
+        returns (welded mesh, vertices removed)."""
         v = np.ascontiguousarray(mesh.vertices, np.float32).copy()
         t = np.ascontiguousarray(mesh.triangles, np.int32).copy()
         m = PrgMesh()
@@ -300,7 +302,7 @@ class Reconstructor:
         m.triangles = t.ctypes.data_as(C.POINTER(C.c_int32))
         m.n_leaf_vertices, m.n_leaf_triangles = int(mesh.n_leaf_vertices), int(mesh.n_leaf_triangles)
         merged = C.c_int64()
-        self._call("prg_weld_mesh", C.byref(m), C.byref(merged))
+        self._call("prg_weld_mesh_noalloc" if noalloc else "prg_weld_mesh", C.byref(m), C.byref(merged))
         return (Mesh(v[: m.n_vertices].copy(), t, mesh.center, mesh.scale, m.n_leaf_vertices, m.n_leaf_triangles),
                 merged.value)
 

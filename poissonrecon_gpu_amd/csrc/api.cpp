@@ -129,6 +129,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         PRG_HIP(hipHostMalloc(&c->cg_host, 4096, hipHostMallocDefault));
         PRG_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->sub_host), 64, hipHostMallocDefault));
         PRG_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, cf.device));
+        prg::weld_noalloc_scratch(cf.device);  // the fixed weld array's address on this device, once
         if (const char* lim = std::getenv("PRG_TILE_LDS_LIMIT")) c->tile_limit = std::max(0, std::atoi(lim));
         c->kt.enabled = cf.timing != 0;
         c->kt.stream = c->stream;
@@ -432,24 +433,37 @@ prg_status prg_subdivision_info(prg_ctx* c, int64_t* n_roots, int64_t* n_virtual
     });
 }
 
-prg_status prg_weld_mesh(prg_ctx* c, prg_mesh* mesh, int64_t* n_merged) {
+namespace {
+static prg_status weld_checked(prg_ctx* c, prg_mesh* mesh, int64_t* n_merged, bool noalloc) {
     return guarded(c, [&]() {
         if (!mesh) return fail(c, PRG_E_ARG, "null mesh");
         const int64_t n = mesh->n_vertices, m = mesh->n_triangles;
         if (n < 0 || m < 0 || (n > 0 && !mesh->vertices) || (m > 0 && !mesh->triangles))
             return fail(c, PRG_E_ARG, "mesh arrays missing");
         if (n >= (int64_t(1) << 31)) return fail(c, PRG_E_ARG, "more than 2^31 - 1 vertices");
+        if (noalloc && (n > PRG_WELD_NOALLOC_MAX_VERTICES || m > PRG_WELD_NOALLOC_MAX_TRIANGLES))
+            return fail(c, PRG_E_ARG, "mesh larger than the fixed weld array (PRG_WELD_NOALLOC_MAX_*)");
         for (int64_t i = 0; i < 3 * m; i++)
             if (mesh->triangles[i] < 0 || mesh->triangles[i] >= n)
                 return fail(c, PRG_E_ARG, "triangle index out of range");
         PRG_HIP(hipSetDevice(c->cfg.device));
         int64_t nv = n, nleaf = mesh->n_leaf_vertices;
-        prg::weld_mesh(c, mesh->vertices, n, mesh->triangles, m, mesh->n_leaf_vertices, &nv, &nleaf);
+        if (noalloc)
+            prg::weld_mesh_noalloc(c, mesh->vertices, n, mesh->triangles, m, mesh->n_leaf_vertices, &nv, &nleaf);
+        else
+            prg::weld_mesh(c, mesh->vertices, n, mesh->triangles, m, mesh->n_leaf_vertices, &nv, &nleaf);
         if (n_merged) *n_merged = n - nv;
         mesh->n_vertices = nv;
         mesh->n_leaf_vertices = nleaf;
         return PRG_OK;
     });
+}
+}  // namespace
+
+prg_status prg_weld_mesh(prg_ctx* c, prg_mesh* mesh, int64_t* n_merged) { return weld_checked(c, mesh, n_merged, false); }
+
+prg_status prg_weld_mesh_noalloc(prg_ctx* c, prg_mesh* mesh, int64_t* n_merged) {
+    return weld_checked(c, mesh, n_merged, true);
 }
 
 void prg_mesh_release(prg_mesh* m) {

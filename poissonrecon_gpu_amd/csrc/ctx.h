@@ -150,6 +150,12 @@ void unpack_points(prg_ctx* c);
 // already checked to lie in [0, n))
 void weld_mesh(prg_ctx* c, float* vertices, int64_t n, int32_t* triangles, int64_t m, int64_t n_leaf,
                int64_t* n_out, int64_t* n_leaf_out);
+// the same without allocation: every device array in the library's fixed
+// per-device weld array (limits in prg.h)
+void weld_mesh_noalloc(prg_ctx* c, float* vertices, int64_t n, int32_t* triangles, int64_t m, int64_t n_leaf,
+                       int64_t* n_out, int64_t* n_leaf_out);
+// the device address of the fixed weld array on `device` (current), looked up once per device
+uint8_t* weld_noalloc_scratch(int device);
 // EdgeCollapse (mode 0) / TriangleCollapse (mode 1) of a host mesh in place (decimate.cpp)
 void decimate(int mode, float edge_ratio, float* vertices, int64_t* n_io, int32_t* triangles, int64_t* m_io);
 // the 27-neighbour planes of every depth (kernels use the parent-neighbour
