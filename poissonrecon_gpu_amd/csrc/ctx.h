@@ -102,7 +102,7 @@ struct prg_ctx {
     float* transform_host = nullptr;  // pinned, 4 floats
     bool transform_pending = false;
     // per-stage temporaries (slots: octree 0-59, divergence 60-75, marching cubes 76-95, warm start 96)
-    prg::DBuf<uint8_t> pool[128];
+    prg::DBuf<uint8_t> pool[144];
     float iso = 0.0f;
     bool iso_ready = false;  // iso computed or set by the caller
     // the isovalue in device memory (read by the marching-cubes kernels);

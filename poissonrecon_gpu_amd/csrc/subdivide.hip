@@ -63,7 +63,8 @@ __constant__ int8_t c_children_vertex_kind[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 
 // slots of the device counter array
 enum { kCntCoarse = 0, kCntCorners = 1, kCntRoots = 2, kCntBricks = 3, kCntVerts = 4, kCntTris = 5, kCntActive = 6,
-       kCntLeaves = 7, kCntErr = 8, kCntSlots = 16 };
+       kCntLeaves = 7, kCntErr = 8, kCntDense = 9, kCntDepBricks = 10, kCntDepPairs = 11, kCntWork = 12,
+       kCntSlots = 16 };
 
 __device__ __forceinline__ int depth_of(int64_t i, const Levels& L, int D) {
     int d = 0;
@@ -252,41 +253,6 @@ __global__ void __launch_bounds__(kBlockS) k_root_of(const int32_t* roots, const
     if (i < *nroots) root_of[roots[i]] = int32_t(i);
 }
 
-// Largest |x| of each depth (one workgroup per depth).
-__global__ void __launch_bounds__(1024) k_depth_absmax(const float* x, Levels L, int D, int64_t M, float* out) {
-    __shared__ float red[1024];
-    const int d = blockIdx.x;
-    const int64_t b = L.base[d], e = d < D ? L.base[d + 1] : M;
-    float m = 0.0f;
-    for (int64_t i = b + threadIdx.x; i < e; i += 1024) m = fmaxf(m, fabsf(x[i]));
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (int(threadIdx.x) < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[d] = red[0];
-}
-
-// Largest |value| of each depth's entries of the corner basis table.
-__global__ void __launch_bounds__(1024) k_table_absmax(const float4* tab, int D, float* out) {
-    __shared__ float red[1024];
-    const int d = blockIdx.x;
-    const int64_t n = int64_t(2) << D;  // (ix, side) entries per depth
-    float m = 0.0f;
-    for (int64_t i = threadIdx.x; i < n; i += 1024) {
-        const float4 t = tab[int64_t(d) * n + i];
-        m = fmaxf(m, fmaxf(fabsf(t.x), fmaxf(fabsf(t.y), fabsf(t.z))));
-    }
-    red[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-        if (int(threadIdx.x) < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[d] = red[0];
-}
-
 struct SubArgs {
     const uint32_t* key;
     const int32_t* neigh;  // 27 planes
@@ -305,9 +271,16 @@ struct SubArgs {
     const int32_t* boff;     // first brick of each root (exclusive scan, nroots + 1 entries)
     const int32_t* root_of;  // per real node: its root number, or -1
     const int32_t* cnt;      // device counters
-    const float* xmax;       // per depth: largest |x|
-    const float* bmax;       // per depth: largest |basis value| in the corner basis table
+    const int32_t* bmap;     // root number of each brick
+    const int32_t* rec;      // per root: its record (kRecWords words, root_record)
 };
+
+// Per-root record, built once per root and loaded by each of its bricks in
+// one coalesced read: the root's 27 neighbours, their child bases and root
+// numbers (group), its ancestor chain's neighbour values (27 per depth
+// 0..r), the root's index, depth and key.
+constexpr int kRecWords = 384;
+constexpr int kRecNB = 0, kRecNBK = 27, kRecNBR = 54, kRecX = 81, kRecRho = 378, kRecR = 379, kRecKey = 380;
 
 // A brick and its root (wave-uniform).
 struct Brick {
@@ -322,30 +295,30 @@ struct Brick {
 
 constexpr int kP = (1 << kBrickLog) + 1;  // grid points per brick side (at most)
 
-// Scratch of the brick certificate (brick_certify): per depth, the
-// coefficients of a 4x4x4 block of nodes and the separable partial sums.
-struct CertLds {
-    float w[3][kP][4], wa[3][kP][4];  // per axis and grid coordinate: basis weights over the block (and |.|)
-    float X4[64], X4a[64];
-    float Z[16][kP], Za[16][kP];
-    float Y[4][kP][kP], Ya[4][kP][kP];
-};
-
 // Blocks of the real nodes below the root that a brick's corner owners see:
 // at most (D - 3 - r) blocks of 4^3 cells and blocks of 5^3, 7^3 and 11^3
 // cells at the three finest depths.
 constexpr int kXBlk = 2304;
 
-// LDS of one brick's wave
-struct BrickLds {
-    union {
-        float G[kBrickPts];  // f - iso at the brick's corners (grid points)
-        CertLds cert;
-    };
+// LDS of one brick's wave: the root's neighbourhood, the brick's basis
+// tables and blocks (BrickCore), and for the evaluation its corner values.
+struct BrickCore {
+    // per depth d, axis a and grid coordinate i: the corner basis table entry
+    // of the owner (own_cell) of grid coordinate org[a] + i, (d, a, i) at
+    // d * 3 kP + a kP + i
+    float4 T[(kMaxDepth + 1) * 3 * kP];
     float X[(kMaxDepth + 1) * 27];     // the root's chain: neighbour values at depths 0..r
     float XB[kXBlk];                   // per depth below the root: its block (xb_off)
     int32_t xb_off[kMaxDepth + 1];     // start of depth dd's block in XB, or -1 (no real node in it)
     int32_t NB[27], NBK[27], NBR[27];  // the root's neighbours, their child bases, their root numbers (group)
+    float tmax[kMaxDepth + 1];         // brick_abs_bound: per depth, the largest |basis value|
+    float xsum[kMaxDepth + 1];         // and the sum of |coefficients|
+    float wlo[(kMaxDepth + 1) * 12];   // brick_interval_sign: per depth, axis and block slot,
+    float whi[(kMaxDepth + 1) * 12];   // the range of the weight over the brick's points
+};
+
+struct BrickLds : BrickCore {
+    float G[kBrickPts];  // f - iso at the brick's corners: (i, j, k) at (k kP + j) kP + i
 };
 
 __device__ __forceinline__ void decode_cell(uint32_t k, int32_t c[3]) {
@@ -354,21 +327,71 @@ __device__ __forceinline__ void decode_cell(uint32_t k, int32_t c[3]) {
     c[2] = int32_t(compact3(k));
 }
 
-// Finds brick b's root (binary search of the brick offsets) and loads the
-// root's neighbours and ancestor chain into LDS; `chain` = 0 skips the chain.
-__device__ void brick_setup(const SubArgs& A, int32_t b, Brick& B, BrickLds& S, bool chain) {
-    const int D = A.D;
-    int32_t lo = 0, hi = A.cnt[kCntRoots] - 1;
+// The record of root q (one wave per root).
+__global__ void __launch_bounds__(kWave) k_root_records(SubArgs A, int32_t* rec) {
+    const int D = A.D, lane = threadIdx.x;
+    const int32_t q = blockIdx.x;
+    const int32_t rho = A.roots[q];
+    const int r = depth_of(rho, A.L, D);
+    int32_t* R = rec + int64_t(q) * kRecWords;
+    if (lane < 27) {
+        const int32_t nb = A.neigh[int64_t(lane) * A.M + rho];
+        R[kRecNB + lane] = nb;
+        R[kRecNBK + lane] = nb < 0 ? -1 : A.child_base[nb];
+        R[kRecNBR + lane] = (nb < 0 || r < kFinerDepth) ? -1 : A.root_of[nb];
+        R[kRecX + 27 * r + lane] = __float_as_int(nb < 0 ? 0.0f : A.x[nb]);
+    }
+    // ancestors' rows (their 27 neighbour values, then the parent), depths r-1 .. 1
+    int32_t a = A.parent[rho];
+    for (int d = r - 1; d >= 1; d--) {
+        const float* Row = A.rows + kRowFloats * (int64_t(a) - A.L.base[1]);
+        if (lane < 27) R[kRecX + 27 * d + lane] = __float_as_int(Row[lane]);
+        a = __builtin_amdgcn_readfirstlane(__float_as_int(Row[27]));
+    }
+    if (lane == 0) {
+        R[kRecX] = __float_as_int(A.x[0]);
+        R[kRecRho] = rho;
+        R[kRecR] = r;
+        R[kRecKey] = int32_t(A.key[rho]);
+    }
+}
+
+// bmap[b] = the root of brick b (binary search of the brick offsets).
+__global__ void __launch_bounds__(kBlockS) k_brick_map(const int32_t* boff, const int32_t* cnt, int64_t nb,
+                                                       int32_t* bmap) {
+    const int64_t b = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
+    if (b >= nb) return;
+    int32_t lo = 0, hi = cnt[kCntRoots] - 1;
     while (lo < hi) {
         const int32_t mid = (lo + hi + 1) >> 1;
-        if (A.boff[mid] <= b) lo = mid;
+        if (boff[mid] <= b) lo = mid;
         else hi = mid - 1;
     }
-    B.q = lo;
-    B.rho = A.roots[lo];
-    B.r = depth_of(B.rho, A.L, D);
+    bmap[b] = lo;
+}
+
+// Loads brick b's root record into LDS and sets the brick's geometry.
+// q_loaded: the root whose record S already holds (the previous brick's, in a
+// kernel that walks consecutive bricks), or -1.
+__device__ __forceinline__ void brick_setup(const SubArgs& A, int32_t b, Brick& B, BrickCore& S, bool chain,
+                                            int32_t q_loaded = -1) {
+    const int D = A.D, lane = threadIdx.x;
+    B.q = __builtin_amdgcn_readfirstlane(A.bmap[b]);
+    const int32_t* R = A.rec + int64_t(B.q) * kRecWords;
+    B.rho = __builtin_amdgcn_readfirstlane(R[kRecRho]);
+    B.r = __builtin_amdgcn_readfirstlane(R[kRecR]);
+    const uint32_t rkey = uint32_t(__builtin_amdgcn_readfirstlane(R[kRecKey]));
+    if (B.q != q_loaded) {
+        for (int w = lane; w < kRecX; w += kWave) {
+            const int32_t v = R[w];
+            if (w < kRecNBK) S.NB[w] = v;
+            else if (w < kRecNBR) S.NBK[w - kRecNBK] = v;
+            else S.NBR[w - kRecNBR] = v;
+        }
+        if (chain)
+            for (int w = lane; w < 27 * (B.r + 1); w += kWave) S.X[w] = __int_as_float(R[kRecX + w]);
+    }
     B.kb = brick_log(D, B.r);
-    const uint32_t rkey = A.key[B.rho];
     int32_t rc[3];
     decode_cell(rkey, rc);
     B.Sr = 1 << (D - B.r);
@@ -376,26 +399,8 @@ __device__ void brick_setup(const SubArgs& A, int32_t b, Brick& B, BrickLds& S, 
         B.R[a] = rc[a] >> (D - B.r);
         B.R0[a] = B.R[a] << (D - B.r);
     }
-    B.bkey = rkey | (uint32_t(b - A.boff[lo]) << (3 * B.kb));
+    B.bkey = rkey | (uint32_t(b - A.boff[B.q]) << (3 * B.kb));
     decode_cell(B.bkey, B.org);
-    const int lane = threadIdx.x;
-    if (lane < 27) {
-        const int32_t nb = A.neigh[int64_t(lane) * A.M + B.rho];
-        S.NB[lane] = nb;
-        S.NBK[lane] = nb < 0 ? -1 : A.child_base[nb];
-        S.NBR[lane] = (nb < 0 || B.r < kFinerDepth) ? -1 : A.root_of[nb];
-        if (chain) S.X[27 * B.r + lane] = nb < 0 ? 0.0f : A.x[nb];
-    }
-    if (chain) {
-        // ancestors' rows (their 27 neighbour values, then the parent), depths r-1 .. 1
-        int32_t a = A.parent[B.rho];
-        for (int d = B.r - 1; d >= 1; d--) {
-            const float* R = A.rows + kRowFloats * (int64_t(a) - A.L.base[1]);
-            if (lane < 27) S.X[27 * d + lane] = R[lane];
-            a = __builtin_amdgcn_readfirstlane(__float_as_int(R[27]));
-        }
-        if (lane == 0) S.X[0] = A.x[0];
-    }
     __syncthreads();
     bool deep = false;
     for (int j = 0; j < 27; j++)
@@ -406,7 +411,7 @@ __device__ void brick_setup(const SubArgs& A, int32_t b, Brick& B, BrickLds& S, 
 // Whether the depth-D cell c (inside the root's 3x3x3 neighbourhood) is a
 // virtual leaf of the brick's group; *q receives its root's neighbour slot
 // (13 for the brick's own root).
-__device__ __forceinline__ bool cell_virtual(const Brick& B, const BrickLds& S, int D, const int32_t c[3], int* slot) {
+__device__ __forceinline__ bool cell_virtual(const Brick& B, const BrickCore& S, int D, const int32_t c[3], int* slot) {
     const int W = 1 << D;
     if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= W || c[1] >= W || c[2] >= W) return false;
     const int sh = D - B.r;
@@ -424,67 +429,126 @@ __device__ __forceinline__ int block_width(int D, int kb, int dd) {
     return (e > 0 ? (1 << e) : 1) + 3;
 }
 
-// Coefficient of the depth-dd cell c for corners of the brick's root: the
-// real node reached from the root's neighbour by c's key bits, or 0 (inside
-// the root: virtual; absent; below a coarse leaf).
-__device__ __forceinline__ float deep_coef(const SubArgs& A, const Brick& B, const BrickLds& S, int dd,
-                                           const int32_t c[3]) {
-    const int32_t W = 1 << dd;
-    if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= W || c[1] >= W || c[2] >= W) return 0.0f;
-    const int up = dd - B.r;
-    const int ox = (c[0] >> up) - B.R[0], oy = (c[1] >> up) - B.R[1], oz = (c[2] >> up) - B.R[2];
-    if (!(ox | oy | oz)) return 0.0f;
-    const int jn = nidx(ox, oy, oz);
-    int32_t cur = S.NBK[jn] < 0 ? -1 : S.NB[jn];
-    for (int l = B.r + 1; l <= dd && cur >= 0; l++) {
-        const int32_t cb = A.child_base[cur];
-        const int s2 = dd - l;
-        cur = cb < 0 ? -1 : cb + ((((c[0] >> s2) & 1) << 2) | (((c[1] >> s2) & 1) << 1) | ((c[2] >> s2) & 1));
+// Along one axis of grid coordinate g, the owner cell for a root whose owners
+// are all its own leaves: the smaller of the two cells around g that lies in
+// the root (Morton keys grow with each coordinate).
+__device__ __forceinline__ int32_t own_cell(int32_t g, int32_t R0) { return g == R0 ? g : g - 1; }
+
+// Loads the brick's basis table entries (S.T) for owners inside its root.
+// The smallest-key leaf around a grid point, when it is a leaf of the
+// brick's own root, is own_cell along every axis, so every corner the
+// brick's root owns reads its three basis values per depth from S.T.
+__device__ __forceinline__ void brick_tables(const SubArgs& A, const Brick& B, BrickCore& S) {
+    const int D = A.D, P = (1 << B.kb) + 1;
+    for (int e = threadIdx.x; e < (D + 1) * kP; e += kWave) {
+        const int d = e / kP, i = e % kP;
+        if (i >= P) continue;
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++) {  // constant indices keep B in registers
+            const int32_t g = B.org[ax] + i, o = own_cell(g, B.R0[ax]);
+            S.T[3 * kP * d + kP * ax + i] = A.btab[((int64_t(d) << D) + o) * 2 + (g - o)];
+        }
     }
-    return cur >= 0 ? A.x[cur] : 0.0f;
 }
 
-// Fills the brick's blocks (depths r+1..D): each cell's coefficient, one
-// walk down the tree per cell instead of one per corner and neighbour.  A
-// depth whose block lies inside the root, or holds no real node, gets none.
-__device__ void brick_blocks(const SubArgs& A, const Brick& B, BrickLds& S) {
+// Fills the brick's blocks (depths r+1..D): each cell's coefficient.  A
+// depth's block is needed when it reaches outside the root into a
+// neighbour with children; the parents of a block's cells lie in the block
+// one depth up, so the depths that need one are r+1 .. some dmax, and the
+// blocks are filled level by level: the cells of depth r+1 from the
+// neighbours' child bases, each deeper cell from its parent's node (one
+// child-base load per cell).  A block holds its cells' node indices until
+// the next depth has read them, then their coefficients.  A depth whose
+// block holds no real node gets none.
+__device__ __forceinline__ void brick_blocks(const SubArgs& A, const Brick& B, BrickCore& S) {
     const int D = A.D, lane = threadIdx.x;
-    int off = 0;
-    for (int dd = B.r + 1; dd <= D; dd++) {
+    int off[kMaxDepth + 1];  // uniform: constant indices below keep it in scalar registers
+    int total = 0, dmax = B.r;
+#pragma unroll
+    for (int dd = 0; dd <= kMaxDepth; dd++) {
+        off[dd] = -1;
+        if (dd > D || dd <= B.r || !B.deep || dmax != dd - 1) continue;
         const int W = block_width(D, B.kb, dd), sh = D - dd, up = dd - B.r;
-        int32_t lo[3];
-        bool inside = true;
+        // the root slots the block touches: filled only if one of them has children
+        int slo[3], shi[3];
         for (int a = 0; a < 3; a++) {
-            lo[a] = (B.org[a] >> sh) - 2;
-            const int32_t rlo = B.R[a] << up, rhi = rlo + (1 << up) - 1;
-            inside = inside && lo[a] >= rlo && lo[a] + W - 1 <= rhi;
+            const int32_t lo = (B.org[a] >> sh) - 2;
+            slo[a] = (lo >> up) - B.R[a];
+            shi[a] = ((lo + W - 1) >> up) - B.R[a];
         }
-        int slot = -1;
-        if (!inside && B.deep) {
-            const int n = W * W * W;
-            bool any = false;
+        bool kids = false;
+        for (int x = slo[0]; x <= shi[0]; x++)
+            for (int y = slo[1]; y <= shi[1]; y++)
+                for (int z = slo[2]; z <= shi[2]; z++)
+                    if ((x | y | z) && S.NBK[nidx(x, y, z)] >= 0) kids = true;
+        if (!kids) continue;
+        off[dd] = total;
+        total += W * W * W;
+        dmax = dd;
+    }
+    int32_t* NI = reinterpret_cast<int32_t*>(S.XB);
+    uint32_t nz = 0;  // depths where this lane found a real node
+#pragma unroll
+    for (int dd = 1; dd <= kMaxDepth + 1; dd++) {
+        if (dd <= kMaxDepth && dd > B.r && dd <= dmax) {
+            // node indices of depth dd
+            const int W = block_width(D, B.kb, dd), sh = D - dd, n = W * W * W;
+            const uint32_t rw = (65536u + uint32_t(W) - 1u) / uint32_t(W);
+            const int32_t lo[3] = {(B.org[0] >> sh) - 2, (B.org[1] >> sh) - 2, (B.org[2] >> sh) - 2};
+            const int32_t width = 1 << dd;
+            const int Wp = dd > B.r + 1 ? block_width(D, B.kb, dd - 1) : 0;
+            const int32_t plo[3] = {(B.org[0] >> (sh + 1)) - 2, (B.org[1] >> (sh + 1)) - 2, (B.org[2] >> (sh + 1)) - 2};
             for (int e = lane; e < n; e += kWave) {
-                const int32_t c[3] = {lo[0] + e / (W * W), lo[1] + (e / W) % W, lo[2] + e % W};
-                const float xv = deep_coef(A, B, S, dd, c);
-                S.XB[off + e] = xv;
-                any = any || xv != 0.0f;
-            }
-            if (__builtin_amdgcn_ballot_w64(any)) {
-                slot = off;
-                off += n;
+                // l / W and l / W^2 for l < W^3 <= 1331 by a 16-bit reciprocal (exact: l / W
+                // has a fractional part of at most (W-1)/W, the reciprocal's excess adds < 0.03)
+                const int q1 = int((uint32_t(e) * rw) >> 16), q2 = int((uint32_t(q1) * rw) >> 16);
+                const int32_t c[3] = {lo[0] + q2, lo[1] + (q1 - q2 * W), lo[2] + (e - q1 * W)};
+                const int oct = ((c[0] & 1) << 2) | ((c[1] & 1) << 1) | (c[2] & 1);
+                int32_t node = -1;
+                if (c[0] >= 0 && c[1] >= 0 && c[2] >= 0 && c[0] < width && c[1] < width && c[2] < width) {
+                    if (dd == B.r + 1) {
+                        const int ox = (c[0] >> 1) - B.R[0], oy = (c[1] >> 1) - B.R[1], oz = (c[2] >> 1) - B.R[2];
+                        const int32_t kids = (ox | oy | oz) ? S.NBK[nidx(ox, oy, oz)] : -1;
+                        node = kids < 0 ? -1 : kids + oct;
+                    } else {
+                        const int32_t pe =
+                            (((c[0] >> 1) - plo[0]) * Wp + ((c[1] >> 1) - plo[1])) * Wp + ((c[2] >> 1) - plo[2]);
+                        const int32_t par = NI[off[dd - 1] + pe];
+                        const int32_t cb = par < 0 ? -1 : A.child_base[par];
+                        node = cb < 0 ? -1 : cb + oct;
+                    }
+                }
+                NI[off[dd] + e] = node;
             }
         }
-        if (lane == 0) S.xb_off[dd] = slot;
+        // coefficients of depth dd - 1, whose node indices depth dd has read
+        const int dc = dd - 1;
+        if (dc > B.r && dc <= dmax) {
+            const int W = block_width(D, B.kb, dc), n = W * W * W;
+            for (int e = lane; e < n; e += kWave) {
+                const int32_t node = NI[off[dc] + e];
+                const float xv = node < 0 ? 0.0f : A.x[node];
+                S.XB[off[dc] + e] = xv;
+                if (xv != 0.0f) nz |= 1u << dc;
+            }
+        }
+        if (dd <= dmax + 1) __syncthreads();  // this depth's node indices are read by the next
+    }
+#pragma unroll
+    for (int dd = 0; dd <= kMaxDepth; dd++) {
+        const bool any = __builtin_amdgcn_ballot_w64((nz >> dd) & 1u) != 0;
+        if (lane == 0) S.xb_off[dd] = any ? off[dd] : -1;
     }
     __syncthreads();
 }
 
-// deep_sum for an owner leaf o of the brick's own root, reading the blocks.
-// Every one of the 27 terms is added: a neighbour that is virtual, absent or
-// outside the domain contributes +-0 (its coefficient or its basis value is
-// 0), which leaves a float sum that started at +0 unchanged, so the result
-// is deep_sum's bit for bit.
-__device__ float deep_sum_lds(const SubArgs& A, const Brick& B, const BrickLds& S, const int32_t o[3], const int s[3]) {
+// deep_sum for an owner leaf o of the brick's own root (the owner of grid
+// point org + gi), reading the blocks and S.T.  Every one of the 27 terms is
+// added: a neighbour that is virtual, absent or outside the domain
+// contributes +-0 (its coefficient or its basis value is 0), which leaves a
+// float sum that started at +0 unchanged, so the result is deep_sum's bit
+// for bit.
+__device__ __forceinline__ float deep_sum_lds(const SubArgs& A, const Brick& B, const BrickCore& S, const int32_t o[3], const int gi[3]) {
     const int D = A.D;
     float f = 0.0f;
     for (int dd = D; dd > B.r; dd--) {
@@ -500,9 +564,9 @@ __device__ float deep_sum_lds(const SubArgs& A, const Brick& B, const BrickLds& 
         const int W = block_width(D, B.kb, dd);
         const int bx0 = ax - 1 - ((B.org[0] >> sh) - 2), by0 = ay - 1 - ((B.org[1] >> sh) - 2),
                   bz0 = az - 1 - ((B.org[2] >> sh) - 2);
-        const float4 tx = A.btab[((int64_t(dd) << D) + o[0]) * 2 + s[0]];
-        const float4 ty = A.btab[((int64_t(dd) << D) + o[1]) * 2 + s[1]];
-        const float4 tz = A.btab[((int64_t(dd) << D) + o[2]) * 2 + s[2]];
+        const float4 tx = S.T[3 * kP * dd + gi[0]];
+        const float4 ty = S.T[3 * kP * dd + kP + gi[1]];
+        const float4 tz = S.T[3 * kP * dd + 2 * kP + gi[2]];
         const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
         const float* Xb = S.XB + slot;
 #pragma unroll
@@ -524,7 +588,7 @@ __device__ float deep_sum_lds(const SubArgs& A, const Brick& B, const BrickLds& 
 // owner's root at depth r; nb(j) its neighbour j and kids(j) that neighbour's
 // child base.
 template <class NbFn, class KidsFn>
-__device__ float deep_sum(const SubArgs& A, int r, const int32_t R[3], const int32_t o[3], const int s[3], NbFn nb,
+__device__ __forceinline__ float deep_sum(const SubArgs& A, int r, const int32_t R[3], const int32_t o[3], const int s[3], NbFn nb,
                           KidsFn kids) {
     const int D = A.D;
     float f = 0.0f;
@@ -542,6 +606,7 @@ __device__ float deep_sum(const SubArgs& A, int r, const int32_t R[3], const int
         const float4 tz = A.btab[((int64_t(dd) << D) + o[2]) * 2 + s[2]];
         const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
         const int32_t width = 1 << dd;
+#pragma unroll
         for (int k = 0; k < 27; k++) {
             const int32_t nx = ax + (k / 9 - 1), ny = ay + ((k / 3) % 3 - 1), nz = az + (k % 3 - 1);
             if (nx < 0 || ny < 0 || nz < 0 || nx >= width || ny >= width || nz >= width) continue;
@@ -566,14 +631,13 @@ __device__ float deep_sum(const SubArgs& A, int r, const int32_t R[3], const int
 }
 
 // The root's chain from LDS (chain_value's order: depth r down to 1, then the
-// root of the tree), continuing from f.
-__device__ __forceinline__ float chain_lds(const SubArgs& A, const BrickLds& S, int r, const int32_t o[3],
-                                           const int s[3], float f) {
-    const int D = A.D;
+// root of the tree), continuing from f, for the owner of grid point org + gi
+// in the brick's own root.
+__device__ __forceinline__ float chain_lds(const BrickCore& S, int r, const int gi[3], float f) {
     for (int d = r; d >= 1; d--) {
-        const float4 tx = A.btab[((int64_t(d) << D) + o[0]) * 2 + s[0]];
-        const float4 ty = A.btab[((int64_t(d) << D) + o[1]) * 2 + s[1]];
-        const float4 tz = A.btab[((int64_t(d) << D) + o[2]) * 2 + s[2]];
+        const float4 tx = S.T[3 * kP * d + gi[0]];
+        const float4 ty = S.T[3 * kP * d + kP + gi[1]];
+        const float4 tz = S.T[3 * kP * d + 2 * kP + gi[2]];
         const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
         const float* X = S.X + 27 * d;
 #pragma unroll
@@ -584,16 +648,37 @@ __device__ __forceinline__ float chain_lds(const SubArgs& A, const BrickLds& S, 
             f = f + tt;
         }
     }
-    const float b0x = A.btab[o[0] * 2 + s[0]].y, b0y = A.btab[o[1] * 2 + s[1]].y, b0z = A.btab[o[2] * 2 + s[2]].y;
+    const float b0x = S.T[gi[0]].y, b0y = S.T[kP + gi[1]].y, b0z = S.T[2 * kP + gi[2]].y;
     float tt = S.X[0] * b0x;
     tt = tt * b0y;
     tt = tt * b0z;
     return f + tt;
 }
 
+// The owner of grid point g (depth-D corner coordinates) of the brick's root
+// cell: the smallest-key virtual leaf of the group around g.  Returns its
+// root's neighbour slot (13: the brick's own root) and its cell in o.
+__device__ __forceinline__ int corner_owner(const Brick& B, const BrickCore& S, int D, const int32_t g[3],
+                                            int32_t o[3]) {
+    uint32_t best = 0xffffffffu;
+    int slot = 13;
+    for (int t = 0; t < 8; t++) {
+        const int32_t c[3] = {g[0] - 1 + (t & 1), g[1] - 1 + ((t >> 1) & 1), g[2] - 1 + (t >> 2)};
+        int sl;
+        if (!cell_virtual(B, S, D, c, &sl)) continue;
+        const uint32_t k = cell_key(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]));
+        if (k < best) {
+            best = k;
+            slot = sl;
+            for (int a = 0; a < 3; a++) o[a] = c[a];
+        }
+    }
+    return slot;
+}
+
 // f - iso at grid point g (depth-D corner coordinates) of the brick's root cell,
 // computed by its owner: the smallest-key virtual leaf around it.
-__device__ float grid_value(const SubArgs& A, const Brick& B, const BrickLds& S, const int32_t g[3]) {
+__device__ __forceinline__ float grid_value(const SubArgs& A, const Brick& B, const BrickCore& S, const int32_t g[3]) {
     const int D = A.D;
     int32_t o[3];
     int side[3];
@@ -607,23 +692,14 @@ __device__ float grid_value(const SubArgs& A, const Brick& B, const BrickLds& S,
             side[a] = 1;
         }
     } else {
-        uint32_t best = 0xffffffffu;
-        for (int t = 0; t < 8; t++) {
-            const int32_t c[3] = {g[0] - 1 + (t & 1), g[1] - 1 + ((t >> 1) & 1), g[2] - 1 + (t >> 2)};
-            int sl;
-            if (!cell_virtual(B, S, D, c, &sl)) continue;
-            const uint32_t k = cell_key(uint32_t(c[0]), uint32_t(c[1]), uint32_t(c[2]));
-            if (k < best) {
-                best = k;
-                slot = sl;
-                for (int a = 0; a < 3; a++) o[a] = c[a];
-            }
-        }
+        slot = corner_owner(B, S, D, g, o);
         for (int a = 0; a < 3; a++) side[a] = g[a] - o[a];
     }
     if (slot == 13) {
-        const float f = B.deep ? deep_sum_lds(A, B, S, o, side) : 0.0f;
-        return chain_lds(A, S, B.r, o, side, f) - *A.iso;
+        // o is own_cell along every axis: S.T holds its basis values
+        const int gi[3] = {g[0] - B.org[0], g[1] - B.org[1], g[2] - B.org[2]};
+        const float f = B.deep ? deep_sum_lds(A, B, S, o, gi) : 0.0f;
+        return chain_lds(S, B.r, gi, f) - *A.iso;
     }
     // owned by a leaf of another root of the group: that root's neighbours and chain
     const int32_t rho2 = S.NB[slot];
@@ -643,10 +719,12 @@ __device__ float grid_value(const SubArgs& A, const Brick& B, const BrickLds& S,
     return chain_value(rho2, B.r, px, py, pz, A.key, A.pnt, A.L, A.parent, A.x, A.bf, D, f) - *A.iso;
 }
 
-__device__ void brick_values(const SubArgs& A, const Brick& B, BrickLds& S) {
+__device__ __forceinline__ void brick_values(const SubArgs& A, const Brick& B, BrickLds& S) {
     const int P = (1 << B.kb) + 1;
-    for (int p = threadIdx.x; p < P * P * P; p += kWave) {
-        const int32_t g[3] = {B.org[0] + p % P, B.org[1] + (p / P) % P, B.org[2] + p / (P * P)};
+    for (int p = threadIdx.x; p < kBrickPts; p += kWave) {
+        const int i = p % kP, j = (p / kP) % kP, k = p / (kP * kP);
+        if (i >= P || j >= P || k >= P) continue;
+        const int32_t g[3] = {B.org[0] + i, B.org[1] + j, B.org[2] + k};
         S.G[p] = grid_value(A, B, S, g);
     }
     __syncthreads();
@@ -655,7 +733,7 @@ __device__ void brick_values(const SubArgs& A, const Brick& B, BrickLds& S) {
 // Owner of edge e of leaf v among the virtual leaves sharing it
 // (initSubdivideEdgeArray, main.cu:1867-2019): its cell and the edge kind seen
 // from it.
-__device__ __forceinline__ void edge_owner(const Brick& B, const BrickLds& S, int D, const int32_t v[3], int e,
+__device__ __forceinline__ void edge_owner(const Brick& B, const BrickCore& S, int D, const int32_t v[3], int e,
                                            int32_t ow[3], int* eo) {
     const int o = e >> 2;
     const int a1 = (o == 0) ? 1 : 0, a2 = (o == 2) ? 1 : 2;
@@ -684,7 +762,6 @@ __device__ __forceinline__ void edge_owner(const Brick& B, const BrickLds& S, in
 // (generateSubdivideVexNums, main.cu:2481-2531; generateTriNums, :2562-2582).
 __device__ __forceinline__ void leaf_cube(const Brick& B, const BrickLds& S, int D, int l, int32_t v[3], float cv[8],
                                           int* cube, int* emask) {
-    const int P = (1 << B.kb) + 1;
     const int lx = int(compact3(uint32_t(l) >> 2)), ly = int(compact3(uint32_t(l) >> 1)), lz = int(compact3(uint32_t(l)));
     v[0] = B.org[0] + lx;
     v[1] = B.org[1] + ly;
@@ -693,7 +770,7 @@ __device__ __forceinline__ void leaf_cube(const Brick& B, const BrickLds& S, int
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const int y = (q >> 1) & 1, xx = (q & 1) ^ y, z = q >> 2;
-        cv[q] = S.G[((lz + z) * P + (ly + y)) * P + (lx + xx)];
+        cv[q] = S.G[((lz + z) * kP + (ly + y)) * kP + (lx + xx)];
         if (cv[q] < 0) cb |= 1 << q;
     }
     *cube = cb;
@@ -736,191 +813,489 @@ __device__ __forceinline__ int wave_excl_scan(int v) {
     return inc - v;
 }
 
-// Owner cell along one axis of grid coordinate g for a root whose owners are
-// all its own leaves: the smaller of the two cells around g that lies in the
-// root (Morton keys grow with each coordinate).
-__device__ __forceinline__ int32_t own_cell(int32_t g, int32_t R0) { return g == R0 ? g : g - 1; }
+// The separable contraction of one depth's coefficients over z and y for
+// the lane's weights: Y[m] = sum_n wy[n] sum_l wz[l] X[m][n][l].
+// contract27: the root's chain at a depth 1..r, whose 27 neighbours all lie
+// around the owners' common ancestor (3 slots per axis, weights (t.x, t.y,
+// t.z)).  contract64: a 4^3 block (cells A - 2 .. A + 1 around the brick's
+// ancestor A) with 4-slot weights.
+__device__ __forceinline__ void contract27(const float* X27, float4 ty, float4 tz, float Y[3]) {
+    const float wy[3] = {ty.x, ty.y, ty.z}, wz[3] = {tz.x, tz.y, tz.z};
+    float xv[27];
+#pragma unroll
+    for (int q = 0; q < 27; q++) xv[q] = X27[q];
+#pragma unroll
+    for (int m = 0; m < 3; m++) {
+        float y = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 3; n++) {
+            float z = 0.0f;
+#pragma unroll
+            for (int l = 0; l < 3; l++) z = __builtin_fmaf(xv[9 * m + 3 * n + l], wz[l], z);
+            y = __builtin_fmaf(wy[n], z, y);
+        }
+        Y[m] = y;
+    }
+}
+
+__device__ __forceinline__ void contract64(const float* X64, const float wy[4], const float wz[4], float Y[4]) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        float xv[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) xv[q] = X64[16 * m + q];
+        float y = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 4; n++) {
+            float z = 0.0f;
+#pragma unroll
+            for (int l = 0; l < 4; l++) z = __builtin_fmaf(xv[4 * n + l], wz[l], z);
+            y = __builtin_fmaf(wy[n], z, y);
+        }
+        Y[m] = y;
+    }
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+__device__ __forceinline__ float wave_fsum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// An upper bound of A(g) = sum over depths and the owner's 27 neighbours of
+// |x_k b_x b_y b_z| for every corner g of the brick owned inside its root:
+// per depth, the sum of |x| over the depth's block (which holds every
+// owner's 27 neighbours) times the cube of the largest |basis value| among
+// the brick's table entries at that depth.  The float sums of non-negative
+// terms are within 0.1% of their exact values.
+__device__ __forceinline__ float brick_abs_bound(const Brick& B, BrickCore& S, int D) {
+    const int lane = threadIdx.x, P = (1 << B.kb) + 1, dlast = D - B.kb;
+    if (lane <= kMaxDepth) S.tmax[lane] = S.xsum[lane] = 0.0f;
+    __syncthreads();
+    // |basis values| (non-negative floats order as their bit patterns)
+    for (int e = lane; e < (D + 1) * 3 * kP; e += kWave) {
+        const int d = e / (3 * kP), i = e % kP;
+        if (i >= P || (d > B.r && S.xb_off[d] < 0)) continue;
+        const float4 t = S.T[e];
+        const float m = fmaxf(fabsf(t.x), fmaxf(fabsf(t.y), fabsf(t.z)));
+        atomicMax(reinterpret_cast<int*>(&S.tmax[d]), __float_as_int(m));
+    }
+    // |coefficients|: the chain, then the blocks
+    for (int e = lane; e < 27 * B.r; e += kWave) atomicAdd(&S.xsum[1 + e / 27], fabsf(S.X[27 + e]));
+    if (lane == 0) atomicAdd(&S.xsum[0], fabsf(S.X[0]));
+    for (int d = B.r + 1; d <= D; d++) {
+        if (S.xb_off[d] < 0) continue;
+        const int W = d <= dlast ? 4 : block_width(D, B.kb, d), n = W * W * W;
+        float sx = 0.0f;
+        for (int e = lane; e < n; e += kWave) sx += fabsf(S.XB[S.xb_off[d] + e]);
+        if (sx != 0.0f) atomicAdd(&S.xsum[d], sx);
+    }
+    __syncthreads();
+    float bound = 0.0f;
+    for (int d = 0; d <= D; d++) {
+        const float tm = S.tmax[d];
+        bound += S.xsum[d] * (tm * tm * tm);
+    }
+    return 1.001f * bound;
+}
+
+// The basis weights over the 4 block slots along axis ax for grid coordinate
+// org[ax] + i at depth d <= D - kb: the owner's three values, placed at its
+// ancestor's offset (-1 or 0) from the brick's ancestor.
+__device__ __forceinline__ void axis_weights4(const Brick& B, const BrickCore& S, int D, int d, int ax, int i,
+                                              float w[4]) {
+    const int32_t g = B.org[ax] + i, o = own_cell(g, B.R0[ax]);
+    const int delta = (o >> (D - d)) - (B.org[ax] >> (D - d));
+    const float4 t = S.T[3 * kP * d + kP * ax + i];
+    if (delta == 0) {
+        w[0] = 0.0f;
+        w[1] = t.x;
+        w[2] = t.y;
+        w[3] = t.z;
+    } else {
+        w[0] = t.x;
+        w[1] = t.y;
+        w[2] = t.z;
+        w[3] = 0.0f;
+    }
+}
+
+// A sign for every corner the brick's root owns from one interval for the
+// whole brick.  With slot weights (zero outside the owner's three per axis),
+// a depth's sum is sum over the block's slots (m, n, l) of
+// X[m][n][l] w_x(i)[m] w_y(j)[n] w_z(k)[l] exactly; each weight lies in its
+// slot's range over the brick's points, so each term lies in X times the
+// product of three ranges.  The finer depths, where the brick spans several
+// cells, are bounded by sum |x| times the cube of their largest basis value
+// (brick_abs_bound's per-depth figures).  The float evaluation differs from
+// the exact sum by at most 351 2^-24 A(g) (the certificate's analysis below)
+// and the interval's own float sums by less than 40 2^-24 of the same bound:
+// the brick is certified when the interval clears iso by 700 2^-24 times
+// the bound of A (plus the rounding of the subtraction).
+__device__ __forceinline__ int brick_interval_sign(const SubArgs& A, const Brick& B, BrickCore& S, float abound) {
+    const int D = A.D, lane = threadIdx.x, P = (1 << B.kb) + 1, dlast = D - B.kb;
+    for (int e = lane; e < (dlast + 1) * 12; e += kWave) {
+        const int d = e / 12, a = (e >> 2) % 3, m = e & 3;
+        if (d > B.r && S.xb_off[d] < 0) continue;
+        float lo = 3.4e38f, hi = -3.4e38f;
+        for (int i = 0; i < P; i++) {
+            float w[4];
+            if (a == 0) axis_weights4(B, S, D, d, 0, i, w);  // constant axes keep B in registers
+            else if (a == 1) axis_weights4(B, S, D, d, 1, i, w);
+            else axis_weights4(B, S, D, d, 2, i, w);
+            const float v = m == 0 ? w[0] : m == 1 ? w[1] : m == 2 ? w[2] : w[3];
+            lo = fminf(lo, v);
+            hi = fmaxf(hi, v);
+        }
+        S.wlo[e] = lo;
+        S.whi[e] = hi;
+    }
+    __syncthreads();
+    const int m = lane >> 4, n = (lane >> 2) & 3, l = lane & 3;
+    float lo = 0.0f, hi = 0.0f;
+    for (int d = 0; d <= dlast; d++) {
+        float x;
+        if (d == 0) {
+            x = (m == 2 && n == 2 && l == 2) ? S.X[0] : 0.0f;
+        } else if (d <= B.r) {
+            x = (m && n && l) ? S.X[27 * d + 9 * (m - 1) + 3 * (n - 1) + (l - 1)] : 0.0f;
+        } else {
+            if (S.xb_off[d] < 0) continue;
+            x = S.XB[S.xb_off[d] + lane];
+        }
+        if (x == 0.0f) continue;
+        const float a1 = S.wlo[12 * d + m], a2 = S.whi[12 * d + m];
+        const float b1 = S.wlo[12 * d + 4 + n], b2 = S.whi[12 * d + 4 + n];
+        const float c1 = S.wlo[12 * d + 8 + l], c2 = S.whi[12 * d + 8 + l];
+        const float u1 = fminf(fminf(a1 * b1, a1 * b2), fminf(a2 * b1, a2 * b2));
+        const float u2 = fmaxf(fmaxf(a1 * b1, a1 * b2), fmaxf(a2 * b1, a2 * b2));
+        const float p1 = fminf(fminf(u1 * c1, u1 * c2), fminf(u2 * c1, u2 * c2));
+        const float p2 = fmaxf(fmaxf(u1 * c1, u1 * c2), fmaxf(u2 * c1, u2 * c2));
+        lo += x >= 0.0f ? x * p1 : x * p2;
+        hi += x >= 0.0f ? x * p2 : x * p1;
+    }
+    lo = wave_fsum(lo);
+    hi = wave_fsum(hi);
+    float fine = 0.0f;
+    for (int d = dlast + 1; d <= D; d++) {
+        if (S.xb_off[d] < 0) continue;
+        const float t = S.tmax[d];
+        fine += S.xsum[d] * (t * t * t);
+    }
+    lo -= 1.001f * fine;
+    hi += 1.001f * fine;
+    const float iso = *A.iso;
+    const float margin = 700.0f * 5.9604645e-8f * abound + 1.2e-7f * (fabsf(lo) + fabsf(hi) + 2.0f * fabsf(iso)) + 1e-15f;
+    if (lo - iso > margin) return 1;
+    if (hi - iso < -margin) return -1;
+    return 0;
+}
+
+// The offsets (bit nidx(delta)) of the bricks holding the owners of the
+// brick's corners that leaves of other roots of the group own.
+__device__ __forceinline__ uint32_t brick_group_deps(const Brick& B, const BrickCore& S, int D) {
+    const int Sb = 1 << B.kb, P = Sb + 1;
+    uint32_t deps = 0;
+    for (int p = threadIdx.x; p < kBrickPts; p += kWave) {
+        const int i = p % kP, j = (p / kP) % kP, k = p / (kP * kP);
+        if (i >= P || j >= P || k >= P) continue;
+        const int32_t g[3] = {B.org[0] + i, B.org[1] + j, B.org[2] + k};
+        bool interior = true;
+        for (int a = 0; a < 3; a++) interior = interior && g[a] > B.R0[a] && g[a] < B.R0[a] + B.Sr;
+        if (interior) continue;
+        int32_t o[3];
+        if (corner_owner(B, S, D, g, o) == 13) continue;
+        int dv[3];
+        for (int a = 0; a < 3; a++) dv[a] = o[a] < B.org[a] ? -1 : (o[a] >= B.org[a] + Sb ? 1 : 0);
+        deps |= 1u << nidx(dv[0], dv[1], dv[2]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) deps |= uint32_t(__shfl_xor(int(deps), o, kWave));
+    return deps;
+}
 
 // Proves, when it can, that every corner value of the brick has one strict
 // sign, so the brick has no crossing edge and no triangle.  For each depth
-// whose nodes are fixed across the brick (the chain depths 0..r and the
-// depths r+1..D-kb, where the whole brick lies in one cell) the sum
-// S(g) = sum_k x_k b_x b_y b_z over the corner owner's 27 neighbours is
-// evaluated exactly as a separable contraction of a 4x4x4 block of
-// coefficients with per-axis weights taken from the same float basis table
-// the evaluation reads (the fourth layer holds the neighbours of owners in the
-// lower brick), together with A(g) = sum |x_k b_x b_y b_z|.  The float value
-// the evaluation computes, a sum of n <= 27 (D+1) + 1 products of three
-// roundings each, differs from S(g) by at most (n+3) 2^-24 A(g) (recursive
-// summation bound) plus the rounding of this contraction; each finer depth
-// contributes at most U(g): the sum of |x| over the 27 neighbours the owner's
-// ancestor sees there (from the brick's blocks) times the depth's largest
-// basis value cubed.  The brick is certified when
-// |S(g) - iso| exceeds 420 2^-24 (A + U) + U with one sign at every corner.
-// Bricks whose corners may be owned by another root of the group are not.
-__device__ bool brick_certify(const SubArgs& A, const Brick& B, BrickLds& S) {
+// the sum S(g) = sum_k x_k b_x b_y b_z over the corner owner's 27 neighbours
+// is evaluated from the same float coefficients and basis table entries the
+// evaluation reads, as a separable contraction: each lane owns the column of
+// grid points (*, j, k), contracts the depth's coefficients over z and y
+// with its own weights, then over x for the nine points of its column.  At
+// the chain depths 1..r every owner has the root's ancestor, whose 27
+// neighbours are the chain; at depths r+1..D-kb (the whole brick in one cell)
+// the owners' ancestors are that cell or the one below it, which a 4x4x4
+// block covers; at the finer depths, where the brick spans several cells, the
+// owner's 27 terms are read from the brick's block point by point.  The float
+// value the evaluation computes, a sum of n <= 27 (D+1) + 1 products of
+// three roundings each, differs from S(g) by at most (n+3) 2^-24 A(g) with
+// A(g) = sum_k |x_k b_x b_y b_z| (recursive summation bound); the
+// contraction's own rounding adds less than 60 2^-24 A(g).  A(g) is bounded
+// for the whole brick by brick_abs_bound.  The brick is certified when
+// |S(g) - iso| exceeds 560 2^-24 times that bound (plus the rounding of the
+// subtraction) with one sign at every corner its root owns.  A corner on the
+// root's boundary may be owned by a leaf of another root of the group: its
+// value is the one that root's brick holding the owner computes for its own
+// corner (the same owner, table entries and order), so the brick records
+// that brick's offset instead (deps) and the sign is compared with that
+// brick's certificate afterwards (k_brick_deps).
+struct Certificate {
+    int sign;       // +1 / -1: every corner the brick's root owns has this sign; 0: not proven
+    uint32_t deps;  // bit nidx(delta): a corner owned by a leaf of another root, in the brick at offset delta
+};
+
+__device__ __forceinline__ Certificate brick_certify(const SubArgs& A, const Brick& B, BrickCore& S) {
     const int D = A.D, kb = B.kb, Sb = 1 << kb, P = Sb + 1;
     const int lane = threadIdx.x;
-    CertLds& L = S.cert;
     bool lo[3], hi[3];
     for (int a = 0; a < 3; a++) {
         lo[a] = B.org[a] == B.R0[a];
         hi[a] = B.org[a] + Sb == B.R0[a] + B.Sr;
     }
+    bool group = false;
     for (int j = 0; j < 27; j++) {
         if (j == 13 || S.NBR[j] < 0) continue;
         const int dv[3] = {j / 9 - 1, (j / 3) % 3 - 1, j % 3 - 1};
         bool touches = true;
         for (int a = 0; a < 3; a++)
             if (dv[a] != 0) touches = touches && (dv[a] < 0 ? lo[a] : hi[a]);
-        if (touches) return false;
-    }
-    constexpr int kPer = (kBrickPts + kWave - 1) / kWave;
-    float sacc[kPer], aacc[kPer];
-#pragma unroll
-    for (int t = 0; t < kPer; t++) sacc[t] = aacc[t] = 0.0f;
-    const int dlast = D - kb;  // deepest depth whose cell holds the whole brick
-    for (int d = 0; d <= dlast; d++) {
-        // the block's coefficients: cells A - 2 .. A + 1 per axis around the
-        // brick's ancestor A at depth d
-        float xv = 0.0f;
-        {
-            const int mx = lane >> 4, my = (lane >> 2) & 3, mz = lane & 3;
-            if (d <= B.r) {
-                if (mx >= 1 && my >= 1 && mz >= 1) {
-                    if (d == 0) xv = (mx == 2 && my == 2 && mz == 2) ? S.X[0] : 0.0f;
-                    else xv = S.X[27 * d + 9 * (mx - 1) + 3 * (my - 1) + (mz - 1)];
-                }
-            } else if (S.xb_off[d] >= 0) {
-                xv = S.XB[S.xb_off[d] + lane];  // a 4^3 block, cells A - 2 .. A + 1
-            }
-        }
-        if (__builtin_amdgcn_ballot_w64(xv != 0.0f) == 0) continue;  // nothing at this depth
-        L.X4[lane] = xv;
-        L.X4a[lane] = fabsf(xv);
-        // per-axis weights: the owner's three basis values, placed at its
-        // ancestor's offset inside the block
-        if (lane < 3 * P) {
-            const int ax = lane / P, i = lane % P;
-            const int32_t g = B.org[ax] + i;
-            const int32_t o = own_cell(g, B.R0[ax]);
-            const int side = g - o;
-            const int delta = (o >> (D - d)) - (B.org[ax] >> (D - d));  // -1 or 0
-            const float4 t = A.btab[((int64_t(d) << D) + o) * 2 + side];
-            const float tv[3] = {t.x, t.y, t.z};
-#pragma unroll
-            for (int m = 0; m < 4; m++) {
-                const int a = m - delta - 1;
-                const float v = (a >= 0 && a <= 2) ? tv[a] : 0.0f;
-                L.w[ax][i][m] = v;
-                L.wa[ax][i][m] = fabsf(v);
-            }
-        }
-        __syncthreads();
-        for (int e = lane; e < 16 * P; e += kWave) {
-            const int mn = e / P, k = e % P;
-            float z = 0.0f, za = 0.0f;
-#pragma unroll
-            for (int l = 0; l < 4; l++) {
-                z = __builtin_fmaf(L.X4[4 * mn + l], L.w[2][k][l], z);
-                za = __builtin_fmaf(L.X4a[4 * mn + l], L.wa[2][k][l], za);
-            }
-            L.Z[mn][k] = z;
-            L.Za[mn][k] = za;
-        }
-        __syncthreads();
-        for (int e = lane; e < 4 * P * P; e += kWave) {
-            const int m = e / (P * P), j = (e / P) % P, k = e % P;
-            float y = 0.0f, ya = 0.0f;
-#pragma unroll
-            for (int n = 0; n < 4; n++) {
-                y = __builtin_fmaf(L.w[1][j][n], L.Z[4 * m + n][k], y);
-                ya = __builtin_fmaf(L.wa[1][j][n], L.Za[4 * m + n][k], ya);
-            }
-            L.Y[m][j][k] = y;
-            L.Ya[m][j][k] = ya;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int t = 0; t < kPer; t++) {
-            const int p = lane + kWave * t;
-            if (p < P * P * P) {
-                const int i = p % P, j = (p / P) % P, k = p / (P * P);
-                float v = 0.0f, va = 0.0f;
-#pragma unroll
-                for (int m = 0; m < 4; m++) {
-                    v = __builtin_fmaf(L.w[0][i][m], L.Y[m][j][k], v);
-                    va = __builtin_fmaf(L.wa[0][i][m], L.Ya[m][j][k], va);
-                }
-                sacc[t] += v;
-                aacc[t] += va;
-            }
-        }
-        __syncthreads();
+        group = group || touches;
     }
     const float iso = *A.iso;
+    const int dlast = D - kb;  // deepest depth whose cell holds the whole brick
+    const float abound = brick_abs_bound(B, S, D);
+    {
+        const int sgn = brick_interval_sign(A, B, S, abound);
+        if (sgn != 0) {
+            Certificate c;
+            c.sign = sgn;
+            c.deps = group ? brick_group_deps(B, S, D) : 0;
+            return c;
+        }
+    }
+    const float marginA = 560.0f * 5.9604645e-8f * abound + 1e-15f;
     bool ok = true, pos = false, neg = false;
+    uint32_t deps = 0;
+    for (int pp = lane; pp < kP * kP; pp += kWave) {
+        const int j = pp % kP, k = pp / kP;
+        if (j >= P || k >= P) continue;
+        float v[kP];
 #pragma unroll
-    for (int t = 0; t < kPer; t++) {
-        const int p = lane + kWave * t;
-        if (p >= P * P * P) continue;
-        const int32_t g[3] = {B.org[0] + p % P, B.org[1] + (p / P) % P, B.org[2] + p / (P * P)};
-        // the finer depths, where the owner's ancestor touches a root face
-        // whose neighbours have children
-        float U = 0.0f;
-        for (int d = max(dlast + 1, B.r + 1); d <= D; d++) {
+        for (int i = 0; i < kP; i++) v[i] = 0.0f;
+        // (points i >= P of a smaller brick are accumulated from unset table
+        // entries and never checked)
+        {  // depth 0: the tree's root, one coefficient
+            const float y = (S.X[0] * S.T[2 * kP + k].y) * S.T[kP + j].y;
+#pragma unroll
+            for (int i = 0; i < kP; i++) v[i] = __builtin_fmaf(S.T[i].y, y, v[i]);
+        }
+        for (int d = 1; d <= B.r; d++) {  // the chain: all owners share their ancestor
+            float Y[3];
+            contract27(S.X + 27 * d, S.T[3 * kP * d + kP + j], S.T[3 * kP * d + 2 * kP + k], Y);
+#pragma unroll
+            for (int i = 0; i < kP; i++) {
+                const float4 t = S.T[3 * kP * d + i];
+                v[i] = __builtin_fmaf(t.x, Y[0], v[i]);
+                v[i] = __builtin_fmaf(t.y, Y[1], v[i]);
+                v[i] = __builtin_fmaf(t.z, Y[2], v[i]);
+            }
+        }
+        for (int d = B.r + 1; d <= dlast; d++) {  // 4^3 blocks of real nodes
+            if (S.xb_off[d] < 0) continue;
+            float w1[4], w2[4], Y[4];
+            axis_weights4(B, S, D, d, 1, j, w1);
+            axis_weights4(B, S, D, d, 2, k, w2);
+            contract64(S.XB + S.xb_off[d], w1, w2, Y);
+#pragma unroll
+            for (int i = 0; i < kP; i++) {
+                float w0[4];
+                axis_weights4(B, S, D, d, 0, i, w0);
+#pragma unroll
+                for (int m = 0; m < 4; m++) v[i] = __builtin_fmaf(w0[m], Y[m], v[i]);
+            }
+        }
+        // the finer depths, where the brick spans several cells: the owner's
+        // 27 terms directly from the blocks (the same products the evaluation
+        // forms, summed here in FP32 with their absolute values)
+        for (int d = dlast + 1; d <= D; d++) {
             const int slot = S.xb_off[d];
             if (slot < 0) continue;
-            // the 27 coefficients the owner's ancestor sees at this depth
             const int sh = D - d, W = block_width(D, kb, d);
-            int b0[3];
-            for (int a = 0; a < 3; a++) b0[a] = (own_cell(g[a], B.R0[a]) >> sh) - 1 - ((B.org[a] >> sh) - 2);
-            float sx = 0.0f;
-            for (int k = 0; k < 27; k++)
-                sx += fabsf(S.XB[slot + ((b0[0] + k / 9) * W + (b0[1] + (k / 3) % 3)) * W + (b0[2] + k % 3)]);
-            const float bm = A.bmax[d];
-            U += 1.0001f * sx * (bm * bm * bm);
+            const int b1 = (own_cell(B.org[1] + j, B.R0[1]) >> sh) - 1 - ((B.org[1] >> sh) - 2);
+            const int b2 = (own_cell(B.org[2] + k, B.R0[2]) >> sh) - 1 - ((B.org[2] >> sh) - 2);
+            const float4 ty = S.T[3 * kP * d + kP + j], tz = S.T[3 * kP * d + 2 * kP + k];
+            const float by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
+#pragma unroll
+            for (int i = 0; i < kP; i++) {
+                if (i >= P) continue;
+                const int b0 = (own_cell(B.org[0] + i, B.R0[0]) >> sh) - 1 - ((B.org[0] >> sh) - 2);
+                const float4 tx = S.T[3 * kP * d + i];
+                const float bx[3] = {tx.x, tx.y, tx.z};
+                float t = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 27; q++) {
+                    const float xv = S.XB[slot + ((b0 + q / 9) * W + (b1 + (q / 3) % 3)) * W + (b2 + q % 3)];
+                    t = __builtin_fmaf(xv * bx[q / 9], by[(q / 3) % 3] * bz[q % 3], t);
+                }
+                v[i] += t;
+            }
         }
-        const float v = sacc[t] - iso;
-        const float margin = 420.0f * 5.9604645e-8f * (aacc[t] + U) + U + 1.2e-7f * (fabsf(sacc[t]) + fabsf(iso)) + 1e-15f;
-        ok = ok && fabsf(v) > margin;
-        pos = pos || v > 0;
-        neg = neg || v < 0;
+        const bool jk_inner = B.org[1] + j > B.R0[1] && B.org[1] + j < B.R0[1] + B.Sr && B.org[2] + k > B.R0[2] &&
+                              B.org[2] + k < B.R0[2] + B.Sr;
+#pragma unroll
+        for (int i = 0; i < kP; i++) {
+            if (i >= P) continue;
+            const int32_t gx = B.org[0] + i;
+            if (group && !(jk_inner && gx > B.R0[0] && gx < B.R0[0] + B.Sr)) {
+                // on the root's boundary: is the owner a leaf of another root?
+                const int32_t g[3] = {gx, B.org[1] + j, B.org[2] + k};
+                int32_t o[3];
+                if (corner_owner(B, S, D, g, o) != 13) {
+                    int dv[3];
+                    for (int a = 0; a < 3; a++) dv[a] = o[a] < B.org[a] ? -1 : (o[a] >= B.org[a] + Sb ? 1 : 0);
+                    deps |= 1u << nidx(dv[0], dv[1], dv[2]);
+                    continue;
+                }
+            }
+            const float val = v[i] - iso;
+            const float margin = marginA + 1.2e-7f * (fabsf(v[i]) + fabsf(iso));
+            ok = ok && fabsf(val) > margin;
+            pos = pos || val > 0;
+            neg = neg || val < 0;
+        }
     }
     const bool lane_good = ok && !(pos && neg);
     const uint64_t good = __builtin_amdgcn_ballot_w64(lane_good);
     const uint64_t anypos = __builtin_amdgcn_ballot_w64(pos), anyneg = __builtin_amdgcn_ballot_w64(neg);
-    return good == ~uint64_t(0) && !(anypos && anyneg);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) deps |= uint32_t(__shfl_xor(int(deps), o, kWave));
+    Certificate c;
+    c.sign = (good == ~uint64_t(0) && !(anypos && anyneg)) ? (anypos ? 1 : -1) : 0;
+    c.deps = deps;
+    return c;
 }
 
-// Pass 1: vertex and triangle counts of every brick.
-__global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, int32_t* bnv, int32_t* bnt) {
+// Pass 0: the certificate for every brick (a fixed grid takes chunks of
+// consecutive bricks from a counter).
+// A certified brick has no output (its counts are 0) unless a brick it
+// depends on has another sign; the others are listed for the evaluation.
+// sign[b] keeps each brick's certificate; a brick with dependencies appends
+// (b, first, count) to dbricks and the dependencies' brick numbers to dpairs.
+__global__ void __launch_bounds__(kWave) k_brick_screen(SubArgs A, int64_t nb, int32_t* bnv, int32_t* bnt,
+                                                        int8_t* sign, int32_t* list, int32_t* dbricks,
+                                                        int32_t* dpairs, int32_t* cnt) {
+    __shared__ BrickCore S;
+    const int lane = threadIdx.x, D = A.D;
+    constexpr int kChunk = 16;  // consecutive bricks per grab: mostly one root, whose record stays in LDS
+    int32_t q_loaded = -1, b = 0, b_end = 0;
+    for (;;) {
+        if (b == b_end) {
+            int32_t c0 = 0;
+            if (lane == 0) c0 = atomicAdd(cnt + kCntWork, kChunk);
+            c0 = __shfl(c0, 0, kWave);
+            if (c0 >= nb) break;
+            b = c0;
+            b_end = int32_t(std::min<int64_t>(nb, int64_t(c0) + kChunk));
+        }
+        Brick B;
+        __syncthreads();  // the previous brick's LDS reads are done
+        brick_setup(A, b, B, S, true, q_loaded);
+        q_loaded = B.q;
+        brick_tables(A, B, S);
+        brick_blocks(A, B, S);
+        const Certificate c = brick_certify(A, B, S);
+        int32_t first = 0;
+        if (lane == 0) {
+            bnv[b] = 0;
+            bnt[b] = 0;
+            sign[b] = int8_t(c.sign);
+            if (c.sign == 0) {
+                list[atomicAdd(cnt + kCntDense, 1)] = b;
+            } else if (c.deps) {
+                const int n = __popc(c.deps);
+                first = atomicAdd(cnt + kCntDepPairs, n);
+                const int32_t e = atomicAdd(cnt + kCntDepBricks, 1);
+                dbricks[3 * e] = b;
+                dbricks[3 * e + 1] = first;
+                dbricks[3 * e + 2] = n;
+            }
+        }
+        first = __shfl(first, 0, kWave);
+        if (c.sign != 0 && lane < 27 && ((c.deps >> lane) & 1u)) {
+            // the brick at offset delta(lane), in the root of the group that holds it
+            const int Sb = 1 << B.kb, sh = D - B.r;
+            const int dv[3] = {lane / 9 - 1, (lane / 3) % 3 - 1, lane % 3 - 1};
+            int32_t cc[3];
+            int ro[3];
+            for (int a = 0; a < 3; a++) {
+                cc[a] = B.org[a] + dv[a] * Sb;
+                ro[a] = (cc[a] >> sh) - B.R[a];
+            }
+            const int slot = nidx(ro[0], ro[1], ro[2]);
+            const int32_t q2 = slot == 13 ? B.q : S.NBR[slot];
+            const uint32_t rmask = 3 * sh >= 32 ? 0xffffffffu : ((1u << (3 * sh)) - 1u);
+            const int32_t b2 =
+                q2 < 0 ? -1
+                       : A.boff[q2] + int32_t((cell_key(uint32_t(cc[0]), uint32_t(cc[1]), uint32_t(cc[2])) & rmask) >>
+                                              (3 * B.kb));
+            dpairs[first + __popc(c.deps & ((1u << lane) - 1u))] = b2;
+        }
+        b++;
+    }
+}
+
+// The certified bricks with corners owned by other roots: certified only when
+// every brick they depend on carries the same sign; the others are listed
+// for the evaluation.
+__global__ void __launch_bounds__(kBlockS) k_brick_deps(const int32_t* dbricks, const int32_t* dpairs,
+                                                        const int8_t* sign, int32_t* list, int32_t* cnt) {
+    const int32_t e = blockIdx.x * kBlockS + threadIdx.x;
+    if (e >= cnt[kCntDepBricks]) return;
+    const int32_t b = dbricks[3 * e], first = dbricks[3 * e + 1], n = dbricks[3 * e + 2];
+    const int8_t sb = sign[b];
+    bool ok = true;
+    for (int t = 0; t < n; t++) {
+        const int32_t b2 = dpairs[first + t];
+        ok = ok && b2 >= 0 && sign[b2] == sb;
+    }
+    if (!ok) list[atomicAdd(cnt + kCntDense, 1)] = b;
+}
+
+// Pass 1: vertex and triangle counts of the bricks the certificate left
+// (a fixed grid walks the list).
+__global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, const int32_t* list, const int32_t* nlist,
+                                                       int32_t* bnv, int32_t* bnt) {
     __shared__ BrickLds S;
-    Brick B;
-    const int32_t b = blockIdx.x;
-    brick_setup(A, b, B, S, true);
-    brick_blocks(A, B, S);
-    if (brick_certify(A, B, S)) {
-        if (threadIdx.x == 0) bnv[b] = bnt[b] = 0;
-        return;
-    }
-    __syncthreads();
-    brick_values(A, B, S);
-    int nv = 0, nt = 0;
-    const int L = 1 << (3 * B.kb);
-    for (int l = threadIdx.x; l < L; l += kWave) {
-        int32_t v[3];
-        float cv[8];
-        int cube, em;
-        leaf_cube(B, S, A.D, l, v, cv, &cube, &em);
-        nv += __popc(uint32_t(em));
-        nt += tri_count(cube);
-    }
-    nv = wave_sum(nv);
-    nt = wave_sum(nt);
-    if (threadIdx.x == 0) {
-        bnv[b] = nv;
-        bnt[b] = nt;
+    const int32_t n = *nlist;
+    for (int32_t it = blockIdx.x; it < n; it += gridDim.x) {
+        Brick B;
+        const int32_t b = list[it];
+        __syncthreads();  // the previous brick's LDS reads are done
+        brick_setup(A, b, B, S, true);
+        brick_tables(A, B, S);
+        brick_blocks(A, B, S);
+        brick_values(A, B, S);
+        int nv = 0, nt = 0;
+        const int L = 1 << (3 * B.kb);
+        for (int l = threadIdx.x; l < L; l += kWave) {
+            int32_t v[3];
+            float cv[8];
+            int cube, em;
+            leaf_cube(B, S, A.D, l, v, cv, &cube, &em);
+            nv += __popc(uint32_t(em));
+            nt += tri_count(cube);
+        }
+        nv = wave_sum(nv);
+        nt = wave_sum(nt);
+        if (threadIdx.x == 0) {
+            bnv[b] = nv;
+            bnt[b] = nt;
+        }
     }
 }
 
@@ -953,6 +1328,7 @@ __global__ void __launch_bounds__(kWave) k_brick_vertices(SubArgs A, const int32
     const int32_t ai = blockIdx.x;
     const int32_t b = alist[ai];
     brick_setup(A, b, B, S, true);
+    brick_tables(A, B, S);
     brick_blocks(A, B, S);
     brick_values(A, B, S);
     const int D = A.D;
@@ -1087,7 +1463,7 @@ void extract_subdivision(prg_ctx* c) {
     const int rd = ensure_coarse_rows(c, s);
     if (rd < D - 1) throw Error{PRG_E_STATE, "coarse rows missing"};
 
-    // pool slots 97..127
+    // pool slots 97..132
     PoolBuf<uint8_t> inter{&c->pool[97]};
     PoolBuf<int32_t> cnt{&c->pool[98]}, flag{&c->pool[99]}, off{&c->pool[100]}, cl{&c->pool[101]},
         oref{&c->pool[102]}, need{&c->pool[103]}, nid{&c->pool[104]}, clist{&c->pool[105]}, rflag{&c->pool[107]},
@@ -1169,19 +1545,25 @@ void extract_subdivision(prg_ctx* c) {
     k_root_of<<<grid_for(nr, kBlockS), kBlockS, 0, s>>>(roots.get(), C + kCntRoots, root_of.get());
     PRG_CHECK_LAUNCH();
 
-    PoolBuf<float> btab{&c->pool[91]}, bounds{&c->pool[127]};
-    float* XB = bounds.reserve(2 * (D + 1));
-    k_depth_absmax<<<D + 1, 1024, 0, s>>>(c->x.get(), Lv, D, M, XB);
-    PRG_CHECK_LAUNCH();
-    k_table_absmax<<<D + 1, 1024, 0, s>>>(reinterpret_cast<const float4*>(btab.get()), D, XB + D + 1);
-    PRG_CHECK_LAUNCH();
-    const SubArgs A{c->key.get(),      c->neigh.get(),  M,
+    PoolBuf<float> btab{&c->pool[91]};
+    SubArgs A{c->key.get(),      c->neigh.get(),  M,
                     c->child_base.get(), c->parent.get(), c->pnt.get(),
                     Lv,                c->x.get(),      c->d_base_fn.get(),
                     reinterpret_cast<const float4*>(btab.get()), c->coarse_rows.get(), D,
                     c->iso_dev.get(),  roots.get(),     boff.get(),
-                    root_of.get(),     C,               XB,
-                    XB + D + 1};
+                    root_of.get(),     C,               nullptr,
+                    nullptr};
+
+    // per-root records and the brick -> root map
+    PoolBuf<int32_t> rec{&c->pool[127]}, bmap{&c->pool[128]};
+    rec.reserve(size_t(nr) * kRecWords);
+    bmap.reserve(size_t(nb));
+    A.rec = rec.get();
+    A.bmap = bmap.get();
+    k_root_records<<<unsigned(nr), kWave, 0, s>>>(A, rec.get());
+    PRG_CHECK_LAUNCH();
+    k_brick_map<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(boff.get(), C, nb, bmap.get());
+    PRG_CHECK_LAUNCH();
 
     // pass 1: counts per brick
     PoolBuf<int32_t> bnv{&c->pool[116]}, bnt{&c->pool[117]}, bvoff{&c->pool[118]}, btoff{&c->pool[119]},
@@ -1194,9 +1576,32 @@ void extract_subdivision(prg_ctx* c) {
     aoff.reserve(nb);
     alist.reserve(nb);
     amap.reserve(nb);
+    // pass 0: certificates; pass 1: counts of the bricks left
+    PoolBuf<int32_t> dlist{&c->pool[129]}, dbricks{&c->pool[130]}, dpairs{&c->pool[131]};
+    PoolBuf<int8_t> bsign{&c->pool[132]};
+    dlist.reserve(nb);
+    dbricks.reserve(3 * nb);
+    dpairs.reserve(26 * nb);
+    bsign.reserve(nb);
     {
-        TimedScope ts(kt, "subdivide_count", 4.0 * double(c->sub_leaves));
-        k_brick_count<<<unsigned(nb), kWave, 0, s>>>(A, bnv.get(), bnt.get());
+        TimedScope ts(kt, "subdivide_screen", 4.0 * double(c->sub_leaves));
+        // a resident grid: every workgroup walks its share of the bricks
+        int per_cu = 0;
+        PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_screen, kWave, 0));
+        const int64_t grid = std::min<int64_t>(nb, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
+        k_brick_screen<<<unsigned(grid), kWave, 0, s>>>(A, nb, bnv.get(), bnt.get(), bsign.get(), dlist.get(),
+                                                        dbricks.get(), dpairs.get(), C);
+        PRG_CHECK_LAUNCH();
+        k_brick_deps<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(dbricks.get(), dpairs.get(), bsign.get(),
+                                                                dlist.get(), C);
+        PRG_CHECK_LAUNCH();
+    }
+    {
+        TimedScope ts(kt, "subdivide_count", 0.0);
+        int per_cu = 0;
+        PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_count, kWave, 0));
+        const int64_t grid = std::min<int64_t>(nb, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
+        k_brick_count<<<unsigned(grid), kWave, 0, s>>>(A, dlist.get(), C + kCntDense, bnv.get(), bnt.get());
         PRG_CHECK_LAUNCH();
     }
     scan_exclusive2(bnv.get(), bvoff.get(), C + kCntVerts, bnt.get(), btoff.get(), C + kCntTris, nb, s, c->scratch,
