@@ -251,6 +251,13 @@ prg_status prg_decimate_mesh(prg_mesh* mesh, int32_t mode, float edge_ratio);
  * virtual depth-D leaves marching cubes ran over. */
 prg_status prg_subdivision_info(prg_ctx* ctx, int64_t* n_roots, int64_t* n_virtual_leaves);
 
+/* After a prg_extract_mesh* call with subdivide != 0: how the subdivision's
+ * bricks (blocks of up to 8^3 virtual leaves) were settled: all of them, those
+ * the one-interval certificate did not settle, and those whose corner values
+ * were evaluated (the exact certificate did not settle them either).  No
+ * counterpart in the reference, which evaluates every virtual leaf. */
+prg_status prg_subdivision_bricks(prg_ctx* ctx, int64_t* n_bricks, int64_t* n_exact, int64_t* n_evaluated);
+
 /* Whole pipeline: build, divergence, solve, isovalue, mesh. */
 prg_status prg_reconstruct(prg_ctx* ctx, int32_t subdivide, prg_mesh* out, prg_solve_stats* per_depth);
 

@@ -100,6 +100,7 @@ SIGNATURES = {
     "prg_weld_mesh_noalloc": (C.c_int, [_P, C.POINTER(PrgMesh), C.POINTER(C.c_int64)]),
     "prg_extract_mesh_into": (C.c_int, [_P, C.c_int32, _P, C.c_int64, _P, C.c_int64, C.POINTER(PrgMesh)]),
     "prg_subdivision_info": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "prg_subdivision_bricks": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "prg_reconstruct": (C.c_int, [_P, C.c_int32, C.POINTER(PrgMesh), C.POINTER(PrgSolveStats)]),
     "prg_stage_times": (C.c_int, [_P, C.POINTER(PrgTimings)]),
     "prg_buffer_size": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_int64)]),
@@ -311,6 +312,12 @@ class Reconstructor:
         a, b = C.c_int64(), C.c_int64()
         self._call("prg_subdivision_info", C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def subdivision_bricks(self) -> tuple[int, int, int]:
+        """(bricks, left by the interval certificate, evaluated) of the last subdivided mesh."""
+        a, b, e = C.c_int64(), C.c_int64(), C.c_int64()
+        self._call("prg_subdivision_bricks", C.byref(a), C.byref(b), C.byref(e))
+        return a.value, b.value, e.value
 
     def dense_grid(self, depth: int, which: int) -> tuple[np.ndarray, float]:
         """Dense (G, G, G) grid of one depth (0 divergence, 1 solution, 2 node mask), and

@@ -433,6 +433,16 @@ prg_status prg_subdivision_info(prg_ctx* c, int64_t* n_roots, int64_t* n_virtual
     });
 }
 
+prg_status prg_subdivision_bricks(prg_ctx* c, int64_t* n_bricks, int64_t* n_exact, int64_t* n_evaluated) {
+    return guarded(c, [&]() {
+        if (!n_bricks || !n_exact || !n_evaluated) return fail(c, PRG_E_ARG, "null output");
+        *n_bricks = c->sub_bricks;
+        *n_exact = c->sub_exact;
+        *n_evaluated = c->sub_evaluated;
+        return PRG_OK;
+    });
+}
+
 namespace {
 static prg_status weld_checked(prg_ctx* c, prg_mesh* mesh, int64_t* n_merged, bool noalloc) {
     return guarded(c, [&]() {

@@ -118,6 +118,7 @@ struct prg_ctx {
     prg::DBuf<float> sub_v;
     prg::DBuf<int32_t> sub_t;
     int64_t sub_nv = 0, sub_nt = 0, sub_roots = 0, sub_leaves = 0;
+    int64_t sub_bricks = 0, sub_exact = 0, sub_evaluated = 0;  // bricks; left by the interval; evaluated
     bool have_sub = false;
 
     // stage timing: start/end events per stage, resolved lazily (when the

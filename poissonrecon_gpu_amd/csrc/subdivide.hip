@@ -64,7 +64,7 @@ __constant__ int8_t c_children_vertex_kind[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 // slots of the device counter array
 enum { kCntCoarse = 0, kCntCorners = 1, kCntRoots = 2, kCntBricks = 3, kCntVerts = 4, kCntTris = 5, kCntActive = 6,
        kCntLeaves = 7, kCntErr = 8, kCntDense = 9, kCntDepBricks = 10, kCntDepPairs = 11, kCntWork = 12,
-       kCntSlots = 16 };
+       kCntExact = 13, kCntSlots = 16 };
 
 __device__ __forceinline__ int depth_of(int64_t i, const Levels& L, int D) {
     int d = 0;
@@ -291,6 +291,7 @@ struct Brick {
     int32_t org[3];   // first depth-D cell of the brick
     uint32_t bkey;    // key of the brick's first leaf
     bool deep;        // a neighbour of the root has children (real nodes below depth r)
+    uint32_t kids;    // bit j: neighbour j (j != 13) has children
 };
 
 constexpr int kP = (1 << kBrickLog) + 1;  // grid points per brick side (at most)
@@ -302,13 +303,14 @@ constexpr int kXBlk = 2304;
 
 // LDS of one brick's wave: the root's neighbourhood, the brick's basis
 // tables and blocks (BrickCore), and for the evaluation its corner values.
-struct BrickCore {
+template <int kXBCap>
+struct BrickCoreT {
     // per depth d, axis a and grid coordinate i: the corner basis table entry
     // of the owner (own_cell) of grid coordinate org[a] + i, (d, a, i) at
     // d * 3 kP + a kP + i
     float4 T[(kMaxDepth + 1) * 3 * kP];
     float X[(kMaxDepth + 1) * 27];     // the root's chain: neighbour values at depths 0..r
-    float XB[kXBlk];                   // per depth below the root: its block (xb_off)
+    float XB[kXBCap];                  // per depth below the root: its block (xb_off)
     int32_t xb_off[kMaxDepth + 1];     // start of depth dd's block in XB, or -1 (no real node in it)
     int32_t NB[27], NBK[27], NBR[27];  // the root's neighbours, their child bases, their root numbers (group)
     float tmax[kMaxDepth + 1];         // brick_abs_bound: per depth, the largest |basis value|
@@ -317,9 +319,20 @@ struct BrickCore {
     float whi[(kMaxDepth + 1) * 12];   // the range of the weight over the brick's points
 };
 
-struct BrickLds : BrickCore {
+using BrickCore = BrickCoreT<kXBlk>;
+// The certificate screen's LDS: blocks up to kXBlkScreen floats (the depths
+// where the brick lies in one cell, and the first finer one); bricks that
+// need more go to the exact certificate, whose LDS holds every block.
+constexpr int kXBlkScreen = 512;
+using BrickScreenLds = BrickCoreT<kXBlkScreen>;
+
+template <int kXBCap>
+struct BrickValsT : BrickCoreT<kXBCap> {
     float G[kBrickPts];  // f - iso at the brick's corners: (i, j, k) at (k kP + j) kP + i
 };
+using BrickLds = BrickValsT<kXBlk>;
+// The vertex pass reads the corner values the count pass stored: no blocks.
+using BrickMeshLds = BrickValsT<1>;
 
 __device__ __forceinline__ void decode_cell(uint32_t k, int32_t c[3]) {
     c[0] = int32_t(compact3(k >> 2));
@@ -373,7 +386,8 @@ __global__ void __launch_bounds__(kBlockS) k_brick_map(const int32_t* boff, cons
 // Loads brick b's root record into LDS and sets the brick's geometry.
 // q_loaded: the root whose record S already holds (the previous brick's, in a
 // kernel that walks consecutive bricks), or -1.
-__device__ __forceinline__ void brick_setup(const SubArgs& A, int32_t b, Brick& B, BrickCore& S, bool chain,
+template <class L>
+__device__ __forceinline__ void brick_setup(const SubArgs& A, int32_t b, Brick& B, L& S, bool chain,
                                             int32_t q_loaded = -1) {
     const int D = A.D, lane = threadIdx.x;
     B.q = __builtin_amdgcn_readfirstlane(A.bmap[b]);
@@ -402,16 +416,16 @@ __device__ __forceinline__ void brick_setup(const SubArgs& A, int32_t b, Brick& 
     B.bkey = rkey | (uint32_t(b - A.boff[B.q]) << (3 * B.kb));
     decode_cell(B.bkey, B.org);
     __syncthreads();
-    bool deep = false;
-    for (int j = 0; j < 27; j++)
-        if (j != 13 && S.NBK[j] >= 0) deep = true;
-    B.deep = deep;
+    const uint64_t kids = __builtin_amdgcn_ballot_w64(lane < 27 && lane != 13 && S.NBK[lane < 27 ? lane : 0] >= 0);
+    B.kids = uint32_t(kids);
+    B.deep = B.kids != 0;
 }
 
 // Whether the depth-D cell c (inside the root's 3x3x3 neighbourhood) is a
 // virtual leaf of the brick's group; *q receives its root's neighbour slot
 // (13 for the brick's own root).
-__device__ __forceinline__ bool cell_virtual(const Brick& B, const BrickCore& S, int D, const int32_t c[3], int* slot) {
+template <class L>
+__device__ __forceinline__ bool cell_virtual(const Brick& B, const L& S, int D, const int32_t c[3], int* slot) {
     const int W = 1 << D;
     if (c[0] < 0 || c[1] < 0 || c[2] < 0 || c[0] >= W || c[1] >= W || c[2] >= W) return false;
     const int sh = D - B.r;
@@ -438,7 +452,8 @@ __device__ __forceinline__ int32_t own_cell(int32_t g, int32_t R0) { return g ==
 // The smallest-key leaf around a grid point, when it is a leaf of the
 // brick's own root, is own_cell along every axis, so every corner the
 // brick's root owns reads its three basis values per depth from S.T.
-__device__ __forceinline__ void brick_tables(const SubArgs& A, const Brick& B, BrickCore& S) {
+template <class L>
+__device__ __forceinline__ void brick_tables(const SubArgs& A, const Brick& B, L& S) {
     const int D = A.D, P = (1 << B.kb) + 1;
     for (int e = threadIdx.x; e < (D + 1) * kP; e += kWave) {
         const int d = e / kP, i = e % kP;
@@ -460,7 +475,9 @@ __device__ __forceinline__ void brick_tables(const SubArgs& A, const Brick& B, B
 // child-base load per cell).  A block holds its cells' node indices until
 // the next depth has read them, then their coefficients.  A depth whose
 // block holds no real node gets none.
-__device__ __forceinline__ void brick_blocks(const SubArgs& A, const Brick& B, BrickCore& S) {
+template <class L>
+__device__ __forceinline__ bool brick_blocks(const SubArgs& A, const Brick& B, L& S) {
+    constexpr int kCap = int(sizeof(S.XB) / sizeof(float));
     const int D = A.D, lane = threadIdx.x;
     int off[kMaxDepth + 1];  // uniform: constant indices below keep it in scalar registers
     int total = 0, dmax = B.r;
@@ -470,18 +487,18 @@ __device__ __forceinline__ void brick_blocks(const SubArgs& A, const Brick& B, B
         if (dd > D || dd <= B.r || !B.deep || dmax != dd - 1) continue;
         const int W = block_width(D, B.kb, dd), sh = D - dd, up = dd - B.r;
         // the root slots the block touches: filled only if one of them has children
-        int slo[3], shi[3];
+        uint32_t in[3];  // per axis, bit t: offset t - 1 is touched
         for (int a = 0; a < 3; a++) {
             const int32_t lo = (B.org[a] >> sh) - 2;
-            slo[a] = (lo >> up) - B.R[a];
-            shi[a] = ((lo + W - 1) >> up) - B.R[a];
+            const int slo = (lo >> up) - B.R[a], shi = ((lo + W - 1) >> up) - B.R[a];
+            in[a] = (slo <= -1 ? 1u : 0u) | (slo <= 0 && shi >= 0 ? 2u : 0u) | (shi >= 1 ? 4u : 0u);
         }
-        bool kids = false;
-        for (int x = slo[0]; x <= shi[0]; x++)
-            for (int y = slo[1]; y <= shi[1]; y++)
-                for (int z = slo[2]; z <= shi[2]; z++)
-                    if ((x | y | z) && S.NBK[nidx(x, y, z)] >= 0) kids = true;
-        if (!kids) continue;
+        uint32_t touched = 0;
+#pragma unroll
+        for (int j = 0; j < 27; j++)
+            if (((in[0] >> (j / 9)) & (in[1] >> ((j / 3) % 3)) & (in[2] >> (j % 3))) & 1u) touched |= 1u << j;
+        if (!(touched & B.kids)) continue;
+        if (total + W * W * W > kCap) return false;  // (uniform) the caller's LDS cannot hold the blocks
         off[dd] = total;
         total += W * W * W;
         dmax = dd;
@@ -540,6 +557,7 @@ __device__ __forceinline__ void brick_blocks(const SubArgs& A, const Brick& B, B
         if (lane == 0) S.xb_off[dd] = any ? off[dd] : -1;
     }
     __syncthreads();
+    return true;
 }
 
 // deep_sum for an owner leaf o of the brick's own root (the owner of grid
@@ -548,7 +566,8 @@ __device__ __forceinline__ void brick_blocks(const SubArgs& A, const Brick& B, B
 // contributes +-0 (its coefficient or its basis value is 0), which leaves a
 // float sum that started at +0 unchanged, so the result is deep_sum's bit
 // for bit.
-__device__ __forceinline__ float deep_sum_lds(const SubArgs& A, const Brick& B, const BrickCore& S, const int32_t o[3], const int gi[3]) {
+template <class L>
+__device__ __forceinline__ float deep_sum_lds(const SubArgs& A, const Brick& B, const L& S, const int32_t o[3], const int gi[3]) {
     const int D = A.D;
     float f = 0.0f;
     for (int dd = D; dd > B.r; dd--) {
@@ -633,7 +652,8 @@ __device__ __forceinline__ float deep_sum(const SubArgs& A, int r, const int32_t
 // The root's chain from LDS (chain_value's order: depth r down to 1, then the
 // root of the tree), continuing from f, for the owner of grid point org + gi
 // in the brick's own root.
-__device__ __forceinline__ float chain_lds(const BrickCore& S, int r, const int gi[3], float f) {
+template <class L>
+__device__ __forceinline__ float chain_lds(const L& S, int r, const int gi[3], float f) {
     for (int d = r; d >= 1; d--) {
         const float4 tx = S.T[3 * kP * d + gi[0]];
         const float4 ty = S.T[3 * kP * d + kP + gi[1]];
@@ -658,7 +678,8 @@ __device__ __forceinline__ float chain_lds(const BrickCore& S, int r, const int 
 // The owner of grid point g (depth-D corner coordinates) of the brick's root
 // cell: the smallest-key virtual leaf of the group around g.  Returns its
 // root's neighbour slot (13: the brick's own root) and its cell in o.
-__device__ __forceinline__ int corner_owner(const Brick& B, const BrickCore& S, int D, const int32_t g[3],
+template <class L>
+__device__ __forceinline__ int corner_owner(const Brick& B, const L& S, int D, const int32_t g[3],
                                             int32_t o[3]) {
     uint32_t best = 0xffffffffu;
     int slot = 13;
@@ -678,7 +699,8 @@ __device__ __forceinline__ int corner_owner(const Brick& B, const BrickCore& S, 
 
 // f - iso at grid point g (depth-D corner coordinates) of the brick's root cell,
 // computed by its owner: the smallest-key virtual leaf around it.
-__device__ __forceinline__ float grid_value(const SubArgs& A, const Brick& B, const BrickCore& S, const int32_t g[3]) {
+template <class L>
+__device__ __forceinline__ float grid_value(const SubArgs& A, const Brick& B, const L& S, const int32_t g[3]) {
     const int D = A.D;
     int32_t o[3];
     int side[3];
@@ -719,7 +741,8 @@ __device__ __forceinline__ float grid_value(const SubArgs& A, const Brick& B, co
     return chain_value(rho2, B.r, px, py, pz, A.key, A.pnt, A.L, A.parent, A.x, A.bf, D, f) - *A.iso;
 }
 
-__device__ __forceinline__ void brick_values(const SubArgs& A, const Brick& B, BrickLds& S) {
+template <class L>
+__device__ __forceinline__ void brick_values(const SubArgs& A, const Brick& B, L& S) {
     const int P = (1 << B.kb) + 1;
     for (int p = threadIdx.x; p < kBrickPts; p += kWave) {
         const int i = p % kP, j = (p / kP) % kP, k = p / (kP * kP);
@@ -733,7 +756,8 @@ __device__ __forceinline__ void brick_values(const SubArgs& A, const Brick& B, B
 // Owner of edge e of leaf v among the virtual leaves sharing it
 // (initSubdivideEdgeArray, main.cu:1867-2019): its cell and the edge kind seen
 // from it.
-__device__ __forceinline__ void edge_owner(const Brick& B, const BrickCore& S, int D, const int32_t v[3], int e,
+template <class L>
+__device__ __forceinline__ void edge_owner(const Brick& B, const L& S, int D, const int32_t v[3], int e,
                                            int32_t ow[3], int* eo) {
     const int o = e >> 2;
     const int a1 = (o == 0) ? 1 : 0, a2 = (o == 2) ? 1 : 2;
@@ -760,7 +784,8 @@ __device__ __forceinline__ void edge_owner(const Brick& B, const BrickCore& S, i
 
 // Cube index and owned crossing edges of leaf l of the brick
 // (generateSubdivideVexNums, main.cu:2481-2531; generateTriNums, :2562-2582).
-__device__ __forceinline__ void leaf_cube(const Brick& B, const BrickLds& S, int D, int l, int32_t v[3], float cv[8],
+template <class L>
+__device__ __forceinline__ void leaf_cube(const Brick& B, const L& S, int D, int l, int32_t v[3], float cv[8],
                                           int* cube, int* emask) {
     const int lx = int(compact3(uint32_t(l) >> 2)), ly = int(compact3(uint32_t(l) >> 1)), lz = int(compact3(uint32_t(l)));
     v[0] = B.org[0] + lx;
@@ -874,7 +899,8 @@ __device__ __forceinline__ float wave_fsum(float v) {
 // owner's 27 neighbours) times the cube of the largest |basis value| among
 // the brick's table entries at that depth.  The float sums of non-negative
 // terms are within 0.1% of their exact values.
-__device__ __forceinline__ float brick_abs_bound(const Brick& B, BrickCore& S, int D) {
+template <class L>
+__device__ __forceinline__ float brick_abs_bound(const Brick& B, L& S, int D) {
     const int lane = threadIdx.x, P = (1 << B.kb) + 1, dlast = D - B.kb;
     if (lane <= kMaxDepth) S.tmax[lane] = S.xsum[lane] = 0.0f;
     __syncthreads();
@@ -898,9 +924,10 @@ __device__ __forceinline__ float brick_abs_bound(const Brick& B, BrickCore& S, i
     }
     __syncthreads();
     float bound = 0.0f;
-    for (int d = 0; d <= D; d++) {
+#pragma unroll
+    for (int d = 0; d <= kMaxDepth; d++) {
         const float tm = S.tmax[d];
-        bound += S.xsum[d] * (tm * tm * tm);
+        if (d <= D) bound += S.xsum[d] * (tm * tm * tm);
     }
     return 1.001f * bound;
 }
@@ -908,7 +935,8 @@ __device__ __forceinline__ float brick_abs_bound(const Brick& B, BrickCore& S, i
 // The basis weights over the 4 block slots along axis ax for grid coordinate
 // org[ax] + i at depth d <= D - kb: the owner's three values, placed at its
 // ancestor's offset (-1 or 0) from the brick's ancestor.
-__device__ __forceinline__ void axis_weights4(const Brick& B, const BrickCore& S, int D, int d, int ax, int i,
+template <class L>
+__device__ __forceinline__ void axis_weights4(const Brick& B, const L& S, int D, int d, int ax, int i,
                                               float w[4]) {
     const int32_t g = B.org[ax] + i, o = own_cell(g, B.R0[ax]);
     const int delta = (o >> (D - d)) - (B.org[ax] >> (D - d));
@@ -938,20 +966,24 @@ __device__ __forceinline__ void axis_weights4(const Brick& B, const BrickCore& S
 // and the interval's own float sums by less than 40 2^-24 of the same bound:
 // the brick is certified when the interval clears iso by 700 2^-24 times
 // the bound of A (plus the rounding of the subtraction).
-__device__ __forceinline__ int brick_interval_sign(const SubArgs& A, const Brick& B, BrickCore& S, float abound) {
+template <class L>
+__device__ __forceinline__ int brick_interval_sign(const SubArgs& A, const Brick& B, L& S, float abound) {
     const int D = A.D, lane = threadIdx.x, P = (1 << B.kb) + 1, dlast = D - B.kb;
     for (int e = lane; e < (dlast + 1) * 12; e += kWave) {
         const int d = e / 12, a = (e >> 2) % 3, m = e & 3;
         if (d > B.r && S.xb_off[d] < 0) continue;
         float lo = 3.4e38f, hi = -3.4e38f;
-        for (int i = 0; i < P; i++) {
+#pragma unroll
+        for (int i = 0; i < kP; i++) {
             float w[4];
             if (a == 0) axis_weights4(B, S, D, d, 0, i, w);  // constant axes keep B in registers
             else if (a == 1) axis_weights4(B, S, D, d, 1, i, w);
             else axis_weights4(B, S, D, d, 2, i, w);
             const float v = m == 0 ? w[0] : m == 1 ? w[1] : m == 2 ? w[2] : w[3];
-            lo = fminf(lo, v);
-            hi = fmaxf(hi, v);
+            if (i < P) {
+                lo = fminf(lo, v);
+                hi = fmaxf(hi, v);
+            }
         }
         S.wlo[e] = lo;
         S.whi[e] = hi;
@@ -959,26 +991,30 @@ __device__ __forceinline__ int brick_interval_sign(const SubArgs& A, const Brick
     __syncthreads();
     const int m = lane >> 4, n = (lane >> 2) & 3, l = lane & 3;
     float lo = 0.0f, hi = 0.0f;
-    for (int d = 0; d <= dlast; d++) {
-        float x;
+#pragma unroll
+    for (int d = 0; d <= kMaxDepth; d++) {
+        float x = 0.0f;
         if (d == 0) {
             x = (m == 2 && n == 2 && l == 2) ? S.X[0] : 0.0f;
         } else if (d <= B.r) {
             x = (m && n && l) ? S.X[27 * d + 9 * (m - 1) + 3 * (n - 1) + (l - 1)] : 0.0f;
-        } else {
-            if (S.xb_off[d] < 0) continue;
-            x = S.XB[S.xb_off[d] + lane];
+        } else if (d <= dlast) {
+            const int o = S.xb_off[d];
+            x = o < 0 ? 0.0f : S.XB[o + lane];
         }
-        if (x == 0.0f) continue;
-        const float a1 = S.wlo[12 * d + m], a2 = S.whi[12 * d + m];
-        const float b1 = S.wlo[12 * d + 4 + n], b2 = S.whi[12 * d + 4 + n];
-        const float c1 = S.wlo[12 * d + 8 + l], c2 = S.whi[12 * d + 8 + l];
-        const float u1 = fminf(fminf(a1 * b1, a1 * b2), fminf(a2 * b1, a2 * b2));
-        const float u2 = fmaxf(fmaxf(a1 * b1, a1 * b2), fmaxf(a2 * b1, a2 * b2));
-        const float p1 = fminf(fminf(u1 * c1, u1 * c2), fminf(u2 * c1, u2 * c2));
-        const float p2 = fmaxf(fmaxf(u1 * c1, u1 * c2), fmaxf(u2 * c1, u2 * c2));
-        lo += x >= 0.0f ? x * p1 : x * p2;
-        hi += x >= 0.0f ? x * p2 : x * p1;
+        if (d <= dlast) {
+            const float a1 = S.wlo[12 * d + m], a2 = S.whi[12 * d + m];
+            const float b1 = S.wlo[12 * d + 4 + n], b2 = S.whi[12 * d + 4 + n];
+            const float c1 = S.wlo[12 * d + 8 + l], c2 = S.whi[12 * d + 8 + l];
+            const float u1 = fminf(fminf(a1 * b1, a1 * b2), fminf(a2 * b1, a2 * b2));
+            const float u2 = fmaxf(fmaxf(a1 * b1, a1 * b2), fmaxf(a2 * b1, a2 * b2));
+            const float p1 = fminf(fminf(u1 * c1, u1 * c2), fminf(u2 * c1, u2 * c2));
+            const float p2 = fmaxf(fmaxf(u1 * c1, u1 * c2), fmaxf(u2 * c1, u2 * c2));
+            if (x != 0.0f) {
+                lo += x >= 0.0f ? x * p1 : x * p2;
+                hi += x >= 0.0f ? x * p2 : x * p1;
+            }
+        }
     }
     lo = wave_fsum(lo);
     hi = wave_fsum(hi);
@@ -999,7 +1035,8 @@ __device__ __forceinline__ int brick_interval_sign(const SubArgs& A, const Brick
 
 // The offsets (bit nidx(delta)) of the bricks holding the owners of the
 // brick's corners that leaves of other roots of the group own.
-__device__ __forceinline__ uint32_t brick_group_deps(const Brick& B, const BrickCore& S, int D) {
+template <class L>
+__device__ __forceinline__ uint32_t brick_group_deps(const Brick& B, const L& S, int D) {
     const int Sb = 1 << B.kb, P = Sb + 1;
     uint32_t deps = 0;
     for (int p = threadIdx.x; p < kBrickPts; p += kWave) {
@@ -1049,9 +1086,11 @@ struct Certificate {
     uint32_t deps;  // bit nidx(delta): a corner owned by a leaf of another root, in the brick at offset delta
 };
 
-__device__ __forceinline__ Certificate brick_certify(const SubArgs& A, const Brick& B, BrickCore& S) {
-    const int D = A.D, kb = B.kb, Sb = 1 << kb, P = Sb + 1;
-    const int lane = threadIdx.x;
+// Whether the brick touches a face, edge or corner its root shares with
+// another root of the group (whose leaves may own corners of the brick).
+template <class L>
+__device__ __forceinline__ bool brick_in_group(const Brick& B, const L& S) {
+    const int Sb = 1 << B.kb;
     bool lo[3], hi[3];
     for (int a = 0; a < 3; a++) {
         lo[a] = B.org[a] == B.R0[a];
@@ -1066,19 +1105,27 @@ __device__ __forceinline__ Certificate brick_certify(const SubArgs& A, const Bri
             if (dv[a] != 0) touches = touches && (dv[a] < 0 ? lo[a] : hi[a]);
         group = group || touches;
     }
+    return group;
+}
+
+// The interval certificate (brick_interval_sign) with the brick's
+// dependencies on other roots' bricks.
+template <class L>
+__device__ __forceinline__ Certificate brick_certify_interval(const SubArgs& A, const Brick& B, L& S) {
+    Certificate c;
+    c.sign = brick_interval_sign(A, B, S, brick_abs_bound(B, S, A.D));
+    c.deps = (c.sign != 0 && brick_in_group(B, S)) ? brick_group_deps(B, S, A.D) : 0;
+    return c;
+}
+
+template <class L>
+__device__ __forceinline__ Certificate brick_certify(const SubArgs& A, const Brick& B, L& S) {
+    const int D = A.D, kb = B.kb, Sb = 1 << kb, P = Sb + 1;
+    const int lane = threadIdx.x;
+    const bool group = brick_in_group(B, S);
     const float iso = *A.iso;
     const int dlast = D - kb;  // deepest depth whose cell holds the whole brick
-    const float abound = brick_abs_bound(B, S, D);
-    {
-        const int sgn = brick_interval_sign(A, B, S, abound);
-        if (sgn != 0) {
-            Certificate c;
-            c.sign = sgn;
-            c.deps = group ? brick_group_deps(B, S, D) : 0;
-            return c;
-        }
-    }
-    const float marginA = 560.0f * 5.9604645e-8f * abound + 1e-15f;
+    const float marginA = 560.0f * 5.9604645e-8f * brick_abs_bound(B, S, D) + 1e-15f;
     bool ok = true, pos = false, neg = false;
     uint32_t deps = 0;
     for (int pp = lane; pp < kP * kP; pp += kWave) {
@@ -1180,17 +1227,60 @@ __device__ __forceinline__ Certificate brick_certify(const SubArgs& A, const Bri
     return c;
 }
 
-// Pass 0: the certificate for every brick (a fixed grid takes chunks of
-// consecutive bricks from a counter).
-// A certified brick has no output (its counts are 0) unless a brick it
-// depends on has another sign; the others are listed for the evaluation.
-// sign[b] keeps each brick's certificate; a brick with dependencies appends
-// (b, first, count) to dbricks and the dependencies' brick numbers to dpairs.
-__global__ void __launch_bounds__(kWave) k_brick_screen(SubArgs A, int64_t nb, int32_t* bnv, int32_t* bnt,
-                                                        int8_t* sign, int32_t* list, int32_t* dbricks,
-                                                        int32_t* dpairs, int32_t* cnt) {
-    __shared__ BrickCore S;
+// Records a brick's certificate: its sign, and either its dependencies
+// (dbricks: (b, first, count); dpairs: the bricks it depends on) or, when
+// not certified, its place in `list`.
+template <class L>
+__device__ __forceinline__ void record_certificate(const SubArgs& A, const Brick& B, const L& S, int32_t b,
+                                                   const Certificate& c, int8_t* sign, int32_t* list, int list_slot,
+                                                   int32_t* dbricks, int32_t* dpairs, int32_t* cnt) {
     const int lane = threadIdx.x, D = A.D;
+    int32_t first = 0;
+    if (lane == 0) {
+        sign[b] = int8_t(c.sign);
+        if (c.sign == 0) {
+            list[atomicAdd(cnt + list_slot, 1)] = b;
+        } else if (c.deps) {
+            const int n = __popc(c.deps);
+            first = atomicAdd(cnt + kCntDepPairs, n);
+            const int32_t e = atomicAdd(cnt + kCntDepBricks, 1);
+            dbricks[3 * e] = b;
+            dbricks[3 * e + 1] = first;
+            dbricks[3 * e + 2] = n;
+        }
+    }
+    first = __shfl(first, 0, kWave);
+    if (c.sign != 0 && lane < 27 && ((c.deps >> lane) & 1u)) {
+        // the brick at offset delta(lane), in the root of the group that holds it
+        const int Sb = 1 << B.kb, sh = D - B.r;
+        const int dv[3] = {lane / 9 - 1, (lane / 3) % 3 - 1, lane % 3 - 1};
+        int32_t cc[3];
+        int ro[3];
+        for (int a = 0; a < 3; a++) {
+            cc[a] = B.org[a] + dv[a] * Sb;
+            ro[a] = (cc[a] >> sh) - B.R[a];
+        }
+        const int slot = nidx(ro[0], ro[1], ro[2]);
+        const int32_t q2 = slot == 13 ? B.q : S.NBR[slot];
+        const uint32_t rmask = 3 * sh >= 32 ? 0xffffffffu : ((1u << (3 * sh)) - 1u);
+        const int32_t b2 =
+            q2 < 0 ? -1
+                   : A.boff[q2] + int32_t((cell_key(uint32_t(cc[0]), uint32_t(cc[1]), uint32_t(cc[2])) & rmask) >>
+                                          (3 * B.kb));
+        dpairs[first + __popc(c.deps & ((1u << lane) - 1u))] = b2;
+    }
+}
+
+// Pass 0: the interval certificate for every brick (a fixed grid takes
+// chunks of consecutive bricks from a counter).  Every brick's counts start
+// at 0; a certified brick keeps them unless a brick it depends on has
+// another sign (k_brick_deps); the others are listed for the exact
+// certificate (k_brick_exact).
+__global__ void __launch_bounds__(kWave) k_brick_screen(SubArgs A, int64_t nb, int32_t* bnv, int32_t* bnt,
+                                                        int8_t* sign, int32_t* xlist, int32_t* dbricks,
+                                                        int32_t* dpairs, int32_t* cnt) {
+    __shared__ BrickScreenLds S;
+    const int lane = threadIdx.x;
     constexpr int kChunk = 16;  // consecutive bricks per grab: mostly one root, whose record stays in LDS
     int32_t q_loaded = -1, b = 0, b_end = 0;
     for (;;) {
@@ -1207,45 +1297,35 @@ __global__ void __launch_bounds__(kWave) k_brick_screen(SubArgs A, int64_t nb, i
         brick_setup(A, b, B, S, true, q_loaded);
         q_loaded = B.q;
         brick_tables(A, B, S);
-        brick_blocks(A, B, S);
-        const Certificate c = brick_certify(A, B, S);
-        int32_t first = 0;
+        Certificate c;
+        c.sign = 0;
+        c.deps = 0;
+        if (brick_blocks(A, B, S)) c = brick_certify_interval(A, B, S);  // else: to the exact certificate
         if (lane == 0) {
             bnv[b] = 0;
             bnt[b] = 0;
-            sign[b] = int8_t(c.sign);
-            if (c.sign == 0) {
-                list[atomicAdd(cnt + kCntDense, 1)] = b;
-            } else if (c.deps) {
-                const int n = __popc(c.deps);
-                first = atomicAdd(cnt + kCntDepPairs, n);
-                const int32_t e = atomicAdd(cnt + kCntDepBricks, 1);
-                dbricks[3 * e] = b;
-                dbricks[3 * e + 1] = first;
-                dbricks[3 * e + 2] = n;
-            }
         }
-        first = __shfl(first, 0, kWave);
-        if (c.sign != 0 && lane < 27 && ((c.deps >> lane) & 1u)) {
-            // the brick at offset delta(lane), in the root of the group that holds it
-            const int Sb = 1 << B.kb, sh = D - B.r;
-            const int dv[3] = {lane / 9 - 1, (lane / 3) % 3 - 1, lane % 3 - 1};
-            int32_t cc[3];
-            int ro[3];
-            for (int a = 0; a < 3; a++) {
-                cc[a] = B.org[a] + dv[a] * Sb;
-                ro[a] = (cc[a] >> sh) - B.R[a];
-            }
-            const int slot = nidx(ro[0], ro[1], ro[2]);
-            const int32_t q2 = slot == 13 ? B.q : S.NBR[slot];
-            const uint32_t rmask = 3 * sh >= 32 ? 0xffffffffu : ((1u << (3 * sh)) - 1u);
-            const int32_t b2 =
-                q2 < 0 ? -1
-                       : A.boff[q2] + int32_t((cell_key(uint32_t(cc[0]), uint32_t(cc[1]), uint32_t(cc[2])) & rmask) >>
-                                              (3 * B.kb));
-            dpairs[first + __popc(c.deps & ((1u << lane) - 1u))] = b2;
-        }
+        record_certificate(A, B, S, b, c, sign, xlist, kCntExact, dbricks, dpairs, cnt);
         b++;
+    }
+}
+
+// Pass 0b: the exact certificate (brick_certify) for the bricks the interval
+// left (a fixed grid walks the list); the bricks it does not certify are
+// listed for the evaluation.
+__global__ void __launch_bounds__(kWave) k_brick_exact(SubArgs A, const int32_t* xlist, int8_t* sign, int32_t* list,
+                                                       int32_t* dbricks, int32_t* dpairs, int32_t* cnt) {
+    __shared__ BrickCore S;
+    const int32_t n = cnt[kCntExact];
+    for (int32_t it = blockIdx.x; it < n; it += gridDim.x) {
+        const int32_t b = xlist[it];
+        Brick B;
+        __syncthreads();  // the previous brick's LDS reads are done
+        brick_setup(A, b, B, S, true);
+        brick_tables(A, B, S);
+        brick_blocks(A, B, S);
+        const Certificate c = brick_certify(A, B, S);
+        record_certificate(A, B, S, b, c, sign, list, kCntDense, dbricks, dpairs, cnt);
     }
 }
 
@@ -1268,8 +1348,10 @@ __global__ void __launch_bounds__(kBlockS) k_brick_deps(const int32_t* dbricks, 
 
 // Pass 1: vertex and triangle counts of the bricks the certificate left
 // (a fixed grid walks the list).
+// The corner values go to gvals (kBrickPts per listed brick) and the
+// brick's place in the list to didx, for the vertex pass.
 __global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, const int32_t* list, const int32_t* nlist,
-                                                       int32_t* bnv, int32_t* bnt) {
+                                                       int32_t* bnv, int32_t* bnt, float* gvals, int32_t* didx) {
     __shared__ BrickLds S;
     const int32_t n = *nlist;
     for (int32_t it = blockIdx.x; it < n; it += gridDim.x) {
@@ -1280,6 +1362,8 @@ __global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, const int32_t*
         brick_tables(A, B, S);
         brick_blocks(A, B, S);
         brick_values(A, B, S);
+        for (int p = threadIdx.x; p < kBrickPts; p += kWave) gvals[int64_t(it) * kBrickPts + p] = S.G[p];
+        if (threadIdx.x == 0) didx[b] = it;
         int nv = 0, nt = 0;
         const int L = 1 << (3 * B.kb);
         for (int l = threadIdx.x; l < L; l += kWave) {
@@ -1322,15 +1406,17 @@ __global__ void __launch_bounds__(kBlockS) k_brick_compact(const int32_t* flag, 
 // generateSubdivideIntersectionPoint, main.cu:2584-2597, 2629-2697).
 __global__ void __launch_bounds__(kWave) k_brick_vertices(SubArgs A, const int32_t* alist, const int32_t* bvoff,
                                                           uint16_t* lofs_tab, uint16_t* emask_tab, uint8_t* cube_tab,
-                                                          float* out_v) {
-    __shared__ BrickLds S;
+                                                          const float* gvals, const int32_t* didx, float* out_v) {
+    __shared__ BrickMeshLds S;
     Brick B;
     const int32_t ai = blockIdx.x;
     const int32_t b = alist[ai];
-    brick_setup(A, b, B, S, true);
-    brick_tables(A, B, S);
-    brick_blocks(A, B, S);
-    brick_values(A, B, S);
+    brick_setup(A, b, B, S, false);
+    {  // the corner values the count pass computed
+        const float* G = gvals + int64_t(didx[b]) * kBrickPts;
+        for (int p = threadIdx.x; p < kBrickPts; p += kWave) S.G[p] = G[p];
+        __syncthreads();
+    }
     const int D = A.D;
     const int L = 1 << (3 * B.kb);
     const int per = (L + kWave - 1) / kWave;
@@ -1458,12 +1544,13 @@ void extract_subdivision(prg_ctx* c) {
     const Levels Lv = levels_of(c);
     c->sub_nv = c->sub_nt = 0;
     c->sub_roots = c->sub_leaves = 0;
+    c->sub_bricks = c->sub_exact = c->sub_evaluated = 0;
     if (baseD <= 1) return;
     if (8 * baseD >= (int64_t(1) << 31)) throw Error{PRG_E_ARG, "too many nodes above the leaf level for subdivision"};
     const int rd = ensure_coarse_rows(c, s);
     if (rd < D - 1) throw Error{PRG_E_STATE, "coarse rows missing"};
 
-    // pool slots 97..132
+    // pool slots 97..135
     PoolBuf<uint8_t> inter{&c->pool[97]};
     PoolBuf<int32_t> cnt{&c->pool[98]}, flag{&c->pool[99]}, off{&c->pool[100]}, cl{&c->pool[101]},
         oref{&c->pool[102]}, need{&c->pool[103]}, nid{&c->pool[104]}, clist{&c->pool[105]}, rflag{&c->pool[107]},
@@ -1577,32 +1664,48 @@ void extract_subdivision(prg_ctx* c) {
     alist.reserve(nb);
     amap.reserve(nb);
     // pass 0: certificates; pass 1: counts of the bricks left
-    PoolBuf<int32_t> dlist{&c->pool[129]}, dbricks{&c->pool[130]}, dpairs{&c->pool[131]};
+    PoolBuf<int32_t> dlist{&c->pool[129]}, dbricks{&c->pool[130]}, dpairs{&c->pool[131]}, xlist{&c->pool[133]};
     PoolBuf<int8_t> bsign{&c->pool[132]};
     dlist.reserve(nb);
+    xlist.reserve(nb);
     dbricks.reserve(3 * nb);
     dpairs.reserve(26 * nb);
     bsign.reserve(nb);
     {
         TimedScope ts(kt, "subdivide_screen", 4.0 * double(c->sub_leaves));
-        // a resident grid: every workgroup walks its share of the bricks
+        // resident grids: every workgroup walks its share of the bricks
         int per_cu = 0;
         PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_screen, kWave, 0));
-        const int64_t grid = std::min<int64_t>(nb, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
-        k_brick_screen<<<unsigned(grid), kWave, 0, s>>>(A, nb, bnv.get(), bnt.get(), bsign.get(), dlist.get(),
+        int64_t grid = std::min<int64_t>(nb, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
+        k_brick_screen<<<unsigned(grid), kWave, 0, s>>>(A, nb, bnv.get(), bnt.get(), bsign.get(), xlist.get(),
                                                         dbricks.get(), dpairs.get(), C);
+        PRG_CHECK_LAUNCH();
+        PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_exact, kWave, 0));
+        grid = std::min<int64_t>(nb, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
+        k_brick_exact<<<unsigned(grid), kWave, 0, s>>>(A, xlist.get(), bsign.get(), dlist.get(), dbricks.get(),
+                                                       dpairs.get(), C);
         PRG_CHECK_LAUNCH();
         k_brick_deps<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(dbricks.get(), dpairs.get(), bsign.get(),
                                                                 dlist.get(), C);
         PRG_CHECK_LAUNCH();
     }
+    PRG_HIP(hipMemcpyAsync(H, C, kCntSlots * 4, hipMemcpyDeviceToHost, s));
+    PRG_HIP(hipStreamSynchronize(s));
+    const int64_t ndense = H[kCntDense];
+    PoolBuf<float> gvals{&c->pool[134]};
+    PoolBuf<int32_t> didx{&c->pool[135]};
+    gvals.reserve(size_t(std::max<int64_t>(1, ndense)) * kBrickPts);
+    didx.reserve(nb);
     {
         TimedScope ts(kt, "subdivide_count", 0.0);
         int per_cu = 0;
         PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_count, kWave, 0));
-        const int64_t grid = std::min<int64_t>(nb, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
-        k_brick_count<<<unsigned(grid), kWave, 0, s>>>(A, dlist.get(), C + kCntDense, bnv.get(), bnt.get());
-        PRG_CHECK_LAUNCH();
+        const int64_t grid = std::min<int64_t>(ndense, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
+        if (grid > 0) {
+            k_brick_count<<<unsigned(grid), kWave, 0, s>>>(A, dlist.get(), C + kCntDense, bnv.get(), bnt.get(),
+                                                           gvals.get(), didx.get());
+            PRG_CHECK_LAUNCH();
+        }
     }
     scan_exclusive2(bnv.get(), bvoff.get(), C + kCntVerts, bnt.get(), btoff.get(), C + kCntTris, nb, s, c->scratch,
                     kt);
@@ -1614,6 +1717,9 @@ void extract_subdivision(prg_ctx* c) {
     PRG_HIP(hipMemcpyAsync(H, C, kCntSlots * 4, hipMemcpyDeviceToHost, s));
     PRG_HIP(hipStreamSynchronize(s));
     const int64_t snv = H[kCntVerts], snt = H[kCntTris], nact = H[kCntActive];
+    c->sub_bricks = nb;
+    c->sub_exact = H[kCntExact];
+    c->sub_evaluated = H[kCntDense];
     if (snv < 0 || snt < 0 || nact < 0) throw Error{PRG_E_NOMEM, "subdivision mesh exceeds int32 counts"};
     if (c->mesh_nv + snv >= (int64_t(1) << 31)) throw Error{PRG_E_NOMEM, "mesh vertex count exceeds int32 indices"};
     if (nact == 0) return;
@@ -1629,7 +1735,7 @@ void extract_subdivision(prg_ctx* c) {
     {
         TimedScope ts(kt, "subdivide_vertices", 12.0 * double(snv));
         k_brick_vertices<<<unsigned(nact), kWave, 0, s>>>(A, alist.get(), bvoff.get(), lofs_tab.get(),
-                                                          emask_tab.get(), cube_tab.get(), SV);
+                                                          emask_tab.get(), cube_tab.get(), gvals.get(), didx.get(), SV);
         PRG_CHECK_LAUNCH();
     }
     {
