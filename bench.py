@@ -66,6 +66,9 @@ def workload(args):
     if args.workload == "superquadrics_2M_d10":
         p, n = prg.synth_superquadrics(args.points, 3, 0.001, 3)
         return p, n, 10
+    if args.workload == "superquadrics_150k_d10":  # sparse: large coarse leaves the surface crosses
+        p, n = prg.synth_superquadrics(args.points if args.points != 2_000_000 else 150_000, 3, 0.001, 21)
+        return p, n, 10
     if args.workload == "torus_500k_d9":
         p, n = prg.synth_torus(args.points, 2)
         return p, n, 9
@@ -329,6 +332,7 @@ def main():
     stage_ms = {k: round(v, 3) for k, v in g.stage_times().items()}
     roots, vleaves = C.c_int64(), C.c_int64()
     g._call("prg_subdivision_info", C.byref(roots), C.byref(vleaves))
+    bricks = g.subdivision_bricks()
     counts, bases = g.level_info()
     # the same step from device-resident points, and the leaf-level mesh alone
     dt_dev, _ = timed_steps(hip, g, dp, dn, N, True, pinned, 1, args.steps)
@@ -372,7 +376,8 @@ def main():
                  "div_from_subtree_sums", "div_leaf", "parent_neighbor_table", "radix_downsweep", "radix_upsweep",
                  "point_values", "corner_value", "neighbors", "levels_fill", "edge_owner", "corner_owner",
                  "scan_exclusive", "gather_points", "morton", "point_weights", "spmv_tile_plan", "div_parent_sums",
-                 "face_marks", "coarse_corner_value", "subdivide_count", "subdivide_vertices", "subdivide_triangles"]:
+                 "face_marks", "coarse_corner_value", "subdivide_screen", "subdivide_count", "subdivide_vertices",
+                 "subdivide_triangles"]:
         try:
             cnt, ms, bpl = gt.kernel_stats(name)
             kern[name] = {"launches": cnt, "total_ms": round(ms, 4), "avg_us": round(1000 * ms / cnt, 2),
@@ -465,7 +470,9 @@ def main():
         "cg_iters": [s["iters"] for s in stats],
         "solve_vs_fp64_oracle": solve_fp64,
         "mesh": {"vertices": int(mesh[0]), "triangles": int(mesh[1]), "subdivision_roots": roots.value,
-                 "virtual_leaves": vleaves.value},
+                 "virtual_leaves": vleaves.value,
+                 "subdivision_bricks": {"bricks": bricks[0], "left_by_interval_certificate": bricks[1],
+                                        "evaluated": bricks[2]}},
         "variants": variants,
         "kernels": kern,
         "roofline": roofline,

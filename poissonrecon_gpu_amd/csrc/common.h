@@ -114,6 +114,11 @@ struct KernelTimer {
         if (idx < 0) return;
         PRG_HIP(hipEventRecord(pending[idx].b, stream));
     }
+    // Counts a launch under `name` without timing it (no events: a record
+    // that only says which variant ran, beside a timed one).
+    void count(const char* name) {
+        if (enabled) recs[name].launches++;
+    }
     // Files a recorded launch under another name with other algorithmic bytes
     // (e.g. one that found its work already finished and moved nothing), before flush().
     void relabel(int idx, const char* name, double bytes = 0.0) {

@@ -1029,10 +1029,10 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                 // 27 int32 neighbour indices, p_i read, (Ap)_i written (116 B)
                 TimedScope ts(c->kt, spmv_name, kSpmvRowBytes * double(rows));
                 h.launched.emplace_back(ts.idx, it);
-                // the instance (cells per thread) that ran, as its own record
+                // the instance (cells per thread) that ran, counted as its own record
                 static const char* kInstName[5] = {"spmv_cells_2", "spmv_cells_4", "spmv_cells_8", "spmv_cells_27",
                                                    "spmv_cells_5"};
-                TimedScope ti(c->kt, kInstName[h.inst < 0 || h.inst > 4 ? 3 : h.inst], 0.0);
+                c->kt.count(kInstName[h.inst < 0 || h.inst > 4 ? 3 : h.inst]);
                 launch_spmv(h.inst, h.nwg, h.lds, h.s, dseg + h.seg0, cnt, it, tol, fixed, max_iters);
             }
             {

@@ -1483,7 +1483,7 @@ __global__ void __launch_bounds__(kWave) k_brick_triangles(SubArgs A, const int3
                                                            const uint16_t* lofs_tab, const uint16_t* emask_tab,
                                                            const uint8_t* cube_tab, int64_t vbase, int32_t* out_t,
                                                            int32_t* err) {
-    __shared__ BrickLds S;
+    __shared__ BrickCoreT<1> S;  // the root's neighbours only
     Brick B;
     const int32_t ai = blockIdx.x;
     const int32_t b = alist[ai];
@@ -1672,7 +1672,7 @@ void extract_subdivision(prg_ctx* c) {
     dpairs.reserve(26 * nb);
     bsign.reserve(nb);
     {
-        TimedScope ts(kt, "subdivide_screen", 4.0 * double(c->sub_leaves));
+        TimedScope ts(kt, "subdivide_screen", 0.0);  // latency- and issue-bound: no byte figure
         // resident grids: every workgroup walks its share of the bricks
         int per_cu = 0;
         PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_screen, kWave, 0));
