@@ -131,6 +131,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         PRG_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, cf.device));
         prg::weld_noalloc_scratch(cf.device);  // the fixed weld array's address on this device, once
         if (const char* lim = std::getenv("PRG_TILE_LDS_LIMIT")) c->tile_limit = std::max(0, std::atoi(lim));
+        if (const char* ev = std::getenv("PRG_SUBDIVIDE_EVALUATE_ALL")) c->sub_certify = std::atoi(ev) == 0;
         c->kt.enabled = cf.timing != 0;
         c->kt.stream = c->stream;
         return PRG_OK;

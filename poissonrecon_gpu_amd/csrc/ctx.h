@@ -135,6 +135,8 @@ struct prg_ctx {
     // listed blocks and points the leaf-level tile kernels stage in LDS, so
     // their per-row and global-memory paths run on ordinary inputs
     int32_t tile_limit = 0;
+    // test knob (PRG_SUBDIVIDE_EVALUATE_ALL=1): evaluate every subdivision brick, no certificates
+    bool sub_certify = true;
 
     int64_t MD() const { return counts.empty() ? 0 : counts[D]; }
     int64_t base_of(int d) const { return bases[d]; }

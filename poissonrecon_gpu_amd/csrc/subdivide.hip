@@ -1278,7 +1278,7 @@ __device__ __forceinline__ void record_certificate(const SubArgs& A, const Brick
 // certificate (k_brick_exact).
 __global__ void __launch_bounds__(kWave) k_brick_screen(SubArgs A, int64_t nb, int32_t* bnv, int32_t* bnt,
                                                         int8_t* sign, int32_t* xlist, int32_t* dbricks,
-                                                        int32_t* dpairs, int32_t* cnt) {
+                                                        int32_t* dpairs, int32_t* cnt, int certify) {
     __shared__ BrickScreenLds S;
     const int lane = threadIdx.x;
     constexpr int kChunk = 16;  // consecutive bricks per grab: mostly one root, whose record stays in LDS
@@ -1300,7 +1300,7 @@ __global__ void __launch_bounds__(kWave) k_brick_screen(SubArgs A, int64_t nb, i
         Certificate c;
         c.sign = 0;
         c.deps = 0;
-        if (brick_blocks(A, B, S)) c = brick_certify_interval(A, B, S);  // else: to the exact certificate
+        if (brick_blocks(A, B, S) && certify) c = brick_certify_interval(A, B, S);  // else: to the exact certificate
         if (lane == 0) {
             bnv[b] = 0;
             bnt[b] = 0;
@@ -1314,7 +1314,7 @@ __global__ void __launch_bounds__(kWave) k_brick_screen(SubArgs A, int64_t nb, i
 // left (a fixed grid walks the list); the bricks it does not certify are
 // listed for the evaluation.
 __global__ void __launch_bounds__(kWave) k_brick_exact(SubArgs A, const int32_t* xlist, int8_t* sign, int32_t* list,
-                                                       int32_t* dbricks, int32_t* dpairs, int32_t* cnt) {
+                                                       int32_t* dbricks, int32_t* dpairs, int32_t* cnt, int certify) {
     __shared__ BrickCore S;
     const int32_t n = cnt[kCntExact];
     for (int32_t it = blockIdx.x; it < n; it += gridDim.x) {
@@ -1324,7 +1324,10 @@ __global__ void __launch_bounds__(kWave) k_brick_exact(SubArgs A, const int32_t*
         brick_setup(A, b, B, S, true);
         brick_tables(A, B, S);
         brick_blocks(A, B, S);
-        const Certificate c = brick_certify(A, B, S);
+        Certificate c;
+        c.sign = 0;
+        c.deps = 0;
+        if (certify) c = brick_certify(A, B, S);
         record_certificate(A, B, S, b, c, sign, list, kCntDense, dbricks, dpairs, cnt);
     }
 }
@@ -1678,12 +1681,12 @@ void extract_subdivision(prg_ctx* c) {
         PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_screen, kWave, 0));
         int64_t grid = std::min<int64_t>(nb, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
         k_brick_screen<<<unsigned(grid), kWave, 0, s>>>(A, nb, bnv.get(), bnt.get(), bsign.get(), xlist.get(),
-                                                        dbricks.get(), dpairs.get(), C);
+                                                        dbricks.get(), dpairs.get(), C, int(c->sub_certify));
         PRG_CHECK_LAUNCH();
         PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_exact, kWave, 0));
         grid = std::min<int64_t>(nb, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
         k_brick_exact<<<unsigned(grid), kWave, 0, s>>>(A, xlist.get(), bsign.get(), dlist.get(), dbricks.get(),
-                                                       dpairs.get(), C);
+                                                       dpairs.get(), C, int(c->sub_certify));
         PRG_CHECK_LAUNCH();
         k_brick_deps<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(dbricks.get(), dpairs.get(), bsign.get(),
                                                                 dlist.get(), C);

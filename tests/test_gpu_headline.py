@@ -218,6 +218,33 @@ def test_leaf_mesh_and_subdivision_tail_bit_exact(case, prg):
     print(f"structural check of {n} subdivision triangles passed")
 
 
+def test_subdivision_certificates_agree_with_evaluation(case, prg, monkeypatch):
+    """The brick certificates only skip bricks without output: the full mesh
+    with them equals, bit for bit, the mesh of a context that evaluates every
+    brick (test setting PRG_SUBDIVIDE_EVALUATE_ALL=1), at the headline size."""
+    g, D = case["g"], case["D"]
+    x, iso = _oracle_solution(case)
+    g.write_buffer(prg.BUF_SOLUTION, x)
+    g.set_isovalue(iso)
+    m = g.extract_mesh(subdivide=True)
+    nb, nx, ne = g.subdivision_bricks()
+    monkeypatch.setenv("PRG_SUBDIVIDE_EVALUATE_ALL", "1")
+    with prg.Reconstructor(max_depth=D) as h:
+        h.set_points(case["p"], case["n"])
+        h.build_octree()
+        h.write_buffer(prg.BUF_SOLUTION, x)
+        h.set_isovalue(iso)
+        m2 = h.extract_mesh(subdivide=True)
+        nb2, nx2, ne2 = h.subdivision_bricks()
+    print(f"bricks {nb}: {nb - nx} settled by the interval, {nx - ne} by the per-corner certificate, {ne} evaluated")
+    assert nb2 == nb and nx2 == nb2 and ne2 == nb2, "the test setting did not evaluate every brick"
+    assert ne < nx < nb, "the certificates settled nothing"
+    assert (m.n_leaf_vertices, m.n_leaf_triangles) == (m2.n_leaf_vertices, m2.n_leaf_triangles)
+    assert m.vertices.shape == m2.vertices.shape and m.triangles.shape == m2.triangles.shape
+    assert np.array_equal(np.nan_to_num(m.vertices).view(np.uint32), np.nan_to_num(m2.vertices).view(np.uint32))
+    assert np.array_equal(m.triangles, m2.triangles)
+
+
 def test_end_to_end_hausdorff(case, prg):
     """Each side's own pipeline (own solve, own isovalue) to the leaf-level mesh;
     symmetric Hausdorff distance of the two meshes (vertices and triangle
