@@ -1166,9 +1166,10 @@ __device__ __forceinline__ Certificate brick_certify(const SubArgs& A, const Bri
                 for (int m = 0; m < 4; m++) v[i] = __builtin_fmaf(w0[m], Y[m], v[i]);
             }
         }
-        // the finer depths, where the brick spans several cells: the owner's
-        // 27 terms directly from the blocks (the same products the evaluation
-        // forms, summed here in FP32 with their absolute values)
+        // the finer depths, where the brick spans several cells: per x slot m
+        // of the block, Y_m = sum over the lane's 3 y and 3 z slots, then
+        // each point adds its owner's three x slots' Y (the slots a point's
+        // owner sees follow from grid coordinates alone: uniform per i)
         for (int d = dlast + 1; d <= D; d++) {
             const int slot = S.xb_off[d];
             if (slot < 0) continue;
@@ -1177,19 +1178,33 @@ __device__ __forceinline__ Certificate brick_certify(const SubArgs& A, const Bri
             const int b2 = (own_cell(B.org[2] + k, B.R0[2]) >> sh) - 1 - ((B.org[2] >> sh) - 2);
             const float4 ty = S.T[3 * kP * d + kP + j], tz = S.T[3 * kP * d + 2 * kP + k];
             const float by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
+            int b0[kP];
+            float bx[kP][3];
 #pragma unroll
             for (int i = 0; i < kP; i++) {
-                if (i >= P) continue;
-                const int b0 = (own_cell(B.org[0] + i, B.R0[0]) >> sh) - 1 - ((B.org[0] >> sh) - 2);
+                b0[i] = (own_cell(B.org[0] + i, B.R0[0]) >> sh) - 1 - ((B.org[0] >> sh) - 2);
                 const float4 tx = S.T[3 * kP * d + i];
-                const float bx[3] = {tx.x, tx.y, tx.z};
-                float t = 0.0f;
+                bx[i][0] = tx.x;
+                bx[i][1] = tx.y;
+                bx[i][2] = tx.z;
+            }
 #pragma unroll
-                for (int q = 0; q < 27; q++) {
-                    const float xv = S.XB[slot + ((b0 + q / 9) * W + (b1 + (q / 3) % 3)) * W + (b2 + q % 3)];
-                    t = __builtin_fmaf(xv * bx[q / 9], by[(q / 3) % 3] * bz[q % 3], t);
+            for (int m = 0; m < 11; m++) {  // W <= 11
+                if (m >= W || m < b0[0] || m > b0[P - 1] + 2) continue;  // (uniform) a slot no point sees
+                const float* Xm = S.XB + slot + m * W * W;
+                float y = 0.0f;
+#pragma unroll
+                for (int n = 0; n < 3; n++) {
+                    float z = 0.0f;
+#pragma unroll
+                    for (int l = 0; l < 3; l++) z = __builtin_fmaf(Xm[(b1 + n) * W + (b2 + l)], bz[l], z);
+                    y = __builtin_fmaf(by[n], z, y);
                 }
-                v[i] += t;
+#pragma unroll
+                for (int i = 0; i < kP; i++) {
+                    const int q = m - b0[i];  // uniform
+                    if (i < P && q >= 0 && q <= 2) v[i] = __builtin_fmaf(q == 0 ? bx[i][0] : q == 1 ? bx[i][1] : bx[i][2], y, v[i]);
+                }
             }
         }
         const bool jk_inner = B.org[1] + j > B.R0[1] && B.org[1] + j < B.R0[1] + B.Sr && B.org[2] + k > B.R0[2] &&
