@@ -298,7 +298,7 @@ def main():
     ap.add_argument("--learn-epochs", type=int, default=6)
     ap.add_argument("--learn-batch", type=int, default=8)
     ap.add_argument("--learn-lr", type=float, default=1e-3)
-    ap.add_argument("--batch", action="store_true", help="also run the 64-cloud batch workload")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-cloud batch workload")
     ap.add_argument("--batch-clouds", type=int, default=64)
     ap.add_argument("--batch-workers", type=int, default=4)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -404,7 +404,7 @@ def main():
     # per launch (FETCH_SIZE x2 + WRITE_SIZE, calibrated on a copy kernel).
     dom = "stencil_spmv_dot_finest"
     k = kern[dom]
-    rows = k["alg_bytes_per_launch"] / SPMV_ROW_BYTES
+    rows = k["alg_bytes_per_launch"] / SPMV_OWN_ROW_BYTES
     sec = k["avg_us"] * 1e-6
     achieved = rows * SPMV_OWN_ROW_BYTES / sec / 1e9
     roofline = {"kernel": dom, "hip_kernel": "k_cg_spmv", "rows_per_launch": int(rows), "bound": "hbm",
@@ -447,7 +447,7 @@ def main():
     hip.hipHostFree(hn)
 
     learned = None if args.no_learned else learned_leg(args)
-    batch = batch_leg(args) if args.batch else None
+    batch = None if args.no_batch else batch_leg(args)
 
     out = {
         "metric": "points/s end-to-end at depth 10; solve ms learned vs plain CG at equal residual",

@@ -54,6 +54,12 @@ prg_status prg_dense_grid(prg_ctx* ctx, int32_t depth, int32_t which, float* gri
  * context does not own it. */
 prg_status prg_set_warm_start_net(prg_ctx* ctx, prg_unet* net);
 
+/* After prg_solve: the depth the network warm-started (-1: none) and whether
+ * its start was rejected because ||b - A x0|| > ||b|| (that depth then
+ * started from x0 = 0, as every depth of the reference does,
+ * main.cu:1237-1329). */
+prg_status prg_warm_start_info(prg_ctx* ctx, int32_t* depth, int32_t* rejected);
+
 #ifdef __cplusplus
 }
 #endif

@@ -127,6 +127,7 @@ SIGNATURES = {
     "prg_unet_train_step": (C.c_int, [_P, _P, _P, _P, _P, C.c_int32, C.c_float, C.POINTER(C.c_double)]),
     "prg_dense_grid": (C.c_int, [_P, C.c_int32, C.c_int32, _P, C.POINTER(C.c_float)]),
     "prg_set_warm_start_net": (C.c_int, [_P, _P]),
+    "prg_warm_start_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "prg_synth_sphere": (C.c_int, [C.c_int64, C.c_uint64, _P, _P]),
     "prg_synth_torus": (C.c_int, [C.c_int64, C.c_uint64, _P, _P]),
     "prg_synth_superquadrics": (C.c_int, [C.c_int64, C.c_int32, C.c_float, C.c_uint64, _P, _P]),
@@ -330,6 +331,12 @@ class Reconstructor:
 
     def set_warm_start_net(self, net: "UNet | None") -> None:
         self._call("prg_set_warm_start_net", net._h if net is not None else None)
+
+    def warm_start_info(self) -> tuple[int, bool]:
+        """(depth the network warm-started or -1, whether its start was rejected) of the last solve."""
+        d, r = C.c_int32(), C.c_int32()
+        self._call("prg_warm_start_info", C.byref(d), C.byref(r))
+        return d.value, bool(r.value)
 
     # ---- introspection ----
     def level_info(self) -> tuple[np.ndarray, np.ndarray]:

@@ -132,6 +132,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         prg::weld_noalloc_scratch(cf.device);  // the fixed weld array's address on this device, once
         if (const char* lim = std::getenv("PRG_TILE_LDS_LIMIT")) c->tile_limit = std::max(0, std::atoi(lim));
         if (const char* ev = std::getenv("PRG_SUBDIVIDE_EVALUATE_ALL")) c->sub_certify = std::atoi(ev) == 0;
+        if (const char* ws = std::getenv("PRG_WARM_START_SCALE")) c->warm_scale = float(std::atof(ws));
         c->kt.enabled = cf.timing != 0;
         c->kt.stream = c->stream;
         return PRG_OK;

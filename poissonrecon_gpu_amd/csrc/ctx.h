@@ -137,6 +137,11 @@ struct prg_ctx {
     int32_t tile_limit = 0;
     // test knob (PRG_SUBDIVIDE_EVALUATE_ALL=1): evaluate every subdivision brick, no certificates
     bool sub_certify = true;
+    // test knob (PRG_WARM_START_SCALE): multiplies the learned warm start's prediction
+    float warm_scale = 1.0f;
+    // last solve: the warm-started depth (-1: none) and whether its start was
+    // rejected (||b - A x0|| > ||b||: the depth started from zero instead)
+    int32_t warm_depth_used = -1, warm_rejected = 0;
 
     int64_t MD() const { return counts.empty() ? 0 : counts[D]; }
     int64_t base_of(int d) const { return bases[d]; }

@@ -42,11 +42,14 @@ constexpr int kBlock = 256;
 constexpr int kMaxBlocks = 2048;
 constexpr int kSmallBlock = 1024;
 constexpr int64_t kSmallRows = 2048;
-// Algorithmic bytes per SpMV row under the reference's CSR-free representation
-// (int32 neighbour indices, FP32 values): 108 B of neighbour indices, p_i read,
-// (Ap)_i written.  The kernels here read 27 parent-neighbour entries per sibling
-// block instead (13.5 B per row), so their compulsory traffic is lower.
-constexpr double kSpmvRowBytes = 116.0;
+// Algorithmic bytes per row of the kernels here: the 27 parent-neighbour
+// entries of a row's 8-row sibling block (13.5 B per row), plus the vectors
+// each reads and writes once per row.  (The reference's representation, 27
+// int32 neighbour indices + p_i read + (Ap)_i written, would be 116 B per
+// SpMV row.)
+constexpr double kPntRowBytes = 13.5;
+constexpr double kSpmvRowBytes = kPntRowBytes + 16.0;  // r, p_(k-1) read; p_k, q written
+constexpr double kResidualRowBytes = kPntRowBytes + 12.0;  // x, b read; r written
 
 struct Stencil {
     float L[27];
@@ -59,6 +62,8 @@ struct CgState {
     double rr;          // r.r at the last stop test
     double rr_hist[2];  // rr_k in slot k & 1
     int32_t done, iters;
+    int32_t restarted;  // a warm start with ||b - A x0|| > ||b|| was replaced by x0 = 0
+    int32_t pad;
 };
 
 // One depth of a chain: its rows, its workgroup ranges in the chain's SpMV and
@@ -172,7 +177,34 @@ __global__ void __launch_bounds__(kVecThreads) k_cg_init(const int32_t* pnt, int
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->done = 0;
         st->iters = 0;
+        st->restarted = 0;
     }
+}
+
+// A warm start worse than none (||b - A x0|| > ||b||, from k_cg_init's
+// partials, summed in a fixed order) is replaced by x0 = 0: xs = 0, r = p = b,
+// and each workgroup's r.r partial becomes its b.b partial (the same grid as
+// k_cg_init, so the same rows per workgroup: the state k_cg_init leaves for
+// x0 = 0, bit for bit).  Every workgroup decides from the same sums; the
+// partials are only rewritten by k_warm_reset_apply after all have decided.
+__global__ void __launch_bounds__(kVecThreads) k_warm_reset_decide(const double* rr_part, const double* bb_part,
+                                                                   int32_t np, CgState* st) {
+    __shared__ double lds[kVecThreads / 64];
+    const double rr = sum_partials<kVecThreads>(rr_part, np, lds);
+    const double bb = sum_partials<kVecThreads>(bb_part, np, lds);
+    if (threadIdx.x == 0) st->restarted = rr > bb ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(kVecThreads) k_warm_reset_apply(int64_t n, const float* b, float* xs, float* r,
+                                                                  float* p, double* rr_part, const double* bb_part,
+                                                                  const CgState* st) {
+    if (!st->restarted) return;
+    for (int64_t t = int64_t(blockIdx.x) * kVecThreads + threadIdx.x; t < n; t += int64_t(gridDim.x) * kVecThreads) {
+        xs[t] = 0.0f;
+        r[t] = b[t];
+        p[t] = b[t];
+    }
+    if (threadIdx.x == 0) rr_part[blockIdx.x] = bb_part[blockIdx.x];
 }
 
 // Tile plan of one depth (once per octree): for each tile of kTile sibling
@@ -509,6 +541,7 @@ __global__ void __launch_bounds__(kVecThreads) k_true_residual_finish(const doub
         out[0] = (bb > 0.0) ? sqrt(RR / bb) : 0.0;
         out[1] = (bb > 0.0) ? sqrt(rec / bb) : 0.0;
         out[2] = double(iters);
+        out[3] = double(st->restarted);
     }
 }
 
@@ -601,6 +634,17 @@ __global__ void __launch_bounds__(kSmallBlock) k_cg_small(SmallJobs jobs, const 
     }
     bb = block_allsum<kSmallBlock>(bb, lds);  // every read of sv above is complete
     rr = block_allsum<kSmallBlock>(rr, lds);
+    // a warm start worse than none (||b - A x0|| > ||b||) is replaced by x0 = 0:
+    // r = b - (+0) = b and r.r = b.b, the state of a zero start
+    const bool restarted = d == warm_depth && rr > bb;
+    if (restarted) {
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            x[k] = 0.0f;
+            r[k] = b[k];
+        }
+        rr = bb;
+    }
     const double tol2 = double(tol) * double(tol) * bb;
     int it = 0;
 #pragma unroll
@@ -689,6 +733,7 @@ __global__ void __launch_bounds__(kSmallBlock) k_cg_small(SmallJobs jobs, const 
         out[4 * d + 0] = (bb > 0.0) ? sqrt(tr / bb) : 0.0;
         out[4 * d + 1] = (bb > 0.0) ? sqrt(rr / bb) : 0.0;
         out[4 * d + 2] = double(it);
+        out[4 * d + 3] = restarted ? 1.0 : 0.0;
     }
 }
 
@@ -1008,11 +1053,19 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                 dbeg[d] = new_event();
                 dend[d] = new_event();
                 PRG_HIP(hipEventRecord(dbeg[d], h.s));
-                TimedScope ts(c->kt, "cg_init", kSpmvRowBytes * double(n));
+                // b read, xs, r, p written; x0 read and A x0 formed unless x0 = 0
+                TimedScope ts(c->kt, "cg_init", (d == warm_depth ? kPntRowBytes + 20.0 : 16.0) * double(n));
                 k_cg_init<<<g.nvwg, kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], n, stencil_of(d),
                                                           c->div.get() + base, X + base, d == warm_depth ? 0 : 1,
                                                           g.XS, g.R, g.P0, g.rr, g.bb, g.st);
                 PRG_CHECK_LAUNCH();
+                if (d == warm_depth) {
+                    k_warm_reset_decide<<<1, kVecThreads, 0, h.s>>>(g.rr, g.bb, g.nvwg, g.st);
+                    PRG_CHECK_LAUNCH();
+                    k_warm_reset_apply<<<g.nvwg, kVecThreads, 0, h.s>>>(n, c->div.get() + base, g.XS, g.R, g.P0,
+                                                                        g.rr, g.bb, g.st);
+                    PRG_CHECK_LAUNCH();
+                }
             }
             h.issued = 0;
             h.batch = kFirstBatch;
@@ -1025,8 +1078,6 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             for (int d : h.depths) rows += c->counts[d];
             const char* spmv_name = h.depths.back() == D ? "stencil_spmv_dot_finest" : "stencil_spmv_dot";
             {
-                // algorithmic bytes of one SpMV row in the reference's representation:
-                // 27 int32 neighbour indices, p_i read, (Ap)_i written (116 B)
                 TimedScope ts(c->kt, spmv_name, kSpmvRowBytes * double(rows));
                 h.launched.emplace_back(ts.idx, it);
                 // the instance (cells per thread) that ran, counted as its own record
@@ -1045,7 +1096,7 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                 for (int i = 0; i < cnt; i++) {
                     const int d = h.depths[i];
                     const Seg& g = hseg[h.seg0 + i];
-                    TimedScope ts(c->kt, "cg_refresh", kSpmvRowBytes * double(g.n));
+                    TimedScope ts(c->kt, "cg_refresh", kResidualRowBytes * double(g.n));
                     h.launched.emplace_back(ts.idx, it);
                     k_cg_refresh<<<g.nvwg, kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], g.n,
                                                                  stencil_of(d), c->div.get() + c->bases[d], g.XS,
@@ -1072,7 +1123,7 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                 const Seg& g = hseg[h.seg0 + i];
                 const int64_t n = c->counts[d], base = c->bases[d];
                 {
-                    TimedScope ts(c->kt, "cg_true_residual", kSpmvRowBytes * double(n));
+                    TimedScope ts(c->kt, "cg_true_residual", kResidualRowBytes * double(n));
                     k_true_residual<<<g.nvwg, kVecThreads, 0, h.s>>>(c->pnt.get() + 27 * c->pnt_off[d], n,
                                                                     stencil_of(d), c->div.get() + base, g.XS, g.bb);
                     PRG_CHECK_LAUNCH();
@@ -1178,6 +1229,8 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
         (void)hipStreamSynchronize(s);
         throw;
     }
+    c->warm_depth_used = warm_depth;
+    c->warm_rejected = warm_depth >= 0 && ov[4 * warm_depth + 3] > 0.5;
     for (int d = 0; d <= D; d++) {
         const int32_t iters = int32_t(ov[4 * d + 2]);
         if (stats) {

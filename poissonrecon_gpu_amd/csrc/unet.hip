@@ -508,12 +508,13 @@ __global__ void k_input_dev(const float* b, const float* mask, const double* ss,
     x[k] = c == 0 ? b[v] * (1.0f / rms_of(ss)) : (c == 1 ? mask[v] : 0.0f);
 }
 
-// x[i] = pred[cell(i)] * scale for the n nodes of one depth
+// x[i] = pred[cell(i)] * scale for the n nodes of one depth (times `extra`,
+// a test setting that is 1 otherwise)
 __global__ void k_gather_nodes(const uint32_t* key, const float* pred, const double* ss, int64_t n, int d, int D,
-                               float* x) {
+                               float extra, float* x) {
     const int64_t i = int64_t(blockIdx.x) * kT + threadIdx.x;
     if (i >= n) return;
-    x[i] = pred[node_cell(key[i], d, D)] * rms_of(ss);
+    x[i] = pred[node_cell(key[i], d, D)] * rms_of(ss) * extra;
 }
 
 unsigned blocks(int64_t n) { return unsigned((n + kT - 1) / kT); }
@@ -800,7 +801,7 @@ void unet_warm_start(prg_ctx* c, int depth, hipStream_t s) {
             forward(n);
         }
         k_gather_nodes<<<blocks(cnt), kT, 0, s>>>(c->key.get() + base, n->pred.get(), ss.get(), cnt, depth, c->D,
-                                                  c->x.get() + base);
+                                                  c->warm_scale, c->x.get() + base);
         PRG_CHECK_LAUNCH();
     } catch (...) {
         n->stream = keep;
@@ -1078,6 +1079,13 @@ prg_status prg_dense_grid(prg_ctx* c, int32_t depth, int32_t which, float* grid,
 prg_status prg_set_warm_start_net(prg_ctx* c, prg_unet* n) {
     if (!c) return PRG_E_ARG;
     c->warm_net = n;
+    return PRG_OK;
+}
+
+prg_status prg_warm_start_info(prg_ctx* c, int32_t* depth, int32_t* rejected) {
+    if (!c || !depth || !rejected) return PRG_E_ARG;
+    *depth = c->warm_depth_used;
+    *rejected = c->warm_rejected;
     return PRG_OK;
 }
 

@@ -119,3 +119,37 @@ def test_learned_warm_start_solves_to_tolerance(prg):
         warm.close()
         plain.close()
         net.close()
+
+
+@pytest.mark.parametrize("grid", [16, 64])
+def test_worse_than_zero_warm_start_falls_back(prg, monkeypatch, grid):
+    """A warm start with ||b - A x0|| > ||b|| is replaced by x0 = 0: with the
+    prediction scaled by 1e6 (test setting PRG_WARM_START_SCALE), the solve
+    equals the plain solve bit for bit, and prg_warm_start_info reports the
+    rejection.  grid 16 warm-starts depth 4 (the one-workgroup small-depth
+    CG), grid 64 depth 6 (the multi-kernel CG of the large depths)."""
+    print(f"\n[warm start fallback: sphere N=35947 D=8, network grid {grid}]", flush=True)
+    p, n = prg.synth_sphere(35947, 1)
+    net = prg.UNet(grid=grid, seed=3)
+    plain = prg.Reconstructor(max_depth=8)
+    monkeypatch.setenv("PRG_WARM_START_SCALE", "1e6")
+    warm = prg.Reconstructor(max_depth=8, warm_start=prg.WARM_LEARNED)
+    try:
+        warm.set_warm_start_net(net)
+        for g in (plain, warm):
+            g.set_points(p, n)
+            g.build_octree()
+            g.compute_divergence()
+        counts, _ = plain.level_info()
+        wd = {16: 4, 64: 6}[grid]
+        assert (counts[wd] > 2048) == (grid == 64), counts.tolist()
+        sp, sw = plain.solve(), warm.solve()
+        assert warm.warm_start_info() == (wd, True)
+        assert [s["iters"] for s in sp] == [s["iters"] for s in sw]
+        assert np.array_equal(plain.read_buffer(prg.BUF_SOLUTION).view(np.uint32),
+                              warm.read_buffer(prg.BUF_SOLUTION).view(np.uint32))
+        assert plain.warm_start_info() == (-1, False)
+    finally:
+        warm.close()
+        plain.close()
+        net.close()
