@@ -723,22 +723,33 @@ __device__ __forceinline__ float grid_value(const SubArgs& A, const Brick& B, co
         const float f = B.deep ? deep_sum_lds(A, B, S, o, gi) : 0.0f;
         return chain_lds(S, B.r, gi, f) - *A.iso;
     }
-    // owned by a leaf of another root of the group: that root's neighbours and chain
-    const int32_t rho2 = S.NB[slot];
-    const int32_t R2[3] = {B.R[0] + slot / 9 - 1, B.R[1] + (slot / 3) % 3 - 1, B.R[2] + slot % 3 - 1};
-    const float f = deep_sum(
-        A, B.r, R2, o, side, [&](int j) { return A.neigh[int64_t(j) * A.M + rho2]; },
-        [&](int j) {
-            const int32_t a = A.neigh[int64_t(j) * A.M + rho2];
-            return a < 0 ? -1 : A.child_base[a];
-        });
-    float c[3];
-    node_center(cell_key(uint32_t(o[0]), uint32_t(o[1]), uint32_t(o[2])), D, D, c);
-    const float hw = 1.0f / float(1 << (D + 1));
-    const float px = c[0] + float(2 * side[0] - 1) * hw;
-    const float py = c[1] + float(2 * side[1] - 1) * hw;
-    const float pz = c[2] + float(2 * side[2] - 1) * hw;
-    return chain_value(rho2, B.r, px, py, pz, A.key, A.pnt, A.L, A.parent, A.x, A.bf, D, f) - *A.iso;
+    // owned by a leaf of another root of the group: that root's record (its
+    // neighbours, their child bases, its chain's values: what the neighbour
+    // planes and chain_value would read), with the owner's basis values from
+    // the corner basis table
+    const int32_t* R2 = A.rec + int64_t(S.NBR[slot]) * kRecWords;
+    const int32_t RC[3] = {B.R[0] + slot / 9 - 1, B.R[1] + (slot / 3) % 3 - 1, B.R[2] + slot % 3 - 1};
+    float f = deep_sum(
+        A, B.r, RC, o, side, [&](int j) { return R2[kRecNB + j]; }, [&](int j) { return R2[kRecNBK + j]; });
+    for (int d = B.r; d >= 1; d--) {
+        const float4 tx = A.btab[((int64_t(d) << D) + o[0]) * 2 + side[0]];
+        const float4 ty = A.btab[((int64_t(d) << D) + o[1]) * 2 + side[1]];
+        const float4 tz = A.btab[((int64_t(d) << D) + o[2]) * 2 + side[2]];
+        const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
+        const int32_t* X = R2 + kRecX + 27 * d;
+#pragma unroll
+        for (int j = 0; j < 27; j++) {
+            float tt = __int_as_float(X[j]) * bx[j / 9];
+            tt = tt * by[(j / 3) % 3];
+            tt = tt * bz[j % 3];
+            f = f + tt;
+        }
+    }
+    const float b0x = A.btab[o[0] * 2 + side[0]].y, b0y = A.btab[o[1] * 2 + side[1]].y, b0z = A.btab[o[2] * 2 + side[2]].y;
+    float tt = __int_as_float(R2[kRecX]) * b0x;
+    tt = tt * b0y;
+    tt = tt * b0z;
+    return f + tt - *A.iso;
 }
 
 template <class L>
