@@ -980,24 +980,40 @@ __device__ __forceinline__ void axis_weights4(const Brick& B, const L& S, int D,
 template <class L>
 __device__ __forceinline__ int brick_interval_sign(const SubArgs& A, const Brick& B, L& S, float abound) {
     const int D = A.D, lane = threadIdx.x, P = (1 << B.kb) + 1, dlast = D - B.kb;
-    for (int e = lane; e < (dlast + 1) * 12; e += kWave) {
-        const int d = e / 12, a = (e >> 2) % 3, m = e & 3;
-        if (d > B.r && S.xb_off[d] < 0) continue;
-        float lo = 3.4e38f, hi = -3.4e38f;
+    // one lane per (depth, axis): the four slots' ranges over the points
+    if (lane < 3 * (dlast + 1)) {
+        const int d = lane / 3, a = lane % 3;
+        if (d <= B.r || S.xb_off[d] >= 0) {
+            // (arithmetic selects: an index by a would put B in scratch memory)
+            const int32_t og = (a == 0) * B.org[0] + (a == 1) * B.org[1] + (a == 2) * B.org[2];
+            const int32_t r0 = (a == 0) * B.R0[0] + (a == 1) * B.R0[1] + (a == 2) * B.R0[2];
+            float lo[4], hi[4];
 #pragma unroll
-        for (int i = 0; i < kP; i++) {
-            float w[4];
-            if (a == 0) axis_weights4(B, S, D, d, 0, i, w);  // constant axes keep B in registers
-            else if (a == 1) axis_weights4(B, S, D, d, 1, i, w);
-            else axis_weights4(B, S, D, d, 2, i, w);
-            const float v = m == 0 ? w[0] : m == 1 ? w[1] : m == 2 ? w[2] : w[3];
-            if (i < P) {
-                lo = fminf(lo, v);
-                hi = fmaxf(hi, v);
+            for (int m = 0; m < 4; m++) {
+                lo[m] = 3.4e38f;
+                hi[m] = -3.4e38f;
+            }
+#pragma unroll
+            for (int i = 0; i < kP; i++) {
+                if (i < P) {
+                    const float4 t = S.T[3 * kP * d + kP * a + i];
+                    // the owner's ancestor is the brick's (slots 1..3) or the one below (0..2)
+                    const int32_t o = own_cell(og + i, r0);
+                    const bool below = d > B.r && (o >> (D - d)) != (og >> (D - d));
+                    const float w0 = below ? t.x : 0.0f, w1 = below ? t.y : t.x, w2 = below ? t.z : t.y,
+                                w3 = below ? 0.0f : t.z;
+                    lo[0] = fminf(lo[0], w0), hi[0] = fmaxf(hi[0], w0);
+                    lo[1] = fminf(lo[1], w1), hi[1] = fmaxf(hi[1], w1);
+                    lo[2] = fminf(lo[2], w2), hi[2] = fmaxf(hi[2], w2);
+                    lo[3] = fminf(lo[3], w3), hi[3] = fmaxf(hi[3], w3);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                S.wlo[12 * d + 4 * a + m] = lo[m];
+                S.whi[12 * d + 4 * a + m] = hi[m];
             }
         }
-        S.wlo[e] = lo;
-        S.whi[e] = hi;
     }
     __syncthreads();
     const int m = lane >> 4, n = (lane >> 2) & 3, l = lane & 3;
