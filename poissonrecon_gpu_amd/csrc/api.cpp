@@ -127,7 +127,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         }
         PRG_HIP(hipHostMalloc(&c->transform_host, 16, hipHostMallocDefault));
         PRG_HIP(hipHostMalloc(&c->cg_host, 4096, hipHostMallocDefault));
-        PRG_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->sub_host), 64, hipHostMallocDefault));
+        PRG_HIP(hipHostMalloc(reinterpret_cast<void**>(&c->sub_host), 128, hipHostMallocDefault));
         PRG_HIP(hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, cf.device));
         prg::weld_noalloc_scratch(cf.device);  // the fixed weld array's address on this device, once
         if (const char* lim = std::getenv("PRG_TILE_LDS_LIMIT")) c->tile_limit = std::max(0, std::atoi(lim));

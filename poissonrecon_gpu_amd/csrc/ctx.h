@@ -130,7 +130,7 @@ struct prg_ctx {
     std::vector<hipEvent_t> ev_pool;  // the solver's events, reused across solves
     int num_cu = 0;                   // compute units of the device
     void* cg_host = nullptr;          // pinned, 4 KiB: CG state read-backs and the segment table's staging
-    int32_t* sub_host = nullptr;      // pinned, 16 ints: the subdivision's counters
+    int32_t* sub_host = nullptr;      // pinned, 32 ints: the subdivision's counters
     // test setting (environment PRG_TILE_LDS_LIMIT at creation): caps the
     // listed blocks and points the leaf-level tile kernels stage in LDS, so
     // their per-row and global-memory paths run on ordinary inputs

@@ -64,7 +64,8 @@ __constant__ int8_t c_children_vertex_kind[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 // slots of the device counter array
 enum { kCntCoarse = 0, kCntCorners = 1, kCntRoots = 2, kCntBricks = 3, kCntVerts = 4, kCntTris = 5, kCntActive = 6,
        kCntLeaves = 7, kCntErr = 8, kCntDense = 9, kCntDepBricks = 10, kCntDepPairs = 11, kCntWork = 12,
-       kCntExact = 13, kCntSlots = 16 };
+       kCntExact = 13, kCntSuper = 14, kCntList2 = 15, kCntWork2 = 16, kCntSlots = 32 };
+static_assert(kCntSlots * 4 <= 128, "the pinned counter copy holds 32 ints");
 
 __device__ __forceinline__ int depth_of(int64_t i, const Levels& L, int D) {
     int d = 0;
@@ -303,12 +304,13 @@ constexpr int kXBlk = 2304;
 
 // LDS of one brick's wave: the root's neighbourhood, the brick's basis
 // tables and blocks (BrickCore), and for the evaluation its corner values.
-template <int kXBCap>
+template <int kXBCap, int kTP = kP>
 struct BrickCoreT {
+    static constexpr int kTPts = kTP;  // grid coordinates per axis in T
     // per depth d, axis a and grid coordinate i: the corner basis table entry
     // of the owner (own_cell) of grid coordinate org[a] + i, (d, a, i) at
     // d * 3 kP + a kP + i
-    float4 T[(kMaxDepth + 1) * 3 * kP];
+    float4 T[(kMaxDepth + 1) * 3 * kTP];
     float X[(kMaxDepth + 1) * 27];     // the root's chain: neighbour values at depths 0..r
     float XB[kXBCap];                  // per depth below the root: its block (xb_off)
     int32_t xb_off[kMaxDepth + 1];     // start of depth dd's block in XB, or -1 (no real node in it)
@@ -325,6 +327,9 @@ using BrickCore = BrickCoreT<kXBlk>;
 // need more go to the exact certificate, whose LDS holds every block.
 constexpr int kXBlkScreen = 512;
 using BrickScreenLds = BrickCoreT<kXBlkScreen>;
+// The super-brick screen's: 2x2x2 bricks, 17 grid coordinates per axis.
+constexpr int kPSuper = 2 * (1 << kBrickLog) + 1;
+using BrickSuperLds = BrickCoreT<kXBlkScreen, kPSuper>;
 
 template <int kXBCap>
 struct BrickValsT : BrickCoreT<kXBCap> {
@@ -385,10 +390,11 @@ __global__ void __launch_bounds__(kBlockS) k_brick_map(const int32_t* boff, cons
 
 // Loads brick b's root record into LDS and sets the brick's geometry.
 // q_loaded: the root whose record S already holds (the previous brick's, in a
-// kernel that walks consecutive bricks), or -1.
+// kernel that walks consecutive bricks), or -1.  up = 1 sets up the
+// super-brick of the 2x2x2 bricks that start at brick b.
 template <class L>
 __device__ __forceinline__ void brick_setup(const SubArgs& A, int32_t b, Brick& B, L& S, bool chain,
-                                            int32_t q_loaded = -1) {
+                                            int32_t q_loaded = -1, int up = 0) {
     const int D = A.D, lane = threadIdx.x;
     B.q = __builtin_amdgcn_readfirstlane(A.bmap[b]);
     const int32_t* R = A.rec + int64_t(B.q) * kRecWords;
@@ -405,7 +411,7 @@ __device__ __forceinline__ void brick_setup(const SubArgs& A, int32_t b, Brick& 
         if (chain)
             for (int w = lane; w < 27 * (B.r + 1); w += kWave) S.X[w] = __int_as_float(R[kRecX + w]);
     }
-    B.kb = brick_log(D, B.r);
+    B.kb = brick_log(D, B.r) + up;  // up = 1: the 2x2x2 bricks from brick b (a multiple of 8 in the root)
     int32_t rc[3];
     decode_cell(rkey, rc);
     B.Sr = 1 << (D - B.r);
@@ -413,7 +419,7 @@ __device__ __forceinline__ void brick_setup(const SubArgs& A, int32_t b, Brick& 
         B.R[a] = rc[a] >> (D - B.r);
         B.R0[a] = B.R[a] << (D - B.r);
     }
-    B.bkey = rkey | (uint32_t(b - A.boff[B.q]) << (3 * B.kb));
+    B.bkey = rkey | (uint32_t(b - A.boff[B.q]) << (3 * (B.kb - up)));
     decode_cell(B.bkey, B.org);
     __syncthreads();
     const uint64_t kids = __builtin_amdgcn_ballot_w64(lane < 27 && lane != 13 && S.NBK[lane < 27 ? lane : 0] >= 0);
@@ -455,13 +461,14 @@ __device__ __forceinline__ int32_t own_cell(int32_t g, int32_t R0) { return g ==
 template <class L>
 __device__ __forceinline__ void brick_tables(const SubArgs& A, const Brick& B, L& S) {
     const int D = A.D, P = (1 << B.kb) + 1;
-    for (int e = threadIdx.x; e < (D + 1) * kP; e += kWave) {
-        const int d = e / kP, i = e % kP;
+    constexpr int TP = L::kTPts;
+    for (int e = threadIdx.x; e < (D + 1) * TP; e += kWave) {
+        const int d = e / TP, i = e % TP;
         if (i >= P) continue;
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) {  // constant indices keep B in registers
             const int32_t g = B.org[ax] + i, o = own_cell(g, B.R0[ax]);
-            S.T[3 * kP * d + kP * ax + i] = A.btab[((int64_t(d) << D) + o) * 2 + (g - o)];
+            S.T[3 * TP * d + TP * ax + i] = A.btab[((int64_t(d) << D) + o) * 2 + (g - o)];
         }
     }
 }
@@ -916,8 +923,9 @@ __device__ __forceinline__ float brick_abs_bound(const Brick& B, L& S, int D) {
     if (lane <= kMaxDepth) S.tmax[lane] = S.xsum[lane] = 0.0f;
     __syncthreads();
     // |basis values| (non-negative floats order as their bit patterns)
-    for (int e = lane; e < (D + 1) * 3 * kP; e += kWave) {
-        const int d = e / (3 * kP), i = e % kP;
+    constexpr int TP = L::kTPts;
+    for (int e = lane; e < (D + 1) * 3 * TP; e += kWave) {
+        const int d = e / (3 * TP), i = e % TP;
         if (i >= P || (d > B.r && S.xb_off[d] < 0)) continue;
         const float4 t = S.T[e];
         const float m = fmaxf(fabsf(t.x), fmaxf(fabsf(t.y), fabsf(t.z)));
@@ -951,7 +959,7 @@ __device__ __forceinline__ void axis_weights4(const Brick& B, const L& S, int D,
                                               float w[4]) {
     const int32_t g = B.org[ax] + i, o = own_cell(g, B.R0[ax]);
     const int delta = (o >> (D - d)) - (B.org[ax] >> (D - d));
-    const float4 t = S.T[3 * kP * d + kP * ax + i];
+    const float4 t = S.T[3 * L::kTPts * d + L::kTPts * ax + i];
     if (delta == 0) {
         w[0] = 0.0f;
         w[1] = t.x;
@@ -994,9 +1002,9 @@ __device__ __forceinline__ int brick_interval_sign(const SubArgs& A, const Brick
                 hi[m] = -3.4e38f;
             }
 #pragma unroll
-            for (int i = 0; i < kP; i++) {
+            for (int i = 0; i < L::kTPts; i++) {
                 if (i < P) {
-                    const float4 t = S.T[3 * kP * d + kP * a + i];
+                    const float4 t = S.T[3 * L::kTPts * d + L::kTPts * a + i];
                     // the owner's ancestor is the brick's (slots 1..3) or the one below (0..2)
                     const int32_t o = own_cell(og + i, r0);
                     const bool below = d > B.r && (o >> (D - d)) != (og >> (D - d));
@@ -1313,27 +1321,111 @@ __device__ __forceinline__ void record_certificate(const SubArgs& A, const Brick
     }
 }
 
-// Pass 0: the interval certificate for every brick (a fixed grid takes
-// chunks of consecutive bricks from a counter).  Every brick's counts start
-// at 0; a certified brick keeps them unless a brick it depends on has
-// another sign (k_brick_deps); the others are listed for the exact
-// certificate (k_brick_exact).
-__global__ void __launch_bounds__(kWave) k_brick_screen(SubArgs A, int64_t nb, int32_t* bnv, int32_t* bnt,
+// Pass 0a: the interval certificate for the super-bricks (2x2x2 bricks) of
+// the roots with at least 8 bricks (a fixed grid takes chunks of consecutive
+// super-bricks from a counter).  A certified super-brick settles its 8
+// bricks (counts 0, its sign); the bricks of the others, and of super-bricks
+// that touch another root of the group (whose leaves may own corners) or
+// whose blocks the LDS cannot hold, are listed for the brick screen.
+__global__ void __launch_bounds__(kWave) k_brick_screen_super(SubArgs A, const int32_t* smap, const int32_t* soff,
+                                                              int32_t* bnv, int32_t* bnt, int8_t* sign,
+                                                              int32_t* list2, int32_t* cnt, int certify) {
+    __shared__ BrickSuperLds S;
+    const int lane = threadIdx.x;
+    const int32_t ns = cnt[kCntSuper];
+    constexpr int kChunk = 8;
+    int32_t q_loaded = -1, sb = 0, s_end = 0;
+    for (;;) {
+        if (sb == s_end) {
+            int32_t c0 = 0;
+            if (lane == 0) c0 = atomicAdd(cnt + kCntWork, kChunk);
+            c0 = __shfl(c0, 0, kWave);
+            if (c0 >= ns) break;
+            sb = c0;
+            s_end = std::min(ns, c0 + kChunk);
+        }
+        const int32_t q = smap[sb];
+        const int32_t b0 = A.boff[q] + 8 * (sb - soff[q]);
+        Brick B;
+        __syncthreads();  // the previous super-brick's LDS reads are done
+        brick_setup(A, b0, B, S, true, q_loaded, 1);
+        q_loaded = B.q;
+        brick_tables(A, B, S);
+        int sgn = 0;
+        if (certify && !brick_in_group(B, S) && brick_blocks(A, B, S))
+            sgn = brick_interval_sign(A, B, S, brick_abs_bound(B, S, A.D));
+        if (sgn != 0) {
+            if (lane < 8) {
+                bnv[b0 + lane] = 0;
+                bnt[b0 + lane] = 0;
+                sign[b0 + lane] = int8_t(sgn);
+            }
+        } else {
+            int32_t at = 0;
+            if (lane == 0) at = atomicAdd(cnt + kCntList2, 8);
+            at = __shfl(at, 0, kWave);
+            if (lane < 8) list2[at + lane] = b0 + lane;
+        }
+        sb++;
+    }
+}
+
+// The bricks of the roots with fewer than 8 bricks, for the brick screen.
+__global__ void __launch_bounds__(kBlockS) k_list_small_root_bricks(const int32_t* bmap, const int32_t* rbricks,
+                                                                    int64_t nb, int32_t* list2, int32_t* cnt) {
+    const int64_t b = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
+    if (b >= nb) return;
+    if (rbricks[bmap[b]] < 8) list2[atomicAdd(cnt + kCntList2, 1)] = int32_t(b);
+}
+
+// Per root: its super-brick count (a root of at least 8 bricks has a
+// multiple of 8, in 2x2x2 blocks of Morton-consecutive bricks).
+__global__ void __launch_bounds__(kBlockS) k_super_count(const int32_t* rbricks, const int32_t* cnt, int64_t nr_max,
+                                                         int32_t* nsup) {
+    const int64_t q = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
+    if (q >= nr_max) return;
+    int32_t n = 0;
+    if (q < cnt[kCntRoots]) n = rbricks[q] >= 8 ? rbricks[q] / 8 : 0;
+    nsup[q] = n;
+}
+
+// smap[s] = the root of super-brick s (binary search of the super-brick offsets).
+__global__ void __launch_bounds__(kBlockS) k_super_map(const int32_t* soff, const int32_t* cnt, int64_t ns_max,
+                                                       int32_t* smap) {
+    const int64_t sb = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
+    if (sb >= ns_max || sb >= cnt[kCntSuper]) return;
+    int32_t lo = 0, hi = cnt[kCntRoots] - 1;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (soff[mid] <= sb) lo = mid;
+        else hi = mid - 1;
+    }
+    smap[sb] = lo;
+}
+
+// Pass 0b: the interval certificate for the listed bricks (a fixed grid takes
+// chunks of the list from a counter).  Every listed brick's counts start at
+// 0; a certified brick keeps them unless a brick it depends on has another
+// sign (k_brick_deps); the others are listed for the exact certificate
+// (k_brick_exact).
+__global__ void __launch_bounds__(kWave) k_brick_screen(SubArgs A, const int32_t* list2, int32_t* bnv, int32_t* bnt,
                                                         int8_t* sign, int32_t* xlist, int32_t* dbricks,
                                                         int32_t* dpairs, int32_t* cnt, int certify) {
     __shared__ BrickScreenLds S;
     const int lane = threadIdx.x;
-    constexpr int kChunk = 16;  // consecutive bricks per grab: mostly one root, whose record stays in LDS
-    int32_t q_loaded = -1, b = 0, b_end = 0;
+    const int32_t nl = cnt[kCntList2];
+    constexpr int kChunk = 16;  // consecutive entries per grab: mostly one root, whose record stays in LDS
+    int32_t q_loaded = -1, it = 0, it_end = 0;
     for (;;) {
-        if (b == b_end) {
+        if (it == it_end) {
             int32_t c0 = 0;
-            if (lane == 0) c0 = atomicAdd(cnt + kCntWork, kChunk);
+            if (lane == 0) c0 = atomicAdd(cnt + kCntWork2, kChunk);
             c0 = __shfl(c0, 0, kWave);
-            if (c0 >= nb) break;
-            b = c0;
-            b_end = int32_t(std::min<int64_t>(nb, int64_t(c0) + kChunk));
+            if (c0 >= nl) break;
+            it = c0;
+            it_end = std::min(nl, c0 + kChunk);
         }
+        const int32_t b = list2[it];
         Brick B;
         __syncthreads();  // the previous brick's LDS reads are done
         brick_setup(A, b, B, S, true, q_loaded);
@@ -1348,7 +1440,7 @@ __global__ void __launch_bounds__(kWave) k_brick_screen(SubArgs A, int64_t nb, i
             bnt[b] = 0;
         }
         record_certificate(A, B, S, b, c, sign, xlist, kCntExact, dbricks, dpairs, cnt);
-        b++;
+        it++;
     }
 }
 
@@ -1595,7 +1687,7 @@ void extract_subdivision(prg_ctx* c) {
     const int rd = ensure_coarse_rows(c, s);
     if (rd < D - 1) throw Error{PRG_E_STATE, "coarse rows missing"};
 
-    // pool slots 97..135
+    // pool slots 97..139
     PoolBuf<uint8_t> inter{&c->pool[97]};
     PoolBuf<int32_t> cnt{&c->pool[98]}, flag{&c->pool[99]}, off{&c->pool[100]}, cl{&c->pool[101]},
         oref{&c->pool[102]}, need{&c->pool[103]}, nid{&c->pool[104]}, clist{&c->pool[105]}, rflag{&c->pool[107]},
@@ -1718,11 +1810,33 @@ void extract_subdivision(prg_ctx* c) {
     bsign.reserve(nb);
     {
         TimedScope ts(kt, "subdivide_screen", 0.0);  // latency- and issue-bound: no byte figure
-        // resident grids: every workgroup walks its share of the bricks
+        // super-bricks of the roots with at least 8 bricks, then the brick
+        // screen over the bricks they leave and those of the smaller roots
+        PoolBuf<int32_t> nsup{&c->pool[136]}, soff{&c->pool[137]}, smap{&c->pool[138]}, list2{&c->pool[139]};
+        nsup.reserve(nr);
+        soff.reserve(nr);
+        smap.reserve(std::max<int64_t>(1, nb / 8));
+        list2.reserve(nb);
+        k_super_count<<<grid_for(nr, kBlockS), kBlockS, 0, s>>>(rbricks.get(), C, nr, nsup.get());
+        PRG_CHECK_LAUNCH();
+        scan_exclusive(nsup.get(), soff.get(), nr, C + kCntSuper, s, c->scratch, kt);
+        k_super_map<<<grid_for(std::max<int64_t>(1, nb / 8), kBlockS), kBlockS, 0, s>>>(soff.get(), C,
+                                                                                      std::max<int64_t>(1, nb / 8),
+                                                                                      smap.get());
+        PRG_CHECK_LAUNCH();
+        k_list_small_root_bricks<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(bmap.get(), rbricks.get(), nb,
+                                                                            list2.get(), C);
+        PRG_CHECK_LAUNCH();
+        // resident grids: every workgroup walks its share
         int per_cu = 0;
+        PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_screen_super, kWave, 0));
+        int64_t grid = std::min<int64_t>(std::max<int64_t>(1, nb / 8), int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
+        k_brick_screen_super<<<unsigned(grid), kWave, 0, s>>>(A, smap.get(), soff.get(), bnv.get(), bnt.get(),
+                                                              bsign.get(), list2.get(), C, int(c->sub_certify));
+        PRG_CHECK_LAUNCH();
         PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_screen, kWave, 0));
-        int64_t grid = std::min<int64_t>(nb, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
-        k_brick_screen<<<unsigned(grid), kWave, 0, s>>>(A, nb, bnv.get(), bnt.get(), bsign.get(), xlist.get(),
+        grid = std::min<int64_t>(nb, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
+        k_brick_screen<<<unsigned(grid), kWave, 0, s>>>(A, list2.get(), bnv.get(), bnt.get(), bsign.get(), xlist.get(),
                                                         dbricks.get(), dpairs.get(), C, int(c->sub_certify));
         PRG_CHECK_LAUNCH();
         PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_exact, kWave, 0));
