@@ -121,6 +121,7 @@ prg_status prg_create(prg_ctx** out, const prg_config* cfg) {
         PRG_HIP(hipEventCreateWithFlags(&c->plan_ev, hipEventDisableTiming));
         PRG_HIP(hipEventCreate(&c->topo_ev));
         PRG_HIP(hipEventCreate(&c->mesh_ev));
+        PRG_HIP(hipEventCreate(&c->mesh_ev2));
         for (auto& e : c->stage_ev) {
             PRG_HIP(hipEventCreate(&e[0]));
             PRG_HIP(hipEventCreate(&e[1]));
@@ -154,6 +155,7 @@ void prg_destroy(prg_ctx* c) {
     if (c->plan_ev) (void)hipEventDestroy(c->plan_ev);
     if (c->topo_ev) (void)hipEventDestroy(c->topo_ev);
     if (c->mesh_ev) (void)hipEventDestroy(c->mesh_ev);
+    if (c->mesh_ev2) (void)hipEventDestroy(c->mesh_ev2);
     for (auto& e : c->stage_ev)
         for (hipEvent_t x : e)
             if (x) (void)hipEventDestroy(x);
@@ -344,7 +346,8 @@ static void build_mesh(prg_ctx* c, int32_t subdivide, prg_mesh* out) {
     out->scale = c->scale;
 }
 
-// Copies the leaf-level batch and the subdivision batches, in that order.
+// Copies the leaf-level batch and the subdivision batches, in that order
+// (the leaf-level batch may be on its way already: out_copied).
 static void copy_mesh(prg_ctx* c, float* vertices, int32_t* triangles) {
     prg::StageTimer st(c, PRG_STAGE_MESH_COPY);
     if (!c->out_copied) {
@@ -353,6 +356,7 @@ static void copy_mesh(prg_ctx* c, float* vertices, int32_t* triangles) {
     }
     if (c->sub_nv) d2h(c, vertices + 3 * c->mesh_nv, c->sub_v.get(), size_t(3 * c->sub_nv) * 4);
     if (c->sub_nt) d2h(c, triangles + 3 * c->mesh_nt, c->sub_t.get(), size_t(3 * c->sub_nt) * 4);
+    if (c->out_copied && c->out_async) PRG_HIP(hipStreamWaitEvent(c->stream, c->mesh_ev, 0));
     PRG_HIP(hipStreamSynchronize(c->stream));
 }
 
@@ -381,19 +385,23 @@ prg_status prg_extract_mesh_into(prg_ctx* c, int32_t subdivide, float* vertices,
         if (!c->have_x) return fail(c, PRG_E_STATE, "prg_solve first");
         if (!c->iso_ready) return fail(c, PRG_E_STATE, "prg_isovalue or prg_set_isovalue first");
         if (!out) return fail(c, PRG_E_ARG, "null mesh");
-        // a leaf-level mesh goes straight into the caller's buffers while it is produced
-        if (!subdivide) {
-            c->out_v = vertices;
-            c->out_t = triangles;
-            c->out_vcap = vcap;
-            c->out_tcap = tcap;
-        }
+        // the leaf-level batch goes straight into the caller's buffers while
+        // it is produced; with the subdivision, its copy overlaps the
+        // subdivision's kernels
+        c->out_v = vertices;
+        c->out_t = triangles;
+        c->out_vcap = vcap;
+        c->out_tcap = tcap;
+        c->out_async = subdivide != 0;
         struct Reset {
             prg_ctx* c;
             ~Reset() {
+                // an exception may leave copies in flight into the caller's buffers
+                if (c->out_copied && c->out_async) (void)hipStreamSynchronize(c->stream2);
                 c->out_v = nullptr;
                 c->out_t = nullptr;
                 c->out_vcap = c->out_tcap = 0;
+                c->out_async = false;
             }
         } reset{c};
         build_mesh(c, subdivide, out);

@@ -91,7 +91,12 @@ struct prg_ctx {
     int32_t* out_t = nullptr;
     int64_t out_vcap = 0, out_tcap = 0;
     bool out_copied = false;
+    // with the subdivision to follow: the leaf-level copies run on the second
+    // stream behind mesh_ev while the subdivision runs (mesh_ev2 orders them
+    // after the triangle emission)
+    bool out_async = false;
     hipEvent_t mesh_ev = nullptr;
+    hipEvent_t mesh_ev2 = nullptr;
     bool topo_pending = false;
     int32_t topo_corners = 0;  // owned corners
     bool neigh_ready = false;  // the 27 neighbour planes (depths 1..D) are built on demand

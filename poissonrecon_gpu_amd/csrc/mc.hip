@@ -619,7 +619,17 @@ void extract_leaf_mesh(prg_ctx* c) {
     c->mesh_nv = NV;
     c->mesh_nt = NT;
     c->out_copied = direct;
-    if (direct) {
+    if (direct && c->out_async) {
+        // the subdivision follows on the context stream: the triangles go out
+        // on the second stream too, behind the vertices, while it runs
+        // (copy_mesh waits for mesh_ev)
+        if (NT) {
+            PRG_HIP(hipEventRecord(c->mesh_ev2, s));
+            PRG_HIP(hipStreamWaitEvent(c->stream2, c->mesh_ev2, 0));
+            PRG_HIP(hipMemcpyAsync(c->out_t, c->mesh_t.get(), size_t(3) * NT * 4, hipMemcpyDeviceToHost, c->stream2));
+        }
+        PRG_HIP(hipEventRecord(c->mesh_ev, c->stream2));
+    } else if (direct) {
         StageTimer st(c, PRG_STAGE_MESH_COPY);
         if (NT) PRG_HIP(hipMemcpyAsync(c->out_t, c->mesh_t.get(), size_t(3) * NT * 4, hipMemcpyDeviceToHost, s));
         if (NV) PRG_HIP(hipStreamWaitEvent(s, c->mesh_ev, 0));
