@@ -30,9 +30,6 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 SPMV_ROW_BYTES = 116.0  # per row, reference representation (int32 neighbour indices, FP32 values)
-# per row, this build's layout: 27 parent-neighbour entries per 8-row sibling
-# block (13.5 B), r_i and p_{k-1,i} read, p_{k,i} and q_i written
-SPMV_OWN_ROW_BYTES = 29.5
 
 
 def _hip():
@@ -397,20 +394,22 @@ def main():
 
     # Dominant kernel of the solve: the CG SpMV at the finest depth (k_cg_spmv;
     # one launch per CG iteration over all M_D rows).  Its algorithmic bytes in
-    # this build's layout (29.5 B/row: the parent-neighbour entries of the
-    # row's sibling block, r_i and p_{k-1,i} read, p_{k,i} and q_i written) give
-    # `achieved`; the reference's CSR representation (116 B/row) is reported
-    # beside it as an equivalent rate, and `traffic` is the measured HBM bytes
-    # per launch (FETCH_SIZE x2 + WRITE_SIZE, calibrated on a copy kernel).
+    # this build's layout (the tile plan: 27 int16 row ids per sibling block, a
+    # count per tile and 4 B per listed block; r_i and p_{k-1,i} read, p_{k,i}
+    # and q_i written) give `achieved`; the reference's CSR representation
+    # (116 B/row) is reported beside it as an equivalent rate, and `traffic` is
+    # the measured HBM bytes per launch (FETCH_SIZE x2 + WRITE_SIZE, calibrated
+    # on a copy kernel).
     dom = "stencil_spmv_dot_finest"
     k = kern[dom]
-    rows = k["alg_bytes_per_launch"] / SPMV_OWN_ROW_BYTES
+    rows = int(counts[D])
     sec = k["avg_us"] * 1e-6
-    achieved = rows * SPMV_OWN_ROW_BYTES / sec / 1e9
-    roofline = {"kernel": dom, "hip_kernel": "k_cg_spmv", "rows_per_launch": int(rows), "bound": "hbm",
+    achieved = k["alg_bytes_per_launch"] / sec / 1e9
+    roofline = {"kernel": dom, "hip_kernel": "k_cg_spmv", "rows_per_launch": rows, "bound": "hbm",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "alg_bytes_per_row": SPMV_OWN_ROW_BYTES,
+                "alg_bytes_per_launch": k["alg_bytes_per_launch"],
+                "alg_bytes_per_row": round(k["alg_bytes_per_launch"] / rows, 3),
                 "reference_representation_rate_gbs": round(rows * SPMV_ROW_BYTES / sec / 1e9, 1),
                 "avg_us": k["avg_us"]}
     traffic_file = ROOT / "profiles" / f"traffic_k_cg_spmv_{args.workload}.json"

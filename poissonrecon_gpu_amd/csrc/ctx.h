@@ -72,13 +72,14 @@ struct prg_ctx {
     prg::DBuf<int16_t> plan_rowid;
     int64_t plan_tile_off[prg::kMaxDepth + 2] = {};
     int32_t plan_maxcount[prg::kMaxDepth + 1] = {};
+    int64_t plan_cells[prg::kMaxDepth + 1] = {};  // per depth: distinct blocks listed over all tiles
     uint64_t tree_gen = 0, plan_gen = 0;
     // per-node rows of the coarse depths (27 neighbour solution values and the
     // parent), for the solution `x_gen` of octree `rows_tree`
     prg::DBuf<float> coarse_rows;
     int32_t rows_depth = 0;
     uint64_t x_gen = 0, rows_x = ~uint64_t(0), rows_tree = ~uint64_t(0);
-    prg::DBuf<int32_t> plan_mx;       // per-depth largest tile count (device)
+    prg::DBuf<int32_t> plan_mx;       // per-depth largest tile count and sum of tile counts (device)
     hipEvent_t plan_ev = nullptr;     // completion of the plans started after an octree build
     bool plan_pending = false;
     // marching-cubes topology of the octree `topo_gen` (corner owners and

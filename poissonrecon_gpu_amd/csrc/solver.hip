@@ -42,13 +42,13 @@ constexpr int kBlock = 256;
 constexpr int kMaxBlocks = 2048;
 constexpr int kSmallBlock = 1024;
 constexpr int64_t kSmallRows = 2048;
-// Algorithmic bytes per row of the kernels here: the 27 parent-neighbour
-// entries of a row's 8-row sibling block (13.5 B per row), plus the vectors
-// each reads and writes once per row.  (The reference's representation, 27
-// int32 neighbour indices + p_i read + (Ap)_i written, would be 116 B per
-// SpMV row.)
+// Algorithmic bytes per row of the kernels that gather through the
+// parent-neighbour table: the 27 entries of a row's 8-row sibling block (13.5 B
+// per row), plus the vectors each reads and writes once per row.  The SpMV
+// reads the tile plans instead (spmv_plan_bytes).  (The reference's
+// representation, 27 int32 neighbour indices + p_i read + (Ap)_i written,
+// would be 116 B per SpMV row.)
 constexpr double kPntRowBytes = 13.5;
-constexpr double kSpmvRowBytes = kPntRowBytes + 16.0;  // r, p_(k-1) read; p_k, q written
 constexpr double kResidualRowBytes = kPntRowBytes + 12.0;  // x, b read; r written
 
 struct Stencil {
@@ -109,12 +109,6 @@ __device__ __forceinline__ float spmv_pnt(const int32_t* __restrict__ pnt, int64
         out = (nb[j] < 0 || S.L[j] == 0.0f) ? out : out + prod;
     }
     return out;
-}
-
-// p_k at row t: r + beta * p_{k-1}
-__device__ __forceinline__ float pdir(const float* __restrict__ r, const float* __restrict__ pold, float bf, int64_t t) {
-    const float tp = bf * pold[t];
-    return r[t] + tp;
 }
 
 // Block-wide sum of a double in a fixed order; every thread gets the total.
@@ -254,45 +248,81 @@ __global__ void __launch_bounds__(kSpmvThreads) k_tile_plan(const int32_t* pnt, 
     if (tid == 0) count[tile] = ucount;
 }
 
-// Largest tile count of every depth (one workgroup per depth; the per-tile
-// counts of depth d are count[off[d] .. off[d + 1]))
+// Largest tile count and the sum of the tile counts of every depth (one
+// workgroup per depth; the per-tile counts of depth d are count[off[d] ..
+// off[d + 1]))
 struct PlanOffsets {
     int64_t off[kMaxDepth + 2];
 };
-__global__ void __launch_bounds__(1024) k_plan_max(const int32_t* count, PlanOffsets P, int32_t* maxcount) {
+__global__ void __launch_bounds__(1024) k_plan_max(const int32_t* count, PlanOffsets P, int32_t* maxcount,
+                                                   long long* sumcount) {
     __shared__ int32_t red[1024 / 64];
+    __shared__ long long reds[1024 / 64];
     const int d = blockIdx.x;
     int32_t m = 0;
-    for (int64_t t = P.off[d] + threadIdx.x; t < P.off[d + 1]; t += 1024) m = max(m, count[t]);
+    long long sm = 0;
+    for (int64_t t = P.off[d] + threadIdx.x; t < P.off[d + 1]; t += 1024) {
+        m = max(m, count[t]);
+        sm += count[t];
+    }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_down(m, o, 64));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    for (int o = 32; o > 0; o >>= 1) {
+        m = max(m, __shfl_down(m, o, 64));
+        sm += __shfl_down(sm, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = m;
+        reds[threadIdx.x >> 6] = sm;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         int32_t r = 0;
-        for (int w = 0; w < 1024 / 64; w++) r = max(r, red[w]);
+        long long rs = 0;
+        for (int w = 0; w < 1024 / 64; w++) {
+            r = max(r, red[w]);
+            rs += reds[w];
+        }
         maxcount[d] = r;
+        sumcount[d] = rs;
     }
 }
 
 // CG iteration `it` (1-based), first kernel.  Every workgroup sums the r.r
 // partials of the previous iteration (rr_{it-1}) and applies the stop test
 // (iteration cap, rr == 0, fixed count, rr <= tol^2 b.b); then q = A p_it over
-// tiles, with p_it = r + beta p_{it-1} formed as the cells are loaded (it == 1:
+// tiles, with p_it = r + beta p_{it-1} formed as the cells reach LDS (it == 1:
 // p_1 = r is already in pnew), each distinct sibling block of a tile loaded
 // once into LDS (see k_tile_plan), and the rows summed from LDS in the order
 // j = 0..26.  Writes this workgroup's partial of p.q.
 //
-// The tile loop is software-pipelined two tiles deep: while tile t is summed,
-// the cell values of tile t + 1 and the plan (count, row ids, block bases) of
-// tile t + 2 are in flight in registers.  K = cells per thread covers the
+// The tile loop is software-pipelined two tiles deep: while a tile is summed,
+// the raw cell values of the workgroup's next tile and the plan (count, row
+// ids, block bases) of the one after are in flight in registers.  K = cells per thread covers the
 // depth's largest tile (8 * count <= K * kSpmvThreads).
-// Instances with at most kSpmvSmallK cells per thread fit 80 VGPRs: six waves
-// per SIMD, three workgroups per CU; the larger ones run two.
+// Instances with at most kSpmvSmallK cells per thread fit 64 VGPRs: eight
+// waves per SIMD, four workgroups per CU when LDS allows; the larger ones run
+// two or three (the launch asks the runtime's occupancy).
 constexpr int kSpmvSmallK = 5;
 
+// Global-memory views of a segment's pointers.  Pointers read from a Seg are
+// generic, and generic (flat) loads count against the LDS wait counter too:
+// every LDS wait of the tile loop would then also wait for the next tile's
+// loads in flight.  Global loads count only against the vector-memory counter.
+template <class T>
+using gmem = __attribute__((address_space(1))) T;
+
+// Raw buffer descriptor of `bytes` bytes at p (the gfx9 data-format word);
+// loads at offsets beyond it return 0.  Ranges are capped just below 4 GiB so
+// that kBufOob always lies outside (a depth holds fewer than 2^30 - 2 rows).
+constexpr int kBufWord3 = 0x00020000;
+constexpr uint32_t kBufOob = 0xFFFFFFFCu;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint64_t bytes) {
+    const uint32_t nr = bytes < 0xFFFFFFF8ull ? uint32_t(bytes) : 0xFFFFFFF8u;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, int(nr), kBufWord3);
+}
+
 template <int K>
-__global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 6 : (K <= 8 ? 4 : 2)) k_cg_spmv(const Seg* __restrict__ segs,
+__global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 8 : (K <= 8 ? 4 : 2)) k_cg_spmv(const Seg* __restrict__ segs,
                                                                                  int32_t nseg, int32_t it,
                                                           float tol, int32_t fixed, int32_t max_iters) {
     static_assert(K * kSpmvThreads <= 8 * kTileEntries, "cells per thread beyond a tile's plan");
@@ -310,13 +340,10 @@ __global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 6 : (K <= 8 ?
     CgState* st = sg.st;
     if (st->done) return;
     const int64_t n = sg.n;
-    const int32_t* __restrict__ count = sg.cnt;
-    const int32_t* __restrict__ ubase = sg.ub;
-    const int16_t* __restrict__ rowid = sg.rw;
-    const float* r = sg.R;
-    float* pnew = ((it - 1) & 1) ? sg.P1 : sg.P0;
-    const float* pold = (it & 1) ? sg.P1 : sg.P0;
-    float* q = sg.Q;
+    // p_k and q of the tile's own rows (it == 1: p_1 is already in place, the
+    // empty range drops those stores)
+    const auto rs_pnew = buf_rsrc(((it - 1) & 1) ? sg.P1 : sg.P0, it == 1 ? 0 : uint64_t(n) * sizeof(float));
+    const auto rs_q = buf_rsrc(sg.Q, uint64_t(n) * sizeof(float));
     const double* rr_part = sg.rr + ((it - 1) & 1) * kMaxWG;
     const double* bb_part = sg.bb;
     const int32_t np = sg.nvwg;
@@ -343,8 +370,26 @@ __global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 6 : (K <= 8 ?
     const int g = tid >> 3, s = tid & 7;
     const int64_t nblk = n >> 3;
     const int64_t ntiles = (nblk + kTile - 1) / kTile;
-    const int64_t per = (ntiles + nblocks - 1) / nblocks;
-    const int64_t tbeg = int64_t(bid) * per, tend = tbeg + per < ntiles ? tbeg + per : ntiles;
+    // Tile schedule.  Workgroups are dealt to the 8 XCDs in turn, so the
+    // segment's workgroups bid = x, x + G, x + 2G, ... (G = min(8, nblocks))
+    // run on one XCD.  Group x takes one contiguous range of tiles, in
+    // proportion to its members, and its members stride through it together:
+    // member l takes tiles tbeg + l, tbeg + l + m, ... (m members).  At any
+    // moment an XCD works on a window of m consecutive tiles, whose
+    // neighbourhoods overlap in that XCD's L2.
+    // (Integer division runs on the vector unit; readfirstlane keeps the
+    // results in scalar registers, where the buffer descriptors need them.)
+    const int G = nblocks < 8 ? nblocks : 8;
+    const int x = __builtin_amdgcn_readfirstlane(bid % G), l = __builtin_amdgcn_readfirstlane(bid / G);
+    const int q8 = __builtin_amdgcn_readfirstlane(nblocks / G), r8 = nblocks - q8 * G;
+    const int before = x * q8 + (x < r8 ? x : r8);  // workgroups of the groups before x
+    const int m = q8 + (x < r8 ? 1 : 0);
+    // tiles split evenly over the workgroups (a depth has fewer than 2^31 tiles)
+    const int32_t per = __builtin_amdgcn_readfirstlane(int32_t(ntiles) / nblocks);
+    const int32_t rem = int32_t(ntiles) - per * nblocks;
+    const int64_t gbeg = int64_t(before) * per + (before < rem ? before : rem);
+    const int64_t gend = int64_t(before + m) * per + (before + m < rem ? before + m : rem);
+    const int64_t tbeg = gbeg + l, tstep = m, tend = gend;
     // this lane's row sits at (1 + sx, 1 + sy, 1 + sz) of its block's grid
     const int c0 = 16 * ((s >> 2) & 1) + 4 * ((s >> 1) & 1) + (s & 1);
     // plan of one tile: cell count, row-id words, block base of each of this thread's cells
@@ -353,45 +398,62 @@ __global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 6 : (K <= 8 ?
         int32_t rw[2];
         int32_t ub[K];
     };
+    // Plan and cell loads go through buffer descriptors: a 32-bit offset from
+    // a uniform base (one address register per load), and offsets past the
+    // descriptor's range read 0 without touching memory: the row-id words
+    // beyond the plan, the cells beyond the tile's count, and every plan entry
+    // of a tile past this workgroup's range (count 0, so none of its cells is
+    // loaded either).  No load sits behind a branch, so the waits before the
+    // values are used count exactly the loads and stores issued after them.
+    // (the range check covers the vector offset only, so the tile's offset goes there)
+    const auto rs_cnt = buf_rsrc(sg.cnt + gbeg, uint64_t(gend - gbeg) * sizeof(int32_t));
     auto load_plan = [&](int64_t tile, Plan& P) {
-        if (tile >= tend) return;
-        P.cnt = count[tile];
-        const int32_t* w = reinterpret_cast<const int32_t*>(rowid + tile * kTileEntries);
-        P.rw[0] = w[tid];
-        P.rw[1] = tid + kSpmvThreads < kRidWords ? w[tid + kSpmvThreads] : 0;
-        const int32_t* tub = ubase + tile * kTileEntries;
+        const bool in = tile < tend;
+        P.cnt = int32_t(__builtin_amdgcn_raw_buffer_load_b32(rs_cnt, uint32_t(4 * (tile - gbeg)), 0, 0));
+        const auto rw = buf_rsrc(sg.rw + tile * kTileEntries, in ? kTileEntries * sizeof(int16_t) : 0);
+        P.rw[0] = int32_t(__builtin_amdgcn_raw_buffer_load_b32(rw, 4 * tid, 0, 0));
+        P.rw[1] = int32_t(__builtin_amdgcn_raw_buffer_load_b32(rw, 4 * (tid + kSpmvThreads), 0, 0));
+        const auto ub = buf_rsrc(sg.ub + tile * kTileEntries, in ? kTileEntries * sizeof(int32_t) : 0);
 #pragma unroll
-        for (int k = 0; k < K; k++) P.ub[k] = tub[(tid + k * kSpmvThreads) >> 3];  // in bounds; masked by cnt later
+        for (int k = 0; k < K; k++)  // entries beyond the count are masked later
+            P.ub[k] = int32_t(__builtin_amdgcn_raw_buffer_load_b32(ub, 4 * ((tid >> 3) + k * (kSpmvThreads / 8)), 0, 0));
     };
-    auto load_cells = [&](int64_t tile, const Plan& P, float v[K]) {
-        if (tile >= tend) return;
+    // raw r and p_(k-1) of the tile's cells; p_k = r + beta p_(k-1) is formed
+    // when the values reach LDS, so nothing waits for these loads before the
+    // next tile.  it == 1: p_1 = r (k_cg_init copies r into p), so the second
+    // load reads r again and the value is taken as it is.
+    const auto rs_r = buf_rsrc(sg.R, uint64_t(n) * sizeof(float));
+    const auto rs_p = buf_rsrc(first ? sg.R : ((it & 1) ? sg.P1 : sg.P0), uint64_t(n) * sizeof(float));
+    auto load_cells = [&](const Plan& P, float vr[K], float vp[K]) {
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const int u = tid + k * kSpmvThreads;
-            if (u < 8 * P.cnt) {
-                const int64_t idx = int64_t(P.ub[k]) + (u & 7);
-                v[k] = first ? pnew[idx] : pdir(r, pold, bf, idx);
-            }
+            const uint32_t off = u < 8 * P.cnt ? 4u * (uint32_t(P.ub[k]) + uint32_t(u & 7)) : kBufOob;
+            vr[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_r, off, 0, 0));
+            vp[k] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs_p, off, 0, 0));
         }
     };
     Plan pa{}, pb{};
-    float va[K];
+    float va[K], vpa[K];
     load_plan(tbeg, pa);
-    load_cells(tbeg, pa, va);
-    load_plan(tbeg + 1, pb);
+    load_cells(pa, va, vpa);
+    load_plan(tbeg + tstep, pb);
     double pq = 0.0;
-    for (int64_t tile = tbeg; tile < tend; tile++) {
-        // tile: values in va, plan in pa; tile + 1: plan in pb
+    for (int64_t tile = tbeg; tile < tend; tile += tstep) {
+        // this tile: values in va, plan in pa; the next one: plan in pb
         rid32[tid] = pa.rw[0];
         if (tid + kSpmvThreads < kRidWords) rid32[tid + kSpmvThreads] = pa.rw[1];
 #pragma unroll
         for (int k = 0; k < K; k++) {
             const int u = tid + k * kSpmvThreads;
-            if (u < 8 * pa.cnt) cellv[u] = va[k];
+            if (u < 8 * pa.cnt) {
+                const float tp = bf * vpa[k];
+                cellv[u] = first ? va[k] : va[k] + tp;
+            }
         }
-        load_cells(tile + 1, pb, va);  // va is free again: tile + 1's values
+        load_cells(pb, va, vpa);  // free again: the next tile's values
         pa = pb;
-        load_plan(tile + 2, pb);
+        load_plan(tile + 2 * tstep, pb);
         __syncthreads();
         // lanes of block g fill its 4x4x4 grid: cell (x, y, z) lies in parent
         // neighbour (x >> 1, y >> 1, z >> 1) at child slot (x & 1, y & 1, z & 1)
@@ -404,8 +466,12 @@ __global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 6 : (K <= 8 ?
             grid[g][cc] = id < 0 ? 0.0f : cellv[8 * id + (((ax & 1) << 2) | ((ay & 1) << 1) | (az & 1))];
         }
         __syncthreads();
+        // blocks past the last one have an empty plan (all-zero grids); their
+        // stores fall outside the descriptors and are dropped, so the stores
+        // are issued without a branch and the loads of the next tiles are
+        // waited for by count
         const int64_t blk = tile * kTile + g;
-        if (blk < nblk) {
+        {
             // absent neighbours hold +0: adding L*0 leaves a float sum
             // unchanged, so the result equals the sum over present terms
             const float* gc = grid[g] + c0;
@@ -415,11 +481,12 @@ __global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 6 : (K <= 8 ?
                 const float prod = sg.L[j] * gc[16 * (j / 9) + 4 * ((j / 3) % 3) + (j % 3)];
                 out = out + prod;
             }
-            const int64_t t = 8 * blk + s;
+            const uint32_t off = blk < nblk ? 4u * uint32_t(8 * blk + s) : kBufOob;
             const float pt = gc[16 + 4 + 1];
-            if (!first) pnew[t] = pt;
-            q[t] = out;
-            pq += double(pt) * double(out);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(pt), rs_pnew, off, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(out), rs_q, off, 0, 0);
+            const double pqt = pq + double(pt) * double(out);
+            pq = blk < nblk ? pqt : pq;
         }
         __syncthreads();
     }
@@ -765,19 +832,42 @@ constexpr int kFirstBatch = 24;      // iterations queued before a chain's first
 constexpr int kMaxSegs = 12;         // depths per solve above kSmallRows rows (D <= 10 has at most 10)
 constexpr size_t kHostStates = 1024;  // pinned read-back area: states first, then the segment table
 constexpr size_t kHostPlanMax = 3840;  // then the plans' per-depth largest tile counts (kMaxDepth + 1 int32)
+constexpr size_t kHostPlanSum = 3888;  // and their per-depth sums of tile counts (kMaxDepth + 1 int64)
+static_assert(kHostPlanMax + (kMaxDepth + 1) * 4 <= kHostPlanSum && kHostPlanSum % 8 == 0, "plan read-back layout");
 
 int32_t* plan_maxcount_host(prg_ctx* c) {
     return reinterpret_cast<int32_t*>(static_cast<uint8_t*>(c->cg_host) + kHostPlanMax);
 }
+long long* plan_sumcount_host(prg_ctx* c) {
+    return reinterpret_cast<long long*>(static_cast<uint8_t*>(c->cg_host) + kHostPlanSum);
+}
+
+// Bytes one SpMV launch must move for depth d in this layout: every tile's
+// row-id words and count, the block bases of the distinct blocks the tiles
+// list, r and p_(k-1) read and p_k and q written per row.
+double spmv_plan_bytes(const prg_ctx* c, int d) {
+    const int64_t nt = c->plan_tile_off[d + 1] - c->plan_tile_off[d];
+    return double(nt) * (kTileEntries * sizeof(int16_t) + sizeof(int32_t)) + 4.0 * double(c->plan_cells[d]) +
+           16.0 * double(c->counts[d]);
+}
 
 // SpMV instance: cells per thread covering the largest tile of the chain's
-// depths.  A chain's launch is one round of workgroups, two per CU, shared
-// among its depths by tile count, each pipelining a run of tiles (four per CU
-// measured the same at depth 10).
+// depths.  A chain's launch is one round of workgroups (as many per CU as
+// stay resident), shared among its depths by tile count, each pipelining a
+// run of tiles.
 int spmv_instance(int kcells) { return kcells <= 2 ? 0 : kcells <= 4 ? 1 : kcells <= 5 ? 4 : kcells <= 8 ? 2 : 3; }
 
-// workgroups per CU of an instance (its waves per SIMD / 2)
-int spmv_wg_per_cu(int inst) { return (inst == 0 || inst == 1 || inst == 4) ? 3 : 2; }
+// resident workgroups per CU of an instance with `lds` bytes of cell values
+int spmv_wg_per_cu(int inst, size_t lds) {
+    const void* fn = inst == 0   ? reinterpret_cast<const void*>(&k_cg_spmv<2>)
+                     : inst == 1 ? reinterpret_cast<const void*>(&k_cg_spmv<4>)
+                     : inst == 2 ? reinterpret_cast<const void*>(&k_cg_spmv<8>)
+                     : inst == 4 ? reinterpret_cast<const void*>(&k_cg_spmv<5>)
+                                 : reinterpret_cast<const void*>(&k_cg_spmv<27>);
+    int n = 0;
+    PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fn, kSpmvThreads, lds));
+    return std::max(1, n);
+}
 
 void launch_spmv(int inst, int32_t nwg, size_t lds, hipStream_t s, const Seg* segs, int32_t nseg, int32_t it,
                  float tol, int32_t fixed, int32_t max_iters) {
@@ -812,7 +902,10 @@ void start_plans(prg_ctx* c) {
     int32_t* cnt = c->plan_count.reserve(off + 1);
     int32_t* ub = c->plan_ubase.reserve(size_t(off) * kTileEntries + 1);
     int16_t* rw = c->plan_rowid.reserve(size_t(off) * kTileEntries + 1);
-    int32_t* mx = c->plan_mx.reserve(kMaxDepth + 1);
+    // largest counts (kMaxDepth + 1 int32), then the sums (kMaxDepth + 1 int64, 8-aligned)
+    constexpr int kSumAt = (kMaxDepth + 2) & ~1;
+    int32_t* mx = c->plan_mx.reserve(kSumAt + 2 * (kMaxDepth + 1));
+    long long* sm = reinterpret_cast<long long*>(mx + kSumAt);
     if (s != c->stream) {
         PRG_HIP(hipEventRecord(c->plan_ev, c->stream));
         PRG_HIP(hipStreamWaitEvent(s, c->plan_ev, 0));
@@ -831,10 +924,11 @@ void start_plans(prg_ctx* c) {
     {
         PlanOffsets P{};
         for (int d = 0; d <= D + 1; d++) P.off[d] = c->plan_tile_off[d];
-        k_plan_max<<<unsigned(D + 1), 1024, 0, s>>>(cnt, P, mx);
+        k_plan_max<<<unsigned(D + 1), 1024, 0, s>>>(cnt, P, mx, sm);
         PRG_CHECK_LAUNCH();
     }
     PRG_HIP(hipMemcpyAsync(plan_maxcount_host(c), mx, size_t(D + 1) * 4, hipMemcpyDeviceToHost, s));
+    PRG_HIP(hipMemcpyAsync(plan_sumcount_host(c), sm, size_t(D + 1) * 8, hipMemcpyDeviceToHost, s));
     PRG_HIP(hipEventRecord(c->plan_ev, s));
     c->plan_pending = true;
     c->plan_gen = c->tree_gen;
@@ -845,7 +939,10 @@ void ensure_plans(prg_ctx* c) {
     if (c->plan_gen != c->tree_gen) start_plans(c);
     if (c->plan_pending) {
         PRG_HIP(hipEventSynchronize(c->plan_ev));
-        for (int d = 0; d <= c->D; d++) c->plan_maxcount[d] = plan_maxcount_host(c)[d];
+        for (int d = 0; d <= c->D; d++) {
+            c->plan_maxcount[d] = plan_maxcount_host(c)[d];
+            c->plan_cells[d] = plan_sumcount_host(c)[d];
+        }
         c->plan_pending = false;
     }
 }
@@ -943,7 +1040,7 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
     if (nsegs > kMaxSegs) throw Error{PRG_E_ARG, "too many large depths"};
     static_assert(kMaxSegs * sizeof(CgState) <= kHostStates, "chain states exceed their pinned area");
     static_assert(kHostStates + kMaxSegs * sizeof(Seg) <= kHostPlanMax, "segment table exceeds the pinned staging area");
-    static_assert(kHostPlanMax + (kMaxDepth + 1) * 4 <= 4092, "plan counts exceed the pinned area (its last 4 bytes hold the owned-corner count)");
+    static_assert(kHostPlanSum + (kMaxDepth + 1) * 8 <= 4092, "plan counts exceed the pinned area (its last 4 bytes hold the owned-corner count)");
     CgState* hst = static_cast<CgState*>(c->cg_host);
     Seg* hseg = reinterpret_cast<Seg*>(static_cast<uint8_t*>(c->cg_host) + kHostStates);
     Seg* dseg = reinterpret_cast<Seg*>(c->cg_seg.reserve(std::max<size_t>(1, size_t(nsegs)) * sizeof(Seg)));
@@ -970,14 +1067,15 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
             h.hst = hst + si;
             h.ev = new_event();
             // the chain's SpMV instance (cells per thread for its largest tile)
-            // and workgroups: one round (two or three per CU), shared by tile count
+            // and workgroups: one resident round, shared by tile count
             int32_t kc = 1;
             for (int i = 0; i < cnt; i++)
                 kc = std::max(kc, (8 * std::max(1, c->plan_maxcount[h.depths[i]]) + kSpmvThreads - 1) / kSpmvThreads);
             h.inst = spmv_instance(kc);
-            const int64_t budget = int64_t(spmv_wg_per_cu(h.inst)) * std::max(1, c->num_cu);
+            size_t lds = 0;  // cell values of the largest tile among the chain's depths
+            for (int i = 0; i < cnt; i++) lds = std::max(lds, size_t(8) * 4 * std::max(1, c->plan_maxcount[h.depths[i]]));
+            const int64_t budget = int64_t(spmv_wg_per_cu(h.inst, lds)) * std::max(1, c->num_cu);
             int32_t wg = 0, vwg = 0;
-            size_t lds = 0;
             for (int i = 0; i < cnt; i++) {
                 const int d = h.depths[i];
                 const int64_t n = c->counts[d], nt = tiles_of(n);
@@ -1005,8 +1103,6 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
                 g.st = h.st + i;
                 const Stencil S = stencil_of(d);
                 for (int j = 0; j < 27; j++) g.L[j] = S.L[j];
-                const int mc = std::max(1, c->plan_maxcount[d]);
-                lds = std::max(lds, size_t(8) * 4 * mc);
             }
             h.nwg = wg;
             h.nvwg = vwg;
@@ -1075,10 +1171,14 @@ prg_status solve_all(prg_ctx* c, prg_solve_stats* stats) {
         auto launch_iter = [&](Chain& h, int it) {
             const int32_t cnt = int32_t(h.depths.size());
             int64_t rows = 0;
-            for (int d : h.depths) rows += c->counts[d];
+            double spmv_bytes = 0.0;
+            for (int d : h.depths) {
+                rows += c->counts[d];
+                spmv_bytes += spmv_plan_bytes(c, d);
+            }
             const char* spmv_name = h.depths.back() == D ? "stencil_spmv_dot_finest" : "stencil_spmv_dot";
             {
-                TimedScope ts(c->kt, spmv_name, kSpmvRowBytes * double(rows));
+                TimedScope ts(c->kt, spmv_name, spmv_bytes);
                 h.launched.emplace_back(ts.idx, it);
                 // the instance (cells per thread) that ran, counted as its own record
                 static const char* kInstName[5] = {"spmv_cells_2", "spmv_cells_4", "spmv_cells_8", "spmv_cells_27",
