@@ -55,9 +55,30 @@ __constant__ int8_t c_parent_face[8][6] = {{0, -1, 2, -1, 4, -1}, {-1, 1, 2, -1,
                                            {-1, 1, -1, 3, 4, -1}, {0, -1, 2, -1, -1, 5}, {-1, 1, 2, -1, -1, 5},
                                            {0, -1, -1, 3, -1, 5}, {0, 1, -1, 3, -1, 5}};
 // faceEdges (MarchingCubes.cuh:734-741) as bit masks of edge kinds
-__constant__ uint16_t c_face_edges[6] = {(1 << 4) | (1 << 6) | (1 << 8) | (1 << 10), (1 << 5) | (1 << 7) | (1 << 9) | (1 << 11),
-                                         (1 << 0) | (1 << 2) | (1 << 8) | (1 << 9),  (1 << 1) | (1 << 3) | (1 << 10) | (1 << 11),
-                                         (1 << 0) | (1 << 1) | (1 << 4) | (1 << 5),  (1 << 2) | (1 << 3) | (1 << 6) | (1 << 7)};
+constexpr uint16_t kFaceEdges[6] = {(1 << 4) | (1 << 6) | (1 << 8) | (1 << 10), (1 << 5) | (1 << 7) | (1 << 9) | (1 << 11),
+                                    (1 << 0) | (1 << 2) | (1 << 8) | (1 << 9),  (1 << 1) | (1 << 3) | (1 << 10) | (1 << 11),
+                                    (1 << 0) | (1 << 1) | (1 << 4) | (1 << 5),  (1 << 2) | (1 << 3) | (1 << 6) | (1 << 7)};
+// Faces of a leaf cube touched by its triangles' edges, per cube
+// configuration: bit j when an edge of face j (kFaceEdges) is an edge of one
+// of the configuration's triangles.  Built at compile time from the
+// marching-cubes triangle table.
+struct CubeFaces {
+    uint8_t f[256];
+};
+constexpr CubeFaces make_cube_faces() {
+    constexpr int8_t tri[256][16] = {PRG_MC_TRIANGLE_TABLE_DATA};
+    CubeFaces t{};
+    for (int cube = 0; cube < 256; cube++) {
+        uint32_t em = 0;
+        for (int k = 0; k < 15 && tri[cube][k] >= 0; k++) em |= 1u << tri[cube][k];
+        uint8_t f = 0;
+        for (int j = 0; j < 6; j++)
+            if (em & kFaceEdges[j]) f = uint8_t(f | (1u << j));
+        t.f[cube] = f;
+    }
+    return t;
+}
+__constant__ CubeFaces c_cube_faces = make_cube_faces();
 // childrenVertexKind (MarchingCubes.cuh:720-723), used literally
 __constant__ int8_t c_children_vertex_kind[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 
@@ -87,42 +108,69 @@ __device__ __forceinline__ int64_t face_rec(int64_t X, int j, const uint32_t* __
     return 6 * X + j;
 }
 
-__device__ __forceinline__ bool has_parent_face(int64_t rec, const uint32_t* __restrict__ key,
-                                                const int32_t* __restrict__ parent, const Levels& L, int D) {
-    const int64_t o = rec / 6;
-    const int kind = int(rec % 6);
-    if (parent[o] == -1) return false;
-    const int d = depth_of(o, L, D);
-    const int son = int((key[o] >> (3 * (D - d))) & 7u);
-    return c_parent_face[son][kind] != -1;
-}
-
 // hasSurfaceIntersection marks from the leaf-level triangles (generateTrianglePos,
 // main.cu:2734-2755): every face of a leaf-level node touched by one of its
 // triangle edges, and the faces up the parent chain while the face record
-// has a parent face.
-__global__ void __launch_bounds__(kBlockS) k_face_marks(const float* cval, int64_t MD, int64_t baseD,
+// has a parent face.  The reference walks each chain from its leaf-level
+// node; here the walk goes level by level.  A node's face j is visited when a
+// chain reaches (node, j); the chain moves on to (parent, j) when the face's
+// record has a parent face (hasParentFace, MarchingCubes.cuh:708-717, on the
+// record's owner and kind).  So the leaf pass marks its own faces and ORs the
+// continuing faces into the parent's byte of `up`; each coarser depth, after
+// the finer one, does the same for the faces its byte collected.  The set of
+// marked records is the reference's.
+__device__ __forceinline__ void face_visit(int64_t Y, uint32_t faces, const uint32_t* __restrict__ key,
+                                           const int32_t* __restrict__ neigh, int64_t M,
+                                           const int32_t* __restrict__ parent, int d, int D, uint8_t* inter,
+                                           uint32_t* up) {
+    const uint32_t ky = key[Y];
+    uint32_t cont = 0;
+    for (int j = 0; j < 6; j++) {
+        if (!((faces >> j) & 1u)) continue;
+        // the face record (face_rec): owned by the smaller key of Y and its
+        // neighbour across the face, both at depth d
+        const int o3 = j >> 1, dl = (j & 1) ? 1 : -1;
+        const int32_t nb = neigh[int64_t(nidx(o3 == 0 ? dl : 0, o3 == 1 ? dl : 0, o3 == 2 ? dl : 0)) * M + Y];
+        int64_t owner = Y;
+        int kind = j;
+        uint32_t ko = ky;
+        if (nb != -1) {
+            const uint32_t kn = key[nb];
+            if (kn < ky) {
+                owner = nb;
+                kind = j ^ 1;
+                ko = kn;
+            }
+        }
+        inter[6 * owner + kind] = 1;
+        // only the root (depth 0) has no parent
+        if (d > 0 && c_parent_face[(ko >> (3 * (D - d))) & 7u][kind] != -1) cont |= 1u << j;
+    }
+    if (cont) {
+        const int64_t P = parent[Y];
+        atomicOr(up + (P >> 2), cont << (8 * (P & 3)));
+    }
+}
+
+// (cubes: the leaf-level cube indices the marching cubes pass stored)
+__global__ void __launch_bounds__(kBlockS) k_face_marks(const uint8_t* cubes, int64_t MD, int64_t baseD,
                                                         const uint32_t* key, const int32_t* neigh, int64_t M,
-                                                        const int32_t* parent, Levels L, int D, uint8_t* inter) {
+                                                        const int32_t* parent, int D, uint8_t* inter, uint32_t* up) {
     const int64_t t = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
     if (t >= MD) return;
-    const int cube = cube_of(cval, t);
-    const int n = 3 * tri_count(cube);
-    if (!n) return;
-    uint32_t em = 0;
-    for (int k = 0; k < n; k++) em |= 1u << c_tri[cube][k];
-    const int64_t X = baseD + t;
-    for (int j = 0; j < 6; j++) {
-        if (!(em & c_face_edges[j])) continue;
-        int64_t rec = face_rec(X, j, key, neigh, M);
-        inter[rec] = 1;
-        int64_t pnode = parent[X];
-        while (has_parent_face(rec, key, parent, L, D)) {
-            rec = face_rec(pnode, j, key, neigh, M);
-            pnode = parent[pnode];
-            inter[rec] = 1;
-        }
-    }
+    const uint32_t faces = c_cube_faces.f[cubes[t]];
+    if (faces) face_visit(baseD + t, faces, key, neigh, M, parent, D, D, inter, up);
+}
+
+// the faces the chains carried up to the nodes of depth d (nodes base .. base + n)
+__global__ void __launch_bounds__(kBlockS) k_face_marks_up(int64_t base, int64_t n, int d, const uint32_t* key,
+                                                           const int32_t* neigh, int64_t M, const int32_t* parent,
+                                                           int D, uint8_t* inter, uint32_t* up) {
+    const int64_t t = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
+    if (t >= n) return;
+    const int64_t Y = base + t;
+    const uint32_t faces = (up[Y >> 2] >> (8 * (Y & 3))) & 0xffu;
+    if (faces) face_visit(Y, faces, key, neigh, M, parent, d, D, inter, up);
 }
 
 __global__ void __launch_bounds__(kBlockS) k_flag_coarse_leaves(const int32_t* child_base, int64_t n, int32_t* flag) {
@@ -1697,14 +1745,28 @@ void extract_subdivision(prg_ctx* c) {
     int32_t* C = cnt.reserve(kCntSlots);
     PRG_HIP(hipMemsetAsync(C, 0, kCntSlots * 4, s));
 
-    // faces touched by the leaf-level surface
+    // faces touched by the leaf-level surface, and up the parent chains
     inter.reserve(6 * M);
     PRG_HIP(hipMemsetAsync(inter.get(), 0, size_t(6 * M), s));
     {
-        TimedScope ts(kt, "face_marks", 32.0 * double(MD));
-        k_face_marks<<<grid_for(MD, kBlockS), kBlockS, 0, s>>>(c->corner_vals.get(), MD, baseD, c->key.get(),
-                                                                c->neigh.get(), M, c->parent.get(), Lv, D, inter.get());
+        PoolBuf<uint32_t> up{&c->pool[140]};
+        up.reserve((M + 3) / 4);
+        PRG_HIP(hipMemsetAsync(up.get(), 0, size_t((M + 3) / 4) * 4, s));
+        // bytes: the cube index of each leaf-level node and the face byte of
+        // each coarser node, read once (the face visits' gathers come on top)
+        TimedScope ts(kt, "face_marks", double(M));
+        // the leaf-level cube indices of the marching cubes pass that ran just before
+        PoolBuf<uint8_t> cubes{&c->pool[94]};
+        k_face_marks<<<grid_for(MD, kBlockS), kBlockS, 0, s>>>(cubes.get(), MD, baseD, c->key.get(),
+                                                                c->neigh.get(), M, c->parent.get(), D, inter.get(),
+                                                                up.get());
         PRG_CHECK_LAUNCH();
+        for (int d = D - 1; d >= 0; d--) {
+            k_face_marks_up<<<grid_for(c->counts[d], kBlockS), kBlockS, 0, s>>>(
+                c->bases[d], c->counts[d], d, c->key.get(), c->neigh.get(), M, c->parent.get(), D, inter.get(),
+                up.get());
+            PRG_CHECK_LAUNCH();
+        }
     }
     // coarse leaves (counts stay on the device; launches are sized by upper bounds)
     flag.reserve(baseD);
