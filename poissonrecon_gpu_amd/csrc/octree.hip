@@ -529,9 +529,8 @@ __global__ void __launch_bounds__(kSmallThreads) k_levels_small(LevelArrays A, S
                                                                 int64_t M, int64_t npoints, int64_t MD,
                                                                 uint32_t* key, int32_t* pidx, int32_t* pnum,
                                                                 int32_t* parent, int32_t* child_base, int32_t* didx,
-                                                                int32_t* dnum, int32_t* neigh, int32_t* pnt) {
+                                                                int32_t* dnum, int32_t* pnt) {
     const int tid = threadIdx.x;
-    if (tid < 27) neigh[int64_t(tid) * M] = (tid == 13) ? 0 : -1;
     if (tid == 0) {
         key[0] = 0;
         pidx[0] = 0;
@@ -610,8 +609,22 @@ void unpack_points(prg_ctx* c) {
     PRG_CHECK_LAUNCH();
 }
 
+// The 27-plane neighbour table (plane j holds neighbour j of every node), for
+// callers that read it (PRG_BUF_NEIGHS and its view); the pipeline's own
+// kernels take neighbours from the parent-neighbour rows.  The root's only
+// neighbour is itself (main.cu:803-808).
+namespace {
+__global__ void k_neighbors_root(int32_t* neigh, int64_t M) {
+    const int j = threadIdx.x;
+    if (j < 27) neigh[int64_t(j) * M] = (j == 13) ? 0 : -1;
+}
+}  // namespace
+
 void ensure_neighbours(prg_ctx* c) {
     if (c->neigh_ready) return;
+    c->neigh.reserve(size_t(27) * c->M);
+    k_neighbors_root<<<1, 64, 0, c->stream>>>(c->neigh.get(), c->M);
+    PRG_CHECK_LAUNCH();
     for (int d = 1; d <= c->D; d++) {
         TimedScope ts(c->kt, "neighbors", 128.0 * double(c->counts[d]));
         k_neighbors<<<grid_for(c->counts[d], kBlock), kBlock, 0, c->stream>>>(
@@ -772,7 +785,6 @@ void build_octree(prg_ctx* c) {
     int32_t* CB = c->child_base.reserve(M);
     int32_t* DI = c->didx.reserve(M);
     int32_t* DN = c->dnum.reserve(M);
-    int32_t* NB = c->neigh.reserve(size_t(27) * M);
     const int64_t MD = counts[D];
     c->pnt_off.assign(D + 2, 0);
     for (int d = 1; d <= D; d++) c->pnt_off[d + 1] = c->pnt_off[d] + counts[d] / 8;
@@ -790,7 +802,7 @@ void build_octree(prg_ctx* c) {
         }
         for (int d = 0; d <= D + 1; d++) SL.pnt_off[d] = c->pnt_off[d];
         TimedScope ts(kt, "levels_small", 44.0 * double(bases[df] + counts[df]));
-        k_levels_small<<<1, kSmallThreads, 0, s>>>(LA, SL, df, D, M, n, MD, K, PI, PN, PA, CB, DI, DN, NB, PT);
+        k_levels_small<<<1, kSmallThreads, 0, s>>>(LA, SL, df, D, M, n, MD, K, PI, PN, PA, CB, DI, DN, PT);
         PRG_CHECK_LAUNCH();
     }
     {
@@ -821,7 +833,7 @@ void build_octree(prg_ctx* c) {
     }
     // 4. parent-neighbour tables (depths 1..D), top-down; the 27-plane
     // neighbour table (depths 1..D) is built on demand
-    c->neigh_ready = (D == 0);
+    c->neigh_ready = false;
     for (int d = df + 1; d <= D; d++) {
         const int64_t nb = counts[d] / 8;
         TimedScope ts(kt, "parent_neighbor_table", 120.0 * double(nb));

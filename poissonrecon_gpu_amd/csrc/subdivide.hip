@@ -43,6 +43,7 @@
 
 #include "ctx.h"
 #include "mc_common.h"
+#include "tile.h"
 
 namespace prg {
 namespace {
@@ -96,14 +97,29 @@ __device__ __forceinline__ int depth_of(int64_t i, const Levels& L, int D) {
 
 __device__ __forceinline__ int nidx(int dx, int dy, int dz) { return 9 * (dx + 1) + 3 * (dy + 1) + (dz + 1); }
 
-// The face record holding face j of node X (initFaceArray / maintainFaceNodePointer,
-// main.cu:2827-2955): owned by the smaller key of X and its neighbour across
-// the face, numbered (owner, kind seen from the owner).
-__device__ __forceinline__ int64_t face_rec(int64_t X, int j, const uint32_t* __restrict__ key,
-                                            const int32_t* __restrict__ neigh, int64_t M) {
+// Neighbour j = nidx(dx, dy, dz) of node i at depth d, or -1 (computeNeighbor,
+// main.cu:492-509): child LUTchild[s][j] of the parent's neighbour
+// LUTparent[s][j], read from the sibling block's parent-neighbour row (the
+// level-local child base of each of the parent's 27 neighbours).  Nodes of a
+// depth come in sibling blocks of 8 in child order, so the slot s is the
+// node's level-local index mod 8.  The root's only neighbour is itself.
+__device__ __forceinline__ int32_t neighbour_of(const int32_t* __restrict__ pnt, const Levels& L, int d, int64_t i,
+                                                int j) {
+    if (d == 0) return j == 13 ? int32_t(i) : -1;
+    const int64_t loc = i - L.base[d];
+    int entry, child;
+    tile_neighbour(int(loc & 7), j, entry, child);
+    const int32_t cb = pnt[27 * (L.pnt_off[d] + (loc >> 3)) + entry];
+    return cb < 0 ? -1 : int32_t(L.base[d] + cb + child);
+}
+
+// The face record holding face j of node X at depth d (initFaceArray /
+// maintainFaceNodePointer, main.cu:2827-2955): owned by the smaller key of X
+// and its neighbour across the face, numbered (owner, kind seen from the owner).
+__device__ __forceinline__ int64_t face_rec(int64_t X, int d, int j, const uint32_t* __restrict__ key,
+                                            const int32_t* __restrict__ pnt, const Levels& L) {
     const int o = j >> 1, dl = (j & 1) ? 1 : -1;
-    const int jj = nidx(o == 0 ? dl : 0, o == 1 ? dl : 0, o == 2 ? dl : 0);
-    const int32_t nb = neigh[int64_t(jj) * M + X];
+    const int32_t nb = neighbour_of(pnt, L, d, X, nidx(o == 0 ? dl : 0, o == 1 ? dl : 0, o == 2 ? dl : 0));
     if (nb != -1 && key[nb] < key[X]) return 6 * int64_t(nb) + (j ^ 1);
     return 6 * X + j;
 }
@@ -120,7 +136,7 @@ __device__ __forceinline__ int64_t face_rec(int64_t X, int j, const uint32_t* __
 // the finer one, does the same for the faces its byte collected.  The set of
 // marked records is the reference's.
 __device__ __forceinline__ void face_visit(int64_t Y, uint32_t faces, const uint32_t* __restrict__ key,
-                                           const int32_t* __restrict__ neigh, int64_t M,
+                                           const int32_t* __restrict__ pnt, const Levels& L,
                                            const int32_t* __restrict__ parent, int d, int D, uint8_t* inter,
                                            uint32_t* up) {
     const uint32_t ky = key[Y];
@@ -130,7 +146,7 @@ __device__ __forceinline__ void face_visit(int64_t Y, uint32_t faces, const uint
         // the face record (face_rec): owned by the smaller key of Y and its
         // neighbour across the face, both at depth d
         const int o3 = j >> 1, dl = (j & 1) ? 1 : -1;
-        const int32_t nb = neigh[int64_t(nidx(o3 == 0 ? dl : 0, o3 == 1 ? dl : 0, o3 == 2 ? dl : 0)) * M + Y];
+        const int32_t nb = neighbour_of(pnt, L, d, Y, nidx(o3 == 0 ? dl : 0, o3 == 1 ? dl : 0, o3 == 2 ? dl : 0));
         int64_t owner = Y;
         int kind = j;
         uint32_t ko = ky;
@@ -154,23 +170,23 @@ __device__ __forceinline__ void face_visit(int64_t Y, uint32_t faces, const uint
 
 // (cubes: the leaf-level cube indices the marching cubes pass stored)
 __global__ void __launch_bounds__(kBlockS) k_face_marks(const uint8_t* cubes, int64_t MD, int64_t baseD,
-                                                        const uint32_t* key, const int32_t* neigh, int64_t M,
+                                                        const uint32_t* key, const int32_t* pnt, Levels L,
                                                         const int32_t* parent, int D, uint8_t* inter, uint32_t* up) {
     const int64_t t = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
     if (t >= MD) return;
     const uint32_t faces = c_cube_faces.f[cubes[t]];
-    if (faces) face_visit(baseD + t, faces, key, neigh, M, parent, D, D, inter, up);
+    if (faces) face_visit(baseD + t, faces, key, pnt, L, parent, D, D, inter, up);
 }
 
 // the faces the chains carried up to the nodes of depth d (nodes base .. base + n)
 __global__ void __launch_bounds__(kBlockS) k_face_marks_up(int64_t base, int64_t n, int d, const uint32_t* key,
-                                                           const int32_t* neigh, int64_t M, const int32_t* parent,
+                                                           const int32_t* pnt, Levels L, const int32_t* parent,
                                                            int D, uint8_t* inter, uint32_t* up) {
     const int64_t t = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
     if (t >= n) return;
     const int64_t Y = base + t;
     const uint32_t faces = (up[Y >> 2] >> (8 * (Y & 3))) & 0xffu;
-    if (faces) face_visit(Y, faces, key, neigh, M, parent, d, D, inter, up);
+    if (faces) face_visit(Y, faces, key, pnt, L, parent, d, D, inter, up);
 }
 
 __global__ void __launch_bounds__(kBlockS) k_flag_coarse_leaves(const int32_t* child_base, int64_t n, int32_t* flag) {
@@ -187,11 +203,12 @@ __global__ void __launch_bounds__(kBlockS) k_compact(const int32_t* flag, const 
 // (initVertexOwner, main.cu:1438-1498); marks each (owner, corner) needed.
 // ncl (device) coarse leaves in cl.
 __global__ void __launch_bounds__(kBlockS) k_coarse_corner_owner(const int32_t* cl, const int32_t* ncl,
-                                                                 const uint32_t* key, const int32_t* neigh, int64_t M,
-                                                                 int32_t* oref, int32_t* need) {
+                                                                 const uint32_t* key, const int32_t* pnt, Levels L,
+                                                                 int D, int32_t* oref, int32_t* need) {
     const int64_t g = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
     if (g >= 8 * int64_t(*ncl)) return;
     const int64_t X = cl[g >> 3];
+    const int dX = depth_of(X, L, D);
     const int j = int(g & 7);
     const int cx = j & 1, cy = (j >> 1) & 1, cz = (j >> 2) & 1;
     uint32_t best = key[X];
@@ -201,7 +218,7 @@ __global__ void __launch_bounds__(kBlockS) k_coarse_corner_owner(const int32_t* 
         for (int b = 0; b < 2; b++)
             for (int e = 0; e < 2; e++) {
                 if (!(a | b | e)) continue;
-                const int32_t nb = neigh[int64_t(nidx(a ? 2 * cx - 1 : 0, b ? 2 * cy - 1 : 0, e ? 2 * cz - 1 : 0)) * M + X];
+                const int32_t nb = neighbour_of(pnt, L, dX, X, nidx(a ? 2 * cx - 1 : 0, b ? 2 * cy - 1 : 0, e ? 2 * cz - 1 : 0));
                 if (nb == -1) continue;
                 if (key[nb] < best) {
                     best = key[nb];
@@ -249,7 +266,7 @@ __global__ void __launch_bounds__(kBlockS) k_coarse_corner_value(const int32_t* 
 // over all n slots counts the roots.
 __global__ void __launch_bounds__(kBlockS) k_coarse_select(const int32_t* cl, const int32_t* ncl, int64_t n,
                                                            const int32_t* oref, const int32_t* nid, const float* cv,
-                                                           const uint32_t* key, const int32_t* neigh, int64_t M,
+                                                           const uint32_t* key, const int32_t* pnt, Levels L, int D,
                                                            const uint8_t* inter, int32_t* rflag) {
     const int64_t c = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
     if (c >= n) return;
@@ -265,8 +282,9 @@ __global__ void __launch_bounds__(kBlockS) k_coarse_select(const int32_t* cl, co
     for (int j = 1; j < 8; j++)
         if (sign * cv[nid[oref[8 * c + j]]] < 0) tri = true;
     bool hit = false;
+    const int dX = depth_of(X, L, D);
     for (int j = 0; j < 6; j++)
-        if (inter[face_rec(X, j, key, neigh, M)]) hit = true;
+        if (inter[face_rec(X, dX, j, key, pnt, L)]) hit = true;
     rflag[c] = (tri || hit) ? 1 : 0;
 }
 
@@ -304,8 +322,6 @@ __global__ void __launch_bounds__(kBlockS) k_root_of(const int32_t* roots, const
 
 struct SubArgs {
     const uint32_t* key;
-    const int32_t* neigh;  // 27 planes
-    int64_t M;
     const int32_t* child_base;
     const int32_t* parent;
     const int32_t* pnt;
@@ -401,7 +417,7 @@ __global__ void __launch_bounds__(kWave) k_root_records(SubArgs A, int32_t* rec)
     const int r = depth_of(rho, A.L, D);
     int32_t* R = rec + int64_t(q) * kRecWords;
     if (lane < 27) {
-        const int32_t nb = A.neigh[int64_t(lane) * A.M + rho];
+        const int32_t nb = neighbour_of(A.pnt, A.L, r, rho, lane);
         R[kRecNB + lane] = nb;
         R[kRecNBK + lane] = nb < 0 ? -1 : A.child_base[nb];
         R[kRecNBR + lane] = (nb < 0 || r < kFinerDepth) ? -1 : A.root_of[nb];
@@ -1719,7 +1735,6 @@ __global__ void __launch_bounds__(kWave) k_brick_triangles(SubArgs A, const int3
 int ensure_coarse_rows(prg_ctx* c, hipStream_t s);
 
 void extract_subdivision(prg_ctx* c) {
-    ensure_neighbours(c);
     ensure_tables(c);
     StageTimer st(c, PRG_STAGE_SUBDIVIDE);
     hipStream_t s = c->stream;
@@ -1757,13 +1772,12 @@ void extract_subdivision(prg_ctx* c) {
         TimedScope ts(kt, "face_marks", double(M));
         // the leaf-level cube indices of the marching cubes pass that ran just before
         PoolBuf<uint8_t> cubes{&c->pool[94]};
-        k_face_marks<<<grid_for(MD, kBlockS), kBlockS, 0, s>>>(cubes.get(), MD, baseD, c->key.get(),
-                                                                c->neigh.get(), M, c->parent.get(), D, inter.get(),
-                                                                up.get());
+        k_face_marks<<<grid_for(MD, kBlockS), kBlockS, 0, s>>>(cubes.get(), MD, baseD, c->key.get(), c->pnt.get(),
+                                                                Lv, c->parent.get(), D, inter.get(), up.get());
         PRG_CHECK_LAUNCH();
         for (int d = D - 1; d >= 0; d--) {
             k_face_marks_up<<<grid_for(c->counts[d], kBlockS), kBlockS, 0, s>>>(
-                c->bases[d], c->counts[d], d, c->key.get(), c->neigh.get(), M, c->parent.get(), D, inter.get(),
+                c->bases[d], c->counts[d], d, c->key.get(), c->pnt.get(), Lv, c->parent.get(), D, inter.get(),
                 up.get());
             PRG_CHECK_LAUNCH();
         }
@@ -1785,7 +1799,7 @@ void extract_subdivision(prg_ctx* c) {
     cv.reserve(8 * baseD);
     PRG_HIP(hipMemsetAsync(need.get(), 0, size_t(8 * baseD) * 4, s));
     k_coarse_corner_owner<<<grid_for(8 * baseD, kBlockS), kBlockS, 0, s>>>(cl.get(), C + kCntCoarse, c->key.get(),
-                                                                            c->neigh.get(), M, oref.get(), need.get());
+                                                                            c->pnt.get(), Lv, D, oref.get(), need.get());
     PRG_CHECK_LAUNCH();
     scan_exclusive(need.get(), nid.get(), 8 * baseD, C + kCntCorners, s, c->scratch, kt);
     k_compact<<<grid_for(8 * baseD, kBlockS), kBlockS, 0, s>>>(need.get(), nid.get(), 8 * baseD, clist.get());
@@ -1807,8 +1821,8 @@ void extract_subdivision(prg_ctx* c) {
     rleaves.reserve(baseD);
     loff.reserve(baseD);
     k_coarse_select<<<grid_for(baseD, kBlockS), kBlockS, 0, s>>>(cl.get(), C + kCntCoarse, baseD, oref.get(),
-                                                                  nid.get(), cv.get(), c->key.get(), c->neigh.get(), M,
-                                                                  inter.get(), rflag.get());
+                                                                  nid.get(), cv.get(), c->key.get(), c->pnt.get(), Lv,
+                                                                  D, inter.get(), rflag.get());
     PRG_CHECK_LAUNCH();
     scan_exclusive(rflag.get(), roff.get(), baseD, C + kCntRoots, s, c->scratch, kt);
     k_compact<<<grid_for(baseD, kBlockS), kBlockS, 0, s>>>(rflag.get(), roff.get(), baseD, ridx.get());
@@ -1832,7 +1846,7 @@ void extract_subdivision(prg_ctx* c) {
     PRG_CHECK_LAUNCH();
 
     PoolBuf<float> btab{&c->pool[91]};
-    SubArgs A{c->key.get(),      c->neigh.get(),  M,
+    SubArgs A{c->key.get(),
                     c->child_base.get(), c->parent.get(), c->pnt.get(),
                     Lv,                c->x.get(),      c->d_base_fn.get(),
                     reinterpret_cast<const float4*>(btab.get()), c->coarse_rows.get(), D,
