@@ -368,13 +368,14 @@ constexpr int kXBlk = 2304;
 
 // LDS of one brick's wave: the root's neighbourhood, the brick's basis
 // tables and blocks (BrickCore), and for the evaluation its corner values.
-template <int kXBCap, int kTP = kP>
+template <int kXBCap, int kTP = kP, bool kHasT = true>
 struct BrickCoreT {
-    static constexpr int kTPts = kTP;  // grid coordinates per axis in T
+    static constexpr int kTPts = kTP;  // grid coordinates per axis (of T, or of the screen's ranges)
     // per depth d, axis a and grid coordinate i: the corner basis table entry
     // of the owner (own_cell) of grid coordinate org[a] + i, (d, a, i) at
-    // d * 3 kP + a kP + i
-    float4 T[(kMaxDepth + 1) * 3 * kTP];
+    // d * 3 kP + a kP + i (the screens reduce the entries straight from the
+    // global table and keep no T)
+    float4 T[kHasT ? (kMaxDepth + 1) * 3 * kTP : 1];
     float X[(kMaxDepth + 1) * 27];     // the root's chain: neighbour values at depths 0..r
     float XB[kXBCap];                  // per depth below the root: its block (xb_off)
     int32_t xb_off[kMaxDepth + 1];     // start of depth dd's block in XB, or -1 (no real node in it)
@@ -390,10 +391,10 @@ using BrickCore = BrickCoreT<kXBlk>;
 // where the brick lies in one cell, and the first finer one); bricks that
 // need more go to the exact certificate, whose LDS holds every block.
 constexpr int kXBlkScreen = 512;
-using BrickScreenLds = BrickCoreT<kXBlkScreen>;
+using BrickScreenLds = BrickCoreT<kXBlkScreen, kP, false>;
 // The super-brick screen's: 2x2x2 bricks, 17 grid coordinates per axis.
 constexpr int kPSuper = 2 * (1 << kBrickLog) + 1;
-using BrickSuperLds = BrickCoreT<kXBlkScreen, kPSuper>;
+using BrickSuperLds = BrickCoreT<kXBlkScreen, kPSuper, false>;
 
 template <int kXBCap>
 struct BrickValsT : BrickCoreT<kXBCap> {
@@ -1015,6 +1016,77 @@ __device__ __forceinline__ float brick_abs_bound(const Brick& B, L& S, int D) {
     return 1.001f * bound;
 }
 
+// brick_abs_bound for the screens, and the ranges brick_interval_sign needs,
+// from the corner basis table in global memory (the screens keep no T): one
+// lane per (depth, axis) reads the owner's entry for each of the brick's grid
+// coordinates (brick_tables' entries), takes the largest |basis value| (for
+// tmax) and, at depths d <= D - kb, each block slot's weight range over the
+// points (wlo, whi).  The figures are those brick_abs_bound and the range
+// pass of the T path compute.
+template <class L>
+__device__ __forceinline__ float brick_screen_bound(const SubArgs& A, const Brick& B, L& S) {
+    const int D = A.D, lane = threadIdx.x, P = (1 << B.kb) + 1, dlast = D - B.kb;
+    if (lane <= kMaxDepth) S.tmax[lane] = S.xsum[lane] = 0.0f;
+    __syncthreads();
+    if (lane < 3 * (D + 1)) {
+        const int d = lane / 3, a = lane % 3;
+        if (d <= B.r || S.xb_off[d] >= 0) {
+            // (arithmetic selects: an index by a would put B in scratch memory)
+            const int32_t og = (a == 0) * B.org[0] + (a == 1) * B.org[1] + (a == 2) * B.org[2];
+            const int32_t r0 = (a == 0) * B.R0[0] + (a == 1) * B.R0[1] + (a == 2) * B.R0[2];
+            float mx = 0.0f, lo[4], hi[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                lo[m] = 3.4e38f;
+                hi[m] = -3.4e38f;
+            }
+#pragma unroll
+            for (int i = 0; i < L::kTPts; i++) {
+                if (i < P) {
+                    const int32_t g = og + i, o = own_cell(g, r0);
+                    const float4 t = A.btab[((int64_t(d) << D) + o) * 2 + (g - o)];
+                    mx = fmaxf(mx, fmaxf(fabsf(t.x), fmaxf(fabsf(t.y), fabsf(t.z))));
+                    // the owner's ancestor is the brick's (slots 1..3) or the one below (0..2)
+                    const bool below = d > B.r && (o >> (D - d)) != (og >> (D - d));
+                    const float w0 = below ? t.x : 0.0f, w1 = below ? t.y : t.x, w2 = below ? t.z : t.y,
+                                w3 = below ? 0.0f : t.z;
+                    lo[0] = fminf(lo[0], w0), hi[0] = fmaxf(hi[0], w0);
+                    lo[1] = fminf(lo[1], w1), hi[1] = fmaxf(hi[1], w1);
+                    lo[2] = fminf(lo[2], w2), hi[2] = fmaxf(hi[2], w2);
+                    lo[3] = fminf(lo[3], w3), hi[3] = fmaxf(hi[3], w3);
+                }
+            }
+            // |basis values| (non-negative floats order as their bit patterns)
+            atomicMax(reinterpret_cast<int*>(&S.tmax[d]), __float_as_int(mx));
+            if (d <= dlast) {
+#pragma unroll
+                for (int m = 0; m < 4; m++) {
+                    S.wlo[12 * d + 4 * a + m] = lo[m];
+                    S.whi[12 * d + 4 * a + m] = hi[m];
+                }
+            }
+        }
+    }
+    // |coefficients|: the chain, then the blocks (as brick_abs_bound)
+    for (int e = lane; e < 27 * B.r; e += kWave) atomicAdd(&S.xsum[1 + e / 27], fabsf(S.X[27 + e]));
+    if (lane == 0) atomicAdd(&S.xsum[0], fabsf(S.X[0]));
+    for (int d = B.r + 1; d <= D; d++) {
+        if (S.xb_off[d] < 0) continue;
+        const int W = d <= dlast ? 4 : block_width(D, B.kb, d), n = W * W * W;
+        float sx = 0.0f;
+        for (int e = lane; e < n; e += kWave) sx += fabsf(S.XB[S.xb_off[d] + e]);
+        if (sx != 0.0f) atomicAdd(&S.xsum[d], sx);
+    }
+    __syncthreads();
+    float bound = 0.0f;
+#pragma unroll
+    for (int d = 0; d <= kMaxDepth; d++) {
+        const float tm = S.tmax[d];
+        if (d <= D) bound += S.xsum[d] * (tm * tm * tm);
+    }
+    return 1.001f * bound;
+}
+
 // The basis weights over the 4 block slots along axis ax for grid coordinate
 // org[ax] + i at depth d <= D - kb: the owner's three values, placed at its
 // ancestor's offset (-1 or 0) from the brick's ancestor.
@@ -1051,43 +1123,8 @@ __device__ __forceinline__ void axis_weights4(const Brick& B, const L& S, int D,
 // the bound of A (plus the rounding of the subtraction).
 template <class L>
 __device__ __forceinline__ int brick_interval_sign(const SubArgs& A, const Brick& B, L& S, float abound) {
-    const int D = A.D, lane = threadIdx.x, P = (1 << B.kb) + 1, dlast = D - B.kb;
-    // one lane per (depth, axis): the four slots' ranges over the points
-    if (lane < 3 * (dlast + 1)) {
-        const int d = lane / 3, a = lane % 3;
-        if (d <= B.r || S.xb_off[d] >= 0) {
-            // (arithmetic selects: an index by a would put B in scratch memory)
-            const int32_t og = (a == 0) * B.org[0] + (a == 1) * B.org[1] + (a == 2) * B.org[2];
-            const int32_t r0 = (a == 0) * B.R0[0] + (a == 1) * B.R0[1] + (a == 2) * B.R0[2];
-            float lo[4], hi[4];
-#pragma unroll
-            for (int m = 0; m < 4; m++) {
-                lo[m] = 3.4e38f;
-                hi[m] = -3.4e38f;
-            }
-#pragma unroll
-            for (int i = 0; i < L::kTPts; i++) {
-                if (i < P) {
-                    const float4 t = S.T[3 * L::kTPts * d + L::kTPts * a + i];
-                    // the owner's ancestor is the brick's (slots 1..3) or the one below (0..2)
-                    const int32_t o = own_cell(og + i, r0);
-                    const bool below = d > B.r && (o >> (D - d)) != (og >> (D - d));
-                    const float w0 = below ? t.x : 0.0f, w1 = below ? t.y : t.x, w2 = below ? t.z : t.y,
-                                w3 = below ? 0.0f : t.z;
-                    lo[0] = fminf(lo[0], w0), hi[0] = fmaxf(hi[0], w0);
-                    lo[1] = fminf(lo[1], w1), hi[1] = fmaxf(hi[1], w1);
-                    lo[2] = fminf(lo[2], w2), hi[2] = fmaxf(hi[2], w2);
-                    lo[3] = fminf(lo[3], w3), hi[3] = fmaxf(hi[3], w3);
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < 4; m++) {
-                S.wlo[12 * d + 4 * a + m] = lo[m];
-                S.whi[12 * d + 4 * a + m] = hi[m];
-            }
-        }
-    }
-    __syncthreads();
+    const int D = A.D, lane = threadIdx.x, dlast = D - B.kb;
+    // (the slot ranges wlo, whi come from brick_screen_bound, which ended with a barrier)
     const int m = lane >> 4, n = (lane >> 2) & 3, l = lane & 3;
     float lo = 0.0f, hi = 0.0f;
 #pragma unroll
@@ -1212,7 +1249,7 @@ __device__ __forceinline__ bool brick_in_group(const Brick& B, const L& S) {
 template <class L>
 __device__ __forceinline__ Certificate brick_certify_interval(const SubArgs& A, const Brick& B, L& S) {
     Certificate c;
-    c.sign = brick_interval_sign(A, B, S, brick_abs_bound(B, S, A.D));
+    c.sign = brick_interval_sign(A, B, S, brick_screen_bound(A, B, S));
     c.deps = (c.sign != 0 && brick_in_group(B, S)) ? brick_group_deps(B, S, A.D) : 0;
     return c;
 }
@@ -1391,7 +1428,7 @@ __device__ __forceinline__ void record_certificate(const SubArgs& A, const Brick
 // bricks (counts 0, its sign); the bricks of the others, and of super-bricks
 // that touch another root of the group (whose leaves may own corners) or
 // whose blocks the LDS cannot hold, are listed for the brick screen.
-__global__ void __launch_bounds__(kWave) k_brick_screen_super(SubArgs A, const int32_t* smap, const int32_t* soff,
+__global__ void __launch_bounds__(kWave, 5) k_brick_screen_super(SubArgs A, const int32_t* smap, const int32_t* soff,
                                                               int32_t* bnv, int32_t* bnt, int8_t* sign,
                                                               int32_t* list2, int32_t* cnt, int certify) {
     __shared__ BrickSuperLds S;
@@ -1414,10 +1451,9 @@ __global__ void __launch_bounds__(kWave) k_brick_screen_super(SubArgs A, const i
         __syncthreads();  // the previous super-brick's LDS reads are done
         brick_setup(A, b0, B, S, true, q_loaded, 1);
         q_loaded = B.q;
-        brick_tables(A, B, S);
         int sgn = 0;
         if (certify && !brick_in_group(B, S) && brick_blocks(A, B, S))
-            sgn = brick_interval_sign(A, B, S, brick_abs_bound(B, S, A.D));
+            sgn = brick_interval_sign(A, B, S, brick_screen_bound(A, B, S));
         if (sgn != 0) {
             if (lane < 8) {
                 bnv[b0 + lane] = 0;
@@ -1472,7 +1508,7 @@ __global__ void __launch_bounds__(kBlockS) k_super_map(const int32_t* soff, cons
 // 0; a certified brick keeps them unless a brick it depends on has another
 // sign (k_brick_deps); the others are listed for the exact certificate
 // (k_brick_exact).
-__global__ void __launch_bounds__(kWave) k_brick_screen(SubArgs A, const int32_t* list2, int32_t* bnv, int32_t* bnt,
+__global__ void __launch_bounds__(kWave, 5) k_brick_screen(SubArgs A, const int32_t* list2, int32_t* bnv, int32_t* bnt,
                                                         int8_t* sign, int32_t* xlist, int32_t* dbricks,
                                                         int32_t* dpairs, int32_t* cnt, int certify) {
     __shared__ BrickScreenLds S;
@@ -1494,7 +1530,6 @@ __global__ void __launch_bounds__(kWave) k_brick_screen(SubArgs A, const int32_t
         __syncthreads();  // the previous brick's LDS reads are done
         brick_setup(A, b, B, S, true, q_loaded);
         q_loaded = B.q;
-        brick_tables(A, B, S);
         Certificate c;
         c.sign = 0;
         c.deps = 0;
