@@ -233,23 +233,96 @@ __global__ void __launch_bounds__(kBlockS) k_coarse_corner_owner(const int32_t* 
 // computeVertexImplicitFunctionValue (main.cu:2259-2326) for a corner of a
 // node above the leaf level: the owner's ancestor chain, then the descent
 // through children[childrenVertexKind[corner]] down to D.
+// chain_value for lanes that start on different nodes at different depths
+// (the owners of coarse-leaf corners, in node order).  Each lane adds its
+// depths d, d - 1, ..., 1 in that order, then the root, as chain_value does.
+// The deepest lanes step first, so the wave's lanes reach each depth
+// together; once every walking lane stands on the same node the rest of the
+// chain runs on one wave-uniform index (its row and coefficients come through
+// the scalar cache once per wave).  A node with leaf-level slots below it
+// (dnum > 0) has a coarse row, whose sum is level_contrib's (row_sum); the
+// others gather through the parent-neighbour rows.
+__device__ __forceinline__ float coarse_chain_value(int64_t nd, int d, bool valid, float px, float py, float pz,
+                                                    const uint32_t* __restrict__ key, const int32_t* __restrict__ pnt,
+                                                    const Levels& L, const int32_t* __restrict__ parent,
+                                                    const int32_t* __restrict__ dnum, const float* __restrict__ x,
+                                                    const float* __restrict__ bf, int D, const float* __restrict__ rows,
+                                                    int rows_depth) {
+    float val = 0.0f;
+    int dm = 0;  // the deepest walking lane's depth
+    for (int t = kMaxDepth; t >= 1; t--)
+        if (__builtin_amdgcn_ballot_w64(valid && d == t) != 0) {
+            dm = t;
+            break;
+        }
+    for (; dm > 0; dm--) {
+        const bool walking = valid && d > 0;
+        const uint64_t wl = __builtin_amdgcn_ballot_w64(walking);
+        const int first = int(__builtin_ctzll(wl));  // (wl != 0 while dm > 0)
+        const int64_t n0 = __builtin_amdgcn_readlane(int32_t(nd), first);
+        if (__builtin_amdgcn_ballot_w64(walking && (d != dm || nd != n0)) == 0) break;
+        if (walking && d == dm) {
+            if (dm <= rows_depth && dnum[nd] > 0) {
+                const float* R = rows + kRowFloats * (nd - L.base[1]);
+                float bx[3], by[3], bz[3];
+                node_basis(nd, dm, px, py, pz, key, bf, D, bx, by, bz);
+                val = row_sum(R, bx, by, bz, val);
+            } else {
+                val = level_contrib(nd, dm, px, py, pz, key, pnt, L, x, bf, D, val);
+            }
+            nd = parent[nd];
+            d--;
+        }
+    }
+    if (dm > 0) {  // every walking lane on node u at depth dm
+        const uint64_t wl = __builtin_amdgcn_ballot_w64(valid && d > 0);
+        int32_t u = __builtin_amdgcn_readlane(int32_t(nd), int(__builtin_ctzll(wl)));
+        for (; dm > 0; dm--) {
+            const bool row = dm <= rows_depth && dnum[u] > 0;
+            float bx[3], by[3], bz[3];
+            if (valid) {
+                if (row) {
+                    node_basis(u, dm, px, py, pz, key, bf, D, bx, by, bz);
+                    val = row_sum(rows + kRowFloats * (int64_t(u) - L.base[1]), bx, by, bz, val);
+                } else {
+                    val = level_contrib(u, dm, px, py, pz, key, pnt, L, x, bf, D, val);
+                }
+            }
+            u = __builtin_amdgcn_readfirstlane(parent[u]);
+        }
+    }
+    float tt = x[0] * basis_value(bf, 0, px);
+    tt = tt * basis_value(bf, 0, py);
+    tt = tt * basis_value(bf, 0, pz);
+    return val + tt;
+}
+
 __global__ void __launch_bounds__(kBlockS) k_coarse_corner_value(const int32_t* list, const int32_t* n_dev,
                                                                  const uint32_t* key, const int32_t* pnt, Levels L,
                                                                  const int32_t* parent, const int32_t* child_base,
-                                                                 const float* x, const float* bf, int D,
+                                                                 const int32_t* dnum, const float* x, const float* bf,
+                                                                 const float* rows, int rows_depth, int D,
                                                                  const float* iso, float* out) {
     const int64_t i = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
-    if (i >= *n_dev) return;
-    const int64_t O = list[i] >> 3;
-    const int jo = list[i] & 7;
-    const int d = depth_of(O, L, D);
-    float c[3];
-    node_center(key[O], d, D, c);
-    const float hw = 1.0f / float(1 << (d + 1));
-    const float px = c[0] + float(2 * (jo & 1) - 1) * hw;
-    const float py = c[1] + float(2 * ((jo & 2) >> 1) - 1) * hw;
-    const float pz = c[2] + float(2 * ((jo & 4) >> 2) - 1) * hw;
-    float f = implicit_value(O, d, px, py, pz, key, pnt, L, parent, x, bf, D);
+    const bool valid = i < *n_dev;
+    if (__builtin_amdgcn_ballot_w64(valid) == 0) return;  // (wave-uniform)
+    int64_t O = 0;
+    int jo = 0, d = 0;
+    float px = 0.5f, py = 0.5f, pz = 0.5f;
+    if (valid) {
+        O = list[i] >> 3;
+        jo = list[i] & 7;
+        d = depth_of(O, L, D);
+        float c[3];
+        node_center(key[O], d, D, c);
+        const float hw = 1.0f / float(1 << (d + 1));
+        px = c[0] + float(2 * (jo & 1) - 1) * hw;
+        py = c[1] + float(2 * ((jo & 2) >> 1) - 1) * hw;
+        pz = c[2] + float(2 * ((jo & 4) >> 2) - 1) * hw;
+    }
+    // (every lane of the wave takes part in the chain walk's wave votes)
+    float f = coarse_chain_value(O, d, valid, px, py, pz, key, pnt, L, parent, dnum, x, bf, D, rows, rows_depth);
+    if (!valid) return;
     const int ex = c_children_vertex_kind[jo];
     int64_t nd = O;
     for (int dd = d + 1; dd <= D; dd++) {
@@ -1843,7 +1916,7 @@ void extract_subdivision(prg_ctx* c) {
         TimedScope ts(kt, "coarse_corner_value", 24.0 * double(baseD));
         k_coarse_corner_value<<<grid_for(8 * baseD, kBlockS), kBlockS, 0, s>>>(
             clist.get(), C + kCntCorners, c->key.get(), c->pnt.get(), Lv, c->parent.get(), c->child_base.get(),
-            c->x.get(), c->d_base_fn.get(), D, c->iso_dev.get(), cv.get());
+            c->dnum.get(), c->x.get(), c->d_base_fn.get(), c->coarse_rows.get(), rd, D, c->iso_dev.get(), cv.get());
         PRG_CHECK_LAUNCH();
     }
     // roots, their bricks and virtual leaves
