@@ -86,7 +86,8 @@ __constant__ int8_t c_children_vertex_kind[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 // slots of the device counter array
 enum { kCntCoarse = 0, kCntCorners = 1, kCntRoots = 2, kCntBricks = 3, kCntVerts = 4, kCntTris = 5, kCntActive = 6,
        kCntLeaves = 7, kCntErr = 8, kCntDense = 9, kCntDepBricks = 10, kCntDepPairs = 11, kCntWork = 12,
-       kCntExact = 13, kCntSuper = 14, kCntList2 = 15, kCntWork2 = 16, kCntSlots = 32 };
+       kCntExact = 13, kCntSuper = 14, kCntList2 = 15, kCntWork2 = 16, kCntExactSorted = 17, kCntWork3 = 18,
+       kCntWork4 = 19, kCntDenseSorted = 20, kCntSlots = 32 };
 static_assert(kCntSlots * 4 <= 128, "the pinned counter copy holds 32 ints");
 
 __device__ __forceinline__ int depth_of(int64_t i, const Levels& L, int D) {
@@ -197,6 +198,13 @@ __global__ void __launch_bounds__(kBlockS) k_flag_coarse_leaves(const int32_t* c
 __global__ void __launch_bounds__(kBlockS) k_compact(const int32_t* flag, const int32_t* off, int64_t n, int32_t* out) {
     const int64_t i = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
     if (i < n && flag[i]) out[off[i]] = int32_t(i);
+}
+
+// mark[list[i]] = 1 for the *n entries of a list of brick numbers below nb
+__global__ void __launch_bounds__(kBlockS) k_mark_listed(const int32_t* list, const int32_t* n, int64_t nb,
+                                                         int32_t* mark) {
+    const int64_t i = int64_t(blockIdx.x) * kBlockS + threadIdx.x;
+    if (i < nb && i < *n) mark[list[i]] = 1;
 }
 
 // Corner owners of the coarse leaves' corners among the same-depth nodes
@@ -1619,15 +1627,31 @@ __global__ void __launch_bounds__(kWave, 5) k_brick_screen(SubArgs A, const int3
 // Pass 0b: the exact certificate (brick_certify) for the bricks the interval
 // left (a fixed grid walks the list); the bricks it does not certify are
 // listed for the evaluation.
+// The list is in brick order (root-major), and a resident grid takes chunks
+// of consecutive entries from a counter, so consecutive bricks mostly share
+// their root and its record stays in LDS.
 __global__ void __launch_bounds__(kWave) k_brick_exact(SubArgs A, const int32_t* xlist, int8_t* sign, int32_t* list,
                                                        int32_t* dbricks, int32_t* dpairs, int32_t* cnt, int certify) {
     __shared__ BrickCore S;
-    const int32_t n = cnt[kCntExact];
-    for (int32_t it = blockIdx.x; it < n; it += gridDim.x) {
-        const int32_t b = xlist[it];
+    const int lane = threadIdx.x;
+    const int32_t n = cnt[kCntExactSorted];
+    // chunks of up to 8 entries, at least about four per workgroup
+    const int32_t chunk = std::max(1, std::min(8, n / (4 * int32_t(gridDim.x))));
+    int32_t q_loaded = -1, it = 0, it_end = 0;
+    for (;;) {
+        if (it == it_end) {
+            int32_t c0 = 0;
+            if (lane == 0) c0 = atomicAdd(cnt + kCntWork3, chunk);
+            c0 = __shfl(c0, 0, kWave);
+            if (c0 >= n) break;
+            it = c0;
+            it_end = std::min(n, c0 + chunk);
+        }
+        const int32_t b = xlist[it++];
         Brick B;
         __syncthreads();  // the previous brick's LDS reads are done
-        brick_setup(A, b, B, S, true);
+        brick_setup(A, b, B, S, true, q_loaded);
+        q_loaded = B.q;
         brick_tables(A, B, S);
         brick_blocks(A, B, S);
         Certificate c;
@@ -1659,15 +1683,30 @@ __global__ void __launch_bounds__(kBlockS) k_brick_deps(const int32_t* dbricks, 
 // (a fixed grid walks the list).
 // The corner values go to gvals (kBrickPts per listed brick) and the
 // brick's place in the list to didx, for the vertex pass.
-__global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, const int32_t* list, const int32_t* nlist,
+// The list is in brick order and a resident grid takes chunks of consecutive
+// entries from a counter (kCntWork4), so the root's record mostly stays in LDS.
+__global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, const int32_t* list, int32_t* cnt,
                                                        int32_t* bnv, int32_t* bnt, float* gvals, int32_t* didx) {
     __shared__ BrickLds S;
-    const int32_t n = *nlist;
-    for (int32_t it = blockIdx.x; it < n; it += gridDim.x) {
+    const int32_t n = cnt[kCntDenseSorted];
+    // chunks of up to 8 entries, at least about four per workgroup (a short
+    // list in long chunks leaves some workgroups with twice the others' work)
+    const int32_t chunk = std::max(1, std::min(8, n / (4 * int32_t(gridDim.x))));
+    int32_t q_loaded = -1, it = 0, it_end = 0;
+    for (;;) {
+        if (it == it_end) {
+            int32_t c0 = 0;
+            if (threadIdx.x == 0) c0 = atomicAdd(cnt + kCntWork4, chunk);
+            c0 = __shfl(c0, 0, kWave);
+            if (c0 >= n) break;
+            it = c0;
+            it_end = std::min(n, c0 + chunk);
+        }
         Brick B;
         const int32_t b = list[it];
         __syncthreads();  // the previous brick's LDS reads are done
-        brick_setup(A, b, B, S, true);
+        brick_setup(A, b, B, S, true, q_loaded);
+        q_loaded = B.q;
         brick_tables(A, B, S);
         brick_blocks(A, B, S);
         brick_values(A, B, S);
@@ -1689,6 +1728,7 @@ __global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, const int32_t*
             bnv[b] = nv;
             bnt[b] = nt;
         }
+        it++;
     }
 }
 
@@ -2023,13 +2063,31 @@ void extract_subdivision(prg_ctx* c) {
         k_brick_screen<<<unsigned(grid), kWave, 0, s>>>(A, list2.get(), bnv.get(), bnt.get(), bsign.get(), xlist.get(),
                                                         dbricks.get(), dpairs.get(), C, int(c->sub_certify));
         PRG_CHECK_LAUNCH();
+        // the exact certificate's list in brick order (marks, scan, compact)
+        PoolBuf<int32_t> xmark{&c->pool[141]}, xoff{&c->pool[142]}, xsorted{&c->pool[143]};
+        xmark.reserve(nb);
+        xoff.reserve(nb);
+        xsorted.reserve(nb);
+        PRG_HIP(hipMemsetAsync(xmark.get(), 0, size_t(nb) * 4, s));
+        k_mark_listed<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(xlist.get(), C + kCntExact, nb, xmark.get());
+        PRG_CHECK_LAUNCH();
+        scan_exclusive(xmark.get(), xoff.get(), nb, C + kCntExactSorted, s, c->scratch, kt);
+        k_compact<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(xmark.get(), xoff.get(), nb, xsorted.get());
+        PRG_CHECK_LAUNCH();
         PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_exact, kWave, 0));
         grid = std::min<int64_t>(nb, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
-        k_brick_exact<<<unsigned(grid), kWave, 0, s>>>(A, xlist.get(), bsign.get(), dlist.get(), dbricks.get(),
+        k_brick_exact<<<unsigned(grid), kWave, 0, s>>>(A, xsorted.get(), bsign.get(), dlist.get(), dbricks.get(),
                                                        dpairs.get(), C, int(c->sub_certify));
         PRG_CHECK_LAUNCH();
         k_brick_deps<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(dbricks.get(), dpairs.get(), bsign.get(),
                                                                 dlist.get(), C);
+        PRG_CHECK_LAUNCH();
+        // the evaluation's list in brick order, in place of the unordered one
+        PRG_HIP(hipMemsetAsync(xmark.get(), 0, size_t(nb) * 4, s));
+        k_mark_listed<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(dlist.get(), C + kCntDense, nb, xmark.get());
+        PRG_CHECK_LAUNCH();
+        scan_exclusive(xmark.get(), xoff.get(), nb, C + kCntDenseSorted, s, c->scratch, kt);
+        k_compact<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(xmark.get(), xoff.get(), nb, dlist.get());
         PRG_CHECK_LAUNCH();
     }
     PRG_HIP(hipMemcpyAsync(H, C, kCntSlots * 4, hipMemcpyDeviceToHost, s));
@@ -2045,8 +2103,8 @@ void extract_subdivision(prg_ctx* c) {
         PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_count, kWave, 0));
         const int64_t grid = std::min<int64_t>(ndense, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
         if (grid > 0) {
-            k_brick_count<<<unsigned(grid), kWave, 0, s>>>(A, dlist.get(), C + kCntDense, bnv.get(), bnt.get(),
-                                                           gvals.get(), didx.get());
+            k_brick_count<<<unsigned(grid), kWave, 0, s>>>(A, dlist.get(), C, bnv.get(), bnt.get(), gvals.get(),
+                                                           didx.get());
             PRG_CHECK_LAUNCH();
         }
     }
