@@ -87,7 +87,7 @@ __constant__ int8_t c_children_vertex_kind[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 enum { kCntCoarse = 0, kCntCorners = 1, kCntRoots = 2, kCntBricks = 3, kCntVerts = 4, kCntTris = 5, kCntActive = 6,
        kCntLeaves = 7, kCntErr = 8, kCntDense = 9, kCntDepBricks = 10, kCntDepPairs = 11, kCntWork = 12,
        kCntExact = 13, kCntSuper = 14, kCntList2 = 15, kCntWork2 = 16, kCntExactSorted = 17, kCntWork3 = 18,
-       kCntWork4 = 19, kCntDenseSorted = 20, kCntSlots = 32 };
+       kCntWork4 = 19, kCntDenseSorted = 20, kCntList2Sorted = 21, kCntSlots = 32 };
 static_assert(kCntSlots * 4 <= 128, "the pinned counter copy holds 32 ints");
 
 __device__ __forceinline__ int depth_of(int64_t i, const Levels& L, int D) {
@@ -1594,7 +1594,7 @@ __global__ void __launch_bounds__(kWave, 5) k_brick_screen(SubArgs A, const int3
                                                         int32_t* dpairs, int32_t* cnt, int certify) {
     __shared__ BrickScreenLds S;
     const int lane = threadIdx.x;
-    const int32_t nl = cnt[kCntList2];
+    const int32_t nl = cnt[kCntList2Sorted];
     constexpr int kChunk = 16;  // consecutive entries per grab: mostly one root, whose record stays in LDS
     int32_t q_loaded = -1, it = 0, it_end = 0;
     for (;;) {
@@ -2058,16 +2058,24 @@ void extract_subdivision(prg_ctx* c) {
         k_brick_screen_super<<<unsigned(grid), kWave, 0, s>>>(A, smap.get(), soff.get(), bnv.get(), bnt.get(),
                                                               bsign.get(), list2.get(), C, int(c->sub_certify));
         PRG_CHECK_LAUNCH();
+        // the lists the passes append to, each put in brick order (marks,
+        // scan, compact) before the next pass walks it
+        PoolBuf<int32_t> xmark{&c->pool[141]}, xoff{&c->pool[142]}, xsorted{&c->pool[143]};
+        xmark.reserve(nb);
+        xoff.reserve(nb);
+        xsorted.reserve(nb);
+        PRG_HIP(hipMemsetAsync(xmark.get(), 0, size_t(nb) * 4, s));
+        k_mark_listed<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(list2.get(), C + kCntList2, nb, xmark.get());
+        PRG_CHECK_LAUNCH();
+        scan_exclusive(xmark.get(), xoff.get(), nb, C + kCntList2Sorted, s, c->scratch, kt);
+        k_compact<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(xmark.get(), xoff.get(), nb, list2.get());
+        PRG_CHECK_LAUNCH();
         PRG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_brick_screen, kWave, 0));
         grid = std::min<int64_t>(nb, int64_t(std::max(1, per_cu)) * std::max(1, c->num_cu));
         k_brick_screen<<<unsigned(grid), kWave, 0, s>>>(A, list2.get(), bnv.get(), bnt.get(), bsign.get(), xlist.get(),
                                                         dbricks.get(), dpairs.get(), C, int(c->sub_certify));
         PRG_CHECK_LAUNCH();
-        // the exact certificate's list in brick order (marks, scan, compact)
-        PoolBuf<int32_t> xmark{&c->pool[141]}, xoff{&c->pool[142]}, xsorted{&c->pool[143]};
-        xmark.reserve(nb);
-        xoff.reserve(nb);
-        xsorted.reserve(nb);
+        // the exact certificate's list
         PRG_HIP(hipMemsetAsync(xmark.get(), 0, size_t(nb) * 4, s));
         k_mark_listed<<<grid_for(nb, kBlockS), kBlockS, 0, s>>>(xlist.get(), C + kCntExact, nb, xmark.get());
         PRG_CHECK_LAUNCH();
