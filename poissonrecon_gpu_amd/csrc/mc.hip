@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(kBlock) k_corner_basis(const float* bf, int D,
 }
 
 // f(corner) - iso for the owned corners (computeVertexImplicitFunctionValue,
-// main.cu:2259-2326): the chain sum of implicit_value at the corner's position,
+// main.cu:2259-2326): the chain sum (chain_value) at the corner's position,
 // with the basis values read from the corner basis table
 __global__ void __launch_bounds__(kBlock, 5) k_corner_value(const int32_t* __restrict__ vlist, int64_t V,
                                                          const uint32_t* __restrict__ key,

@@ -251,13 +251,6 @@ __device__ __forceinline__ float leaf_chain_value(int64_t nd, float px, float py
     return val + tt;
 }
 
-__device__ __forceinline__ float implicit_value(int64_t nd, int d, float px, float py, float pz,
-                                                const uint32_t* __restrict__ key, const int32_t* __restrict__ pnt,
-                                                const Levels& L, const int32_t* __restrict__ parent,
-                                                const float* __restrict__ x, const float* __restrict__ bf, int D) {
-    return chain_value(nd, d, px, py, pz, key, pnt, L, parent, x, bf, D, 0.0f);
-}
-
 // node centre (getNodeCenterAllDepth, main.cu:1403-1420) at the leaf level
 __device__ __forceinline__ void leaf_center(uint32_t k, int D, float c[3]) {
     c[0] = c[1] = c[2] = 0.5f;
@@ -280,14 +273,6 @@ __device__ __forceinline__ void node_center(uint32_t k, int d, int D, float c[3]
         c[2] = ((k >> (3 * i)) & 1u) ? c[2] + w : c[2] - w;
         w = w / 2;
     }
-}
-
-__device__ __forceinline__ int cube_of(const float* cval, int64_t t) {
-    int cube = 0;
-#pragma unroll
-    for (int q = 0; q < 8; q++)
-        if (cval[8 * t + q] < 0) cube |= 1 << q;
-    return cube;
 }
 
 __device__ __forceinline__ int tri_count(int cube) {
