@@ -141,7 +141,6 @@ constexpr int kMaxWG = 1024;       // workgroups of a depth in the vector kernel
 constexpr int kMaxSpmvWG = 2048;   // workgroups of a depth in the SpMV (its partial count)
 constexpr int kSpmvThreads = kTileThreads;  // one row per lane, kTile sibling blocks per tile
 constexpr int kHashSlots = 2048;          // > kTileEntries: linear probing always finds a slot
-constexpr int kGridRow = 72;              // floats per block grid row in the SpMV (64 cells + bank padding)
 static_assert(kHashSlots > kTileEntries, "hash must hold every entry of a tile");
 
 // r = b - A x0, p = r, xs = x0 (the solve works on the 16-byte-aligned copy
@@ -329,11 +328,15 @@ __global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 8 : (K <= 8 ?
     constexpr int kRidWords = kTileEntries / 2;  // row ids as 32-bit words
     extern __shared__ float cellv[];
     __shared__ int32_t rid32[kRidWords];
-    // each block's 4x4x4 neighbourhood, (x, y, z) = (c >> 4, (c >> 2) & 3, c & 3);
-    // rows padded to 72 floats: the 8 blocks of a wave start 8 banks apart, so
-    // a stencil read (ds_read2 over 64 lanes) meets at most 2 lanes per bank
-    // and the fill writes none (64-float rows put all 8 blocks in one bank set)
-    __shared__ float grid[kTile][kGridRow];
+    // each block's 4x4x4 neighbourhood, (x, y, z) = (c >> 4, (c >> 2) & 3, c & 3),
+    // block g's at grid + 272 (g >> 2) + 64 (g & 3) + {0, 2, 8, 10}[g & 3]: the
+    // four blocks of a half-wave (g >> 2 fixed) in windows that do not
+    // overlap.  Their lanes' rows sit at c0 in {0, 1, 4, 5, 16, 17, 20, 21},
+    // and the four shifts move those sets onto disjoint banks (the group
+    // stride, 16 mod 32, maps the sets onto each other), so a stencil read
+    // (ds_read2, constant offsets) meets one lane per bank; the fill writes
+    // meet at most 2.
+    __shared__ float grid[(kTile / 4) * 272];
     __shared__ double lds[kSpmvThreads / 64];
     const int16_t* rid = reinterpret_cast<const int16_t*>(rid32);
     const Seg& sg = segs[seg_of(segs, nseg, blockIdx.x, false)];
@@ -392,6 +395,7 @@ __global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 8 : (K <= 8 ?
     const int64_t tbeg = gbeg + l, tstep = m, tend = gend;
     // this lane's row sits at (1 + sx, 1 + sy, 1 + sz) of its block's grid
     const int c0 = 16 * ((s >> 2) & 1) + 4 * ((s >> 1) & 1) + (s & 1);
+    float* const gbase = grid + 272 * (g >> 2) + 64 * (g & 3) + ((g & 1) ? 2 : 0) + ((g & 2) ? 8 : 0);
     // plan of one tile: cell count, row-id words, block base of each of this thread's cells
     struct Plan {
         int32_t cnt;
@@ -463,7 +467,7 @@ __global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 8 : (K <= 8 ?
             const int cc = s + 8 * m;
             const int ax = (cc >> 4) + 1, ay = ((cc >> 2) & 3) + 1, az = (cc & 3) + 1;
             const int id = rid[27 * g + 9 * (ax >> 1) + 3 * (ay >> 1) + (az >> 1)];
-            grid[g][cc] = id < 0 ? 0.0f : cellv[8 * id + (((ax & 1) << 2) | ((ay & 1) << 1) | (az & 1))];
+            gbase[cc] = id < 0 ? 0.0f : cellv[8 * id + (((ax & 1) << 2) | ((ay & 1) << 1) | (az & 1))];
         }
         __syncthreads();
         // blocks past the last one have an empty plan (all-zero grids); their
@@ -474,7 +478,7 @@ __global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 8 : (K <= 8 ?
         {
             // absent neighbours hold +0: adding L*0 leaves a float sum
             // unchanged, so the result equals the sum over present terms
-            const float* gc = grid[g] + c0;
+            const float* gc = gbase + c0;
             float out = 0.0f;
 #pragma unroll
             for (int j = 0; j < 27; j++) {
