@@ -303,16 +303,13 @@ __global__ void __launch_bounds__(1024) k_plan_max(const int32_t* count, PlanOff
 // two or three (the launch asks the runtime's occupancy).
 constexpr int kSpmvSmallK = 5;
 
-// Global-memory views of a segment's pointers.  Pointers read from a Seg are
-// generic, and generic (flat) loads count against the LDS wait counter too:
-// every LDS wait of the tile loop would then also wait for the next tile's
-// loads in flight.  Global loads count only against the vector-memory counter.
-template <class T>
-using gmem = __attribute__((address_space(1))) T;
-
 // Raw buffer descriptor of `bytes` bytes at p (the gfx9 data-format word);
-// loads at offsets beyond it return 0.  Ranges are capped just below 4 GiB so
-// that kBufOob always lies outside (a depth holds fewer than 2^30 - 2 rows).
+// loads at offsets beyond it return 0 and stores there are dropped.  Ranges
+// are capped just below 4 GiB so that kBufOob always lies outside (a depth
+// holds fewer than 2^30 - 2 rows).  The SpMV's vector and plan traffic goes
+// through these rather than through the Seg's pointers: those are generic,
+// and generic (flat) loads count against the LDS wait counter too, so every
+// LDS wait of the tile loop would also wait for the next tile's loads.
 constexpr int kBufWord3 = 0x00020000;
 constexpr uint32_t kBufOob = 0xFFFFFFFCu;
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint64_t bytes) {
