@@ -449,6 +449,12 @@ constexpr int kXBlk = 2304;
 
 // LDS of one brick's wave: the root's neighbourhood, the brick's basis
 // tables and blocks (BrickCore), and for the evaluation its corner values.
+// One basis table entry of the brick (the owner's three values along one
+// axis): 12 bytes, the corner basis table's fourth float is never read.
+struct BasisT {
+    float x, y, z;
+};
+
 template <int kXBCap, int kTP = kP, bool kHasT = true>
 struct BrickCoreT {
     static constexpr int kTPts = kTP;  // grid coordinates per axis (of T, or of the screen's ranges)
@@ -456,15 +462,17 @@ struct BrickCoreT {
     // of the owner (own_cell) of grid coordinate org[a] + i, (d, a, i) at
     // d * 3 kP + a kP + i (the screens reduce the entries straight from the
     // global table and keep no T)
-    float4 T[kHasT ? (kMaxDepth + 1) * 3 * kTP : 1];
+    BasisT T[kHasT ? (kMaxDepth + 1) * 3 * kTP : 1];
     float X[(kMaxDepth + 1) * 27];     // the root's chain: neighbour values at depths 0..r
     float XB[kXBCap];                  // per depth below the root: its block (xb_off)
     int32_t xb_off[kMaxDepth + 1];     // start of depth dd's block in XB, or -1 (no real node in it)
     int32_t NB[27], NBK[27], NBR[27];  // the root's neighbours, their child bases, their root numbers (group)
     float tmax[kMaxDepth + 1];         // brick_abs_bound: per depth, the largest |basis value|
     float xsum[kMaxDepth + 1];         // and the sum of |coefficients|
-    float wlo[(kMaxDepth + 1) * 12];   // brick_interval_sign: per depth, axis and block slot,
-    float whi[(kMaxDepth + 1) * 12];   // the range of the weight over the brick's points
+    // brick_interval_sign (the screens, which keep no T): per depth, axis and
+    // block slot, the range of the weight over the brick's points
+    float wlo[kHasT ? 1 : (kMaxDepth + 1) * 12];
+    float whi[kHasT ? 1 : (kMaxDepth + 1) * 12];
 };
 
 using BrickCore = BrickCoreT<kXBlk>;
@@ -614,7 +622,8 @@ __device__ __forceinline__ void brick_tables(const SubArgs& A, const Brick& B, L
 #pragma unroll
         for (int ax = 0; ax < 3; ax++) {  // constant indices keep B in registers
             const int32_t g = B.org[ax] + i, o = own_cell(g, B.R0[ax]);
-            S.T[3 * TP * d + TP * ax + i] = A.btab[((int64_t(d) << D) + o) * 2 + (g - o)];
+            const float4 t = A.btab[((int64_t(d) << D) + o) * 2 + (g - o)];
+            S.T[3 * TP * d + TP * ax + i] = BasisT{t.x, t.y, t.z};
         }
     }
 }
@@ -736,9 +745,9 @@ __device__ __forceinline__ float deep_sum_lds(const SubArgs& A, const Brick& B, 
         const int W = block_width(D, B.kb, dd);
         const int bx0 = ax - 1 - ((B.org[0] >> sh) - 2), by0 = ay - 1 - ((B.org[1] >> sh) - 2),
                   bz0 = az - 1 - ((B.org[2] >> sh) - 2);
-        const float4 tx = S.T[3 * kP * dd + gi[0]];
-        const float4 ty = S.T[3 * kP * dd + kP + gi[1]];
-        const float4 tz = S.T[3 * kP * dd + 2 * kP + gi[2]];
+        const BasisT tx = S.T[3 * kP * dd + gi[0]];
+        const BasisT ty = S.T[3 * kP * dd + kP + gi[1]];
+        const BasisT tz = S.T[3 * kP * dd + 2 * kP + gi[2]];
         const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
         const float* Xb = S.XB + slot;
 #pragma unroll
@@ -808,9 +817,9 @@ __device__ __forceinline__ float deep_sum(const SubArgs& A, int r, const int32_t
 template <class L>
 __device__ __forceinline__ float chain_lds(const L& S, int r, const int gi[3], float f) {
     for (int d = r; d >= 1; d--) {
-        const float4 tx = S.T[3 * kP * d + gi[0]];
-        const float4 ty = S.T[3 * kP * d + kP + gi[1]];
-        const float4 tz = S.T[3 * kP * d + 2 * kP + gi[2]];
+        const BasisT tx = S.T[3 * kP * d + gi[0]];
+        const BasisT ty = S.T[3 * kP * d + kP + gi[1]];
+        const BasisT tz = S.T[3 * kP * d + 2 * kP + gi[2]];
         const float bx[3] = {tx.x, tx.y, tx.z}, by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
         const float* X = S.X + 27 * d;
 #pragma unroll
@@ -1008,7 +1017,7 @@ __device__ __forceinline__ int wave_excl_scan(int v) {
 // around the owners' common ancestor (3 slots per axis, weights (t.x, t.y,
 // t.z)).  contract64: a 4^3 block (cells A - 2 .. A + 1 around the brick's
 // ancestor A) with 4-slot weights.
-__device__ __forceinline__ void contract27(const float* X27, float4 ty, float4 tz, float Y[3]) {
+__device__ __forceinline__ void contract27(const float* X27, BasisT ty, BasisT tz, float Y[3]) {
     const float wy[3] = {ty.x, ty.y, ty.z}, wz[3] = {tz.x, tz.y, tz.z};
     float xv[27];
 #pragma unroll
@@ -1073,7 +1082,7 @@ __device__ __forceinline__ float brick_abs_bound(const Brick& B, L& S, int D) {
     for (int e = lane; e < (D + 1) * 3 * TP; e += kWave) {
         const int d = e / (3 * TP), i = e % TP;
         if (i >= P || (d > B.r && S.xb_off[d] < 0)) continue;
-        const float4 t = S.T[e];
+        const BasisT t = S.T[e];
         const float m = fmaxf(fabsf(t.x), fmaxf(fabsf(t.y), fabsf(t.z)));
         atomicMax(reinterpret_cast<int*>(&S.tmax[d]), __float_as_int(m));
     }
@@ -1106,6 +1115,7 @@ __device__ __forceinline__ float brick_abs_bound(const Brick& B, L& S, int D) {
 // pass of the T path compute.
 template <class L>
 __device__ __forceinline__ float brick_screen_bound(const SubArgs& A, const Brick& B, L& S) {
+    static_assert(sizeof(S.wlo) == (kMaxDepth + 1) * 12 * sizeof(float), "the screens' LDS holds the slot ranges");
     const int D = A.D, lane = threadIdx.x, P = (1 << B.kb) + 1, dlast = D - B.kb;
     if (lane <= kMaxDepth) S.tmax[lane] = S.xsum[lane] = 0.0f;
     __syncthreads();
@@ -1176,7 +1186,7 @@ __device__ __forceinline__ void axis_weights4(const Brick& B, const L& S, int D,
                                               float w[4]) {
     const int32_t g = B.org[ax] + i, o = own_cell(g, B.R0[ax]);
     const int delta = (o >> (D - d)) - (B.org[ax] >> (D - d));
-    const float4 t = S.T[3 * L::kTPts * d + L::kTPts * ax + i];
+    const BasisT t = S.T[3 * L::kTPts * d + L::kTPts * ax + i];
     if (delta == 0) {
         w[0] = 0.0f;
         w[1] = t.x;
@@ -1363,7 +1373,7 @@ __device__ __forceinline__ Certificate brick_certify(const SubArgs& A, const Bri
             contract27(S.X + 27 * d, S.T[3 * kP * d + kP + j], S.T[3 * kP * d + 2 * kP + k], Y);
 #pragma unroll
             for (int i = 0; i < kP; i++) {
-                const float4 t = S.T[3 * kP * d + i];
+                const BasisT t = S.T[3 * kP * d + i];
                 v[i] = __builtin_fmaf(t.x, Y[0], v[i]);
                 v[i] = __builtin_fmaf(t.y, Y[1], v[i]);
                 v[i] = __builtin_fmaf(t.z, Y[2], v[i]);
@@ -1393,14 +1403,14 @@ __device__ __forceinline__ Certificate brick_certify(const SubArgs& A, const Bri
             const int sh = D - d, W = block_width(D, kb, d);
             const int b1 = (own_cell(B.org[1] + j, B.R0[1]) >> sh) - 1 - ((B.org[1] >> sh) - 2);
             const int b2 = (own_cell(B.org[2] + k, B.R0[2]) >> sh) - 1 - ((B.org[2] >> sh) - 2);
-            const float4 ty = S.T[3 * kP * d + kP + j], tz = S.T[3 * kP * d + 2 * kP + k];
+            const BasisT ty = S.T[3 * kP * d + kP + j], tz = S.T[3 * kP * d + 2 * kP + k];
             const float by[3] = {ty.x, ty.y, ty.z}, bz[3] = {tz.x, tz.y, tz.z};
             int b0[kP];
             float bx[kP][3];
 #pragma unroll
             for (int i = 0; i < kP; i++) {
                 b0[i] = (own_cell(B.org[0] + i, B.R0[0]) >> sh) - 1 - ((B.org[0] >> sh) - 2);
-                const float4 tx = S.T[3 * kP * d + i];
+                const BasisT tx = S.T[3 * kP * d + i];
                 bx[i][0] = tx.x;
                 bx[i][1] = tx.y;
                 bx[i][2] = tx.z;
@@ -1630,7 +1640,7 @@ __global__ void __launch_bounds__(kWave, 5) k_brick_screen(SubArgs A, const int3
 // The list is in brick order (root-major), and a resident grid takes chunks
 // of consecutive entries from a counter, so consecutive bricks mostly share
 // their root and its record stays in LDS.
-__global__ void __launch_bounds__(kWave) k_brick_exact(SubArgs A, const int32_t* xlist, int8_t* sign, int32_t* list,
+__global__ void __launch_bounds__(kWave, 3) k_brick_exact(SubArgs A, const int32_t* xlist, int8_t* sign, int32_t* list,
                                                        int32_t* dbricks, int32_t* dpairs, int32_t* cnt, int certify) {
     __shared__ BrickCore S;
     const int lane = threadIdx.x;
