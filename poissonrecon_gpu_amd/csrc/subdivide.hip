@@ -486,13 +486,11 @@ using BrickScreenLds = BrickCoreT<kXBlkScreen, kP, false>;
 constexpr int kPSuper = 2 * (1 << kBrickLog) + 1;
 using BrickSuperLds = BrickCoreT<kXBlkScreen, kPSuper, false>;
 
-template <int kXBCap>
-struct BrickValsT : BrickCoreT<kXBCap> {
+// The vertex and triangle passes read the corner values the count pass
+// stored (the count pass keeps them over its blocks): no blocks.
+struct BrickMeshLds : BrickCoreT<1> {
     float G[kBrickPts];  // f - iso at the brick's corners: (i, j, k) at (k kP + j) kP + i
 };
-using BrickLds = BrickValsT<kXBlk>;
-// The vertex pass reads the corner values the count pass stored: no blocks.
-using BrickMeshLds = BrickValsT<1>;
 
 __device__ __forceinline__ void decode_cell(uint32_t k, int32_t c[3]) {
     c[0] = int32_t(compact3(k >> 2));
@@ -915,16 +913,46 @@ __device__ __forceinline__ float grid_value(const SubArgs& A, const Brick& B, co
     return f + tt - *A.iso;
 }
 
+// The brick's corner values (grid_value; 0 at the points a smaller brick
+// does not have), stored over its blocks, which nothing reads once the values
+// are known: each lane holds its values in registers until every lane's
+// block reads are done.  The blocks' LDS is then the corner values' (G), so
+// the evaluation needs no LDS beyond the certificate's.  The values also go
+// to gv (kBrickPts floats).
+constexpr int kValRounds = (kBrickPts + kWave - 1) / kWave;
+
 template <class L>
-__device__ __forceinline__ void brick_values(const SubArgs& A, const Brick& B, L& S) {
+__device__ __forceinline__ float* brick_values(const SubArgs& A, const Brick& B, L& S, float* gv) {
+    static_assert(sizeof(S.XB) >= kBrickPts * sizeof(float), "the corner values fit over the blocks");
     const int P = (1 << B.kb) + 1;
-    for (int p = threadIdx.x; p < kBrickPts; p += kWave) {
+    float held[kValRounds];
+#pragma unroll
+    for (int t = 0; t < kValRounds; t++) held[t] = 0.0f;
+    for (int t = 0; t < kValRounds; t++) {
+        const int p = threadIdx.x + kWave * t;
         const int i = p % kP, j = (p / kP) % kP, k = p / (kP * kP);
-        if (i >= P || j >= P || k >= P) continue;
-        const int32_t g[3] = {B.org[0] + i, B.org[1] + j, B.org[2] + k};
-        S.G[p] = grid_value(A, B, S, g);
+        float v = 0.0f;
+        if (p < kBrickPts && i < P && j < P && k < P) {
+            const int32_t g[3] = {B.org[0] + i, B.org[1] + j, B.org[2] + k};
+            v = grid_value(A, B, S, g);
+        }
+        // (a register queue: constant indices keep held out of scratch memory)
+#pragma unroll
+        for (int q = 0; q + 1 < kValRounds; q++) held[q] = held[q + 1];
+        held[kValRounds - 1] = v;
+    }
+    __syncthreads();  // the blocks' last reads
+    float* G = S.XB;
+#pragma unroll
+    for (int t = 0; t < kValRounds; t++) {
+        const int p = threadIdx.x + kWave * t;
+        if (p < kBrickPts) {
+            G[p] = held[t];
+            gv[p] = held[t];
+        }
     }
     __syncthreads();
+    return G;
 }
 
 // Owner of edge e of leaf v among the virtual leaves sharing it
@@ -959,8 +987,8 @@ __device__ __forceinline__ void edge_owner(const Brick& B, const L& S, int D, co
 // Cube index and owned crossing edges of leaf l of the brick
 // (generateSubdivideVexNums, main.cu:2481-2531; generateTriNums, :2562-2582).
 template <class L>
-__device__ __forceinline__ void leaf_cube(const Brick& B, const L& S, int D, int l, int32_t v[3], float cv[8],
-                                          int* cube, int* emask) {
+__device__ __forceinline__ void leaf_cube(const Brick& B, const L& S, const float* G, int D, int l, int32_t v[3],
+                                          float cv[8], int* cube, int* emask) {
     const int lx = int(compact3(uint32_t(l) >> 2)), ly = int(compact3(uint32_t(l) >> 1)), lz = int(compact3(uint32_t(l)));
     v[0] = B.org[0] + lx;
     v[1] = B.org[1] + ly;
@@ -969,7 +997,7 @@ __device__ __forceinline__ void leaf_cube(const Brick& B, const L& S, int D, int
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const int y = (q >> 1) & 1, xx = (q & 1) ^ y, z = q >> 2;
-        cv[q] = S.G[((lz + z) * kP + (ly + y)) * kP + (lx + xx)];
+        cv[q] = G[((lz + z) * kP + (ly + y)) * kP + (lx + xx)];
         if (cv[q] < 0) cb |= 1 << q;
     }
     *cube = cb;
@@ -1698,7 +1726,7 @@ __global__ void __launch_bounds__(kBlockS) k_brick_deps(const int32_t* dbricks, 
 // entries from a counter (kCntWork4), so the root's record mostly stays in LDS.
 __global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, const int32_t* list, int32_t* cnt,
                                                        int32_t* bnv, int32_t* bnt, float* gvals, int32_t* didx) {
-    __shared__ BrickLds S;
+    __shared__ BrickCore S;
     const int32_t n = cnt[kCntDenseSorted];
     // chunks of up to 8 entries, at least about four per workgroup (a short
     // list in long chunks leaves some workgroups with twice the others' work)
@@ -1720,8 +1748,7 @@ __global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, const int32_t*
         q_loaded = B.q;
         brick_tables(A, B, S);
         brick_blocks(A, B, S);
-        brick_values(A, B, S);
-        for (int p = threadIdx.x; p < kBrickPts; p += kWave) gvals[int64_t(it) * kBrickPts + p] = S.G[p];
+        const float* G = brick_values(A, B, S, gvals + int64_t(it) * kBrickPts);
         if (threadIdx.x == 0) didx[b] = it;
         int nv = 0, nt = 0;
         const int L = 1 << (3 * B.kb);
@@ -1729,7 +1756,7 @@ __global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, const int32_t*
             int32_t v[3];
             float cv[8];
             int cube, em;
-            leaf_cube(B, S, A.D, l, v, cv, &cube, &em);
+            leaf_cube(B, S, G, A.D, l, v, cv, &cube, &em);
             nv += __popc(uint32_t(em));
             nt += tri_count(cube);
         }
@@ -1786,7 +1813,7 @@ __global__ void __launch_bounds__(kWave) k_brick_vertices(SubArgs A, const int32
         int32_t v[3];
         float cv[8];
         int cube, em;
-        leaf_cube(B, S, D, l, v, cv, &cube, &em);
+        leaf_cube(B, S, S.G, D, l, v, cv, &cube, &em);
         emask_tab[int64_t(ai) * kBrickLeaves + l] = uint16_t(em);
         cube_tab[int64_t(ai) * kBrickLeaves + l] = uint8_t(cube);
         cnt += __popc(uint32_t(em));
@@ -1800,7 +1827,7 @@ __global__ void __launch_bounds__(kWave) k_brick_vertices(SubArgs A, const int32
         int32_t v[3];
         float cv[8];
         int cube, em2;
-        leaf_cube(B, S, D, l, v, cv, &cube, &em2);
+        leaf_cube(B, S, S.G, D, l, v, cv, &cube, &em2);
         float c[3];
         node_center(B.bkey | uint32_t(l), D, D, c);
         for (int e = 0; e < 12; e++) {
