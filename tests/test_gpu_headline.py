@@ -32,7 +32,7 @@ from mesh_checks import check_mc_structure
 
 pytestmark = pytest.mark.gpu
 
-CONFIGS = {
+WORKLOADS = {
     "torus_500k_d9": (lambda prg: prg.synth_torus(500000, 2), 9, 7),
     "superquadrics_2M_d10": (lambda prg: prg.synth_superquadrics(2000000, 3, 0.001, 3), 10, 8),
 }
@@ -40,10 +40,10 @@ BUFS_OCTREE = ["BUF_POINTS", "BUF_NORMALS", "BUF_POINT_LEAF", "BUF_KEY", "BUF_PI
                "BUF_CHILDREN", "BUF_DIDX", "BUF_DNUM", "BUF_NEIGHS"]
 
 
-@pytest.fixture(scope="module", params=list(CONFIGS))
+@pytest.fixture(scope="module", params=list(WORKLOADS))
 def case(request, prg):
     name = request.param
-    mk, D, min_root = CONFIGS[name]
+    mk, D, min_root = WORKLOADS[name]
     p, n = mk(prg)
     print(f"\n[headline case {name}: N={len(p)} D={D}]", flush=True)
     o = orc.Oracle(D)
