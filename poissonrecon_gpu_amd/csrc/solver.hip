@@ -350,21 +350,6 @@ __global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 8 : (K <= 8 ?
     double* pq_part = sg.pq;
     const int bid = int(blockIdx.x) - sg.wg0, nblocks = sg.nwg;
     const int k1 = it - 1;
-    const double rr = sum_partials<kSpmvThreads>(rr_part, np, lds);
-    const double bb = (it == 1) ? sum_partials<kSpmvThreads>(bb_part, np, lds) : st->bb;
-    const double tol2 = double(tol) * double(tol) * bb;
-    const bool stop = (bb <= 0.0) || (k1 >= max_iters) || (rr == 0.0) || (fixed > 0 ? (k1 >= fixed) : (rr <= tol2));
-    const float bf = (it == 1) ? 0.0f : float(rr / st->rr_hist[(k1 - 1) & 1]);
-    if (bid == 0 && threadIdx.x == 0) {
-        if (it == 1) st->bb = bb;
-        st->rr_hist[k1 & 1] = rr;
-        st->rr = rr;
-        if (stop) {
-            st->done = 1;
-            st->iters = k1;
-        }
-    }
-    if (stop) return;
     const bool first = (it == 1);
     const int tid = threadIdx.x;
     const int g = tid >> 3, s = tid & 7;
@@ -437,6 +422,22 @@ __global__ void __launch_bounds__(kSpmvThreads, K <= kSpmvSmallK ? 8 : (K <= 8 ?
     Plan pa{}, pb{};
     float va[K], vpa[K];
     load_plan(tbeg, pa);
+    // (the first plan is in flight while the partial sums load and reduce)
+    const double rr = sum_partials<kSpmvThreads>(rr_part, np, lds);
+    const double bb = (it == 1) ? sum_partials<kSpmvThreads>(bb_part, np, lds) : st->bb;
+    const double tol2 = double(tol) * double(tol) * bb;
+    const bool stop = (bb <= 0.0) || (k1 >= max_iters) || (rr == 0.0) || (fixed > 0 ? (k1 >= fixed) : (rr <= tol2));
+    const float bf = (it == 1) ? 0.0f : float(rr / st->rr_hist[(k1 - 1) & 1]);
+    if (bid == 0 && threadIdx.x == 0) {
+        if (it == 1) st->bb = bb;
+        st->rr_hist[k1 & 1] = rr;
+        st->rr = rr;
+        if (stop) {
+            st->done = 1;
+            st->iters = k1;
+        }
+    }
+    if (stop) return;
     load_cells(pa, va, vpa);
     load_plan(tbeg + tstep, pb);
     double pq = 0.0;
