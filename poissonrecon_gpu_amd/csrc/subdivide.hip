@@ -1668,7 +1668,9 @@ __global__ void __launch_bounds__(kWave, 5) k_brick_screen(SubArgs A, const int3
 // listed for the evaluation.
 // The list is in brick order (root-major), and a resident grid takes chunks
 // of consecutive entries from a counter, so consecutive bricks mostly share
-// their root and its record stays in LDS.
+// their root and its record stays in LDS.  Compiled for three waves per SIMD
+// (168 VGPRs, a few spilled; the LDS allows eleven workgroups per CU): the
+// waves wait on memory about half their time, and two per SIMD hid less.
 __global__ void __launch_bounds__(kWave, 3) k_brick_exact(SubArgs A, const int32_t* xlist, int8_t* sign, int32_t* list,
                                                        int32_t* dbricks, int32_t* dpairs, int32_t* cnt, int certify) {
     __shared__ BrickCore S;
@@ -1721,7 +1723,9 @@ __global__ void __launch_bounds__(kBlockS) k_brick_deps(const int32_t* dbricks, 
 // Pass 1: vertex and triangle counts of the bricks the certificate left
 // (a fixed grid walks the list).
 // The corner values go to gvals (kBrickPts per listed brick) and the
-// brick's place in the list to didx, for the vertex pass.
+// brick's place in the list to didx, for the vertex pass; in LDS they take
+// the place of the blocks (brick_values), so the pass needs the
+// certificate's LDS only.
 // The list is in brick order and a resident grid takes chunks of consecutive
 // entries from a counter (kCntWork4), so the root's record mostly stays in LDS.
 __global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, const int32_t* list, int32_t* cnt,
