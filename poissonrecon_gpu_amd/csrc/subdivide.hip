@@ -463,8 +463,10 @@ struct BrickCoreT {
     // of the owner (own_cell) of grid coordinate org[a] + i, (d, a, i) at
     // d * 3 kP + a kP + i (the screens reduce the entries straight from the
     // global table and keep no T)
-    BasisT T[kHasT ? (kMaxDepth + 1) * 3 * kTP : 1];
-    float X[(kMaxDepth + 1) * 27];     // the root's chain: neighbour values at depths 0..r
+    // (a layout without blocks, kXBCap = 1, is for the mesh passes, which
+    // evaluate nothing: no table and no chain)
+    BasisT T[kHasT && kXBCap > 1 ? (kMaxDepth + 1) * 3 * kTP : 1];
+    float X[kXBCap > 1 ? (kMaxDepth + 1) * 27 : 1];  // the root's chain: neighbour values at depths 0..r
     float XB[kXBCap];                  // per depth below the root: its block (xb_off)
     int32_t xb_off[kMaxDepth + 1];     // start of depth dd's block in XB, or -1 (no real node in it)
     int32_t NB[27], NBK[27], NBR[27];  // the root's neighbours, their child bases, their root numbers (group)
@@ -541,13 +543,14 @@ __global__ void __launch_bounds__(kBlockS) k_brick_map(const int32_t* boff, cons
     bmap[b] = lo;
 }
 
-// Loads brick b's root record into LDS and sets the brick's geometry.
+// Loads brick b's root record into LDS (with the chain's neighbour values
+// when kChain) and sets the brick's geometry.
 // q_loaded: the root whose record S already holds (the previous brick's, in a
 // kernel that walks consecutive bricks), or -1.  up = 1 sets up the
 // super-brick of the 2x2x2 bricks that start at brick b.
-template <class L>
-__device__ __forceinline__ void brick_setup(const SubArgs& A, int32_t b, Brick& B, L& S, bool chain,
-                                            int32_t q_loaded = -1, int up = 0) {
+template <bool kChain, class L>
+__device__ __forceinline__ void brick_setup(const SubArgs& A, int32_t b, Brick& B, L& S, int32_t q_loaded = -1,
+                                            int up = 0) {
     const int D = A.D, lane = threadIdx.x;
     B.q = __builtin_amdgcn_readfirstlane(A.bmap[b]);
     const int32_t* R = A.rec + int64_t(B.q) * kRecWords;
@@ -561,8 +564,10 @@ __device__ __forceinline__ void brick_setup(const SubArgs& A, int32_t b, Brick& 
             else if (w < kRecNBR) S.NBK[w - kRecNBK] = v;
             else S.NBR[w - kRecNBR] = v;
         }
-        if (chain)
+        if constexpr (kChain) {
+            static_assert(sizeof(S.X) == (kMaxDepth + 1) * 27 * sizeof(float), "the layout holds the chain");
             for (int w = lane; w < 27 * (B.r + 1); w += kWave) S.X[w] = __int_as_float(R[kRecX + w]);
+        }
     }
     B.kb = brick_log(D, B.r) + up;  // up = 1: the 2x2x2 bricks from brick b (a multiple of 8 in the root)
     int32_t rc[3];
@@ -1569,7 +1574,7 @@ __global__ void __launch_bounds__(kWave, 5) k_brick_screen_super(SubArgs A, cons
         const int32_t b0 = A.boff[q] + 8 * (sb - soff[q]);
         Brick B;
         __syncthreads();  // the previous super-brick's LDS reads are done
-        brick_setup(A, b0, B, S, true, q_loaded, 1);
+        brick_setup<true>(A, b0, B, S, q_loaded, 1);
         q_loaded = B.q;
         int sgn = 0;
         if (certify && !brick_in_group(B, S) && brick_blocks(A, B, S))
@@ -1648,7 +1653,7 @@ __global__ void __launch_bounds__(kWave, 5) k_brick_screen(SubArgs A, const int3
         const int32_t b = list2[it];
         Brick B;
         __syncthreads();  // the previous brick's LDS reads are done
-        brick_setup(A, b, B, S, true, q_loaded);
+        brick_setup<true>(A, b, B, S, q_loaded);
         q_loaded = B.q;
         Certificate c;
         c.sign = 0;
@@ -1691,7 +1696,7 @@ __global__ void __launch_bounds__(kWave, 3) k_brick_exact(SubArgs A, const int32
         const int32_t b = xlist[it++];
         Brick B;
         __syncthreads();  // the previous brick's LDS reads are done
-        brick_setup(A, b, B, S, true, q_loaded);
+        brick_setup<true>(A, b, B, S, q_loaded);
         q_loaded = B.q;
         brick_tables(A, B, S);
         brick_blocks(A, B, S);
@@ -1748,7 +1753,7 @@ __global__ void __launch_bounds__(kWave) k_brick_count(SubArgs A, const int32_t*
         Brick B;
         const int32_t b = list[it];
         __syncthreads();  // the previous brick's LDS reads are done
-        brick_setup(A, b, B, S, true, q_loaded);
+        brick_setup<true>(A, b, B, S, q_loaded);
         q_loaded = B.q;
         brick_tables(A, B, S);
         brick_blocks(A, B, S);
@@ -1802,7 +1807,7 @@ __global__ void __launch_bounds__(kWave) k_brick_vertices(SubArgs A, const int32
     Brick B;
     const int32_t ai = blockIdx.x;
     const int32_t b = alist[ai];
-    brick_setup(A, b, B, S, false);
+    brick_setup<false>(A, b, B, S);
     {  // the corner values the count pass computed
         const float* G = gvals + int64_t(didx[b]) * kBrickPts;
         for (int p = threadIdx.x; p < kBrickPts; p += kWave) S.G[p] = G[p];
@@ -1878,7 +1883,7 @@ __global__ void __launch_bounds__(kWave) k_brick_triangles(SubArgs A, const int3
     Brick B;
     const int32_t ai = blockIdx.x;
     const int32_t b = alist[ai];
-    brick_setup(A, b, B, S, false);
+    brick_setup<false>(A, b, B, S);
     const int D = A.D;
     const int L = 1 << (3 * B.kb);
     const int per = (L + kWave - 1) / kWave;
